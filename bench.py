@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""MLUPS and HBM-roofline fraction of the D3Q19 BGK 256^3 time step: FP32-shifted vs FP64.
+
+Workload: D3Q19 BGK, 256^3 triply periodic Taylor-Green vortex (u0 = 0.05, k = 2 pi/256,
+rho from the TGV pressure field, tau = 0.55), populations resident in HBM before timing.
+A step is one fused AA-pattern stream-collide kernel over all 16,777,216 cells.
+
+Timed: W warm-up steps, then exactly K steps between two stream synchronisations
+(wall clock), with a HIP event pair around every step launch on the grid's own stream
+(mean kernel duration -> achieved algorithmic GB/s).  Algorithmic bytes per cell per step
+are 2*Q*s (each population read once and written once): 304 B FP64, 152 B FP32-shifted,
+76 B FP16-shifted.
+
+Also reported: the same step in all three forms, the velocity L2 error and kinetic-energy
+ratio of the shifted forms against FP64 after the configured 5,000 steps, and a CPU
+baseline (the FP64 oracle restating LUMA's two-sweep AoS algorithm, OpenMP over x, on the
+host's cores, on a bounded number of steps of the same 256^3 workload).
+
+Prints one JSON line.  Only one GPU is used; --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS & %HBM roofline, D3Q19 256³: FP32-shifted vs FP64 at stated error"
+PEAK_GBS = 8000.0            # MI355X HBM3E
+ELEM = {"fp64": 8, "fp32s": 4, "fp16s": 2}
+PMC_FILE = ROOT / "profiles" / "tgv256_step_pmc.json"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_form(case, form, steps, warmup):
+    from luma_amd.grid import GpuGrid
+
+    ncell = case.ncell
+    with GpuGrid.from_case(case, form) as g:
+        g.run(warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        _, ker_ms = g.run_timed(steps)
+        g.sync()
+        t1 = time.perf_counter()
+    wall = t1 - t0
+    bpc = 2 * case.Q * ELEM[form]
+    achieved = bpc * ncell / (ker_ms * 1e-3) / 1e9
+    return {
+        "mlups": ncell * steps / wall / 1e6,
+        "ms_per_step": wall * 1e3 / steps,
+        "kernel_ms_mean": ker_ms,
+        "bytes_per_cell_step": bpc,
+        "achieved_gbs": achieved,
+        "roofline_frac": achieved / PEAK_GBS,
+    }
+
+
+def error_run(case, nsteps):
+    """Run every form nsteps from the same state; u error and KE of shifted forms vs FP64."""
+    import numpy as np
+
+    from luma_amd.grid import GpuGrid
+
+    out = {}
+    with GpuGrid.from_case(case, "fp64") as g64:
+        ke0 = g64.moments()["ke"]
+        g64.run(nsteps - 1)
+        g64.set_store_macro(True)
+        g64.run(1)
+        m64 = g64.moments()
+        _, u64 = g64.download_macro()
+    out["fp64"] = {"ke_ratio": m64["ke"] / ke0}
+    norm = np.linalg.norm(u64)
+    for form in ("fp32s", "fp16s"):
+        with GpuGrid.from_case(case, form) as g:
+            g.run(nsteps - 1)
+            g.set_store_macro(True)
+            g.run(1)
+            m = g.moments()
+            _, u = g.download_macro()
+        out[form] = {"u_l2_rel_vs_fp64": float(np.linalg.norm(u - u64) / norm),
+                     "ke_ratio": m["ke"] / ke0,
+                     "ke_rel_vs_fp64": m["ke"] / m64["ke"] - 1.0}
+        del u
+    nu = (0.55 - 0.5) / 3.0
+    k2 = 2 * (2 * math.pi / case.N) ** 2
+    out["ke_ratio_viscous_decay_estimate"] = math.exp(-2 * nu * k2 * nsteps)
+    return out
+
+
+def cpu_baseline(case, target_s=15.0):
+    from oracle.oracle import OracleGrid
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    g = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega)
+    g.init_feq(case.rho, case.u)
+    t0 = time.perf_counter()
+    g.run(1, threads)
+    one = time.perf_counter() - t0
+    n = int(max(2, min(50, target_s / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    g.run(n, threads)
+    dt = time.perf_counter() - t0
+    g.close()
+    return {"value": case.ncell * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"D3Q19 BGK TGV {case.N}^3 FP64, {n} steps after 1 warm-up: oracle restating "
+                      f"LUMA's LBM_multi_opt (AoS f/fNew pull, two sweeps), OpenMP over x"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="cube edge (cells)")
+    ap.add_argument("--error-steps", type=int, default=5000, help="0 skips the error run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--forms", default="fp64,fp32s,fp16s")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              f"(the multi-GPU slab decomposition is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    from luma_amd import setups
+
+    case = setups.taylor_green(args.n, u0=0.05, tau=0.55, dims=3)
+    forms = [f for f in args.forms.split(",") if f]
+    res = {}
+    for form in forms:
+        log(f"[bench] timing {form} {args.n}^3: warmup {args.warmup}, steps {args.steps}")
+        res[form] = time_form(case, form, args.steps, args.warmup)
+        log(f"[bench] {form}: {res[form]['mlups']:.0f} MLUPS, kernel {res[form]['kernel_ms_mean']:.3f} ms, "
+            f"{res[form]['achieved_gbs']:.0f} GB/s ({100 * res[form]['roofline_frac']:.1f}% of HBM)")
+    err = None
+    if args.error_steps > 0 and "fp64" in forms:
+        log(f"[bench] error run: {args.error_steps} steps per form")
+        err = error_run(case, args.error_steps)
+        log(f"[bench] errors: {json.dumps(err)}")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(case)
+        log(f"[bench] cpu: {cpu['value']:.1f} MLUPS on {cpu['cores']} threads")
+
+    head = "fp32s" if "fp32s" in res else forms[0]
+    r = res[head]
+    traffic = None
+    if PMC_FILE.exists():
+        pmc = json.loads(PMC_FILE.read_text())
+        traffic = pmc.get(head, {}).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"D3Q19 BGK Taylor-Green {args.n}^3 triply periodic, u0=0.05, tau=0.55",
+                   "form": head, "cells": case.ncell, "storage": {"fp64": "f64", "fp32s": "f32 shifted (f - w rho0)",
+                                                                  "fp16s": "f16 shifted (f - w rho0)"}[head],
+                   "bytes_per_cell_step": r["bytes_per_cell_step"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["roofline_frac"], 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "forms": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in res.items()},
+    }
+    if "fp64" in res and "fp32s" in res:
+        line["fp32s_over_fp64"] = round(res["fp32s"]["mlups"] / res["fp64"]["mlups"], 3)
+    if err is not None:
+        line["error"] = {"steps": args.error_steps, **err}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

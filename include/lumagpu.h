@@ -1,0 +1,121 @@
+/*
+ * lumagpu.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann time step.
+ *
+ * The reference has no FFI layer: its boundary is the C++ class API of GridObj and
+ * ObjectManager (LUMA/inc/GridObj.h:44-188, LUMA/inc/ObjectManager.h:141-194).  Each entry
+ * point below names the reference member it replaces.  All arrays crossing this boundary
+ * use LUMA's own host layouts:
+ *   site id       id = k + j*K + i*K*M            (LUMA/inc/IVector.h:94-101)
+ *   populations   f[v + Q*id], LUMA's c_opt order (LUMA/src/stdafx.cpp:81-153)
+ *   vectors       u[d + D*id], force_xyz[d + D*id]; scalars rho[id]
+ *   labels        int eType values                (LUMA/inc/Enumerations.h:84-96)
+ * Pointers are borrowed for the duration of the call only.  Every function returns 0 on
+ * success or a negative LUMAGPU_E* code; lumagpu_last_error() then holds a message
+ * (thread-local).  Calls on one grid are ordered on that grid's HIP stream; functions that
+ * return host data synchronise that stream.  A grid handle is used by one host thread.
+ */
+#ifndef LUMAGPU_H
+#define LUMAGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUMAGPU_ABI_VERSION 1
+
+#define LUMAGPU_OK 0
+#define LUMAGPU_EINVAL (-1)       /* bad argument or descriptor */
+#define LUMAGPU_EHIP (-2)         /* HIP runtime error */
+#define LUMAGPU_ENOMEM (-3)       /* device allocation failed */
+#define LUMAGPU_EUNSUPPORTED (-4) /* configuration not supported by this build */
+#define LUMAGPU_ESTATE (-5)       /* call out of order (e.g. step before state upload) */
+#define LUMAGPU_EPHYS (-6)        /* physically invalid input (omega >= 2, ...) */
+
+/* storage/arithmetic form of the populations */
+#define LUMAGPU_FP64 0   /* plain FP64 storage and arithmetic, LUMA's operation order */
+#define LUMAGPU_FP32S 1  /* FP32 storage of f - w*rho0, FP32 collision */
+#define LUMAGPU_FP16S 2  /* FP16 storage of f - w*rho0, FP32 collision */
+
+/* moments returned by lumagpu_moments, all FP64 */
+#define LUMAGPU_MOMENT_MASS 0
+#define LUMAGPU_MOMENT_MOMX 1
+#define LUMAGPU_MOMENT_MOMY 2
+#define LUMAGPU_MOMENT_MOMZ 3
+#define LUMAGPU_MOMENT_KE 4
+#define LUMAGPU_NUM_MOMENTS 5
+
+typedef struct lumagpu_grid lumagpu_grid;
+
+/* Replaces the compile-time configuration of LUMA/inc/definitions.h for one grid
+ * (GridObj(int level), GridObj_init_grids.cpp:155-384 / :390-536). */
+typedef struct lumagpu_grid_desc {
+    int dims;             /* L_DIMS: 2 or 3 */
+    int Q;                /* L_NUM_VELS: 9 (D2Q9) or 19 (D3Q19) */
+    int N, M, K;          /* N_lim, M_lim, K_lim (K = 1 in 2D) */
+    int precision;        /* LUMAGPU_FP64 / FP32S / FP16S */
+    int regularised;      /* L_REGULARISED_BOUNDARIES */
+    int forced;           /* L_IBM_ON || L_GRAVITY_ON: force_xyz enters macro and collision */
+    int fz_compat;        /* 3D: add F_z/2 to x-momentum as GridObj_ops_lbm_optimised.cpp:833 */
+    int level;            /* refinement level */
+    double omega;         /* GridObj::omega */
+    double rho_in;        /* L_RHOIN */
+    double rho0;          /* reference density of the shifted forms */
+    double xmax, ymax, zmax;   /* GridManager::global_edges[e?Max][0] of level 0 */
+    double wall_thk[6];        /* L_WALL_THICKNESS_{LEFT,RIGHT,BOTTOM,TOP,FRONT,BACK} */
+    int coarse_lims[6];        /* CoarseLimsX/Y/Z[min,max] on the parent (level > 0) */
+} lumagpu_grid_desc;
+
+/* ---- lifetime ------------------------------------------------------------------------ */
+/* GridObj::GridObj(int level) + LBM_initGrid allocation (GridObj.cpp:51-67) */
+int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out);
+/* GridObj::~GridObj (GridObj.cpp:39-43); children must be destroyed by the caller first */
+int lumagpu_grid_destroy(lumagpu_grid *g);
+
+/* ---- state upload / download --------------------------------------------------------- */
+/* LatTyp + XPos/YPos/ZPos (GridObj_init_grids.cpp:234-273; labels of LBM_initBoundLab) */
+int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, const double *ypos,
+                       const double *zpos);
+/* ux_in/uy_in/uz_in, one value per j (GridObj_init_grids.cpp:1322-1360) */
+int lumagpu_set_inlet(lumagpu_grid *g, const double *ux_in, const double *uy_in, const double *uz_in);
+/* rho and u, then f = feq(rho, u) (GridObj_init_grids.cpp:315-333) */
+int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u);
+/* GridObj::f after a step (post-collision populations) and the macro arrays rho, u */
+int lumagpu_upload_f(lumagpu_grid *g, const double *f);
+int lumagpu_upload_macro(lumagpu_grid *g, const double *rho, const double *u);
+int lumagpu_download_f(lumagpu_grid *g, double *f);
+int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u);
+/* GridObj::force_xyz as left by the caller (body force); requires desc.forced */
+int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz);
+
+/* ---- time step ----------------------------------------------------------------------- */
+/* GridObj::LBM_multi_opt(int subcycle) (GridObj.h:188, GridObj_ops_lbm_optimised.cpp:36-193),
+ * children first, without IBM work. */
+int lumagpu_step(lumagpu_grid *g, int subcycle);
+/* nsteps consecutive lumagpu_step(g, 0) calls */
+int lumagpu_run(lumagpu_grid *g, int nsteps);
+/* as lumagpu_run, timing with HIP events on the grid's stream: total wall time of the
+ * nsteps and the mean duration of one step kernel (event pair around each launch) */
+int lumagpu_run_timed(lumagpu_grid *g, int nsteps, double *ms_total, double *ms_kernel_mean);
+/* write rho and u from every step (LUMA keeps them in GridObj::rho/u); off by default */
+int lumagpu_set_store_macro(lumagpu_grid *g, int on);
+/* completed steps on this grid (GridObj::t) */
+int lumagpu_get_time(const lumagpu_grid *g, int *t);
+int lumagpu_sync(lumagpu_grid *g);
+
+/* ---- reductions (FP64, deterministic) --------------------------------------------------
+ * mass, momentum and kinetic energy of the post-collision state; LUMA has no counterpart
+ * (its logs hold per-marker forces only, ObjectManager_ops_io.cpp:65-128). */
+int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]);
+
+/* ---- diagnostics --------------------------------------------------------------------- */
+const char *lumagpu_last_error(void);   /* replaces L_ERROR's log line (stdafx.h:119-149) */
+int lumagpu_abi_version(void);
+/* bytes of population storage per cell and the population array size in bytes */
+int lumagpu_footprint(const lumagpu_grid *g, size_t *bytes_per_cell, size_t *f_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUMAGPU_H */

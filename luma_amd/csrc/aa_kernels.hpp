@@ -1,0 +1,644 @@
+// aa_kernels.hpp -- fused stream-collide kernels for the LBM time step, AA-pattern in place.
+//
+// One population array A (SoA: A[v*ps + cell], cell = x + nx*(y + ny*z), x = LUMA's i) is
+// updated in place.  Two kernels alternate:
+//   odd  kernel (array in "post-local" layout):  incoming f_v(x) = A[opp v][x - c_v];
+//                 outgoing f*_v(x) -> A[v][x + c_v]
+//   even kernel (array in "post-neighbour" layout): incoming f_v(x) = A[v][x];
+//                 outgoing f*_v(x) -> A[opp v][x]
+// Each population is read once and written once per step; no second array exists.  The
+// host uploads LUMA's post-collision f (GridObj::f after the swap) in the post-local layout
+// so the first step is an odd step.  Halfway bounce-back (LUMA's HWBB pull,
+// GridObj_ops_lbm_optimised.cpp:238-243) is folded into the odd kernel: a population whose
+// destination is solid is written to the cell's own opposite slot, where both the next even
+// step and the odd step's own read find it.
+//
+// Per-site arithmetic has two forms:
+//   * exact (FP64 storage): the expression trees of LUMA's _LBM_macro_opt (:800-893),
+//     _LBM_equilibrium_opt (:674-705), _LBM_forceGrid_opt (:928-990), _LBM_collide_opt
+//     (:765-790) and _LBM_regularised_opt (:313-510), operand for operand.  The translation
+//     unit that instantiates it is compiled with -ffp-contract=off, so results match a
+//     non-FMA x86-64 build of LUMA bit for bit.
+//   * shifted (FP32 / FP16 storage): populations are stored as f_v - w_v*rho0 and the
+//     collision works on those deviations in FP32; moments and equilibrium are the same
+//     polynomials rearranged around rho0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lattice.hpp"
+
+namespace lumagpu {
+
+struct P64 {
+    using S = double;
+    using T = double;
+    static constexpr bool kShift = false;
+};
+struct P32S {
+    using S = float;
+    using T = float;
+    static constexpr bool kShift = true;
+};
+struct P16S {
+    using S = _Float16;
+    using T = float;
+    static constexpr bool kShift = true;
+};
+
+struct StepArgs {
+    void *f;                      // population array
+    size_t ps;                    // population stride in elements
+    int nx, ny, nz;               // x = LUMA i (N_lim), y = j (M_lim), z = k (K_lim)
+    const uint8_t *flags;         // per-cell flag byte (masked kernels only)
+    const double *force;          // force_xyz, SoA [d*ncell + cell], read where flagged
+    const double *uin;            // inlet profile [a*ny + j]: ux_in, uy_in, uz_in
+    double *rho_out;              // macro output [cell] (store kernels)
+    double *u_out;                // macro output [d*ncell + cell]
+    size_t ncell;
+    double omega, rho0, rho_in;
+    double cs2, den2;             // cs*cs and 2*cs^2*cs^2, evaluated as LUMA does
+    int regularised;              // L_REGULARISED_BOUNDARIES
+    int fz_compat;                // 3D macro quirk (:833): F_z/2 added to x-momentum
+};
+
+// ----------------------------------------------------------------------------------------
+// storage helpers
+template <class P>
+__device__ __forceinline__ typename P::T ld(const typename P::S *p) {
+    return static_cast<typename P::T>(*p);
+}
+template <class P>
+__device__ __forceinline__ void st(typename P::S *p, typename P::T v) {
+    *p = static_cast<typename P::S>(v);
+}
+
+__device__ __forceinline__ size_t cell_of(int x, int y, int z, int nx, int ny) {
+    return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z);
+}
+
+// Periodic neighbour coordinates of (x,y,z): n[a][c+1] for displacement c in {-1,0,1}.
+struct Nbr {
+    int xs[3], ys[3], zs[3];
+    __device__ __forceinline__ void init(int x, int y, int z, int nx, int ny, int nz) {
+        xs[0] = (x == 0) ? nx - 1 : x - 1;
+        xs[1] = x;
+        xs[2] = (x == nx - 1) ? 0 : x + 1;
+        ys[0] = (y == 0) ? ny - 1 : y - 1;
+        ys[1] = y;
+        ys[2] = (y == ny - 1) ? 0 : y + 1;
+        zs[0] = (z == 0) ? nz - 1 : z - 1;
+        zs[1] = z;
+        zs[2] = (z == nz - 1) ? 0 : z + 1;
+    }
+    __device__ __forceinline__ size_t at(int dx, int dy, int dz, int nx, int ny) const {
+        return cell_of(xs[dx + 1], ys[dy + 1], zs[dz + 1], nx, ny);
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// exact (LUMA-order) per-site arithmetic
+
+// _LBM_equilibrium_opt, GridObj_ops_lbm_optimised.cpp:674-705
+template <int Q, class T>
+__device__ __forceinline__ T feq_exact(int v, T rho, const T *u, T cs2, T den2) {
+    using L = Lattice<Q>;
+    const int c0 = L::c(v, 0), c1 = L::c(v, 1), c2 = L::c(v, 2);
+    T A, B;
+    if constexpr (L::D == 3) {
+        A = (T(c0) * u[0]) + (T(c1) * u[1]) + (T(c2) * u[2]);
+        B = (T(c0 * c0) - cs2) * (u[0] * u[0]) + (T(c1 * c1) - cs2) * (u[1] * u[1]) +
+            (T(c2 * c2) - cs2) * (u[2] * u[2]) + T(2 * c0 * c1) * u[0] * u[1] +
+            T(2 * c0 * c2) * u[0] * u[2] + T(2 * c1 * c2) * u[1] * u[2];
+    } else {
+        A = (T(c0) * u[0]) + (T(c1) * u[1]);
+        B = (T(c0 * c0) - cs2) * (u[0] * u[0]) + (T(c1 * c1) - cs2) * (u[1] * u[1]) +
+            T(2 * c0 * c1) * u[0] * u[1];
+    }
+    return rho * T(L::w(v)) * (T(1.0) + (A / cs2) + (B / den2));
+}
+
+// _LBM_macro_opt, :800-847
+template <int Q, class T>
+__device__ __forceinline__ void macro_exact(const T *fin, const double *F, bool fz_compat, T &rho,
+                                            T *u) {
+    using L = Lattice<Q>;
+    T r = T(0), rx = T(0), ry = T(0), rz = T(0);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        r += fin[v];
+        rx += T(L::c(v, 0)) * fin[v];
+        ry += T(L::c(v, 1)) * fin[v];
+        if constexpr (L::D == 3) rz += T(L::c(v, 2)) * fin[v];
+    }
+    if (F) {
+        rx += T(0.5) * T(F[0]);
+        ry += T(0.5) * T(F[1]);
+        if constexpr (L::D == 3) {
+            if (fz_compat) rx += T(0.5) * T(F[2]);
+            else rz += T(0.5) * T(F[2]);
+        }
+    }
+    u[0] = rx / r;
+    u[1] = ry / r;
+    u[2] = T(0);
+    if constexpr (L::D == 3) u[2] = rz / r;
+    rho = r;
+}
+
+// _LBM_forceGrid_opt, :928-990 (one direction)
+template <int Q, class T>
+__device__ __forceinline__ T guo_exact(int v, const T *u, const double *F, T omega, T cs2) {
+    using L = Lattice<Q>;
+    T beta = T(0);
+    const T lambda = (T(1) - T(0.5) * omega) * (T(L::w(v)) / cs2);
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) beta += (T(L::c(v, d)) * u[d]);
+    beta = beta * (T(1) / cs2);
+    T fi = T(0);
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) fi += T(F[d]) * (T(L::c(v, d)) * (T(1) + beta) - u[d]);
+    fi *= lambda;
+    return fi;
+}
+
+// _LBM_collide_opt (BGK), :765-790; F == nullptr means no force term
+template <int Q, class T>
+__device__ __forceinline__ void collide_exact(const T *fin, T rho, const T *u, const double *F,
+                                              T omega, T cs2, T den2, T *fout) {
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const T feq = feq_exact<Q, T>(v, rho, u, cs2, den2);
+        if (F) fout[v] = fin[v] + (omega * (feq - fin[v]) + guo_exact<Q, T>(v, u, F, omega, cs2));
+        else fout[v] = fin[v] + omega * (feq - fin[v]);
+    }
+}
+
+// _LBM_regularised_opt for a velocity site on a face (edgeCount == 1), :313-510.
+// fa holds absolute populations after streaming; on return it holds the regularised set and
+// rho/u the site's macroscopic values.
+template <int Q, class T>
+__device__ __forceinline__ void regularised_face(T *fa, int ncode, const T *vel_in, T rho_in,
+                                                 T cs2, T den2, T &rho, T *u) {
+    using L = Lattice<Q>;
+    const int ndir = (ncode - 1) >> 1;              // 0:x 1:y 2:z
+    const int nsgn = ((ncode - 1) & 1) ? -1 : 1;    // inward normal component
+    T f_plus = T(0), f_zero = T(0);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int cn = L::c(v, ndir);
+        if (cn == -nsgn) f_plus += fa[v];
+        else if (cn == 0) f_zero += fa[v];
+    }
+    T vel[3] = {vel_in[0], vel_in[1], vel_in[2]};
+    T nv = vel[ndir];
+    if (nsgn == -1) nv *= T(-1.0);
+    (void)rho_in;
+    const T dens = (T(1.0) / (T(1.0) - nv)) * (T(2.0) * f_plus + f_zero);
+    rho = dens;
+    u[0] = vel[0];
+    u[1] = vel[1];
+    u[2] = (L::D == 3) ? vel[2] : T(0);
+    T Sxx = 0, Syy = 0, Sxy = 0, Szz = 0, Sxz = 0, Syz = 0;
+    // Unknown populations first (non-equilibrium bounce-back), in v order as LUMA does:
+    // each v's update reads fa[opp v], which for an unknown v is a known population.
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int c0 = L::c(v, 0), c1 = L::c(v, 1), c2 = L::c(v, 2);
+        const int o = L::opp(v);
+        if (L::c(v, ndir) == nsgn)
+            fa[v] = feq_exact<Q, T>(v, rho, u, cs2, den2) + (fa[o] - feq_exact<Q, T>(o, rho, u, cs2, den2));
+        const T fneq = fa[v] - feq_exact<Q, T>(v, rho, u, cs2, den2);
+        Sxx += T(c0 * c0) * fneq;
+        Syy += T(c1 * c1) * fneq;
+        Sxy += T(c0 * c1) * fneq;
+        if constexpr (L::D == 3) {
+            Szz += T(c2 * c2) * fneq;
+            Sxz += T(c0 * c2) * fneq;
+            Syz += T(c1 * c2) * fneq;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int c0 = L::c(v, 0), c1 = L::c(v, 1), c2 = L::c(v, 2);
+        fa[v] = feq_exact<Q, T>(v, rho, u, cs2, den2) +
+                (T(L::w(v)) / den2) *
+                    (((T(c0 * c0) - cs2) * Sxx) + ((T(c1 * c1) - cs2) * Syy) +
+                     ((T(c2 * c2) - cs2) * Szz) + (T(2.0) * T(c0) * T(c1) * Sxy) +
+                     (T(2.0) * T(c0) * T(c2) * Sxz) + (T(2.0) * T(c1) * T(c2) * Syz));
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// shifted (deviation) per-site arithmetic, FP32 compute
+
+template <int Q>
+__device__ __forceinline__ void macro_shift(const float *fin, const double *F, bool fz_compat,
+                                            float &drho, float *j) {
+    using L = Lattice<Q>;
+    float r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        r += fin[v];
+        if (L::c(v, 0) == 1) jx += fin[v];
+        if (L::c(v, 0) == -1) jx -= fin[v];
+        if (L::c(v, 1) == 1) jy += fin[v];
+        if (L::c(v, 1) == -1) jy -= fin[v];
+        if (L::c(v, 2) == 1) jz += fin[v];
+        if (L::c(v, 2) == -1) jz -= fin[v];
+    }
+    if (F) {
+        jx += 0.5f * float(F[0]);
+        jy += 0.5f * float(F[1]);
+        if constexpr (L::D == 3) {
+            if (fz_compat) jx += 0.5f * float(F[2]);
+            else jz += 0.5f * float(F[2]);
+        }
+    }
+    drho = r;
+    j[0] = jx;
+    j[1] = jy;
+    j[2] = jz;
+}
+
+// BGK on deviations: feq_v - w_v rho0 = w_v (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u)),
+// Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).
+template <int Q>
+__device__ __forceinline__ void collide_shift(const float *fin, float drho, float rho,
+                                              const float *u, const double *F, float omega,
+                                              float *fout) {
+    using L = Lattice<Q>;
+    const float usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    float Fx = 0.f, Fy = 0.f, Fz = 0.f, uF = 0.f;
+    const float gpre = 1.0f - 0.5f * omega;
+    if (F) {
+        Fx = float(F[0]);
+        Fy = float(F[1]);
+        if constexpr (L::D == 3) Fz = float(F[2]);
+        uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+    }
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const float cx = float(L::c(v, 0)), cy = float(L::c(v, 1)), cz = float(L::c(v, 2));
+        const float w = float(L::w(v));
+        const float cu = cx * u[0] + cy * u[1] + cz * u[2];
+        const float feq = w * (drho + rho * (3.0f * cu + 4.5f * cu * cu - usq15));
+        float o = fin[v] + omega * (feq - fin[v]);
+        if (F) {
+            const float cF = cx * Fx + cy * Fy + cz * Fz;
+            o += gpre * w * (3.0f * (cF - uF) + 9.0f * cu * cF);
+        }
+        fout[v] = o;
+    }
+}
+
+// equilibrium of (rho, u) in the storage convention of P (absolute or shifted)
+template <int Q, class P>
+__device__ __forceinline__ typename P::T feq_store(int v, double rho, const double *u,
+                                                   const StepArgs &a) {
+    double fe = feq_exact<Q, double>(v, rho, u, a.cs2, a.den2);
+    if constexpr (P::kShift) fe -= Lattice<Q>::w(v) * a.rho0;
+    return static_cast<typename P::T>(fe);
+}
+
+// ----------------------------------------------------------------------------------------
+// one-site update shared by all step kernels: macro, force, collide.
+template <int Q, class P>
+__device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *fout,
+                                            const double *F, const StepArgs &a, double &rho_o,
+                                            double *u_o, bool collide) {
+    using T = typename P::T;
+    if constexpr (!P::kShift) {
+        T rho, u[3];
+        macro_exact<Q, T>(fin, F, a.fz_compat != 0, rho, u);
+        if (collide) collide_exact<Q, T>(fin, rho, u, F, T(a.omega), T(a.cs2), T(a.den2), fout);
+        else
+            for (int v = 0; v < Q; ++v) fout[v] = fin[v];
+        rho_o = rho;
+        u_o[0] = u[0];
+        u_o[1] = u[1];
+        u_o[2] = u[2];
+    } else {
+        float drho, j[3], u[3];
+        macro_shift<Q>(fin, F, a.fz_compat != 0, drho, j);
+        const float rho = float(a.rho0) + drho;
+        const float inv = 1.0f / rho;
+        u[0] = j[0] * inv;
+        u[1] = j[1] * inv;
+        u[2] = j[2] * inv;
+        if (collide) collide_shift<Q>(fin, drho, rho, u, F, float(a.omega), fout);
+        else
+            for (int v = 0; v < Q; ++v) fout[v] = fin[v];
+        rho_o = double(float(a.rho0) + drho);
+        u_o[0] = u[0];
+        u_o[1] = u[1];
+        u_o[2] = u[2];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void store_macro(const StepArgs &a, size_t cell, double rho, const double *u) {
+    a.rho_out[cell] = rho;
+    a.u_out[cell] = u[0];
+    a.u_out[a.ncell + cell] = u[1];
+    if constexpr (D == 3) a.u_out[2 * a.ncell + cell] = u[2];
+}
+
+// ----------------------------------------------------------------------------------------
+// periodic box, no labels: the metric kernel
+template <int Q, class P, bool ODD, bool STORE>
+__global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
+    using L = Lattice<Q>;
+    using S = typename P::S;
+    using T = typename P::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    S *__restrict__ f = static_cast<S *>(a.f);
+    const size_t ps = a.ps;
+    const int nx = a.nx, ny = a.ny;
+    const size_t cell = cell_of(x, y, z, nx, ny);
+    Nbr nb;
+    if constexpr (ODD) nb.init(x, y, z, nx, ny, a.nz);
+
+    T fin[Q], fout[Q];
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        if constexpr (ODD)
+            fin[v] = ld<P>(f + (size_t)L::opp(v) * ps + nb.at(-L::c(v, 0), -L::c(v, 1), -L::c(v, 2), nx, ny));
+        else
+            fin[v] = ld<P>(f + (size_t)v * ps + cell);
+    }
+    double rho, u[3];
+    site_update<Q, P>(fin, fout, nullptr, a, rho, u, true);
+    if constexpr (STORE) store_macro<L::D>(a, cell, rho, u);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        if constexpr (ODD)
+            st<P>(f + (size_t)v * ps + nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), nx, ny), fout[v]);
+        else
+            st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// labelled grid: solid (HWBB), regularised or forced-equilibrium velocity sites, body force.
+template <int Q, class P, bool ODD, bool STORE>
+__global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
+    using L = Lattice<Q>;
+    using S = typename P::S;
+    using T = typename P::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const int nx = a.nx, ny = a.ny;
+    const size_t cell = cell_of(x, y, z, nx, ny);
+    const unsigned fl = a.flags[cell];
+    const int ty = fl & kTypeMask;
+    if (ty == kSolid || ty == kRefined || (!a.regularised && ty == kVelocity)) return;
+    S *__restrict__ f = static_cast<S *>(a.f);
+    const size_t ps = a.ps;
+    Nbr nb;
+    nb.init(x, y, z, nx, ny, a.nz);
+
+    T fin[Q], fout[Q];
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+        const size_t src = nb.at(-cx, -cy, -cz, nx, ny);
+        const int sty = a.flags[src] & kTypeMask;
+        if (!a.regularised && sty == kVelocity) {
+            // forced-equilibrium velocity source, :254-270; its rho, u never change
+            const int js = nb.ys[1 - cy];
+            const double us[3] = {a.uin[js], a.uin[ny + js], (L::D == 3) ? a.uin[2 * ny + js] : 0.0};
+            fin[v] = feq_store<Q, P>(v, a.rho_in, us, a);
+        } else if constexpr (ODD) {
+            fin[v] = (sty == kSolid) ? ld<P>(f + (size_t)v * ps + cell)
+                                     : ld<P>(f + (size_t)L::opp(v) * ps + src);
+        } else {
+            fin[v] = ld<P>(f + (size_t)v * ps + cell);
+        }
+    }
+
+    const double *F = nullptr;
+    double Fv[3];
+    if (fl & kForcedBit) {
+        Fv[0] = a.force[cell];
+        Fv[1] = a.force[a.ncell + cell];
+        Fv[2] = (L::D == 3) ? a.force[2 * a.ncell + cell] : 0.0;
+        F = Fv;
+    }
+
+    double rho, u[3];
+    if (ty == kVelocity) {
+        // regularised velocity face: BC gives rho, u; no macro on this site (:101-107)
+        const int ncode = (fl >> kNormalShift) & 7;
+        T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
+        T fa[Q];
+#pragma unroll
+        for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
+        T rr, uu[3];
+        regularised_face<Q, T>(fa, ncode, vin, T(a.rho_in), T(a.cs2), T(a.den2), rr, uu);
+        if constexpr (!P::kShift) {
+            collide_exact<Q, T>(fa, rr, uu, F, T(a.omega), T(a.cs2), T(a.den2), fout);
+        } else {
+#pragma unroll
+            for (int v = 0; v < Q; ++v) fin[v] = fa[v] - T(L::w(v) * a.rho0);
+            collide_shift<Q>(fin, rr - float(a.rho0), rr, uu, F, float(a.omega), fout);
+        }
+        rho = rr;
+        u[0] = uu[0];
+        u[1] = uu[1];
+        u[2] = uu[2];
+    } else {
+        site_update<Q, P>(fin, fout, F, a, rho, u, ty != kTL2C);
+    }
+    if constexpr (STORE) {
+        if (ty == kFluid || ty == kTL2F || ty == kVelocity) store_macro<L::D>(a, cell, rho, u);
+    }
+
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        if constexpr (ODD) {
+            const size_t dst = nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), nx, ny);
+            if ((a.flags[dst] & kTypeMask) == kSolid)
+                st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
+            else
+                st<P>(f + (size_t)v * ps + dst, fout[v]);
+        } else {
+            st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// layout conversion between LUMA's AoS f[v + Q*id] (id = k + j*K + i*K*M) and the device.
+
+__device__ __forceinline__ size_t luma_id(int x, int y, int z, int ny, int nz) {
+    return (size_t)z + (size_t)y * nz + (size_t)x * nz * ny;
+}
+
+// f = feq(rho, u) from LUMA-layout rho[id], u[d + D*id]   (GridObj_init_grids.cpp:315-333)
+template <int Q, class P>
+__global__ void k_init_feq(StepArgs a, const double *rho, const double *u) {
+    using L = Lattice<Q>;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const size_t cell = cell_of(x, y, z, a.nx, a.ny);
+    const size_t id = luma_id(x, y, z, a.ny, a.nz);
+    const double us[3] = {u[L::D * id], u[L::D * id + 1], (L::D == 3) ? u[L::D * id + 2] : 0.0};
+    typename P::S *f = static_cast<typename P::S *>(a.f);
+    for (int v = 0; v < Q; ++v)
+        st<P>(f + (size_t)L::opp(v) * a.ps + cell, feq_store<Q, P>(v, rho[id], us, a));
+}
+
+// upload post-collision f (AoS) into the post-local layout: A[opp v][x] = f_v(x)
+template <int Q, class P>
+__global__ void k_upload_aos(StepArgs a, const double *faos) {
+    using L = Lattice<Q>;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const size_t cell = cell_of(x, y, z, a.nx, a.ny);
+    const size_t id = luma_id(x, y, z, a.ny, a.nz);
+    typename P::S *f = static_cast<typename P::S *>(a.f);
+    for (int v = 0; v < Q; ++v) {
+        double fv = faos[v + Q * id];
+        if constexpr (P::kShift) fv -= L::w(v) * a.rho0;
+        st<P>(f + (size_t)L::opp(v) * a.ps + cell, typename P::T(fv));
+    }
+}
+
+// gather the post-collision f*_v(x) from the current layout
+template <int Q, class P>
+__device__ __forceinline__ double gather_post(const StepArgs &a, int x, int y, int z, int v, int parity,
+                                              bool masked) {
+    using L = Lattice<Q>;
+    const typename P::S *f = static_cast<const typename P::S *>(a.f);
+    const size_t cell = cell_of(x, y, z, a.nx, a.ny);
+    double fv;
+    if (parity == 0) {
+        fv = double(ld<P>(f + (size_t)L::opp(v) * a.ps + cell));
+    } else {
+        Nbr nb;
+        nb.init(x, y, z, a.nx, a.ny, a.nz);
+        const size_t dst = nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), a.nx, a.ny);
+        if (masked && (a.flags[dst] & kTypeMask) == kSolid)
+            fv = double(ld<P>(f + (size_t)L::opp(v) * a.ps + cell));
+        else
+            fv = double(ld<P>(f + (size_t)v * a.ps + dst));
+    }
+    if constexpr (P::kShift) fv += L::w(v) * a.rho0;
+    return fv;
+}
+
+template <int Q, class P>
+__global__ void k_download_aos(StepArgs a, double *faos, int parity, int masked) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const size_t id = luma_id(x, y, z, a.ny, a.nz);
+    for (int v = 0; v < Q; ++v) faos[v + Q * id] = gather_post<Q, P>(a, x, y, z, v, parity, masked != 0);
+}
+
+// Macro arrays (device SoA) -> LUMA layout
+template <int D>
+__global__ void k_macro_to_luma(const double *rho_s, const double *u_s, size_t ncell, int nx, int ny,
+                                int nz, double *rho_l, double *u_l) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    const size_t cell = cell_of(x, y, z, nx, ny);
+    const size_t id = luma_id(x, y, z, ny, nz);
+    rho_l[id] = rho_s[cell];
+    for (int d = 0; d < D; ++d) u_l[d + D * id] = u_s[d * ncell + cell];
+}
+
+template <int D>
+__global__ void k_macro_from_luma(double *rho_s, double *u_s, size_t ncell, int nx, int ny, int nz,
+                                  const double *rho_l, const double *u_l) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    const size_t cell = cell_of(x, y, z, nx, ny);
+    const size_t id = luma_id(x, y, z, ny, nz);
+    rho_s[cell] = rho_l[id];
+    for (int d = 0; d < D; ++d) u_s[d * ncell + cell] = u_l[d + D * id];
+}
+
+// ----------------------------------------------------------------------------------------
+// FP64 moments of the post-collision state: mass, momentum (3), kinetic energy.
+// Deterministic: fixed-shape wave/block trees, per-block partials summed in block order.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+constexpr int kMomentCount = 5;
+
+template <int Q, class P>
+__global__ __launch_bounds__(256) void k_moments(StepArgs a, int parity, int masked, double *partials) {
+    using L = Lattice<Q>;
+    __shared__ double sh[4][kMomentCount];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    double m[kMomentCount] = {0, 0, 0, 0, 0};
+    if (x < a.nx) {
+        const size_t cell = cell_of(x, y, z, a.nx, a.ny);
+        const int ty = masked ? (a.flags[cell] & kTypeMask) : kFluid;
+        if (ty != kSolid && ty != kRefined) {
+            double r = 0, j[3] = {0, 0, 0};
+            for (int v = 0; v < Q; ++v) {
+                const double fv = gather_post<Q, P>(a, x, y, z, v, parity, masked != 0);
+                r += fv;
+                j[0] += L::c(v, 0) * fv;
+                j[1] += L::c(v, 1) * fv;
+                j[2] += L::c(v, 2) * fv;
+            }
+            m[0] = r;
+            m[1] = j[0];
+            m[2] = j[1];
+            m[3] = j[2];
+            m[4] = 0.5 * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]) / r;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kMomentCount; ++k) {
+        const double s = wave_sum(m[k]);
+        if (lane == 0) sh[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMomentCount) {
+        double s = 0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) s += sh[w][threadIdx.x];
+        const size_t b = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+        partials[b * kMomentCount + threadIdx.x] = s;
+    }
+}
+
+static __global__ void k_sum_partials(const double *partials, size_t nblocks, double *out) {
+    // one block of 256: strided partial sums, then a fixed tree
+    __shared__ double sh[4][kMomentCount];
+    double acc[kMomentCount] = {0, 0, 0, 0, 0};
+    for (size_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < kMomentCount; ++k) acc[k] += partials[b * kMomentCount + k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < kMomentCount; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0) sh[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMomentCount) {
+        double s = 0;
+        for (int w = 0; w < 4; ++w) s += sh[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+}  // namespace lumagpu

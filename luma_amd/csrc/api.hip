@@ -1,0 +1,471 @@
+// api.hip -- the C ABI (include/lumagpu.h): grid handles, state transfer and the step driver.
+//
+// A grid owns one population array in the AA in-place layout, a per-cell flag byte (only
+// for labelled grids), optional FP64 macro arrays and force field, and one HIP stream at the
+// default priority.  `parity` records which layout the population array is in:
+//   0  post-local: A[opp v][x] = f_v(x)   (after upload, init, or an even step)
+//   1  post-neighbour: A[v][x + c_v] = f_v(x), or A[opp v][x] where x + c_v is solid
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lumagpu.h"
+#include "launch.hpp"
+
+using namespace lumagpu;
+
+struct lumagpu_grid {
+    lumagpu_grid_desc d;
+    int Q = 0, D = 0;
+    size_t ncell = 0, ps = 0, elem = 0;
+    void *f = nullptr;
+    uint8_t *flags = nullptr;
+    double *force = nullptr;
+    double *uin = nullptr;
+    double *rho = nullptr, *u = nullptr;
+    double *partials = nullptr;
+    double *stage = nullptr;       // LUMA-layout staging (f AoS sized)
+    size_t stage_bytes = 0;
+    double *moments_out = nullptr;
+    hipStream_t stream = nullptr;
+    int masked = 0;
+    int has_state = 0;
+    int store_macro = 0;
+    int parity = 0;
+    int t = 0;
+    std::vector<uint8_t> hflags;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(LUMAGPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+StepArgs make_args(const lumagpu_grid *g) {
+    StepArgs a{};
+    a.f = g->f;
+    a.ps = g->ps;
+    a.nx = g->d.N;
+    a.ny = g->d.M;
+    a.nz = g->d.K;
+    a.flags = g->flags;
+    a.force = g->force;
+    a.uin = g->uin;
+    a.rho_out = g->rho;
+    a.u_out = g->u;
+    a.ncell = g->ncell;
+    a.omega = g->d.omega;
+    a.rho0 = g->d.rho0;
+    a.rho_in = g->d.rho_in;
+    const double cs = 1.0 / std::sqrt(3.0);    // LUMA/src/stdafx.cpp:152
+    a.cs2 = cs * cs;
+    a.den2 = 2.0 * (cs * cs) * (cs * cs);
+    a.regularised = g->d.regularised;
+    a.fz_compat = g->d.fz_compat;
+    return a;
+}
+
+hipError_t launch_step(const lumagpu_grid *g, const LaunchSpec &ls, const StepArgs &a) {
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_step_f64(ls, a, g->stream);
+        case LUMAGPU_FP32S: return launch_step_f32s(ls, a, g->stream);
+        default: return launch_step_f16s(ls, a, g->stream);
+    }
+}
+
+hipError_t launch_aux(const lumagpu_grid *g, int op, const double *p0, const double *p1, double *out) {
+    const StepArgs a = make_args(g);
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_aux_f64(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
+        case LUMAGPU_FP32S: return launch_aux_f32s(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
+        default: return launch_aux_f16s(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
+    }
+}
+
+int ensure_stage(lumagpu_grid *g, size_t bytes) {
+    if (g->stage_bytes >= bytes) return 0;
+    if (g->stage) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipFree(g->stage));
+        g->stage = nullptr;
+        g->stage_bytes = 0;
+    }
+    if (hipMalloc(&g->stage, bytes) != hipSuccess) return fail(LUMAGPU_ENOMEM, "staging allocation failed");
+    g->stage_bytes = bytes;
+    return 0;
+}
+
+int check_grid(const lumagpu_grid *g) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    return 0;
+}
+
+// GridUtils::isWithinDomainWall, LUMA/src/GridUtils.cpp:1369-1430: returns edge count and
+// the code of the last inward normal found (1..6 = +x,-x,+y,-y,+z,-z).
+int wall_normal(const lumagpu_grid_desc &d, double px, double py, double pz, int *code) {
+    const double *th = d.wall_thk;
+    int cnt = 0;
+    *code = 0;
+    if (px > 0.0 && px < th[0]) { *code = 1; cnt++; }
+    if (px < d.xmax && px > d.xmax - th[1]) { *code = 2; cnt++; }
+    if (py > 0.0 && py < th[2]) { *code = 3; cnt++; }
+    if (py < d.ymax && py > d.ymax - th[3]) { *code = 4; cnt++; }
+    if (d.dims == 3) {
+        if (pz > 0.0 && pz < th[4]) { *code = 5; cnt++; }
+        if (pz < d.zmax && pz > d.zmax - th[5]) { *code = 6; cnt++; }
+    }
+    return cnt;
+}
+
+int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
+    (void)subcycle;
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
+    LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0};
+    StepArgs a = make_args(g);
+    if (e0) HIP_TRY(hipEventRecord(e0, g->stream));
+    hipError_t e = launch_step(g, ls, a);
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("step launch: ") + hipGetErrorString(e));
+    if (e1) HIP_TRY(hipEventRecord(e1, g->stream));
+    g->parity ^= 1;
+    g->t += 1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lumagpu_last_error(void) { return g_last_error.c_str(); }
+int lumagpu_abi_version(void) { return LUMAGPU_ABI_VERSION; }
+
+int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
+    if (!desc || !out) return fail(LUMAGPU_EINVAL, "null descriptor or output pointer");
+    *out = nullptr;
+    const lumagpu_grid_desc &d = *desc;
+    if (!((d.dims == 2 && d.Q == 9) || (d.dims == 3 && d.Q == 19)))
+        return fail(LUMAGPU_EUNSUPPORTED, "lattice must be D2Q9 or D3Q19");
+    if (d.N < 3 || d.M < 3 || d.K < 1 || (d.dims == 2 && d.K != 1) || (d.dims == 3 && d.K < 3))
+        return fail(LUMAGPU_EINVAL, "grid extents too small (need >= 3 per active dimension; K = 1 in 2D)");
+    if (d.precision < LUMAGPU_FP64 || d.precision > LUMAGPU_FP16S)
+        return fail(LUMAGPU_EINVAL, "unknown precision");
+    // GridObj_init_grids.cpp:353-356
+    if (!(d.omega > 0.0) || d.omega >= 2.0)
+        return fail(LUMAGPU_EPHYS, "LBM relaxation frequency omega too large (must be 0 < omega < 2)");
+    if (d.level != 0) return fail(LUMAGPU_EUNSUPPORTED, "refinement levels are not supported by this build yet");
+
+    lumagpu_grid *g = new lumagpu_grid();
+    g->d = d;
+    g->Q = d.Q;
+    g->D = d.dims;
+    g->ncell = (size_t)d.N * d.M * d.K;
+    g->ps = (g->ncell + 255) / 256 * 256;
+    g->elem = d.precision == LUMAGPU_FP64 ? 8 : d.precision == LUMAGPU_FP32S ? 4 : 2;
+    auto cleanup = [&](int code, const std::string &m) {
+        lumagpu_grid_destroy(g);
+        return fail(code, m);
+    };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(LUMAGPU_EHIP, "hipStreamCreate failed");
+    if (hipMalloc(&g->f, g->ps * g->Q * g->elem) != hipSuccess)
+        return cleanup(LUMAGPU_ENOMEM, "population array allocation failed");
+    if (hipMalloc(&g->rho, g->ncell * sizeof(double)) != hipSuccess ||
+        hipMalloc(&g->u, g->ncell * g->D * sizeof(double)) != hipSuccess)
+        return cleanup(LUMAGPU_ENOMEM, "macro array allocation failed");
+    if (hipMalloc(&g->uin, 3 * (size_t)d.M * sizeof(double)) != hipSuccess)
+        return cleanup(LUMAGPU_ENOMEM, "inlet array allocation failed");
+    (void)hipMemsetAsync(g->uin, 0, 3 * (size_t)d.M * sizeof(double), g->stream);
+    const dim3 gr = cell_grid(d.N, d.M, d.K);
+    const size_t nblocks = (size_t)gr.x * gr.y * gr.z;
+    if (hipMalloc(&g->partials, nblocks * kMomentCount * sizeof(double)) != hipSuccess ||
+        hipMalloc(&g->moments_out, kMomentCount * sizeof(double)) != hipSuccess)
+        return cleanup(LUMAGPU_ENOMEM, "reduction buffers allocation failed");
+    if (d.forced) {
+        if (hipMalloc(&g->force, g->ncell * g->D * sizeof(double)) != hipSuccess)
+            return cleanup(LUMAGPU_ENOMEM, "force array allocation failed");
+        (void)hipMemsetAsync(g->force, 0, g->ncell * g->D * sizeof(double), g->stream);
+    }
+    *out = g;
+    return 0;
+}
+
+int lumagpu_grid_destroy(lumagpu_grid *g) {
+    if (!g) return 0;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    (void)hipFree(g->f);
+    (void)hipFree(g->flags);
+    (void)hipFree(g->force);
+    (void)hipFree(g->uin);
+    (void)hipFree(g->rho);
+    (void)hipFree(g->u);
+    (void)hipFree(g->partials);
+    (void)hipFree(g->stage);
+    (void)hipFree(g->moments_out);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, const double *ypos,
+                       const double *zpos) {
+    if (int r = check_grid(g)) return r;
+    if (!lattyp) return fail(LUMAGPU_EINVAL, "null label array");
+    const int N = g->d.N, M = g->d.M, K = g->d.K;
+    // keep the force bits a previous lumagpu_set_force call placed
+    std::vector<uint8_t> prev = std::move(g->hflags);
+    if (prev.size() != g->ncell) prev.assign(g->ncell, 0);
+    g->hflags.assign(g->ncell, 0);
+    int any = 0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < M; ++j)
+            for (int k = 0; k < K; ++k) {
+                const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+                const int ty = lattyp[id];
+                uint8_t fl = 0;
+                switch (ty) {
+                    case kSolid:
+                    case kFluid: fl = (uint8_t)ty; break;
+                    case kVelocity: {
+                        fl = (uint8_t)ty;
+                        if (g->d.regularised) {
+                            if (!xpos || !ypos || !zpos)
+                                return fail(LUMAGPU_EINVAL, "positions are required for regularised velocity sites");
+                            int code = 0;
+                            const int cnt = wall_normal(g->d, xpos[i], ypos[j], zpos[k], &code);
+                            if (cnt == 0)
+                                return fail(LUMAGPU_EINVAL,
+                                            "Velocity site not outside domain walls is currently not supported.");
+                            if (cnt > 1)
+                                return fail(LUMAGPU_EUNSUPPORTED,
+                                            "regularised velocity site on an edge or corner is not supported");
+                            fl |= (uint8_t)(code << kNormalShift);
+                        }
+                        break;
+                    }
+                    default:
+                        return fail(LUMAGPU_EUNSUPPORTED,
+                                    "site label " + std::to_string(ty) + " is not supported by this build yet");
+                }
+                if (ty != kFluid) any = 1;
+                const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
+                g->hflags[cell] = (uint8_t)(fl | (prev[cell] & kForcedBit));
+            }
+    if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
+    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->masked = any || g->d.forced;
+    return 0;
+}
+
+int lumagpu_set_inlet(lumagpu_grid *g, const double *ux, const double *uy, const double *uz) {
+    if (int r = check_grid(g)) return r;
+    const size_t M = g->d.M;
+    std::vector<double> h(3 * M, 0.0);
+    for (size_t j = 0; j < M; ++j) {
+        h[j] = ux ? ux[j] : 0.0;
+        h[M + j] = uy ? uy[j] : 0.0;
+        h[2 * M + j] = uz ? uz[j] : 0.0;
+    }
+    HIP_TRY(hipMemcpyAsync(g->uin, h.data(), 3 * M * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u) {
+    if (int r = check_grid(g)) return r;
+    if (!rho || !u) return fail(LUMAGPU_EINVAL, "null rho or u");
+    const size_t nb = g->ncell * (1 + g->D) * sizeof(double);
+    if (int r = ensure_stage(g, nb)) return r;
+    double *drho = g->stage, *du = g->stage + g->ncell;
+    HIP_TRY(hipMemcpyAsync(drho, rho, g->ncell * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(du, u, g->ncell * g->D * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(launch_aux(g, kAuxInitFeq, drho, du, nullptr));
+    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    if (g->D == 2) k_macro_from_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
+    else k_macro_from_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->parity = 0;
+    g->has_state = 1;
+    return 0;
+}
+
+int lumagpu_upload_f(lumagpu_grid *g, const double *f) {
+    if (int r = check_grid(g)) return r;
+    if (!f) return fail(LUMAGPU_EINVAL, "null f");
+    const size_t nb = g->ncell * g->Q * sizeof(double);
+    if (int r = ensure_stage(g, nb)) return r;
+    HIP_TRY(hipMemcpyAsync(g->stage, f, nb, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(launch_aux(g, kAuxUpload, g->stage, nullptr, nullptr));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->parity = 0;
+    g->has_state = 1;
+    return 0;
+}
+
+int lumagpu_upload_macro(lumagpu_grid *g, const double *rho, const double *u) {
+    if (int r = check_grid(g)) return r;
+    if (!rho || !u) return fail(LUMAGPU_EINVAL, "null rho or u");
+    const size_t nb = g->ncell * (1 + g->D) * sizeof(double);
+    if (int r = ensure_stage(g, nb)) return r;
+    double *drho = g->stage, *du = g->stage + g->ncell;
+    HIP_TRY(hipMemcpyAsync(drho, rho, g->ncell * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(du, u, g->ncell * g->D * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    if (g->D == 2) k_macro_from_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
+    else k_macro_from_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_download_f(lumagpu_grid *g, double *f) {
+    if (int r = check_grid(g)) return r;
+    if (!f) return fail(LUMAGPU_EINVAL, "null f");
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "no state to download");
+    const size_t nb = g->ncell * g->Q * sizeof(double);
+    if (int r = ensure_stage(g, nb)) return r;
+    HIP_TRY(launch_aux(g, kAuxDownload, nullptr, nullptr, g->stage));
+    HIP_TRY(hipMemcpyAsync(f, g->stage, nb, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u) {
+    if (int r = check_grid(g)) return r;
+    if (!rho || !u) return fail(LUMAGPU_EINVAL, "null rho or u");
+    const size_t nb = g->ncell * (1 + g->D) * sizeof(double);
+    if (int r = ensure_stage(g, nb)) return r;
+    double *drho = g->stage, *du = g->stage + g->ncell;
+    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    if (g->D == 2) k_macro_to_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
+    else k_macro_to_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(rho, drho, g->ncell * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(u, du, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz) {
+    if (int r = check_grid(g)) return r;
+    if (!g->d.forced) return fail(LUMAGPU_ESTATE, "grid was created without forcing (desc.forced = 0)");
+    if (!force_xyz) return fail(LUMAGPU_EINVAL, "null force");
+    const int N = g->d.N, M = g->d.M, K = g->d.K, D = g->D;
+    if (g->hflags.empty()) g->hflags.assign(g->ncell, (uint8_t)kFluid);
+    std::vector<double> soa(g->ncell * D);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < M; ++j)
+            for (int k = 0; k < K; ++k) {
+                const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+                const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
+                bool nz = false;
+                for (int d = 0; d < D; ++d) {
+                    const double v = force_xyz[d + D * id];
+                    soa[d * g->ncell + cell] = v;
+                    nz = nz || v != 0.0;
+                }
+                g->hflags[cell] = (uint8_t)((g->hflags[cell] & ~kForcedBit) | (nz ? kForcedBit : 0));
+            }
+    if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
+    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->force, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->masked = 1;
+    return 0;
+}
+
+int lumagpu_set_store_macro(lumagpu_grid *g, int on) {
+    if (int r = check_grid(g)) return r;
+    g->store_macro = on ? 1 : 0;
+    return 0;
+}
+
+int lumagpu_step(lumagpu_grid *g, int subcycle) {
+    if (int r = check_grid(g)) return r;
+    if (subcycle < 0 || subcycle > 1) return fail(LUMAGPU_EINVAL, "subcycle must be 0 or 1");
+    return apply_step(g, subcycle, nullptr, nullptr);
+}
+
+int lumagpu_run(lumagpu_grid *g, int nsteps) {
+    if (int r = check_grid(g)) return r;
+    if (nsteps < 0) return fail(LUMAGPU_EINVAL, "negative step count");
+    for (int s = 0; s < nsteps; ++s)
+        if (int r = apply_step(g, 0, nullptr, nullptr)) return r;
+    return 0;
+}
+
+int lumagpu_run_timed(lumagpu_grid *g, int nsteps, double *ms_total, double *ms_kernel_mean) {
+    if (int r = check_grid(g)) return r;
+    if (nsteps <= 0) return fail(LUMAGPU_EINVAL, "step count must be positive");
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; ++s) rc = apply_step(g, 0, ev[2 * s], ev[2 * s + 1]);
+    if (!rc) {
+        HIP_TRY(hipEventSynchronize(ev.back()));
+        float tot = 0.f;
+        HIP_TRY(hipEventElapsedTime(&tot, ev.front(), ev.back()));
+        double sum = 0.0;
+        for (int s = 0; s < nsteps; ++s) {
+            float k = 0.f;
+            HIP_TRY(hipEventElapsedTime(&k, ev[2 * s], ev[2 * s + 1]));
+            sum += k;
+        }
+        if (ms_total) *ms_total = tot;
+        if (ms_kernel_mean) *ms_kernel_mean = sum / nsteps;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int lumagpu_get_time(const lumagpu_grid *g, int *t) {
+    if (int r = check_grid(g)) return r;
+    if (t) *t = g->t;
+    return 0;
+}
+
+int lumagpu_sync(lumagpu_grid *g) {
+    if (int r = check_grid(g)) return r;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]) {
+    if (int r = check_grid(g)) return r;
+    if (!out) return fail(LUMAGPU_EINVAL, "null output");
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "no state");
+    HIP_TRY(launch_aux(g, kAuxMoments, nullptr, nullptr, g->partials));
+    const dim3 gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    const size_t nblocks = (size_t)gr.x * gr.y * gr.z;
+    k_sum_partials<<<1, 256, 0, g->stream>>>(g->partials, nblocks, g->moments_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, g->moments_out, kMomentCount * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_footprint(const lumagpu_grid *g, size_t *bytes_per_cell, size_t *f_bytes) {
+    if (int r = check_grid(g)) return r;
+    if (bytes_per_cell) *bytes_per_cell = g->elem * g->Q;
+    if (f_bytes) *f_bytes = g->ps * g->Q * g->elem;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// launch.hpp -- host-side launchers for the step kernels, one set per storage form.
+//
+// The FP64 launchers live in lbm_f64.hip (built with -ffp-contract=off so the exact
+// arithmetic keeps LUMA's rounding); the shifted forms in lbm_f32.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "aa_kernels.hpp"
+
+namespace lumagpu {
+
+struct LaunchSpec {
+    int Q;          // 9 or 19
+    bool masked;    // labelled kernel
+    bool odd;       // neighbour-access step
+    bool store;     // write rho, u
+};
+
+// Each returns hipSuccess or the launch error.
+hipError_t launch_step_f64(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
+hipError_t launch_step_f32s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
+hipError_t launch_step_f16s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
+
+hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                          double *out, int parity, int masked, hipStream_t s);
+hipError_t launch_aux_f32s(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                           double *out, int parity, int masked, hipStream_t s);
+hipError_t launch_aux_f16s(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                           double *out, int parity, int masked, hipStream_t s);
+
+// aux ops
+enum AuxOp : int {
+    kAuxInitFeq = 0,     // p0 = rho (LUMA id), p1 = u (LUMA id)
+    kAuxUpload = 1,      // p0 = f AoS
+    kAuxDownload = 2,    // out = f AoS
+    kAuxMoments = 3,     // out = per-block partials
+};
+
+inline dim3 cell_block(int nx) {
+    int bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
+    return dim3(bx, 1, 1);
+}
+inline dim3 cell_grid(int nx, int ny, int nz) {
+    const dim3 b = cell_block(nx);
+    return dim3((nx + b.x - 1) / b.x, ny, nz);
+}
+
+template <int Q, class P>
+hipError_t launch_step_q(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
+    const dim3 b = cell_block(a.nx), g = cell_grid(a.nx, a.ny, a.nz);
+    if (ls.masked) {
+        if (ls.odd) {
+            if (ls.store) k_aa_masked<Q, P, true, true><<<g, b, 0, s>>>(a);
+            else k_aa_masked<Q, P, true, false><<<g, b, 0, s>>>(a);
+        } else {
+            if (ls.store) k_aa_masked<Q, P, false, true><<<g, b, 0, s>>>(a);
+            else k_aa_masked<Q, P, false, false><<<g, b, 0, s>>>(a);
+        }
+    } else {
+        if (ls.odd) {
+            if (ls.store) k_aa_periodic<Q, P, true, true><<<g, b, 0, s>>>(a);
+            else k_aa_periodic<Q, P, true, false><<<g, b, 0, s>>>(a);
+        } else {
+            if (ls.store) k_aa_periodic<Q, P, false, true><<<g, b, 0, s>>>(a);
+            else k_aa_periodic<Q, P, false, false><<<g, b, 0, s>>>(a);
+        }
+    }
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_step_p(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
+    if (ls.Q == 9) return launch_step_q<9, P>(ls, a, s);
+    if (ls.Q == 19) return launch_step_q<19, P>(ls, a, s);
+    return hipErrorInvalidValue;
+}
+
+template <int Q, class P>
+hipError_t launch_aux_q(int op, const StepArgs &a, const double *p0, const double *p1, double *out,
+                        int parity, int masked, hipStream_t s) {
+    const dim3 b = cell_block(a.nx), g = cell_grid(a.nx, a.ny, a.nz);
+    switch (op) {
+        case kAuxInitFeq: k_init_feq<Q, P><<<g, b, 0, s>>>(a, p0, p1); break;
+        case kAuxUpload: k_upload_aos<Q, P><<<g, b, 0, s>>>(a, p0); break;
+        case kAuxDownload: k_download_aos<Q, P><<<g, b, 0, s>>>(a, out, parity, masked); break;
+        case kAuxMoments: k_moments<Q, P><<<g, b, 0, s>>>(a, parity, masked, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_aux_p(int op, int Q, const StepArgs &a, const double *p0, const double *p1, double *out,
+                        int parity, int masked, hipStream_t s) {
+    if (Q == 9) return launch_aux_q<9, P>(op, a, p0, p1, out, parity, masked, s);
+    if (Q == 19) return launch_aux_q<19, P>(op, a, p0, p1, out, parity, masked, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace lumagpu

@@ -1,0 +1,21 @@
+// lbm_f32.hip -- shifted-storage instantiations (FP32 and FP16 storage, FP32 collision).
+#include "launch.hpp"
+
+namespace lumagpu {
+
+hipError_t launch_step_f32s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
+    return launch_step_p<P32S>(ls, a, s);
+}
+hipError_t launch_step_f16s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
+    return launch_step_p<P16S>(ls, a, s);
+}
+hipError_t launch_aux_f32s(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                           double *out, int parity, int masked, hipStream_t s) {
+    return launch_aux_p<P32S>(op, Q, a, p0, p1, out, parity, masked, s);
+}
+hipError_t launch_aux_f16s(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                           double *out, int parity, int masked, hipStream_t s) {
+    return launch_aux_p<P16S>(op, Q, a, p0, p1, out, parity, masked, s);
+}
+
+}  // namespace lumagpu

@@ -1,0 +1,16 @@
+// lbm_f64.hip -- FP64 instantiation of the step kernels.  Built with -ffp-contract=off:
+// the exact per-site arithmetic must round as LUMA's non-FMA CPU build does.
+#include "launch.hpp"
+
+namespace lumagpu {
+
+hipError_t launch_step_f64(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
+    return launch_step_p<P64>(ls, a, s);
+}
+
+hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                          double *out, int parity, int masked, hipStream_t s) {
+    return launch_aux_p<P64>(op, Q, a, p0, p1, out, parity, masked, s);
+}
+
+}  // namespace lumagpu

@@ -1,0 +1,151 @@
+"""Python handle over the C ABI: one GPU grid in LUMA's host layouts.
+
+Method names follow GridObj (LUMA/inc/GridObj.h): LBM_multi_opt(subcycle) is the time step.
+Arrays are numpy float64 shaped (N, M, K[, Q or D]) in C order, i.e. LUMA's f[v + Q*id].
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import GridDesc, call
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class GpuGrid:
+    def __init__(self, dims, Q, N, M, K=1, *, precision="fp64", omega=1.0, regularised=True,
+                 forced=False, fz_compat=True, rho_in=1.0, rho0=1.0, edges=(1.0, 1.0, 1.0),
+                 wall_thk=(0.0,) * 6, level=0, coarse_lims=(0,) * 6):
+        self.lib = _abi.load()
+        d = GridDesc()
+        d.dims, d.Q, d.N, d.M, d.K = dims, Q, N, M, K
+        d.precision = _abi.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        d.regularised, d.forced, d.fz_compat, d.level = int(regularised), int(forced), int(fz_compat), level
+        d.omega, d.rho_in, d.rho0 = omega, rho_in, rho0
+        d.xmax, d.ymax, d.zmax = edges
+        for i in range(6):
+            d.wall_thk[i] = wall_thk[i]
+            d.coarse_lims[i] = coarse_lims[i]
+        self.desc = d
+        self.dims, self.Q, self.shape = dims, Q, (N, M, K)
+        h = C.c_void_p()
+        call("lumagpu_grid_create", C.byref(d), C.byref(h))
+        self.h = h
+
+    @classmethod
+    def from_case(cls, case, precision="fp64", store_macro=False):
+        g = cls(case.dims, case.Q, case.N, case.M, case.K, precision=precision, omega=case.omega,
+                regularised=case.regularised, forced=case.forced, fz_compat=case.fz_compat,
+                rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk)
+        if case.lattyp is not None:
+            g.set_labels(case.lattyp, case.xpos, case.ypos, case.zpos)
+        if case.ux_in is not None:
+            g.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+        if case.force is not None:
+            g.set_force(case.force)
+        g.init_equilibrium(case.rho, case.u)
+        g.set_store_macro(store_macro)
+        return g
+
+    # lifetime -------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lumagpu_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # state ----------------------------------------------------------------------------
+    def set_labels(self, lattyp, xpos=None, ypos=None, zpos=None):
+        lt = np.ascontiguousarray(lattyp, dtype=np.int32)
+        x, y, z = _f64(xpos), _f64(ypos), _f64(zpos)
+        call("lumagpu_set_labels", self.h, lt.ctypes.data_as(_ip), _dptr(x), _dptr(y), _dptr(z))
+
+    def set_inlet(self, ux, uy, uz):
+        a, b, c = _f64(ux), _f64(uy), _f64(uz)
+        call("lumagpu_set_inlet", self.h, _dptr(a), _dptr(b), _dptr(c))
+
+    def init_equilibrium(self, rho, u):
+        r, uu = _f64(rho), _f64(u)
+        call("lumagpu_init_equilibrium", self.h, _dptr(r), _dptr(uu))
+
+    def upload_f(self, f):
+        a = _f64(f)
+        call("lumagpu_upload_f", self.h, _dptr(a))
+
+    def upload_macro(self, rho, u):
+        r, uu = _f64(rho), _f64(u)
+        call("lumagpu_upload_macro", self.h, _dptr(r), _dptr(uu))
+
+    def download_f(self):
+        N, M, K = self.shape
+        f = np.empty((N, M, K, self.Q))
+        call("lumagpu_download_f", self.h, _dptr(f))
+        return f
+
+    def download_macro(self):
+        N, M, K = self.shape
+        rho = np.empty((N, M, K))
+        u = np.empty((N, M, K, self.dims))
+        call("lumagpu_download_macro", self.h, _dptr(rho), _dptr(u))
+        return rho, u
+
+    def set_force(self, F):
+        a = _f64(F)
+        call("lumagpu_set_force", self.h, _dptr(a))
+
+    def set_store_macro(self, on=True):
+        call("lumagpu_set_store_macro", self.h, int(bool(on)))
+
+    # stepping -------------------------------------------------------------------------
+    def LBM_multi_opt(self, subcycle=0):
+        call("lumagpu_step", self.h, int(subcycle))
+
+    def run(self, nsteps):
+        call("lumagpu_run", self.h, int(nsteps))
+
+    def run_timed(self, nsteps):
+        tot, ker = C.c_double(), C.c_double()
+        call("lumagpu_run_timed", self.h, int(nsteps), C.byref(tot), C.byref(ker))
+        return tot.value, ker.value
+
+    def sync(self):
+        call("lumagpu_sync", self.h)
+
+    @property
+    def t(self):
+        t = C.c_int()
+        call("lumagpu_get_time", self.h, C.byref(t))
+        return t.value
+
+    def moments(self):
+        out = (C.c_double * _abi.NUM_MOMENTS)()
+        call("lumagpu_moments", self.h, out)
+        return {"mass": out[0], "momentum": (out[1], out[2], out[3]), "ke": out[4]}
+
+    def footprint(self):
+        a, b = C.c_size_t(), C.c_size_t()
+        call("lumagpu_footprint", self.h, C.byref(a), C.byref(b))
+        return a.value, b.value

@@ -1,0 +1,190 @@
+"""Initial states and labels of the benchmark cases, built the way LUMA builds them.
+
+Everything here is host-side set-up in LUMA's layouts (arrays shaped (N, M, K[, D]) in C
+order, i.e. id = k + j*K + i*K*M).  It restates the initialisation code the reference runs
+before the time loop:
+  positions       GridObj_init_grids.cpp:234-240 (GridUtils::linspace, GridUtils.h:289-309)
+  wall labels     LBM_initBoundLab, GridObj_init_grids.cpp:983-1097, precedence :1372-1377
+  inlet profile   _LBM_initSetInletProfile, :1322-1360
+  velocity        LBM_initVelocity (L_NO_FLOW), :37-133; density LBM_initRho, :138-151
+  omega           GridObj_init_grids.cpp:336-344 with GridUnits::nud2nulbm (GridUnits.h:128-131)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+SOLID, FLUID, REFINED, TL2C, TL2F, BFL, VELOCITY, PRESSURE, SLIP, EXTRAP_RIGHT = range(10)
+CS = 1.0 / math.sqrt(3.0)
+
+
+@dataclass
+class Case:
+    name: str
+    dims: int
+    Q: int
+    N: int
+    M: int
+    K: int
+    omega: float
+    rho: np.ndarray
+    u: np.ndarray
+    lattyp: Optional[np.ndarray] = None
+    xpos: Optional[np.ndarray] = None
+    ypos: Optional[np.ndarray] = None
+    zpos: Optional[np.ndarray] = None
+    ux_in: Optional[np.ndarray] = None
+    uy_in: Optional[np.ndarray] = None
+    uz_in: Optional[np.ndarray] = None
+    force: Optional[np.ndarray] = None
+    regularised: bool = True
+    forced: bool = False
+    fz_compat: bool = True
+    rho_in: float = 1.0
+    edges: tuple = (1.0, 1.0, 1.0)
+    wall_thk: tuple = (0.0,) * 6
+    info: dict = field(default_factory=dict)
+
+    @property
+    def ncell(self):
+        return self.N * self.M * self.K
+
+
+def linspace(lo: float, hi: float, n: int) -> np.ndarray:
+    """GridUtils::linspace: lo + spacing*i with spacing = (hi - lo)/(n - 1)."""
+    n = max(n, 2)
+    spacing = (hi - lo) / float(n - 1)
+    return lo + spacing * np.arange(n, dtype=np.float64)
+
+
+def omega_from_re(re: float, dt: float, dh: float) -> float:
+    nu = ((1.0 / re) * dt) / (dh * dh)
+    return 1.0 / ((nu / (CS * CS)) + 0.5)
+
+
+def bc_precedence(current: int, desired: int) -> int:
+    if current == SOLID or desired == SOLID:
+        return SOLID
+    if current == VELOCITY:
+        return VELOCITY
+    return desired
+
+
+def label_walls(N, M, K, dims, xpos, ypos, zpos, edges, thk, walls):
+    """LBM_initBoundLab: walls = dict(left, right, bottom, top[, front, back]) of eType."""
+    lt = np.full((N, M, K), FLUID, dtype=np.int32)
+    prec = np.vectorize(bc_precedence, otypes=[np.int32])
+
+    def apply(mask_axis, idx, wall):
+        sl = [slice(None)] * 3
+        sl[mask_axis] = idx
+        lt[tuple(sl)] = prec(lt[tuple(sl)], wall)
+
+    for i in range(N):
+        if xpos[i] <= thk[0]:
+            apply(0, i, walls["left"])
+    for i in range(N):
+        if xpos[i] >= edges[0] - thk[1]:
+            apply(0, i, walls["right"])
+    if dims == 3:
+        for k in range(K):
+            if zpos[k] <= thk[4]:
+                apply(2, k, walls["front"])
+        for k in range(K):
+            if zpos[k] >= edges[2] - thk[5]:
+                apply(2, k, walls["back"])
+    for j in range(M):
+        if ypos[j] <= thk[2]:
+            apply(1, j, walls["bottom"])
+    for j in range(M):
+        if ypos[j] >= edges[1] - thk[3]:
+            apply(1, j, walls["top"])
+    return lt
+
+
+def taylor_green(n: int | tuple, u0: float = 0.05, tau: float = 0.55, dims: int = 3,
+                 rho0: float = 1.0) -> Case:
+    """Triply (doubly) periodic Taylor-Green vortex, cell centres at x = i.
+
+    u = u0 (sin kx cos ky cos kz, -cos kx sin ky cos kz, 0), k = 2 pi / n per axis;
+    rho = rho0 + (p - p0)/cs^2, p - p0 = rho0 u0^2/16 (cos 2kx + cos 2ky)(cos 2kz + 2) in 3D
+    and rho0 u0^2/4 (cos 2kx + cos 2ky) in 2D.
+    """
+    if isinstance(n, int):
+        n = (n, n, n) if dims == 3 else (n, n, 1)
+    N, M, K = n
+    kx, ky = 2 * math.pi / N, 2 * math.pi / M
+    x = np.arange(N, dtype=np.float64)[:, None, None]
+    y = np.arange(M, dtype=np.float64)[None, :, None]
+    if dims == 3:
+        kz = 2 * math.pi / K
+        z = np.arange(K, dtype=np.float64)[None, None, :]
+        ux = u0 * np.sin(kx * x) * np.cos(ky * y) * np.cos(kz * z)
+        uy = -u0 * np.cos(kx * x) * np.sin(ky * y) * np.cos(kz * z)
+        uz = np.zeros((N, M, K))
+        dp = rho0 * u0 * u0 / 16.0 * (np.cos(2 * kx * x) + np.cos(2 * ky * y)) * (np.cos(2 * kz * z) + 2.0)
+        u = np.stack(np.broadcast_arrays(ux, uy, uz), axis=-1)
+        Q = 19
+    else:
+        ux = u0 * np.sin(kx * x) * np.cos(ky * y)
+        uy = -u0 * np.cos(kx * x) * np.sin(ky * y)
+        dp = rho0 * u0 * u0 / 4.0 * (np.cos(2 * kx * x) + np.cos(2 * ky * y))
+        u = np.stack(np.broadcast_arrays(ux, uy), axis=-1)
+        Q = 9
+    rho = np.broadcast_to(rho0 + dp / (CS * CS), (N, M, K)).copy()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return Case(name=f"tgv{dims}d_{N}x{M}x{K}", dims=dims, Q=Q, N=N, M=M, K=K, omega=1.0 / tau,
+                rho=rho, u=u, info={"u0": u0, "tau": tau, "k": (kx, ky)})
+
+
+def lid_driven_cavity(n: int = 256, re: float = 1280.0, u_lid_lbm: float = 0.1,
+                      regularised: bool = True) -> Case:
+    """D2Q9 lid-driven cavity as LUMA sets it up.
+
+    L_RESOLUTION = n, L_BX = L_BY = 1 (dh = 1/n), L_TIMESTEP = u_lid_lbm*dh with L_UX0 = 1,
+    L_RE = re (n = 256, re = 1280 gives tau = 0.56).  Left, right and bottom walls are
+    eSolid, the top wall eVelocity; wall thickness one cell; L_NO_FLOW initial state.
+    """
+    N = M = n
+    K = 1
+    dh = 1.0 / n
+    dt = u_lid_lbm * dh
+    edges = (1.0, 1.0, 0.0)
+    thk = (dh, dh, dh, dh, dh, dh)
+    xpos = linspace(dh / 2.0, edges[0] - dh / 2.0, N)
+    ypos = linspace(dh / 2.0, edges[1] - dh / 2.0, M)
+    zpos = np.array([0.0])
+    lt = label_walls(N, M, K, 2, xpos, ypos, zpos, edges, thk,
+                     dict(left=SOLID, right=SOLID, bottom=SOLID, top=VELOCITY))
+    u_lbm = (1.0 * dt) / dh              # GridUnits::ud2ulbm(L_UX0)
+    ux_in = np.full(M, u_lbm)
+    uy_in = np.zeros(M)
+    uz_in = np.zeros(M)
+    u = np.zeros((N, M, K, 2))
+    vel = lt == VELOCITY
+    u[..., 0][vel] = np.broadcast_to(ux_in[None, :, None], (N, M, K))[vel]
+    rho = np.ones((N, M, K))
+    return Case(name=f"cavity_{n}", dims=2, Q=9, N=N, M=M, K=K, omega=omega_from_re(re, dt, dh),
+                rho=rho, u=u, lattyp=lt, xpos=xpos, ypos=ypos, zpos=zpos, ux_in=ux_in, uy_in=uy_in,
+                uz_in=uz_in, regularised=regularised, edges=edges, wall_thk=thk,
+                info={"re": re, "dh": dh, "dt": dt, "u_lid": u_lbm})
+
+
+def body_force_channel(N: int, M: int, K: int = 1, dims: int = 2, tau: float = 0.8,
+                       g: float = 1e-6) -> Case:
+    """Channel with solid walls at j = 0 and j = M-1, periodic in x (and z), driven by a
+    uniform body force g along x (Poiseuille flow)."""
+    Q = 9 if dims == 2 else 19
+    lt = np.full((N, M, K), FLUID, dtype=np.int32)
+    lt[:, 0, :] = SOLID
+    lt[:, M - 1, :] = SOLID
+    F = np.zeros((N, M, K, dims))
+    F[..., 0] = np.where(lt == FLUID, g, 0.0)
+    rho = np.ones((N, M, K))
+    u = np.zeros((N, M, K, dims))
+    return Case(name=f"poiseuille{dims}d_{N}x{M}x{K}", dims=dims, Q=Q, N=N, M=M, K=K, omega=1.0 / tau,
+                rho=rho, u=u, lattyp=lt, force=F, forced=True, regularised=True,
+                info={"g": g, "tau": tau})

@@ -1,0 +1,164 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/lbm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the luma_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+
+SOLID, FLUID, REFINED, TL2C, TL2F, BFL, VELOCITY, PRESSURE, SLIP, EXTRAP_RIGHT = range(10)
+
+
+class OraDesc(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int), ("Q", C.c_int), ("N", C.c_int), ("M", C.c_int), ("K", C.c_int),
+        ("regularised", C.c_int), ("forced", C.c_int), ("fz_compat", C.c_int), ("level", C.c_int),
+        ("omega", C.c_double), ("rho_in", C.c_double),
+        ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+        ("wall_thk", C.c_double * 6), ("coarse_lims", C.c_int * 6),
+    ]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            from oracle.build import build_oracle
+            build_oracle()
+        lib = C.CDLL(str(LIB))
+        protos = {
+            "ora_create": ([C.POINTER(OraDesc)], _vp), "ora_destroy": ([_vp], None),
+            "ora_add_child": ([_vp, _vp], C.c_int),
+            "ora_set_labels": ([_vp, _ip], None), "ora_set_positions": ([_vp, _dp, _dp, _dp], None),
+            "ora_set_inlet": ([_vp, _dp, _dp, _dp], None), "ora_set_macro": ([_vp, _dp, _dp], None),
+            "ora_set_force": ([_vp, _dp], None), "ora_set_f": ([_vp, _dp], None),
+            "ora_init_feq": ([_vp], None), "ora_get_f": ([_vp, _dp], None),
+            "ora_get_macro": ([_vp, _dp, _dp], None), "ora_get_force": ([_vp, _dp], None),
+            "ora_get_t": ([_vp], C.c_int), "ora_multi_opt": ([_vp, C.c_int], None),
+            "ora_sweep1": ([_vp, C.c_int], None), "ora_sweep2": ([_vp], None),
+            "ora_reset_forces": ([_vp], None), "ora_run": ([_vp, C.c_int, C.c_int], None),
+            "ora_equilibrium": ([_vp, C.c_int, C.c_int], C.c_double),
+            "ora_lattice": ([C.c_int, _ip, _dp, _ip], None), "ora_cs": ([], C.c_double),
+        }
+        for name, (args, res) in protos.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def lattice(Q):
+    lib = _load()
+    c = np.zeros(Q * 3, np.int32)
+    w = np.zeros(Q)
+    o = np.zeros(Q, np.int32)
+    lib.ora_lattice(Q, c.ctypes.data_as(_ip), w.ctypes.data_as(_dp), o.ctypes.data_as(_ip))
+    return c.reshape(Q, 3), w, o
+
+
+class OracleGrid:
+    """LUMA-layout FP64 grid: arrays shaped (N, M, K[, Q or D]) in C order."""
+
+    def __init__(self, dims, Q, N, M, K=1, omega=1.0, regularised=True, forced=False, fz_compat=True,
+                 rho_in=1.0, edges=(1.0, 1.0, 1.0), wall_thk=(0.0,) * 6, level=0, coarse_lims=(0,) * 6):
+        self.lib = _load()
+        d = OraDesc()
+        d.dims, d.Q, d.N, d.M, d.K = dims, Q, N, M, K
+        d.regularised, d.forced, d.fz_compat, d.level = int(regularised), int(forced), int(fz_compat), level
+        d.omega, d.rho_in = omega, rho_in
+        d.xmax, d.ymax, d.zmax = edges
+        for i in range(6):
+            d.wall_thk[i] = wall_thk[i]
+            d.coarse_lims[i] = coarse_lims[i]
+        self.desc = d
+        self.dims, self.Q, self.shape = dims, Q, (N, M, K)
+        self.h = self.lib.ora_create(C.byref(d))
+        self._owned = True
+
+    def close(self):
+        if self.h and self._owned:
+            self.lib.ora_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_child(self, child: "OracleGrid"):
+        assert self.lib.ora_add_child(self.h, child.h) == 0
+        child._owned = False   # destroyed with the parent
+        self._child = child
+
+    def set_labels(self, lattyp):
+        a = np.ascontiguousarray(lattyp, dtype=np.int32)
+        self.lib.ora_set_labels(self.h, a.ctypes.data_as(_ip))
+
+    def set_positions(self, x, y, z):
+        (x, px), (y, py), (z, pz) = _d(x), _d(y), _d(z)
+        self.lib.ora_set_positions(self.h, px, py, pz)
+
+    def set_inlet(self, ux, uy, uz):
+        (a, pa), (b, pb), (c, pc) = _d(ux), _d(uy), _d(uz)
+        self.lib.ora_set_inlet(self.h, pa, pb, pc)
+
+    def set_macro(self, rho, u):
+        (r, pr), (uu, pu) = _d(rho), _d(u)
+        self.lib.ora_set_macro(self.h, pr, pu)
+
+    def set_force(self, F):
+        a, p = _d(F)
+        self.lib.ora_set_force(self.h, p)
+
+    def set_f(self, f):
+        a, p = _d(f)
+        self.lib.ora_set_f(self.h, p)
+
+    def init_feq(self, rho, u):
+        self.set_macro(rho, u)
+        self.lib.ora_init_feq(self.h)
+
+    def get_f(self):
+        N, M, K = self.shape
+        f = np.empty((N, M, K, self.Q))
+        self.lib.ora_get_f(self.h, f.ctypes.data_as(_dp))
+        return f
+
+    def get_macro(self):
+        N, M, K = self.shape
+        rho = np.empty((N, M, K))
+        u = np.empty((N, M, K, self.dims))
+        self.lib.ora_get_macro(self.h, rho.ctypes.data_as(_dp), u.ctypes.data_as(_dp))
+        return rho, u
+
+    def step(self, subcycle=0):
+        self.lib.ora_multi_opt(self.h, subcycle)
+
+    def run(self, n, nthreads=1):
+        self.lib.ora_run(self.h, n, nthreads)
+
+    @property
+    def t(self):
+        return self.lib.ora_get_t(self.h)
+
+    def equilibrium(self, id_, v):
+        return self.lib.ora_equilibrium(self.h, id_, v)

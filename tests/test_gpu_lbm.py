@@ -1,0 +1,175 @@
+"""GPU parity of the LBM step (through the C ABI) against the FP64 CPU oracle.
+
+FP64 form: bit-for-bit equality with the oracle (same operation order, no FMA contraction),
+which is stricter than the 1e-11 relative bound required of it.  Shifted forms: velocity
+error against the FP64 form, with the bound written in each test.
+"""
+import numpy as np
+import pytest
+
+from luma_amd import setups
+from luma_amd.grid import GpuGrid
+from oracle.oracle import OracleGrid
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_for(case):
+    g = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega,
+                   regularised=case.regularised, forced=case.forced, fz_compat=case.fz_compat,
+                   rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk)
+    if case.lattyp is not None:
+        g.set_labels(case.lattyp)
+        if case.xpos is not None:
+            g.set_positions(case.xpos, case.ypos, case.zpos)
+    if case.ux_in is not None:
+        g.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+    g.init_feq(case.rho, case.u)
+    if case.force is not None:
+        g.set_force(case.force)
+    return g
+
+
+def active(case):
+    """Sites whose populations LUMA defines after a step: not solid, and not a velocity
+    site skipped by the forced-equilibrium treatment (GridObj_ops_lbm_optimised.cpp:91-95)."""
+    if case.lattyp is None:
+        return np.ones((case.N, case.M, case.K), bool)
+    m = case.lattyp != setups.SOLID
+    if not case.regularised:
+        m &= case.lattyp != setups.VELOCITY
+    return m
+
+
+def assert_bitwise(case, nsteps):
+    print(f"[parity] {case.name} steps={nsteps}", flush=True)
+    ref = oracle_for(case)
+    with GpuGrid.from_case(case, "fp64", store_macro=True) as g:
+        ref.run(nsteps)
+        g.run(nsteps)
+        assert g.t == nsteps == ref.t
+        m = active(case)
+        fg, fr = g.download_f(), ref.get_f()
+        rg, ug = g.download_macro()
+        rr, ur = ref.get_macro()
+    diff = np.abs(fg[m] - fr[m]).max()
+    assert diff == 0.0, f"f differs by {diff}"
+    assert np.array_equal(rg[m], rr[m])
+    assert np.array_equal(ug[m], ur[m])
+
+
+def test_tgv3d_fp64_bitwise():
+    assert_bitwise(setups.taylor_green((20, 18, 16), u0=0.05, tau=0.55, dims=3), 61)
+
+
+def test_tgv2d_fp64_bitwise():
+    assert_bitwise(setups.taylor_green((32, 24, 1), u0=0.05, tau=0.56, dims=2), 200)
+
+
+def test_cavity_regularised_fp64_bitwise():
+    assert_bitwise(setups.lid_driven_cavity(48, re=1280.0 * 48 / 256), 400)
+
+
+def test_cavity_forced_equilibrium_fp64_bitwise():
+    c = setups.lid_driven_cavity(40, re=1280.0 * 40 / 256, regularised=False)
+    assert_bitwise(c, 300)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_body_force_channel_fp64_bitwise(dims):
+    c = setups.body_force_channel(9, 12, 5 if dims == 3 else 1, dims=dims, tau=0.7, g=2e-5)
+    assert_bitwise(c, 150)
+
+
+@pytest.mark.parametrize("compat", [True, False])
+def test_random_force_field_3d_fp64_bitwise(compat):
+    c = setups.taylor_green((10, 9, 8), dims=3)
+    rng = np.random.default_rng(4)
+    c.force = 1e-4 * rng.standard_normal((10, 9, 8, 3))
+    c.force[rng.random((10, 9, 8)) < 0.5] = 0.0        # sparse forcing, as IB support is
+    c.forced, c.fz_compat = True, compat
+    assert_bitwise(c, 40)
+
+
+def test_closed_box_hwbb_fp64_bitwise():
+    N, M, K = 11, 9, 8
+    lt = np.full((N, M, K), setups.FLUID, np.int32)
+    lt[0], lt[-1], lt[:, 0], lt[:, -1], lt[:, :, 0], lt[:, :, -1] = [setups.SOLID] * 6
+    c = setups.taylor_green((N, M, K), dims=3)
+    c.u[lt == setups.SOLID] = 0.0
+    c.lattyp = lt
+    assert_bitwise(c, 50)
+
+
+def test_upload_download_roundtrip_and_restart():
+    c = setups.taylor_green((12, 10, 8), dims=3)
+    rng = np.random.default_rng(9)
+    with GpuGrid.from_case(c, "fp64") as a, GpuGrid.from_case(c, "fp64") as b:
+        f0 = a.download_f()
+        f0 = f0 * (1 + 1e-3 * rng.standard_normal(f0.shape))
+        a.upload_f(f0)
+        assert np.array_equal(a.download_f(), f0)
+        a.run(7)                                    # odd count: neighbour layout
+        fa = a.download_f()
+        b.upload_f(fa)
+        assert np.array_equal(b.download_f(), fa)
+        a.run(5)
+        b.run(5)
+        assert np.array_equal(a.download_f(), b.download_f())
+
+
+def _u_error(case, prec, nsteps):
+    with GpuGrid.from_case(case, "fp64", store_macro=True) as g64, \
+            GpuGrid.from_case(case, prec, store_macro=True) as gs:
+        g64.run(nsteps)
+        gs.run(nsteps)
+        _, u64 = g64.download_macro()
+        _, us = gs.download_macro()
+    return np.linalg.norm(us - u64) / np.linalg.norm(u64)
+
+
+def test_fp32_shifted_tgv3d_velocity_error():
+    err = _u_error(setups.taylor_green((32, 32, 32), dims=3), "fp32s", 500)
+    print("fp32s rel L2 u error", err)
+    assert err < 1e-5
+
+
+def test_fp16_shifted_tgv3d_velocity_error():
+    err = _u_error(setups.taylor_green((32, 32, 32), dims=3), "fp16s", 500)
+    print("fp16s rel L2 u error", err)
+    assert err < 2e-2
+
+
+def test_fp32_shifted_cavity_velocity_error():
+    err = _u_error(setups.lid_driven_cavity(48, re=1280.0 * 48 / 256), "fp32s", 400)
+    print("fp32s cavity rel L2 u error", err)
+    assert err < 1e-4
+
+
+def test_moments_conservation_and_tgv_decay():
+    n, u0, tau = 48, 0.01, 0.8
+    c = setups.taylor_green((n, n, 1), u0=u0, tau=tau, dims=2)
+    with GpuGrid.from_case(c, "fp64") as g:
+        m0 = g.moments()
+        g.run(200)
+        m1 = g.moments()
+    assert abs(m1["mass"] - m0["mass"]) < 1e-13 * m0["mass"]
+    assert max(abs(x) for x in m1["momentum"]) < 1e-13
+    nu = (tau - 0.5) / 3.0
+    k2 = 2 * (2 * np.pi / n) ** 2
+    ratio = m1["ke"] / m0["ke"]
+    assert abs(ratio / np.exp(-2 * nu * k2 * 200) - 1) < 2e-3
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32s"])
+def test_full_size_256_mass_conservation(prec):
+    # the metric workload at its real size: D3Q19 TGV 256^3
+    c = setups.taylor_green(256, dims=3)
+    print(f"[full] 256^3 {prec}", flush=True)
+    with GpuGrid.from_case(c, prec) as g:
+        m0 = g.moments()
+        g.run(20)
+        m1 = g.moments()
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    assert abs(m1["mass"] - m0["mass"]) < tol * m0["mass"]
+    assert m1["ke"] < m0["ke"]
