@@ -28,6 +28,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "lattice.hpp"
 
 namespace lumagpu {
@@ -60,19 +63,40 @@ struct StepArgs {
     size_t ncell;
     double omega, rho0, rho_in;
     double cs2, den2;             // cs*cs and 2*cs^2*cs^2, evaluated as LUMA does
+    float om_hi, om_lo;           // omega as a float pair (shifted forms)
     int regularised;              // L_REGULARISED_BOUNDARIES
     int fz_compat;                // 3D macro quirk (:833): F_z/2 added to x-momentum
 };
 
 // ----------------------------------------------------------------------------------------
 // storage helpers
-template <class P>
+// Population traffic is streamed: every slot is read once and written once per step.  Row-
+// aligned accesses (c_x = 0, and every access of the even kernel) use non-temporal loads and
+// stores; the x-shifted accesses of the odd kernel share cache lines between neighbouring
+// waves and stay cached, since a non-temporal fetch there is fetched again by the next wave.
+// Measured on gfx950 at 256^3 FP32: all cached 5.3 TB/s, all non-temporal 5.2 TB/s, this
+// split 6.0 TB/s.
+// The cache policy is a template argument: a run-time flag, even one that folds after
+// unrolling, lets the compiler merge both branches into one load and drop the hint.
+template <class P, bool NT = true>
 __device__ __forceinline__ typename P::T ld(const typename P::S *p) {
-    return static_cast<typename P::T>(*p);
+    if constexpr (NT) return static_cast<typename P::T>(__builtin_nontemporal_load(p));
+    else return static_cast<typename P::T>(*p);
 }
-template <class P>
+template <class P, bool NT = true>
 __device__ __forceinline__ void st(typename P::S *p, typename P::T v) {
-    *p = static_cast<typename P::S>(v);
+    if constexpr (NT) __builtin_nontemporal_store(static_cast<typename P::S>(v), p);
+    else *p = static_cast<typename P::S>(v);
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1.
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 __device__ __forceinline__ size_t cell_of(int x, int y, int z, int nx, int ny) {
@@ -263,16 +287,32 @@ __device__ __forceinline__ void macro_shift(const float *fin, const double *F, b
     j[2] = jz;
 }
 
-// BGK on deviations: feq_v - w_v rho0 = w_v (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u)),
+// A constant carried as an unevaluated sum hi + lo of two floats.  Multiplying through
+// fmaf(hi, x, lo*x) rounds once, so the product is as accurate as with the exact constant.
+// This matters for the lattice weights and omega: rounding w_v to a float breaks the
+// equilibrium's moment identities (sum w = 1, sum w c c = I/3) by ~1e-8 the same way every
+// step, and near tau = 1/2 a rounded omega shifts the viscosity by ~1e-6 relative; both
+// errors then accumulate coherently over thousands of steps.
+struct Split {
+    float hi, lo;
+};
+__host__ __device__ constexpr Split split_of(double x) {
+    return Split{float(x), float(x - double(float(x)))};
+}
+__device__ __forceinline__ float mul_split(Split s, float x) { return fmaf(s.hi, x, s.lo * x); }
+
+// BGK on deviations, with rho = rho0 + drho:
+//   feq_v - w_v rho0 = w_v (drho + rho P_v),  P_v = 3 c.u + 4.5 (c.u)^2 - 1.5 u.u,
+// evaluated as w_v (drho + rho0 P_v + drho P_v) so rho itself is never rounded;
 // Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).
 template <int Q>
-__device__ __forceinline__ void collide_shift(const float *fin, float drho, float rho,
-                                              const float *u, const double *F, float omega,
+__device__ __forceinline__ void collide_shift(const float *fin, float drho, float rho0,
+                                              const float *u, const double *F, Split omega,
                                               float *fout) {
     using L = Lattice<Q>;
     const float usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     float Fx = 0.f, Fy = 0.f, Fz = 0.f, uF = 0.f;
-    const float gpre = 1.0f - 0.5f * omega;
+    const float gpre = 1.0f - 0.5f * (omega.hi + omega.lo);
     if (F) {
         Fx = float(F[0]);
         Fy = float(F[1]);
@@ -282,13 +322,15 @@ __device__ __forceinline__ void collide_shift(const float *fin, float drho, floa
 #pragma unroll
     for (int v = 0; v < Q; ++v) {
         const float cx = float(L::c(v, 0)), cy = float(L::c(v, 1)), cz = float(L::c(v, 2));
-        const float w = float(L::w(v));
+        const Split w = split_of(L::w(v));
         const float cu = cx * u[0] + cy * u[1] + cz * u[2];
-        const float feq = w * (drho + rho * (3.0f * cu + 4.5f * cu * cu - usq15));
-        float o = fin[v] + omega * (feq - fin[v]);
+        const float P = 3.0f * cu + 4.5f * cu * cu - usq15;
+        const float feq = mul_split(w, drho + (rho0 * P + drho * P));
+        const float dlt = feq - fin[v];
+        float o = fmaf(omega.hi, dlt, fmaf(omega.lo, dlt, fin[v]));
         if (F) {
             const float cF = cx * Fx + cy * Fy + cz * Fz;
-            o += gpre * w * (3.0f * (cF - uF) + 9.0f * cu * cF);
+            o += gpre * mul_split(w, 3.0f * (cF - uF) + 9.0f * cu * cF);
         }
         fout[v] = o;
     }
@@ -323,12 +365,15 @@ __device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *f
     } else {
         float drho, j[3], u[3];
         macro_shift<Q>(fin, F, a.fz_compat != 0, drho, j);
-        const float rho = float(a.rho0) + drho;
-        const float inv = 1.0f / rho;
-        u[0] = j[0] * inv;
-        u[1] = j[1] * inv;
-        u[2] = j[2] * inv;
-        if (collide) collide_shift<Q>(fin, drho, rho, u, F, float(a.omega), fout);
+        // u = j / (rho0 + drho) = (j - j*q)/rho0 with q = drho/(rho0 + drho): the rounding
+        // of rho0 + drho only reaches the small correction term
+        const float rho0 = float(a.rho0);
+        const float q = drho / (rho0 + drho);
+        const float r0inv = float(1.0 / a.rho0);
+        u[0] = (j[0] - j[0] * q) * r0inv;
+        u[1] = (j[1] - j[1] * q) * r0inv;
+        u[2] = (j[2] - j[2] * q) * r0inv;
+        if (collide) collide_shift<Q>(fin, drho, rho0, u, F, Split{a.om_hi, a.om_lo}, fout);
         else
             for (int v = 0; v < Q; ++v) fout[v] = fin[v];
         rho_o = double(float(a.rho0) + drho);
@@ -364,23 +409,25 @@ __global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
     if constexpr (ODD) nb.init(x, y, z, nx, ny, a.nz);
 
     T fin[Q], fout[Q];
-#pragma unroll
-    for (int v = 0; v < Q; ++v) {
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD)
-            fin[v] = ld<P>(f + (size_t)L::opp(v) * ps + nb.at(-L::c(v, 0), -L::c(v, 1), -L::c(v, 2), nx, ny));
+            fin[v] = ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + nb.at(-cx, -cy, -cz, nx, ny));
         else
             fin[v] = ld<P>(f + (size_t)v * ps + cell);
-    }
+    });
     double rho, u[3];
     site_update<Q, P>(fin, fout, nullptr, a, rho, u, true);
     if constexpr (STORE) store_macro<L::D>(a, cell, rho, u);
-#pragma unroll
-    for (int v = 0; v < Q; ++v) {
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD)
-            st<P>(f + (size_t)v * ps + nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), nx, ny), fout[v]);
+            st<P, cx == 0>(f + (size_t)v * ps + nb.at(cx, cy, cz, nx, ny), fout[v]);
         else
             st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
-    }
+    });
 }
 
 // ----------------------------------------------------------------------------------------
@@ -404,9 +451,9 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     nb.init(x, y, z, nx, ny, a.nz);
 
     T fin[Q], fout[Q];
-#pragma unroll
-    for (int v = 0; v < Q; ++v) {
-        const int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         const size_t src = nb.at(-cx, -cy, -cz, nx, ny);
         const int sty = a.flags[src] & kTypeMask;
         if (!a.regularised && sty == kVelocity) {
@@ -416,11 +463,11 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
             fin[v] = feq_store<Q, P>(v, a.rho_in, us, a);
         } else if constexpr (ODD) {
             fin[v] = (sty == kSolid) ? ld<P>(f + (size_t)v * ps + cell)
-                                     : ld<P>(f + (size_t)L::opp(v) * ps + src);
+                                     : ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + src);
         } else {
             fin[v] = ld<P>(f + (size_t)v * ps + cell);
         }
-    }
+    });
 
     const double *F = nullptr;
     double Fv[3];
@@ -446,7 +493,7 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
         } else {
 #pragma unroll
             for (int v = 0; v < Q; ++v) fin[v] = fa[v] - T(L::w(v) * a.rho0);
-            collide_shift<Q>(fin, rr - float(a.rho0), rr, uu, F, float(a.omega), fout);
+            collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
         }
         rho = rr;
         u[0] = uu[0];
@@ -459,18 +506,19 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
         if (ty == kFluid || ty == kTL2F || ty == kVelocity) store_macro<L::D>(a, cell, rho, u);
     }
 
-#pragma unroll
-    for (int v = 0; v < Q; ++v) {
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD) {
-            const size_t dst = nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), nx, ny);
+            const size_t dst = nb.at(cx, cy, cz, nx, ny);
             if ((a.flags[dst] & kTypeMask) == kSolid)
                 st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
             else
-                st<P>(f + (size_t)v * ps + dst, fout[v]);
+                st<P, cx == 0>(f + (size_t)v * ps + dst, fout[v]);
         } else {
             st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
         }
-    }
+    });
 }
 
 // ----------------------------------------------------------------------------------------
@@ -492,7 +540,7 @@ __global__ void k_init_feq(StepArgs a, const double *rho, const double *u) {
     const double us[3] = {u[L::D * id], u[L::D * id + 1], (L::D == 3) ? u[L::D * id + 2] : 0.0};
     typename P::S *f = static_cast<typename P::S *>(a.f);
     for (int v = 0; v < Q; ++v)
-        st<P>(f + (size_t)L::opp(v) * a.ps + cell, feq_store<Q, P>(v, rho[id], us, a));
+        st<P, false>(f + (size_t)L::opp(v) * a.ps + cell, feq_store<Q, P>(v, rho[id], us, a));
 }
 
 // upload post-collision f (AoS) into the post-local layout: A[opp v][x] = f_v(x)
@@ -508,7 +556,7 @@ __global__ void k_upload_aos(StepArgs a, const double *faos) {
     for (int v = 0; v < Q; ++v) {
         double fv = faos[v + Q * id];
         if constexpr (P::kShift) fv -= L::w(v) * a.rho0;
-        st<P>(f + (size_t)L::opp(v) * a.ps + cell, typename P::T(fv));
+        st<P, false>(f + (size_t)L::opp(v) * a.ps + cell, typename P::T(fv));
     }
 }
 
@@ -521,15 +569,15 @@ __device__ __forceinline__ double gather_post(const StepArgs &a, int x, int y, i
     const size_t cell = cell_of(x, y, z, a.nx, a.ny);
     double fv;
     if (parity == 0) {
-        fv = double(ld<P>(f + (size_t)L::opp(v) * a.ps + cell));
+        fv = double(ld<P, false>(f + (size_t)L::opp(v) * a.ps + cell));
     } else {
         Nbr nb;
         nb.init(x, y, z, a.nx, a.ny, a.nz);
         const size_t dst = nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), a.nx, a.ny);
         if (masked && (a.flags[dst] & kTypeMask) == kSolid)
-            fv = double(ld<P>(f + (size_t)L::opp(v) * a.ps + cell));
+            fv = double(ld<P, false>(f + (size_t)L::opp(v) * a.ps + cell));
         else
-            fv = double(ld<P>(f + (size_t)v * a.ps + dst));
+            fv = double(ld<P, false>(f + (size_t)v * a.ps + dst));
     }
     if constexpr (P::kShift) fv += L::w(v) * a.rho0;
     return fv;

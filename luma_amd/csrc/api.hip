@@ -75,6 +75,8 @@ StepArgs make_args(const lumagpu_grid *g) {
     const double cs = 1.0 / std::sqrt(3.0);    // LUMA/src/stdafx.cpp:152
     a.cs2 = cs * cs;
     a.den2 = 2.0 * (cs * cs) * (cs * cs);
+    a.om_hi = float(g->d.omega);
+    a.om_lo = float(g->d.omega - double(a.om_hi));
     a.regularised = g->d.regularised;
     a.fz_compat = g->d.fz_compat;
     return a;
