@@ -73,6 +73,11 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out);
 /* GridObj::~GridObj (GridObj.cpp:39-43); children must be destroyed by the caller first */
 int lumagpu_grid_destroy(lumagpu_grid *g);
 
+/* GridObj::LBM_addSubGrid / LBM_initSubGrid (GridObj.cpp:92-118, GridObj_init_grids.cpp:390-536):
+ * attach `child` (level + 1, extents 2*(coarse_lims max - min + 1)) as the refined region of
+ * `parent`.  Stepping the parent then steps the child twice first, as LBM_multi_opt does. */
+int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child);
+
 /* ---- state upload / download --------------------------------------------------------- */
 /* LatTyp + XPos/YPos/ZPos (GridObj_init_grids.cpp:234-273; labels of LBM_initBoundLab) */
 int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, const double *ypos,

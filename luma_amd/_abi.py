@@ -48,6 +48,7 @@ _vp = C.c_void_p
 _PROTOS = {
     "lumagpu_grid_create": [C.POINTER(GridDesc), C.POINTER(_vp)],
     "lumagpu_grid_destroy": [_vp],
+    "lumagpu_grid_add_child": [_vp, _vp],
     "lumagpu_set_labels": [_vp, _ip, _dp, _dp, _dp],
     "lumagpu_set_inlet": [_vp, _dp, _dp, _dp],
     "lumagpu_init_equilibrium": [_vp, _dp, _dp],

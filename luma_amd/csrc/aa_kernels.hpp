@@ -66,6 +66,21 @@ struct StepArgs {
     float om_hi, om_lo;           // omega as a float pair (shifted forms)
     int regularised;              // L_REGULARISED_BOUNDARIES
     int fz_compat;                // 3D macro quirk (:833): F_z/2 added to x-momentum
+    // --- refinement coupling (GridObj::parentGrid / subGrid) -----------------------------
+    const void *pf;               // parent population array (explode source)
+    size_t pps;
+    int pnx, pny, pnz, pparity;
+    const uint8_t *pflags;
+    int clx, cly, clz;            // this grid's CoarseLims{X,Y,Z}[eMinimum] on the parent
+    const void *cf;               // child population array (coalesce source), post-local layout
+    size_t cps;
+    int cnx, cny, cnz;
+    int cclx, ccly, cclz;         // the child's CoarseLims{X,Y,Z}[eMinimum] on this grid
+    double *crho, *cu;            // child macro arrays (TL2F copy, :850-891)
+    size_t cncell;
+    int subcycle;
+    // --- ExtrapolateRight sources: xr[v*(ny*nz) + y + ny*z] = f*_v two cells left of (N-1, y, z)
+    const double *xr;
 };
 
 // ----------------------------------------------------------------------------------------
@@ -431,7 +446,100 @@ __global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
 }
 
 // ----------------------------------------------------------------------------------------
-// labelled grid: solid (HWBB), regularised or forced-equilibrium velocity sites, body force.
+// labelled grids
+
+// Post-collision f*_v at (x,y,z) of a grid stored in layout `parity` (0: post-local,
+// 1: post-neighbour with solid redirect), in storage convention.
+template <int Q, class P>
+__device__ __forceinline__ typename P::T post_of(const void *fv, size_t ps, int nx, int ny, int nz,
+                                                 const uint8_t *flags, int parity, int x, int y, int z,
+                                                 int v) {
+    using L = Lattice<Q>;
+    const typename P::S *f = static_cast<const typename P::S *>(fv);
+    const size_t cell = cell_of(x, y, z, nx, ny);
+    if (parity == 0) return ld<P, false>(f + (size_t)L::opp(v) * ps + cell);
+    int dx = x + L::c(v, 0), dy = y + L::c(v, 1), dz = z + L::c(v, 2);
+    dx = dx < 0 ? nx - 1 : (dx >= nx ? 0 : dx);
+    dy = dy < 0 ? ny - 1 : (dy >= ny ? 0 : dy);
+    dz = dz < 0 ? nz - 1 : (dz >= nz ? 0 : dz);
+    const size_t dst = cell_of(dx, dy, dz, nx, ny);
+    if (flags && (flags[dst] & kTypeMask) == kSolid) return ld<P, false>(f + (size_t)L::opp(v) * ps + cell);
+    return ld<P, false>(f + (size_t)v * ps + dst);
+}
+
+// Incoming (streamed, pre-collision) populations of one site: _LBM_stream_opt (:206-297) with
+// LUMA's source precedence: solid (HWBB), ExtrapolateRight, forced-equilibrium velocity,
+// explode from the parent (subcycle 0), coalesce from the child, regular pull.
+template <int Q, class P, bool ODD>
+__device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z, size_t cell, const Nbr &nb,
+                                          int ty, typename P::T *fin) {
+    using L = Lattice<Q>;
+    using S = typename P::S;
+    using T = typename P::T;
+    const S *__restrict__ f = static_cast<const S *>(a.f);
+    const size_t ps = a.ps;
+    const int nx = a.nx, ny = a.ny;
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+        const size_t src = nb.at(-cx, -cy, -cz, nx, ny);
+        const int sty = a.flags[src] & kTypeMask;
+        if (sty == kSolid) {
+            // HWBB: f*_{opp v}(x) of the previous step, in this cell's slot v in either layout
+            fin[v] = ld<P>(f + (size_t)v * ps + cell);
+        } else if (sty == kExtrapRight) {
+            fin[v] = T(a.xr[(size_t)v * ny * a.nz + nb.ys[1 - cy] + (size_t)ny * nb.zs[1 - cz]]);
+        } else if (!a.regularised && sty == kVelocity) {
+            // forced-equilibrium velocity source, :254-270; its rho, u never change
+            const int js = nb.ys[1 - cy];
+            const double us[3] = {a.uin[js], a.uin[ny + js], (L::D == 3) ? a.uin[2 * ny + js] : 0.0};
+            fin[v] = feq_store<Q, P>(v, a.rho_in, us, a);
+        } else if (a.pf && sty == kTL2C && a.subcycle == 0) {
+            // explode, :646-663: the parent's post-collision value at the coarse cell of the source
+            const int sx = nb.xs[1 - cx], sy = nb.ys[1 - cy], sz = nb.zs[1 - cz];
+            fin[v] = post_of<Q, P>(a.pf, a.pps, a.pnx, a.pny, a.pnz, a.pflags, a.pparity, (sx >> 1) + a.clx,
+                                   (sy >> 1) + a.cly, (L::D == 3) ? (sz >> 1) + a.clz : 0, v);
+        } else if (a.cf && sty == kRefined && ty == kTL2F) {
+            // coalesce, :591-636: average of the child's values over this cell's 2^D children
+            const S *cfp = static_cast<const S *>(a.cf);
+            const int fx = 2 * (x - a.cclx), fy = 2 * (y - a.ccly), fz = (L::D == 3) ? 2 * (z - a.cclz) : 0;
+            T acc = T(0);
+            for (int ii = 0; ii < 2; ++ii)
+                for (int jj = 0; jj < 2; ++jj)
+                    for (int kk = 0; kk < (L::D == 3 ? 2 : 1); ++kk)
+                        acc += ld<P, false>(cfp + (size_t)L::opp(v) * a.cps +
+                                            cell_of(fx + ii, fy + jj, fz + kk, a.cnx, a.cny));
+            if constexpr (L::D == 3) acc /= T(8.0);
+            else acc /= T(4.0);
+            fin[v] = acc;
+        } else if constexpr (ODD) {
+            fin[v] = ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + src);
+        } else {
+            fin[v] = ld<P>(f + (size_t)v * ps + cell);
+        }
+    });
+}
+
+// Regularised velocity face for a gathered site: rho, u of the BC and the regularised
+// populations in storage convention.
+template <int Q, class P>
+__device__ __forceinline__ void velocity_site(const StepArgs &a, unsigned fl, int y, typename P::T *fin,
+                                              typename P::T &rr, typename P::T *uu) {
+    using L = Lattice<Q>;
+    using T = typename P::T;
+    const int ny = a.ny;
+    const int ncode = (fl >> kNormalShift) & 7;
+    T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
+    T fa[Q];
+#pragma unroll
+    for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
+    regularised_face<Q, T>(fa, ncode, vin, T(a.rho_in), T(a.cs2), T(a.den2), rr, uu);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) fin[v] = P::kShift ? fa[v] - T(L::w(v) * a.rho0) : fa[v];
+}
+
+// labelled grid: solid (HWBB), velocity (regularised or forced equilibrium), ExtrapolateRight
+// sources, refinement transition layers, body force.
 template <int Q, class P, bool ODD, bool STORE>
 __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     using L = Lattice<Q>;
@@ -444,30 +552,15 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     const size_t cell = cell_of(x, y, z, nx, ny);
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
-    if (ty == kSolid || ty == kRefined || (!a.regularised && ty == kVelocity)) return;
+    // sites LUMA's sweep 1 skips (:91-95); outlet sites are not read by any other site
+    if (ty == kSolid || ty == kRefined || ty == kExtrapRight || (!a.regularised && ty == kVelocity)) return;
     S *__restrict__ f = static_cast<S *>(a.f);
     const size_t ps = a.ps;
     Nbr nb;
     nb.init(x, y, z, nx, ny, a.nz);
 
     T fin[Q], fout[Q];
-    static_for<Q>([&](auto vc) {
-        constexpr int v = decltype(vc)::value;
-        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
-        const size_t src = nb.at(-cx, -cy, -cz, nx, ny);
-        const int sty = a.flags[src] & kTypeMask;
-        if (!a.regularised && sty == kVelocity) {
-            // forced-equilibrium velocity source, :254-270; its rho, u never change
-            const int js = nb.ys[1 - cy];
-            const double us[3] = {a.uin[js], a.uin[ny + js], (L::D == 3) ? a.uin[2 * ny + js] : 0.0};
-            fin[v] = feq_store<Q, P>(v, a.rho_in, us, a);
-        } else if constexpr (ODD) {
-            fin[v] = (sty == kSolid) ? ld<P>(f + (size_t)v * ps + cell)
-                                     : ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + src);
-        } else {
-            fin[v] = ld<P>(f + (size_t)v * ps + cell);
-        }
-    });
+    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
 
     const double *F = nullptr;
     double Fv[3];
@@ -480,19 +573,12 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
 
     double rho, u[3];
     if (ty == kVelocity) {
-        // regularised velocity face: BC gives rho, u; no macro on this site (:101-107)
-        const int ncode = (fl >> kNormalShift) & 7;
-        T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
-        T fa[Q];
-#pragma unroll
-        for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
+        // regularised velocity face: the BC gives rho, u; no macro on this site (:101-107)
         T rr, uu[3];
-        regularised_face<Q, T>(fa, ncode, vin, T(a.rho_in), T(a.cs2), T(a.den2), rr, uu);
+        velocity_site<Q, P>(a, fl, y, fin, rr, uu);
         if constexpr (!P::kShift) {
-            collide_exact<Q, T>(fa, rr, uu, F, T(a.omega), T(a.cs2), T(a.den2), fout);
+            collide_exact<Q, T>(fin, rr, uu, F, T(a.omega), T(a.cs2), T(a.den2), fout);
         } else {
-#pragma unroll
-            for (int v = 0; v < Q; ++v) fin[v] = fa[v] - T(L::w(v) * a.rho0);
             collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
         }
         rho = rr;
@@ -500,10 +586,22 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
         u[1] = uu[1];
         u[2] = uu[2];
     } else {
+        // TL2C sites are streamed but not collided (:144)
         site_update<Q, P>(fin, fout, F, a, rho, u, ty != kTL2C);
     }
     if constexpr (STORE) {
         if (ty == kFluid || ty == kTL2F || ty == kVelocity) store_macro<L::D>(a, cell, rho, u);
+        if (ty == kTL2F && a.crho) {
+            // child TL cluster gets this site's rho, u (:850-891)
+            const int fx = 2 * (x - a.cclx), fy = 2 * (y - a.ccly), fz = (L::D == 3) ? 2 * (z - a.cclz) : 0;
+            for (int ii = 0; ii < 2; ++ii)
+                for (int jj = 0; jj < 2; ++jj)
+                    for (int kk = 0; kk < (L::D == 3 ? 2 : 1); ++kk) {
+                        const size_t cc = cell_of(fx + ii, fy + jj, fz + kk, a.cnx, a.cny);
+                        a.crho[cc] = rho;
+                        for (int d = 0; d < L::D; ++d) a.cu[d * a.cncell + cc] = u[d];
+                    }
+        }
     }
 
     static_for<Q>([&](auto vc) {
@@ -519,6 +617,54 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
             st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
         }
     });
+}
+
+// Sweep-1 macroscopic values (stream + BC + macro, :72-107, force reset to zero) at a list of
+// sites, without touching the populations: the IB interpolation reads them.  Sites of other
+// types keep the rho, u their arrays hold, as LUMA's arrays do.
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, const int *cells) {
+    using L = Lattice<Q>;
+    using T = typename P::T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsites) return;
+    const size_t cell = (size_t)cells[i];
+    const int x = (int)(cell % a.nx), y = (int)((cell / a.nx) % a.ny), z = (int)(cell / ((size_t)a.nx * a.ny));
+    const unsigned fl = a.flags[cell];
+    const int ty = fl & kTypeMask;
+    const bool macro_type = ty == kFluid || ty == kTL2F;
+    const bool vel_type = a.regularised && ty == kVelocity;
+    if (!macro_type && !vel_type) return;
+    Nbr nb;
+    nb.init(x, y, z, a.nx, a.ny, a.nz);
+    T fin[Q];
+    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
+    double rho, u[3];
+    if (vel_type) {
+        T rr, uu[3];
+        velocity_site<Q, P>(a, fl, y, fin, rr, uu);
+        rho = rr;
+        u[0] = uu[0];
+        u[1] = uu[1];
+        u[2] = uu[2];
+    } else {
+        T fout[Q];
+        site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);
+    }
+    store_macro<L::D>(a, cell, rho, u);
+}
+
+// Side buffer for ExtrapolateRight sources (:246-251): xr[v][y, z] = f*_v(N-3, y, z), taken
+// from the current layout before the step kernel overwrites it.
+template <int Q, class P>
+__global__ void k_extrap_right(StepArgs a, int parity, double *xr) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (y >= a.ny) return;
+    const size_t plane = (size_t)a.ny * a.nz;
+    for (int v = 0; v < Q; ++v)
+        xr[(size_t)v * plane + y + (size_t)a.ny * z] =
+            double(post_of<Q, P>(a.f, a.ps, a.nx, a.ny, a.nz, a.flags, parity, a.nx - 3, y, z, v));
 }
 
 // ----------------------------------------------------------------------------------------

@@ -32,6 +32,10 @@ struct lumagpu_grid {
     size_t stage_bytes = 0;
     double *moments_out = nullptr;
     hipStream_t stream = nullptr;
+    lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
+    lumagpu_grid *child = nullptr;    // GridObj::subGrid[0] (non-owning; one region per level)
+    double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
+    int has_outlet = 0;
     int masked = 0;
     int has_state = 0;
     int store_macro = 0;
@@ -79,6 +83,34 @@ StepArgs make_args(const lumagpu_grid *g) {
     a.om_lo = float(g->d.omega - double(a.om_hi));
     a.regularised = g->d.regularised;
     a.fz_compat = g->d.fz_compat;
+    if (g->parent) {
+        const lumagpu_grid *p = g->parent;
+        a.pf = p->f;
+        a.pps = p->ps;
+        a.pnx = p->d.N;
+        a.pny = p->d.M;
+        a.pnz = p->d.K;
+        a.pparity = p->parity;
+        a.pflags = p->flags;
+        a.clx = g->d.coarse_lims[0];
+        a.cly = g->d.coarse_lims[2];
+        a.clz = g->d.coarse_lims[4];
+    }
+    if (g->child) {
+        const lumagpu_grid *c = g->child;
+        a.cf = c->f;
+        a.cps = c->ps;
+        a.cnx = c->d.N;
+        a.cny = c->d.M;
+        a.cnz = c->d.K;
+        a.cclx = c->d.coarse_lims[0];
+        a.ccly = c->d.coarse_lims[2];
+        a.cclz = c->d.coarse_lims[4];
+        a.crho = c->rho;
+        a.cu = c->u;
+        a.cncell = c->ncell;
+    }
+    a.xr = g->xr;
     return a;
 }
 
@@ -134,11 +166,45 @@ int wall_normal(const lumagpu_grid_desc &d, double px, double py, double pz, int
     return cnt;
 }
 
+hipError_t launch_extrap(const lumagpu_grid *g) {
+    const StepArgs a = make_args(g);
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_aux_f64(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP32S: return launch_aux_f32s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
+        default: return launch_aux_f16s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
+    }
+}
+
 int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
-    (void)subcycle;
     if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
+    // GridObj_ops_lbm_optimised.cpp:45-49: two sub-steps of the child first
+    if (g->child) {
+        lumagpu_grid *c = g->child;
+        for (int sc = 0; sc < 2; ++sc)
+            if (int r = apply_step(c, sc, nullptr, nullptr)) return r;
+        // the child reads this grid's populations, so order the two streams
+        hipEvent_t done;
+        HIP_TRY(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(done, c->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, done, 0));
+        HIP_TRY(hipEventDestroy(done));
+        if (c->parity != 0) return fail(LUMAGPU_ESTATE, "child grid out of step with its parent");
+    }
+    if (g->parent) {
+        // this grid reads the parent's populations: wait for the parent's previous work
+        hipEvent_t pe;
+        HIP_TRY(hipEventCreateWithFlags(&pe, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(pe, g->parent->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, pe, 0));
+        HIP_TRY(hipEventDestroy(pe));
+    }
+    if (g->has_outlet) {
+        hipError_t e = launch_extrap(g);
+        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
+    }
     LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0};
     StepArgs a = make_args(g);
+    a.subcycle = subcycle;
     if (e0) HIP_TRY(hipEventRecord(e0, g->stream));
     hipError_t e = launch_step(g, ls, a);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("step launch: ") + hipGetErrorString(e));
@@ -168,7 +234,7 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     // GridObj_init_grids.cpp:353-356
     if (!(d.omega > 0.0) || d.omega >= 2.0)
         return fail(LUMAGPU_EPHYS, "LBM relaxation frequency omega too large (must be 0 < omega < 2)");
-    if (d.level != 0) return fail(LUMAGPU_EUNSUPPORTED, "refinement levels are not supported by this build yet");
+    if (d.level < 0) return fail(LUMAGPU_EINVAL, "negative refinement level");
 
     lumagpu_grid *g = new lumagpu_grid();
     g->d = d;
@@ -217,6 +283,9 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->partials);
     (void)hipFree(g->stage);
     (void)hipFree(g->moments_out);
+    (void)hipFree(g->xr);
+    if (g->parent && g->parent->child == g) g->parent->child = nullptr;
+    if (g->child && g->child->parent == g) g->child->parent = nullptr;
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return 0;
@@ -231,7 +300,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     std::vector<uint8_t> prev = std::move(g->hflags);
     if (prev.size() != g->ncell) prev.assign(g->ncell, 0);
     g->hflags.assign(g->ncell, 0);
-    int any = 0;
+    int any = 0, outlet = 0;
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < M; ++j)
             for (int k = 0; k < K; ++k) {
@@ -258,9 +327,20 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                         }
                         break;
                     }
+                    case kRefined:
+                    case kTL2C:
+                    case kTL2F: fl = (uint8_t)ty; break;
+                    case kExtrapRight:
+                        // LUMA pulls from two sites to the left of the source (:246-251); the
+                        // outlet must be the last x column for that to stay on the grid
+                        if (i != N - 1 || N < 4)
+                            return fail(LUMAGPU_EUNSUPPORTED, "ExtrapolateRight sites must form the last x column");
+                        fl = (uint8_t)ty;
+                        outlet = 1;
+                        break;
                     default:
                         return fail(LUMAGPU_EUNSUPPORTED,
-                                    "site label " + std::to_string(ty) + " is not supported by this build yet");
+                                    "site label " + std::to_string(ty) + " is not supported by this build");
                 }
                 if (ty != kFluid) any = 1;
                 const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
@@ -269,8 +349,56 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
     HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    if (outlet && !g->xr &&
+        hipMalloc(&g->xr, (size_t)g->Q * g->d.M * g->d.K * sizeof(double)) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "outlet buffer allocation failed");
     HIP_TRY(hipStreamSynchronize(g->stream));
+    g->has_outlet = outlet;
     g->masked = any || g->d.forced;
+    return 0;
+}
+
+int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child) {
+    if (int r = check_grid(parent)) return r;
+    if (int r = check_grid(child)) return r;
+    if (parent->child) return fail(LUMAGPU_EUNSUPPORTED, "one refined region per grid is supported");
+    if (child->parent) return fail(LUMAGPU_ESTATE, "grid already has a parent");
+    const lumagpu_grid_desc &p = parent->d, &c = child->d;
+    if (c.level != p.level + 1) return fail(LUMAGPU_EINVAL, "child level must be parent level + 1");
+    if (c.precision != p.precision || c.Q != p.Q) return fail(LUMAGPU_EINVAL, "child and parent must share lattice and precision");
+    // LBM_initSubGrid (GridObj_init_grids.cpp:416-425): N = 2*(CoarseLimsMax - CoarseLimsMin + 1)
+    const int ext[3] = {c.N, c.M, c.K};
+    const int pext[3] = {p.N, p.M, p.K};
+    for (int a = 0; a < c.dims; ++a) {
+        const int lo = c.coarse_lims[2 * a], hi = c.coarse_lims[2 * a + 1];
+        if (lo < 0 || hi >= pext[a] || hi < lo || ext[a] != 2 * (hi - lo + 1))
+            return fail(LUMAGPU_EINVAL, "child extents do not match its coarse limits on the parent");
+    }
+    // Only transition-layer sites may read refined ones (coalesce).  Elsewhere LUMA would pull
+    // the refined sites' stale, still-collided populations (:144, :289-292), which this build
+    // does not keep, so such hierarchies are refused.
+    if (parent->hflags.size() == parent->ncell) {
+        const int N = p.N, M = p.M, K = p.K;
+        auto ty = [&](int i, int j, int k) {
+            i = (i + N) % N; j = (j + M) % M; k = (k + K) % K;
+            return parent->hflags[(size_t)i + (size_t)N * ((size_t)j + (size_t)M * k)] & kTypeMask;
+        };
+        for (int k = 0; k < K; ++k)
+            for (int j = 0; j < M; ++j)
+                for (int i = 0; i < N; ++i) {
+                    const int t = ty(i, j, k);
+                    if (t == kTL2F || t == kRefined || t == kSolid) continue;
+                    for (int dk = (p.dims == 3 ? -1 : 0); dk <= (p.dims == 3 ? 1 : 0); ++dk)
+                        for (int dj = -1; dj <= 1; ++dj)
+                            for (int di = -1; di <= 1; ++di)
+                                if (ty(i + di, j + dj, k + dk) == kRefined)
+                                    return fail(LUMAGPU_EUNSUPPORTED,
+                                                "a site other than eTransitionToFiner borders a refined site; "
+                                                "keep the refined region off the domain boundaries");
+                }
+    }
+    parent->child = child;
+    child->parent = parent;
     return 0;
 }
 

@@ -35,6 +35,7 @@ enum AuxOp : int {
     kAuxUpload = 1,      // p0 = f AoS
     kAuxDownload = 2,    // out = f AoS
     kAuxMoments = 3,     // out = per-block partials
+    kAuxExtrapRight = 4, // out = ExtrapolateRight side buffer
 };
 
 inline dim3 cell_block(int nx) {
@@ -85,6 +86,11 @@ hipError_t launch_aux_q(int op, const StepArgs &a, const double *p0, const doubl
         case kAuxUpload: k_upload_aos<Q, P><<<g, b, 0, s>>>(a, p0); break;
         case kAuxDownload: k_download_aos<Q, P><<<g, b, 0, s>>>(a, out, parity, masked); break;
         case kAuxMoments: k_moments<Q, P><<<g, b, 0, s>>>(a, parity, masked, out); break;
+        case kAuxExtrapRight: {
+            const dim3 bb(64), gg((a.ny + 63) / 64, a.nz);
+            k_extrap_right<Q, P><<<gg, bb, 0, s>>>(a, parity, out);
+            break;
+        }
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

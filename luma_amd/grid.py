@@ -48,7 +48,8 @@ class GpuGrid:
     def from_case(cls, case, precision="fp64", store_macro=False):
         g = cls(case.dims, case.Q, case.N, case.M, case.K, precision=precision, omega=case.omega,
                 regularised=case.regularised, forced=case.forced, fz_compat=case.fz_compat,
-                rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk)
+                rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk,
+                level=case.info.get("level", 0), coarse_lims=case.info.get("coarse_lims", (0,) * 6))
         if case.lattyp is not None:
             g.set_labels(case.lattyp, case.xpos, case.ypos, case.zpos)
         if case.ux_in is not None:
@@ -76,6 +77,11 @@ class GpuGrid:
 
     def __exit__(self, *exc):
         self.close()
+
+    def add_child(self, child: "GpuGrid"):
+        """GridObj::LBM_addSubGrid: the child is stepped twice inside every step of this grid."""
+        call("lumagpu_grid_add_child", self.h, child.h)
+        self._child = child
 
     # state ----------------------------------------------------------------------------
     def set_labels(self, lattyp, xpos=None, ypos=None, zpos=None):

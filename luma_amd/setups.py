@@ -159,10 +159,12 @@ def lid_driven_cavity(n: int = 256, re: float = 1280.0, u_lid_lbm: float = 0.1,
     zpos = np.array([0.0])
     lt = label_walls(N, M, K, 2, xpos, ypos, zpos, edges, thk,
                      dict(left=SOLID, right=SOLID, bottom=SOLID, top=VELOCITY))
-    u_lbm = (1.0 * dt) / dh              # GridUnits::ud2ulbm(L_UX0)
-    ux_in = np.full(M, u_lbm)
-    uy_in = np.zeros(M)
-    uz_in = np.zeros(M)
+    def inlet_fn(y, dt_, dh_):
+        # uniform profile from L_UX0 = 1: GridUnits::ud2ulbm = (u * dt) / dh
+        return np.full(len(y), (1.0 * dt_) / dh_), np.zeros(len(y)), np.zeros(len(y))
+
+    ux_in, uy_in, uz_in = inlet_fn(ypos, dt, dh)
+    u_lbm = ux_in[0]
     u = np.zeros((N, M, K, 2))
     vel = lt == VELOCITY
     u[..., 0][vel] = np.broadcast_to(ux_in[None, :, None], (N, M, K))[vel]
@@ -170,7 +172,7 @@ def lid_driven_cavity(n: int = 256, re: float = 1280.0, u_lid_lbm: float = 0.1,
     return Case(name=f"cavity_{n}", dims=2, Q=9, N=N, M=M, K=K, omega=omega_from_re(re, dt, dh),
                 rho=rho, u=u, lattyp=lt, xpos=xpos, ypos=ypos, zpos=zpos, ux_in=ux_in, uy_in=uy_in,
                 uz_in=uz_in, regularised=regularised, edges=edges, wall_thk=thk,
-                info={"re": re, "dh": dh, "dt": dt, "u_lid": u_lbm})
+                info={"re": re, "dh": dh, "dt": dt, "u_lid": u_lbm, "inlet_fn": inlet_fn})
 
 
 def body_force_channel(N: int, M: int, K: int = 1, dims: int = 2, tau: float = 0.8,
@@ -188,3 +190,89 @@ def body_force_channel(N: int, M: int, K: int = 1, dims: int = 2, tau: float = 0
     return Case(name=f"poiseuille{dims}d_{N}x{M}x{K}", dims=dims, Q=Q, N=N, M=M, K=K, omega=1.0 / tau,
                 rho=rho, u=u, lattyp=lt, force=F, forced=True, regularised=True,
                 info={"g": g, "tau": tau})
+
+
+def position_vector(start: float, dh: float, n: int, end_lim: float, start_lim: float) -> np.ndarray:
+    """GridObj::LBM_initPositionVector (GridObj_init_grids.cpp:902-965): sequential sums of dh,
+    wrapping to start_lim + dh/2 past end_lim."""
+    arr = [start]
+    while len(arr) < n:
+        nxt = arr[-1] + dh
+        arr.append(start_lim + dh / 2.0 if nxt > end_lim else nxt)
+    return np.array(arr, dtype=np.float64)
+
+
+def add_refined_region(parent: Case, lims, dh_parent: float, dt_parent: float, parent_origin=(0.0, 0.0, 0.0),
+                       level: int = 1) -> Case:
+    """Attach a 2:1 refined region covering parent cells lims = (i0, i1, j0, j1[, k0, k1]).
+
+    Restates LBM_initRefinedLab (GridObj_init_grids.cpp:1106-1224) with transition layers on
+    every side: parent sites on the region's outer ring become eTransitionToFiner, the rest
+    eRefined (fluid sites only); each child site takes eTransitionToCoarser under a TL2F parent,
+    eFluid under a refined one, and the parent's label otherwise.  Sizes, positions, omega and
+    the initial state follow LBM_initSubGrid (:390-536) and LBM_initGridToGridMappings (:545-).
+    The parent's labels are modified in place.
+    """
+    D = parent.dims
+    lims = list(lims) + ([0, 0] if D == 2 else [])
+    i0, i1, j0, j1, k0, k1 = lims
+    if parent.lattyp is None:
+        parent.lattyp = np.full((parent.N, parent.M, parent.K), FLUID, np.int32)
+    if parent.xpos is None:
+        parent.xpos = parent_origin[0] + dh_parent / 2 + dh_parent * np.arange(parent.N)
+        parent.ypos = parent_origin[1] + dh_parent / 2 + dh_parent * np.arange(parent.M)
+        parent.zpos = (parent_origin[2] + dh_parent / 2 + dh_parent * np.arange(parent.K)) if D == 3 else np.array([0.0])
+    px, py, pz = parent.xpos, parent.ypos, parent.zpos
+    edges = [px[i0] - dh_parent / 2, px[i1] + dh_parent / 2, py[j0] - dh_parent / 2, py[j1] + dh_parent / 2,
+             (pz[k0] - dh_parent / 2) if D == 3 else 0.0, (pz[k1] + dh_parent / 2) if D == 3 else 0.0]
+    lt = parent.lattyp
+    for i in range(parent.N):
+        for j in range(parent.M):
+            for k in range(parent.K):
+                inside = edges[0] < px[i] < edges[1] and edges[2] < py[j] < edges[3]
+                if D == 3:
+                    inside = inside and edges[4] < pz[k] < edges[5]
+                if not inside:
+                    continue
+                tl = (px[i] < edges[0] + dh_parent or px[i] > edges[1] - dh_parent or
+                      py[j] < edges[2] + dh_parent or py[j] > edges[3] - dh_parent)
+                if D == 3:
+                    tl = tl or pz[k] < edges[4] + dh_parent or pz[k] > edges[5] - dh_parent
+                if lt[i, j, k] == FLUID:
+                    lt[i, j, k] = TL2F if tl else REFINED
+    dh = dh_parent / 2.0
+    dt = dt_parent / 2.0
+    N, M = 2 * (i1 - i0 + 1), 2 * (j1 - j0 + 1)
+    K = 2 * (k1 - k0 + 1) if D == 3 else 1
+    xpos = position_vector(px[i0] - dh / 2.0, dh, N, edges[1], edges[0])
+    ypos = position_vector(py[j0] - dh / 2.0, dh, M, edges[3], edges[2])
+    zpos = position_vector(pz[k0] - dh / 2.0, dh, K, edges[5], edges[4]) if D == 3 else np.array([0.0])
+    clt = np.empty((N, M, K), np.int32)
+    for i in range(N):
+        for j in range(M):
+            for k in range(K):
+                pl = lt[(i // 2) + i0, (j // 2) + j0, ((k // 2) + k0) if D == 3 else 0]
+                clt[i, j, k] = TL2C if pl == TL2F else (FLUID if pl == REFINED else pl)
+    omega = 1.0 / (((1.0 / parent.omega - 0.5) * 2.0) + 0.5)
+    child = Case(name=parent.name + f"_L{level}", dims=D, Q=parent.Q, N=N, M=M, K=K, omega=omega,
+                 rho=np.full((N, M, K), parent.rho_in), u=np.zeros((N, M, K, D)), lattyp=clt,
+                 xpos=xpos, ypos=ypos, zpos=zpos, regularised=parent.regularised, forced=parent.forced,
+                 fz_compat=parent.fz_compat, rho_in=parent.rho_in, edges=parent.edges, wall_thk=parent.wall_thk,
+                 info={"level": level, "coarse_lims": (i0, i1, j0, j1, k0, k1), "dh": dh, "dt": dt,
+                       "region_edges": edges})
+    if "inlet_fn" in parent.info:
+        # _LBM_initSetInletProfile evaluated on the child's positions, dt and dh
+        child.ux_in, child.uy_in, child.uz_in = parent.info["inlet_fn"](ypos, dt, dh)
+        child.info["inlet_fn"] = parent.info["inlet_fn"]
+        vel = clt == VELOCITY
+        child.u[..., 0][vel] = np.broadcast_to(child.ux_in[None, :, None], (N, M, K))[vel]
+    return child
+
+
+def velocity_field_on(child: Case, parent: Case, init: str = "interp"):
+    """Initial child velocity: copy the parent's u at the enclosing coarse cell."""
+    i0, _, j0, _, k0, _ = child.info["coarse_lims"]
+    for i in range(child.N):
+        for j in range(child.M):
+            for k in range(child.K):
+                child.u[i, j, k] = parent.u[(i // 2) + i0, (j // 2) + j0, ((k // 2) + k0) if child.dims == 3 else 0]

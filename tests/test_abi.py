@@ -45,7 +45,7 @@ def _desc(**kw):
     (dict(Q=27), _abi.EUNSUPPORTED, "D2Q9 or D3Q19"),
     (dict(dims=2, Q=9, K=4), _abi.EINVAL, "K = 1 in 2D"),
     (dict(precision=7), _abi.EINVAL, "precision"),
-    (dict(level=1), _abi.EUNSUPPORTED, "refinement"),
+    (dict(level=-1), _abi.EINVAL, "negative refinement level"),
 ])
 def test_invalid_descriptor_is_rejected(kw, code, msg):
     lib = _abi.load()
