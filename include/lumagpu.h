@@ -65,6 +65,8 @@ typedef struct lumagpu_grid_desc {
     double xmax, ymax, zmax;   /* GridManager::global_edges[e?Max][0] of level 0 */
     double wall_thk[6];        /* L_WALL_THICKNESS_{LEFT,RIGHT,BOTTOM,TOP,FRONT,BACK} */
     int coarse_lims[6];        /* CoarseLimsX/Y/Z[min,max] on the parent (level > 0) */
+    double dh;                 /* GridObj::dh (IB support search and distances) */
+    double origin[3];          /* global_edges[e?Min] of this grid's level and region */
 } lumagpu_grid_desc;
 
 /* ---- lifetime ------------------------------------------------------------------------ */
@@ -113,6 +115,29 @@ int lumagpu_sync(lumagpu_grid *g);
  * mass, momentum and kinetic energy of the post-collision state; LUMA has no counterpart
  * (its logs hold per-marker forces only, ObjectManager_ops_io.cpp:65-128). */
 int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]);
+
+/* ---- immersed boundary (ObjectManager, LUMA/src/ObjectManager_ops_ibm.cpp) ------------
+ * Bodies live on one grid; their step (interpolate, force, spread, macroscopic update,
+ * :33-54) runs inside lumagpu_step between the stream/macro and force/collide halves.  The
+ * grid must be created with desc.forced = 1 (L_IBM_ON).  Positions use the units of the
+ * grid's XPos/YPos/ZPos (lumagpu_set_labels / lumagpu_set_positions) and desc.dh. */
+/* Body<IBMarker>::addMarker (Body.h:144-255) + IBMarker ctor (IBMarker.cpp:56-86):
+ * a body of n markers at pos[3*m + a]; epsilon 0, ds 1, markerVel 0 until initialised. */
+int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int flexible, int *body);
+/* ObjectManager::ibm_initialise (:190-246): support stencils, ds and epsilon of every body */
+int lumagpu_ibm_initialise(lumagpu_grid *g);
+/* IBMarker::markerVel for every marker of a body (vel[3*m + a]) */
+int lumagpu_ibm_set_velocity(lumagpu_grid *g, int body, const double *vel);
+/* IBMarker::epsilon for every marker of a body (replaces the ibm_findEpsilon result) */
+int lumagpu_ibm_set_epsilon(lumagpu_grid *g, int body, const double *eps);
+/* Marker state after the last IB step, 16 doubles per marker: position[3], markerVel[3],
+ * force_xyz[3], interpMom[3], interpRho, epsilon, ds, number of support sites. */
+int lumagpu_ibm_get_markers(lumagpu_grid *g, int body, double *rec);
+/* FP64 sum over the body's markers of force_xyz * epsilon * volDepth * ds (the per-marker
+ * terms of ObjectManager::io_writeLiftDrag, ObjectManager_ops_io.cpp:98-121) */
+int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]);
+/* XPos/YPos/ZPos of a grid that has no labels to set */
+int lumagpu_set_positions(lumagpu_grid *g, const double *xpos, const double *ypos, const double *zpos);
 
 /* ---- diagnostics --------------------------------------------------------------------- */
 const char *lumagpu_last_error(void);   /* replaces L_ERROR's log line (stdafx.h:119-149) */

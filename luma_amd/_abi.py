@@ -31,6 +31,7 @@ class GridDesc(C.Structure):
         ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
         ("wall_thk", C.c_double * 6),
         ("coarse_lims", C.c_int * 6),
+        ("dh", C.c_double), ("origin", C.c_double * 3),
     ]
 
 
@@ -64,6 +65,13 @@ _PROTOS = {
     "lumagpu_get_time": [_vp, _ip],
     "lumagpu_sync": [_vp],
     "lumagpu_moments": [_vp, _dp],
+    "lumagpu_set_positions": [_vp, _dp, _dp, _dp],
+    "lumagpu_ibm_add_body": [_vp, C.c_int, _dp, C.c_int, _ip],
+    "lumagpu_ibm_initialise": [_vp],
+    "lumagpu_ibm_set_velocity": [_vp, C.c_int, _dp],
+    "lumagpu_ibm_set_epsilon": [_vp, C.c_int, _dp],
+    "lumagpu_ibm_get_markers": [_vp, C.c_int, _dp],
+    "lumagpu_ibm_body_force": [_vp, C.c_int, _dp],
     "lumagpu_last_error": [],
     "lumagpu_abi_version": [],
     "lumagpu_footprint": [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],

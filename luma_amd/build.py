@@ -32,6 +32,7 @@ UNITS = [
     ("lbm_f64.hip", ["-ffp-contract=off"]),
     ("lbm_f32.hip", ["-ffp-contract=fast"]),
     ("api.hip", ["-ffp-contract=off"]),
+    ("ibm.hip", ["-ffp-contract=off"]),
 ]
 
 

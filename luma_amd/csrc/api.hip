@@ -13,107 +13,21 @@
 #include <string>
 #include <vector>
 
-#include "../../include/lumagpu.h"
-#include "launch.hpp"
+#include "internal.hpp"
 
 using namespace lumagpu;
 
-struct lumagpu_grid {
-    lumagpu_grid_desc d;
-    int Q = 0, D = 0;
-    size_t ncell = 0, ps = 0, elem = 0;
-    void *f = nullptr;
-    uint8_t *flags = nullptr;
-    double *force = nullptr;
-    double *uin = nullptr;
-    double *rho = nullptr, *u = nullptr;
-    double *partials = nullptr;
-    double *stage = nullptr;       // LUMA-layout staging (f AoS sized)
-    size_t stage_bytes = 0;
-    double *moments_out = nullptr;
-    hipStream_t stream = nullptr;
-    lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
-    lumagpu_grid *child = nullptr;    // GridObj::subGrid[0] (non-owning; one region per level)
-    double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
-    int has_outlet = 0;
-    int masked = 0;
-    int has_state = 0;
-    int store_macro = 0;
-    int parity = 0;
-    int t = 0;
-    std::vector<uint8_t> hflags;
-};
-
+namespace lumagpu {
 namespace {
-
 thread_local std::string g_last_error;
-
+}
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
     return code;
 }
+}  // namespace lumagpu
 
-#define HIP_TRY(expr)                                                                        \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(LUMAGPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-StepArgs make_args(const lumagpu_grid *g) {
-    StepArgs a{};
-    a.f = g->f;
-    a.ps = g->ps;
-    a.nx = g->d.N;
-    a.ny = g->d.M;
-    a.nz = g->d.K;
-    a.flags = g->flags;
-    a.force = g->force;
-    a.uin = g->uin;
-    a.rho_out = g->rho;
-    a.u_out = g->u;
-    a.ncell = g->ncell;
-    a.omega = g->d.omega;
-    a.rho0 = g->d.rho0;
-    a.rho_in = g->d.rho_in;
-    const double cs = 1.0 / std::sqrt(3.0);    // LUMA/src/stdafx.cpp:152
-    a.cs2 = cs * cs;
-    a.den2 = 2.0 * (cs * cs) * (cs * cs);
-    a.om_hi = float(g->d.omega);
-    a.om_lo = float(g->d.omega - double(a.om_hi));
-    a.regularised = g->d.regularised;
-    a.fz_compat = g->d.fz_compat;
-    if (g->parent) {
-        const lumagpu_grid *p = g->parent;
-        a.pf = p->f;
-        a.pps = p->ps;
-        a.pnx = p->d.N;
-        a.pny = p->d.M;
-        a.pnz = p->d.K;
-        a.pparity = p->parity;
-        a.pflags = p->flags;
-        a.clx = g->d.coarse_lims[0];
-        a.cly = g->d.coarse_lims[2];
-        a.clz = g->d.coarse_lims[4];
-    }
-    if (g->child) {
-        const lumagpu_grid *c = g->child;
-        a.cf = c->f;
-        a.cps = c->ps;
-        a.cnx = c->d.N;
-        a.cny = c->d.M;
-        a.cnz = c->d.K;
-        a.cclx = c->d.coarse_lims[0];
-        a.ccly = c->d.coarse_lims[2];
-        a.cclz = c->d.coarse_lims[4];
-        a.crho = c->rho;
-        a.cu = c->u;
-        a.cncell = c->ncell;
-    }
-    a.xr = g->xr;
-    return a;
-}
-
+namespace {
 hipError_t launch_step(const lumagpu_grid *g, const LaunchSpec &ls, const StepArgs &a) {
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_step_f64(ls, a, g->stream);
@@ -202,6 +116,9 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
         hipError_t e = launch_extrap(g);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
     }
+    // ObjectManager::ibm_apply between sweep 1 and sweep 2 (:120-121)
+    if (g->ibm)
+        if (int r = ibm_apply(g)) return r;
     LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0};
     StepArgs a = make_args(g);
     a.subcycle = subcycle;
@@ -284,6 +201,7 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->stage);
     (void)hipFree(g->moments_out);
     (void)hipFree(g->xr);
+    ibm_destroy(g);
     if (g->parent && g->parent->child == g) g->parent->child = nullptr;
     if (g->child && g->child->parent == g) g->child->parent = nullptr;
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -295,6 +213,11 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                        const double *zpos) {
     if (int r = check_grid(g)) return r;
     if (!lattyp) return fail(LUMAGPU_EINVAL, "null label array");
+    if (xpos && ypos && zpos) {
+        g->xpos.assign(xpos, xpos + g->d.N);
+        g->ypos.assign(ypos, ypos + g->d.M);
+        g->zpos.assign(zpos, zpos + g->d.K);
+    }
     const int N = g->d.N, M = g->d.M, K = g->d.K;
     // keep the force bits a previous lumagpu_set_force call placed
     std::vector<uint8_t> prev = std::move(g->hflags);

@@ -1,0 +1,515 @@
+// ibm.hip -- immersed-boundary bodies of a grid: host set-up (support stencils, ds, epsilon)
+// in FP64 and the device step (support-site macro, interpolate + force, spread).
+//
+// Host set-up restates ObjectManager::ibm_findSupport, ibm_initialiseSupport, ibm_computeDs,
+// ibm_findEpsilon and ibm_deltaKernel (LUMA/src/ObjectManager_ops_ibm.cpp:255-981) for a
+// serial run; the dense epsilon system is solved here by LU with partial pivoting in the order
+// of LAPACK's unblocked dgetf2/dgetrs('T'), which the reference calls through
+// GridUtils::solveLinearSystem (GridUtils.cpp:356-395).  Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ibm_kernels.hpp"
+#include "internal.hpp"
+
+namespace lumagpu {
+
+namespace {
+
+constexpr int kSmax = 27;
+
+struct Marker {
+    double pos[3] = {0, 0, 0}, vel[3] = {0, 0, 0};
+    double eps = 0.0, ds = 1.0, dilation = 1.0, area = 1.0;
+    int ns = 0;
+    int sijk[kSmax][3];
+    double sxyz[kSmax][3];
+    double delta[kSmax];
+};
+
+struct Body {
+    std::vector<Marker> m;
+    int flexible = 0;
+    int first = 0;   // index of the first marker in the grid-wide marker arrays
+};
+
+template <class T>
+int dev_upload(T **d, const std::vector<T> &h, hipStream_t s) {
+    if (*d) {
+        (void)hipFree(*d);
+        *d = nullptr;
+    }
+    if (h.empty()) return 0;
+    if (hipMalloc(d, h.size() * sizeof(T)) != hipSuccess) return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
+    if (hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(LUMAGPU_EHIP, "IB buffer upload failed");
+    return 0;
+}
+
+// ibm_deltaKernel, :255-272: Roma et al. three-point kernel
+double delta_kernel(double radius, double dilation) {
+    const double mag_r = std::fabs(radius) / dilation;
+    if (mag_r > 1.5) return 0.0;
+    if (mag_r > 0.5) return (5.0 - (3.0 * mag_r) - std::sqrt(-3.0 * std::pow(1.0 - mag_r, 2) + 1.0)) / 6.0;
+    return (1.0 + std::sqrt(1.0 - 3.0 * std::pow(mag_r, 2))) / 3.0;
+}
+
+}  // namespace
+
+struct IbmSystem {
+    std::vector<Body> bodies;
+    int nm = 0;
+    int S = 9;
+    int initialised = 0;
+    // device copies
+    int *d_scell = nullptr, *d_ns = nullptr, *d_sites = nullptr, *d_sptr = nullptr, *d_sent = nullptr;
+    double *d_sdelta = nullptr, *d_vel = nullptr, *d_eps = nullptr, *d_ds = nullptr;
+    double *d_interp = nullptr, *d_force = nullptr, *d_out = nullptr;
+    int nsites = 0;
+    std::vector<int> sites;   // host copy of the distinct support cells
+
+    ~IbmSystem() {
+        for (void *p : {(void *)d_scell, (void *)d_ns, (void *)d_sites, (void *)d_sptr, (void *)d_sent,
+                        (void *)d_sdelta, (void *)d_vel, (void *)d_eps, (void *)d_ds, (void *)d_interp,
+                        (void *)d_force, (void *)d_out})
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+IbmArgs ibm_args(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    IbmArgs b{};
+    b.nm = s->nm;
+    b.S = s->S;
+    b.D = g->D;
+    b.scell = s->d_scell;
+    b.sdelta = s->d_sdelta;
+    b.ns = s->d_ns;
+    b.vel = s->d_vel;
+    b.eps = s->d_eps;
+    b.ds = s->d_ds;
+    b.interp = s->d_interp;
+    b.force = s->d_force;
+    b.rho = g->rho;
+    b.u = g->u;
+    b.fxyz = g->force;
+    b.flags = g->flags;
+    b.ncell = g->ncell;
+    b.nsites = s->nsites;
+    b.sites = s->d_sites;
+    b.sptr = s->d_sptr;
+    b.sent = s->d_sent;
+    return b;
+}
+
+// GridUtils::getEnclosingVoxel (serial), GridUtils.cpp:1630-1719
+int enclosing(const lumagpu_grid *g, double x, int axis) {
+    return (int)std::floor((x - g->d.origin[axis]) / g->d.dh);
+}
+
+// GridUtils::isWithinDomain, GridUtils.cpp:1311-1330 (level-0 edges from 0)
+bool within_domain(const lumagpu_grid *g, const double p[3]) {
+    if (p[0] < 0.0 || p[0] > g->d.xmax || p[1] < 0.0 || p[1] > g->d.ymax) return false;
+    if (g->D == 3 && (p[2] < 0.0 || p[2] > g->d.zmax)) return false;
+    return true;
+}
+
+// ibm_initialiseSupport, :462-491
+double support_delta(const lumagpu_grid *g, const Marker &m, const double est[3]) {
+    const double dh = g->d.dh;
+    const double dx = (est[0] - m.pos[0]) / dh, dy = (est[1] - m.pos[1]) / dh;
+    if (g->D == 3) {
+        const double dz = (est[2] - m.pos[2]) / dh;
+        return delta_kernel(dx, m.dilation) * delta_kernel(dy, m.dilation) * delta_kernel(dz, m.dilation);
+    }
+    return delta_kernel(dx, m.dilation) * delta_kernel(dy, m.dilation);
+}
+
+// ibm_findSupport, :279-453: the enclosing voxel first, then every site of the +-5 cage
+// within 1.5 dilations on each axis and inside the domain.
+int find_support(const lumagpu_grid *g, Body &b, int ib) {
+    const int D = g->D, N = g->d.N, M = g->d.M, K = g->d.K;
+    const double dh = g->d.dh;
+    for (size_t mi = 0; mi < b.m.size(); ++mi) {
+        Marker &m = b.m[mi];
+        m.ns = 0;
+        const int in = enclosing(g, m.pos[0], 0), jn = enclosing(g, m.pos[1], 1);
+        const int kn = (D == 3) ? enclosing(g, m.pos[2], 2) : 0;
+        if (in < 0 || jn < 0 || kn < 0 || in >= N || jn >= M || kn >= K)
+            return fail(LUMAGPU_EPHYS, "Body " + std::to_string(ib) +
+                                           " is no-longer inside the domain! Simulation has likely crashed.");
+        const double nearpos[3] = {g->xpos[in], g->ypos[jn], (D == 3) ? g->zpos[kn] : 0.0};
+        auto add = [&](int i, int j, int k, const double p[3]) {
+            m.sijk[m.ns][0] = i;
+            m.sijk[m.ns][1] = j;
+            m.sijk[m.ns][2] = k;
+            for (int a = 0; a < 3; ++a) m.sxyz[m.ns][a] = p[a];
+            m.delta[m.ns] = support_delta(g, m, p);
+            m.ns++;
+        };
+        add(in, jn, kn, nearpos);
+        for (int i = in - 5; i <= in + 5; i++)
+            for (int j = jn - 5; j <= jn + 5; j++)
+                for (int k = (D == 3 ? kn - 5 : 0); k <= (D == 3 ? kn + 5 : 0); k++) {
+                    double est[3] = {nearpos[0] + (i - in) * dh, nearpos[1] + (j - jn) * dh, 0.0};
+                    if (D == 3) est[2] = nearpos[2] + (k - kn) * dh;
+                    bool cage = (std::fabs(m.pos[0] - est[0]) / dh < 1.5 * m.dilation) &&
+                                (std::fabs(m.pos[1] - est[1]) / dh < 1.5 * m.dilation);
+                    if (D == 3) cage = cage && (std::fabs(m.pos[2] - est[2]) / dh < 1.5 * m.dilation);
+                    if (!cage || !within_domain(g, est)) continue;
+                    if (i == in && j == jn && (D != 3 || k == kn)) continue;
+                    if (m.ns >= kSmax) return fail(LUMAGPU_EINVAL, "IB support stencil overflow");
+                    if (i < 0 || j < 0 || k < 0 || i >= N || j >= M || k >= K)
+                        return fail(LUMAGPU_EUNSUPPORTED, "IB support stencil leaves the grid of body " +
+                                                              std::to_string(ib));
+                    add(i, j, k, est);
+                }
+    }
+    return 0;
+}
+
+// ibm_computeDs, :927-981
+void compute_ds(const lumagpu_grid *g, Body &b) {
+    for (size_t mi = 0; mi < b.m.size(); ++mi) {
+        double ds = 10.0;
+        for (size_t n = 0; n < b.m.size(); ++n) {
+            if (n == mi) continue;
+            double acc = 0.0;
+            for (int a = 0; a < 3; ++a) acc += std::pow(b.m[mi].pos[a] - b.m[n].pos[a], 2);
+            const double dist = std::sqrt(acc) / g->d.dh;
+            if (dist < ds) ds = dist;
+        }
+        b.m[mi].ds = ds;
+    }
+}
+
+// A x = rhs for row-major A: factor the column-major view (A^T) with row pivoting, then solve
+// through its transpose; the operation order of dgetf2 followed by dgetrs('T').
+int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
+    auto M = [&](int i, int j) -> double & { return a[(size_t)j * n + i]; };
+    std::vector<int> piv(n);
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        double best = std::fabs(M(j, j));
+        for (int i = j + 1; i < n; ++i)
+            if (std::fabs(M(i, j)) > best) {
+                best = std::fabs(M(i, j));
+                p = i;
+            }
+        piv[j] = p;
+        if (M(p, j) == 0.0) return fail(LUMAGPU_EPHYS, "singular IB epsilon system");
+        if (p != j)
+            for (int k = 0; k < n; ++k) std::swap(M(j, k), M(p, k));
+        const double r = 1.0 / M(j, j);
+        for (int i = j + 1; i < n; ++i) M(i, j) *= r;
+        for (int k = j + 1; k < n; ++k) {
+            const double t = M(j, k);
+            if (t != 0.0)
+                for (int i = j + 1; i < n; ++i) M(i, k) -= M(i, j) * t;
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        double t = x[j];
+        for (int i = 0; i < j; ++i) t -= M(i, j) * x[i];
+        x[j] = t / M(j, j);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        double t = x[j];
+        for (int i = j + 1; i < n; ++i) t -= M(i, j) * x[i];
+        x[j] = t;
+    }
+    for (int j = n - 1; j >= 0; --j)
+        if (piv[j] != j) std::swap(x[j], x[piv[j]]);
+    return 0;
+}
+
+// ibm_findEpsilon, :773-920: A_IJ = ds_J * sum_s delta_I(s) delta(x_J - x_s) A_I, A eps = 1
+int find_epsilon(const lumagpu_grid *g, Body &b) {
+    const int n = (int)b.m.size(), D = g->D;
+    const double dh = g->d.dh;
+    std::vector<double> A((size_t)n * n, 0.0), rhs(n, 1.0);
+    for (int I = 0; I < n; ++I) {
+        const Marker &mI = b.m[I];
+        for (int J = 0; J < n; ++J) {
+            const Marker &mJ = b.m[J];
+            double acc = 0.0;
+            for (int k = 0; k < mI.ns; ++k) {
+                double dJ;
+                if (D == 3)
+                    dJ = delta_kernel((mJ.pos[0] - mI.sxyz[k][0]) / dh, mJ.dilation) *
+                         delta_kernel((mJ.pos[1] - mI.sxyz[k][1]) / dh, mJ.dilation) *
+                         delta_kernel((mJ.pos[2] - mI.sxyz[k][2]) / dh, mJ.dilation);
+                else
+                    dJ = delta_kernel((mJ.pos[0] - mI.sxyz[k][0]) / dh, mJ.dilation) *
+                         delta_kernel((mJ.pos[1] - mI.sxyz[k][1]) / dh, mJ.dilation);
+                acc += mI.delta[k] * dJ * mI.area;
+            }
+            A[(size_t)I * n + J] = acc * mJ.ds;
+        }
+    }
+    if (int r = lu_solve(n, A, rhs)) return r;
+    for (int i = 0; i < n; ++i) b.m[i].eps = rhs[i];
+    return 0;
+}
+
+// Pack markers and supports for the device; build the per-site gather lists in LUMA's spread
+// order; clear sites that are no longer supports.
+int upload(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    const int S = s->S, N = g->d.N, M = g->d.M;
+    int nm = 0;
+    for (auto &b : s->bodies) {
+        b.first = nm;
+        nm += (int)b.m.size();
+    }
+    s->nm = nm;
+    std::vector<int> scell((size_t)nm * S, 0), ns(nm);
+    std::vector<double> sdelta((size_t)nm * S, 0.0), vel((size_t)nm * 3), eps(nm), ds(nm);
+    std::vector<int> site_of(g->ncell, -1), order;
+    std::vector<std::vector<int>> entries;
+    for (const auto &b : s->bodies)
+        for (size_t mi = 0; mi < b.m.size(); ++mi) {
+            const Marker &m = b.m[mi];
+            const int gm = b.first + (int)mi;
+            ns[gm] = m.ns;
+            for (int a = 0; a < 3; ++a) vel[3 * gm + a] = m.vel[a];
+            eps[gm] = m.eps;
+            ds[gm] = m.ds;
+            for (int k = 0; k < m.ns; ++k) {
+                const int cell = m.sijk[k][0] + N * (m.sijk[k][1] + M * m.sijk[k][2]);
+                scell[(size_t)gm * S + k] = cell;
+                sdelta[(size_t)gm * S + k] = m.delta[k];
+                if (site_of[cell] < 0) {
+                    site_of[cell] = (int)order.size();
+                    order.push_back(cell);
+                    entries.emplace_back();
+                }
+                entries[site_of[cell]].push_back(gm * S + k);
+            }
+        }
+    std::vector<int> sptr(order.size() + 1, 0), sent;
+    for (size_t i = 0; i < order.size(); ++i) {
+        sptr[i + 1] = sptr[i] + (int)entries[i].size();
+        sent.insert(sent.end(), entries[i].begin(), entries[i].end());
+    }
+    // sites that were supports before and are not now lose their force and flag
+    std::vector<int> gone;
+    for (int c : s->sites)
+        if (site_of[c] < 0) gone.push_back(c);
+    int *d_gone = nullptr;
+    if (int r = dev_upload(&d_gone, gone, g->stream)) return r;
+    if (!gone.empty()) {
+        IbmArgs b = ibm_args(g);
+        k_ibm_clear<<<((int)gone.size() + 127) / 128, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
+        HIP_TRY(hipGetLastError());
+    }
+    for (int c : order) g->hflags[c] |= kForcedBit;
+    for (int c : gone) g->hflags[c] &= (uint8_t)~kForcedBit;
+    std::vector<double> zeros((size_t)nm * 4, 0.0);
+    int r = 0;
+    if ((r = dev_upload(&s->d_scell, scell, g->stream)) || (r = dev_upload(&s->d_ns, ns, g->stream)) ||
+        (r = dev_upload(&s->d_sdelta, sdelta, g->stream)) || (r = dev_upload(&s->d_vel, vel, g->stream)) ||
+        (r = dev_upload(&s->d_eps, eps, g->stream)) || (r = dev_upload(&s->d_ds, ds, g->stream)) ||
+        (r = dev_upload(&s->d_sites, order, g->stream)) || (r = dev_upload(&s->d_sptr, sptr, g->stream)) ||
+        (r = dev_upload(&s->d_sent, sent, g->stream)) || (r = dev_upload(&s->d_interp, zeros, g->stream)))
+        return r;
+    std::vector<double> fz((size_t)nm * 3, 0.0);
+    if ((r = dev_upload(&s->d_force, fz, g->stream))) return r;
+    if (!s->d_out && hipMalloc(&s->d_out, 3 * sizeof(double)) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    (void)hipFree(d_gone);
+    s->sites = order;
+    s->nsites = (int)order.size();
+    return 0;
+}
+
+hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int n, const int *cells) {
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_sites_f64(g->Q, a, odd, n, cells, g->stream);
+        case LUMAGPU_FP32S: return launch_sites_f32s(g->Q, a, odd, n, cells, g->stream);
+        default: return launch_sites_f16s(g->Q, a, odd, n, cells, g->stream);
+    }
+}
+
+IbmSystem *system_of(lumagpu_grid *g) {
+    if (!g->ibm) {
+        g->ibm = new IbmSystem();
+        g->ibm->S = (g->D == 3) ? 27 : 9;
+    }
+    return g->ibm;
+}
+
+int check_body(lumagpu_grid *g, int body) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->ibm || body < 0 || body >= (int)g->ibm->bodies.size()) return fail(LUMAGPU_EINVAL, "no such IB body");
+    return 0;
+}
+
+}  // namespace
+
+int ensure_flags(lumagpu_grid *g) {
+    if (g->hflags.size() != g->ncell) g->hflags.assign(g->ncell, (uint8_t)kFluid);
+    if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
+    return 0;
+}
+
+void ibm_destroy(lumagpu_grid *g) {
+    delete g->ibm;
+    g->ibm = nullptr;
+}
+
+// ObjectManager::ibm_apply (:33-54) for rigid bodies: the sweep-1 macroscopic values at the
+// support sites, interpolation and marker force, spread into force_xyz.  The macroscopic
+// update at the support sites (ibm_updateMacroscopic, :696-766) happens in the step kernel,
+// which recomputes u with +F/2 wherever the force bit is set.
+int ibm_apply(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    if (!s || s->nm == 0) return 0;
+    if (!s->initialised) return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
+    StepArgs a = make_args(g);
+    hipError_t e = launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites);
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB site macro launch: ") + hipGetErrorString(e));
+    IbmArgs b = ibm_args(g);
+    k_ibm_interp_force<<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b);
+    HIP_TRY(hipGetLastError());
+    k_ibm_spread<<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace lumagpu
+
+using namespace lumagpu;
+
+extern "C" {
+
+int lumagpu_set_positions(lumagpu_grid *g, const double *x, const double *y, const double *z) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!x || !y || !z) return fail(LUMAGPU_EINVAL, "null position array");
+    g->xpos.assign(x, x + g->d.N);
+    g->ypos.assign(y, y + g->d.M);
+    g->zpos.assign(z, z + g->d.K);
+    return 0;
+}
+
+int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int flexible, int *body) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->d.forced) return fail(LUMAGPU_ESTATE, "IB bodies need a grid created with desc.forced = 1");
+    if (nmarkers < 2 || !pos) return fail(LUMAGPU_EINVAL, "an IB body needs at least two markers");
+    if (flexible) return fail(LUMAGPU_EUNSUPPORTED, "flexible IB bodies are not supported by this entry point");
+    IbmSystem *s = system_of(g);
+    Body b;
+    b.flexible = flexible;
+    b.m.resize(nmarkers);
+    for (int i = 0; i < nmarkers; ++i)
+        for (int a = 0; a < 3; ++a) b.m[i].pos[a] = pos[3 * i + a];
+    s->bodies.push_back(std::move(b));
+    s->initialised = 0;
+    if (body) *body = (int)s->bodies.size() - 1;
+    return 0;
+}
+
+int lumagpu_ibm_initialise(lumagpu_grid *g) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->ibm || g->ibm->bodies.empty()) return fail(LUMAGPU_ESTATE, "no IB bodies on this grid");
+    if (!(g->d.dh > 0.0)) return fail(LUMAGPU_EINVAL, "desc.dh must be positive for IB bodies");
+    if ((int)g->xpos.size() != g->d.N || (int)g->ypos.size() != g->d.M || (int)g->zpos.size() != g->d.K)
+        return fail(LUMAGPU_ESTATE, "grid positions are not set (lumagpu_set_labels or lumagpu_set_positions)");
+    if (int r = ensure_flags(g)) return r;
+    IbmSystem *s = g->ibm;
+    for (size_t i = 0; i < s->bodies.size(); ++i)
+        if (int r = find_support(g, s->bodies[i], (int)i)) return r;
+    for (auto &b : s->bodies) compute_ds(g, b);
+    for (auto &b : s->bodies)
+        if (int r = find_epsilon(g, b)) return r;
+    if (int r = upload(g)) return r;
+    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->masked = 1;
+    s->initialised = 1;
+    return 0;
+}
+
+int lumagpu_ibm_set_velocity(lumagpu_grid *g, int body, const double *vel) {
+    if (int r = check_body(g, body)) return r;
+    if (!vel) return fail(LUMAGPU_EINVAL, "null velocity array");
+    Body &b = g->ibm->bodies[body];
+    for (size_t i = 0; i < b.m.size(); ++i)
+        for (int a = 0; a < 3; ++a) b.m[i].vel[a] = vel[3 * i + a];
+    if (g->ibm->initialised) {
+        std::vector<double> v(3 * b.m.size());
+        for (size_t i = 0; i < b.m.size(); ++i)
+            for (int a = 0; a < 3; ++a) v[3 * i + a] = b.m[i].vel[a];
+        HIP_TRY(hipMemcpyAsync(g->ibm->d_vel + 3 * b.first, v.data(), v.size() * sizeof(double),
+                               hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    return 0;
+}
+
+int lumagpu_ibm_set_epsilon(lumagpu_grid *g, int body, const double *eps) {
+    if (int r = check_body(g, body)) return r;
+    if (!eps) return fail(LUMAGPU_EINVAL, "null epsilon array");
+    if (!g->ibm->initialised) return fail(LUMAGPU_ESTATE, "call lumagpu_ibm_initialise first");
+    Body &b = g->ibm->bodies[body];
+    for (size_t i = 0; i < b.m.size(); ++i) b.m[i].eps = eps[i];
+    HIP_TRY(hipMemcpyAsync(g->ibm->d_eps + b.first, eps, b.m.size() * sizeof(double), hipMemcpyHostToDevice,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_ibm_get_markers(lumagpu_grid *g, int body, double *rec) {
+    if (int r = check_body(g, body)) return r;
+    if (!rec) return fail(LUMAGPU_EINVAL, "null output");
+    IbmSystem *s = g->ibm;
+    const Body &b = s->bodies[body];
+    const size_t n = b.m.size();
+    std::vector<double> interp(4 * n, 0.0), force(3 * n, 0.0);
+    if (s->initialised) {
+        HIP_TRY(hipMemcpyAsync(interp.data(), s->d_interp + 4 * b.first, 4 * n * sizeof(double),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(force.data(), s->d_force + 3 * b.first, 3 * n * sizeof(double),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    for (size_t i = 0; i < n; ++i) {
+        double *o = rec + 16 * i;
+        const Marker &m = b.m[i];
+        for (int a = 0; a < 3; ++a) {
+            o[a] = m.pos[a];
+            o[3 + a] = m.vel[a];
+            o[6 + a] = force[3 * i + a];
+            o[9 + a] = interp[4 * i + 1 + a];
+        }
+        o[12] = interp[4 * i];
+        o[13] = m.eps;
+        o[14] = m.ds;
+        o[15] = m.ns;
+    }
+    return 0;
+}
+
+int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]) {
+    if (int r = check_body(g, body)) return r;
+    if (!out) return fail(LUMAGPU_EINVAL, "null output");
+    IbmSystem *s = g->ibm;
+    if (!s->initialised) return fail(LUMAGPU_ESTATE, "call lumagpu_ibm_initialise first");
+    const Body &b = s->bodies[body];
+    IbmArgs a = ibm_args(g);
+    k_ibm_body_force<<<1, 64, 0, g->stream>>>(a, b.first, b.first + (int)b.m.size(), s->d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, s->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// ibm_kernels.hpp -- immersed-boundary device kernels (FP64 in every storage form).
+//
+// Markers of all bodies on one grid are stored back to back in body order, each with up to S
+// support slots (S = 9 in 2D, 27 in 3D).  The spread is a gather: every distinct support site
+// owns the list of (marker, slot) entries that touch it, kept in LUMA's accumulation order
+// (body, marker, support), so each site sums its contributions in exactly the order of
+// ObjectManager::ibm_spread (ObjectManager_ops_ibm.cpp:621-689) -- no atomics, no colouring
+// passes, and bit-for-bit the reference's sum.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.hpp"
+
+namespace lumagpu {
+
+struct IbmArgs {
+    int nm, S, D;
+    const int *scell;      // [nm*S] support cell index (device cell order)
+    const double *sdelta;  // [nm*S] delta value of each support
+    const int *ns;         // [nm] number of supports
+    const double *vel;     // [nm*3] markerVel
+    const double *eps;     // [nm] epsilon
+    const double *ds;      // [nm] ds
+    double *interp;        // [nm*4] interpRho, interpMom[3]
+    double *force;         // [nm*3] marker force_xyz
+    const double *rho;     // grid rho [cell]
+    const double *u;       // grid u [d*ncell + cell]
+    double *fxyz;          // grid force_xyz [d*ncell + cell]
+    uint8_t *flags;        // grid flag bytes
+    size_t ncell;
+    int nsites;
+    const int *sites;      // [nsites] distinct support cells
+    const int *sptr;       // [nsites+1] entry ranges
+    const int *sent;       // entries: m*S + slot
+};
+
+// ibm_interpolate (:498-591) and ibm_computeForce (:598-614); one thread per marker, sums in
+// support order.  local_area = 1 (IBMarker.cpp:61).
+__global__ void k_ibm_interp_force(IbmArgs b) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= b.nm) return;
+    const double area = 1.0;
+    double rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < b.ns[m]; ++k) {
+        const size_t cell = (size_t)b.scell[m * b.S + k];
+        const double dl = b.sdelta[m * b.S + k];
+        const double r = b.rho[cell];
+        rho += r * dl * area;
+        for (int d = 0; d < b.D; ++d) mom[d] += r * b.u[d * b.ncell + cell] * dl * area;
+    }
+    b.interp[4 * m] = rho;
+    for (int d = 0; d < 3; ++d) b.interp[4 * m + 1 + d] = mom[d];
+    for (int d = 0; d < b.D; ++d) b.force[3 * m + d] = 2.0 * (mom[d] - rho * b.vel[3 * m + d]) / 1.0;
+}
+
+// ibm_spread (:621-689) as a per-site gather; the site's force starts from the reset value 0
+// (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
+// step kernel adds it in the macro (+F/2) and the Guo term.
+__global__ void k_ibm_spread(IbmArgs b) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.nsites) return;
+    const size_t cell = (size_t)b.sites[s];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int e = b.sptr[s]; e < b.sptr[s + 1]; ++e) {
+        const int code = b.sent[e];
+        const int m = code / b.S;
+        const double dl = b.sdelta[code];
+        const double volWidth = b.eps[m];
+        const double volDepth = (b.D == 3) ? b.ds[m] : 1.0;
+        for (int d = 0; d < b.D; ++d) acc[d] -= dl * b.force[3 * m + d] * volWidth * volDepth * b.ds[m];
+    }
+    for (int d = 0; d < b.D; ++d) b.fxyz[d * b.ncell + cell] = acc[d];
+    b.flags[cell] |= kForcedBit;
+}
+
+// Clear the force and flag of sites that stop being supports.
+__global__ void k_ibm_clear(IbmArgs b, int n, const int *cells) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const size_t cell = (size_t)cells[s];
+    for (int d = 0; d < b.D; ++d) b.fxyz[d * b.ncell + cell] = 0.0;
+    b.flags[cell] &= (uint8_t)~kForcedBit;
+}
+
+// FP64 sum over one body's markers of force * epsilon * volDepth * ds, the per-marker values
+// LUMA writes to Body_<id>_LD (ObjectManager_ops_io.cpp:98-121).  One 64-wide wave: strided
+// lane sums, then a fixed xor-shuffle tree, so the result does not depend on timing.
+__global__ void k_ibm_body_force(IbmArgs b, int m0, int m1, double *out) {
+    const int lane = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int m = m0 + lane; m < m1; m += 64) {
+        const double w = b.eps[m] * ((b.D == 3) ? b.ds[m] : 1.0) * b.ds[m];
+        for (int d = 0; d < 3; ++d) acc[d] += b.force[3 * m + d] * w;
+    }
+    for (int d = 0; d < 3; ++d) {
+        double v = acc[d];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) out[d] = v;
+    }
+}
+
+}  // namespace lumagpu

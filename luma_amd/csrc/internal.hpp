@@ -1,0 +1,117 @@
+// internal.hpp -- the grid handle behind the C ABI and helpers shared by the host units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/lumagpu.h"
+#include "launch.hpp"
+
+namespace lumagpu {
+struct IbmSystem;
+}
+
+struct lumagpu_grid {
+    lumagpu_grid_desc d;
+    int Q = 0, D = 0;
+    size_t ncell = 0, ps = 0, elem = 0;
+    void *f = nullptr;
+    uint8_t *flags = nullptr;
+    double *force = nullptr;
+    double *uin = nullptr;
+    double *rho = nullptr, *u = nullptr;
+    double *partials = nullptr;
+    double *stage = nullptr;       // LUMA-layout staging (f AoS sized)
+    size_t stage_bytes = 0;
+    double *moments_out = nullptr;
+    hipStream_t stream = nullptr;
+    lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
+    lumagpu_grid *child = nullptr;    // GridObj::subGrid[0] (non-owning; one region per level)
+    double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
+    int has_outlet = 0;
+    int masked = 0;
+    int has_state = 0;
+    int store_macro = 0;
+    int parity = 0;
+    int t = 0;
+    std::vector<uint8_t> hflags;
+    std::vector<double> xpos, ypos, zpos;   // GridObj::XPos/YPos/ZPos (IB support search)
+    lumagpu::IbmSystem *ibm = nullptr;      // IB bodies owned by this grid
+};
+
+namespace lumagpu {
+
+// thread-local message behind lumagpu_last_error()
+int fail(int code, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(LUMAGPU_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+inline StepArgs make_args(const lumagpu_grid *g) {
+    StepArgs a{};
+    a.f = g->f;
+    a.ps = g->ps;
+    a.nx = g->d.N;
+    a.ny = g->d.M;
+    a.nz = g->d.K;
+    a.flags = g->flags;
+    a.force = g->force;
+    a.uin = g->uin;
+    a.rho_out = g->rho;
+    a.u_out = g->u;
+    a.ncell = g->ncell;
+    a.omega = g->d.omega;
+    a.rho0 = g->d.rho0;
+    a.rho_in = g->d.rho_in;
+    const double cs = 1.0 / std::sqrt(3.0);    // LUMA/src/stdafx.cpp:152
+    a.cs2 = cs * cs;
+    a.den2 = 2.0 * (cs * cs) * (cs * cs);
+    a.om_hi = float(g->d.omega);
+    a.om_lo = float(g->d.omega - double(a.om_hi));
+    a.regularised = g->d.regularised;
+    a.fz_compat = g->d.fz_compat;
+    if (g->parent) {
+        const lumagpu_grid *p = g->parent;
+        a.pf = p->f;
+        a.pps = p->ps;
+        a.pnx = p->d.N;
+        a.pny = p->d.M;
+        a.pnz = p->d.K;
+        a.pparity = p->parity;
+        a.pflags = p->flags;
+        a.clx = g->d.coarse_lims[0];
+        a.cly = g->d.coarse_lims[2];
+        a.clz = g->d.coarse_lims[4];
+    }
+    if (g->child) {
+        const lumagpu_grid *c = g->child;
+        a.cf = c->f;
+        a.cps = c->ps;
+        a.cnx = c->d.N;
+        a.cny = c->d.M;
+        a.cnz = c->d.K;
+        a.cclx = c->d.coarse_lims[0];
+        a.ccly = c->d.coarse_lims[2];
+        a.cclz = c->d.coarse_lims[4];
+        a.crho = c->rho;
+        a.cu = c->u;
+        a.cncell = c->ncell;
+    }
+    a.xr = g->xr;
+    return a;
+}
+
+// IB step between the stream/macro and the force/collide halves (ObjectManager::ibm_apply);
+// defined in ibm.hip
+int ibm_apply(lumagpu_grid *g);
+void ibm_destroy(lumagpu_grid *g);
+int ensure_flags(lumagpu_grid *g);
+
+}  // namespace lumagpu

@@ -29,6 +29,25 @@ hipError_t launch_aux_f32s(int op, int Q, const StepArgs &a, const double *p0, c
 hipError_t launch_aux_f16s(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                            double *out, int parity, int masked, hipStream_t s);
 
+// k_site_macro over a list of cells (IB support sites), in the layout the next step reads
+hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
+hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
+hipError_t launch_sites_f16s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
+
+template <class P>
+hipError_t launch_sites_p(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 b(128), g((n + 127) / 128);
+    if (Q == 9) {
+        if (odd) k_site_macro<9, P, true><<<g, b, 0, s>>>(a, n, cells);
+        else k_site_macro<9, P, false><<<g, b, 0, s>>>(a, n, cells);
+    } else {
+        if (odd) k_site_macro<19, P, true><<<g, b, 0, s>>>(a, n, cells);
+        else k_site_macro<19, P, false><<<g, b, 0, s>>>(a, n, cells);
+    }
+    return hipGetLastError();
+}
+
 // aux ops
 enum AuxOp : int {
     kAuxInitFeq = 0,     // p0 = rho (LUMA id), p1 = u (LUMA id)

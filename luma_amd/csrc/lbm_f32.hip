@@ -18,4 +18,11 @@ hipError_t launch_aux_f16s(int op, int Q, const StepArgs &a, const double *p0, c
     return launch_aux_p<P16S>(op, Q, a, p0, p1, out, parity, masked, s);
 }
 
+hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
+    return launch_sites_p<P32S>(Q, a, odd, n, cells, s);
+}
+hipError_t launch_sites_f16s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
+    return launch_sites_p<P16S>(Q, a, odd, n, cells, s);
+}
+
 }  // namespace lumagpu

@@ -13,4 +13,8 @@ hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, co
     return launch_aux_p<P64>(op, Q, a, p0, p1, out, parity, masked, s);
 }
 
+hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
+    return launch_sites_p<P64>(Q, a, odd, n, cells, s);
+}
+
 }  // namespace lumagpu

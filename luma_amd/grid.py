@@ -27,7 +27,7 @@ def _f64(a):
 class GpuGrid:
     def __init__(self, dims, Q, N, M, K=1, *, precision="fp64", omega=1.0, regularised=True,
                  forced=False, fz_compat=True, rho_in=1.0, rho0=1.0, edges=(1.0, 1.0, 1.0),
-                 wall_thk=(0.0,) * 6, level=0, coarse_lims=(0,) * 6):
+                 wall_thk=(0.0,) * 6, level=0, coarse_lims=(0,) * 6, dh=1.0, origin=(0.0, 0.0, 0.0)):
         self.lib = _abi.load()
         d = GridDesc()
         d.dims, d.Q, d.N, d.M, d.K = dims, Q, N, M, K
@@ -38,7 +38,11 @@ class GpuGrid:
         for i in range(6):
             d.wall_thk[i] = wall_thk[i]
             d.coarse_lims[i] = coarse_lims[i]
+        d.dh = dh
+        for i in range(3):
+            d.origin[i] = origin[i]
         self.desc = d
+        self.nmarkers = []
         self.dims, self.Q, self.shape = dims, Q, (N, M, K)
         h = C.c_void_p()
         call("lumagpu_grid_create", C.byref(d), C.byref(h))
@@ -49,7 +53,8 @@ class GpuGrid:
         g = cls(case.dims, case.Q, case.N, case.M, case.K, precision=precision, omega=case.omega,
                 regularised=case.regularised, forced=case.forced, fz_compat=case.fz_compat,
                 rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk,
-                level=case.info.get("level", 0), coarse_lims=case.info.get("coarse_lims", (0,) * 6))
+                level=case.info.get("level", 0), coarse_lims=case.info.get("coarse_lims", (0,) * 6),
+                dh=case.info.get("dh", 1.0), origin=case.info.get("origin", (0.0, 0.0, 0.0)))
         if case.lattyp is not None:
             g.set_labels(case.lattyp, case.xpos, case.ypos, case.zpos)
         if case.ux_in is not None:
@@ -155,3 +160,37 @@ class GpuGrid:
         a, b = C.c_size_t(), C.c_size_t()
         call("lumagpu_footprint", self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    # immersed boundary (ObjectManager) ------------------------------------------------
+    def set_positions(self, x, y, z):
+        a, b, c = _f64(x), _f64(y), _f64(z)
+        call("lumagpu_set_positions", self.h, _dptr(a), _dptr(b), _dptr(c))
+
+    def ibm_add_body(self, pos, flexible=False):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        body = C.c_int()
+        call("lumagpu_ibm_add_body", self.h, len(p), _dptr(p), int(flexible), C.byref(body))
+        self.nmarkers.append(len(p))
+        return body.value
+
+    def ibm_initialise(self):
+        call("lumagpu_ibm_initialise", self.h)
+
+    def ibm_set_velocity(self, body, vel):
+        v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        call("lumagpu_ibm_set_velocity", self.h, int(body), _dptr(v))
+
+    def ibm_set_epsilon(self, body, eps):
+        e = _f64(eps)
+        call("lumagpu_ibm_set_epsilon", self.h, int(body), _dptr(e))
+
+    def ibm_markers(self, body):
+        rec = np.empty((self.nmarkers[body], 16))
+        call("lumagpu_ibm_get_markers", self.h, int(body), _dptr(rec))
+        return {"pos": rec[:, 0:3], "vel": rec[:, 3:6], "force": rec[:, 6:9], "mom": rec[:, 9:12],
+                "rho": rec[:, 12], "eps": rec[:, 13], "ds": rec[:, 14], "ns": rec[:, 15].astype(int)}
+
+    def ibm_body_force(self, body):
+        out = (C.c_double * 3)()
+        call("lumagpu_ibm_body_force", self.h, int(body), out)
+        return np.array(out[:])

@@ -259,7 +259,7 @@ def add_refined_region(parent: Case, lims, dh_parent: float, dt_parent: float, p
                  xpos=xpos, ypos=ypos, zpos=zpos, regularised=parent.regularised, forced=parent.forced,
                  fz_compat=parent.fz_compat, rho_in=parent.rho_in, edges=parent.edges, wall_thk=parent.wall_thk,
                  info={"level": level, "coarse_lims": (i0, i1, j0, j1, k0, k1), "dh": dh, "dt": dt,
-                       "region_edges": edges})
+                       "region_edges": edges, "origin": (edges[0], edges[2], edges[4])})
     if "inlet_fn" in parent.info:
         # _LBM_initSetInletProfile evaluated on the child's positions, dt and dh
         child.ux_in, child.uy_in, child.uz_in = parent.info["inlet_fn"](ypos, dt, dh)
@@ -276,3 +276,82 @@ def velocity_field_on(child: Case, parent: Case, init: str = "interp"):
         for j in range(child.M):
             for k in range(child.K):
                 child.u[i, j, k] = parent.u[(i // 2) + i0, (j // 2) + j0, ((k // 2) + k0) if child.dims == 3 else 0]
+
+
+# ---- immersed-boundary bodies ----------------------------------------------------------
+
+def circle_markers(centre, radius, dh, luma_skip_first=True, n=None):
+    """Body<IBMarker> circle (Body.h:230-243): n = floor(2 pi r / dh) angles from linspace(0,
+    2 pi - 2 pi/n, n); LUMA keeps angles 1..n-1 (luma_skip_first), giving n - 1 markers."""
+    if n is None:
+        n = int(math.floor(2.0 * math.pi * radius / dh))
+    theta = linspace(0.0, 2.0 * math.pi - (2.0 * math.pi / float(n)), n)
+    if luma_skip_first:
+        theta = theta[1:]
+    return np.stack([centre[0] + radius * np.cos(theta), centre[1] + radius * np.sin(theta),
+                     np.full(len(theta), centre[2] if len(centre) > 2 else 0.0)], axis=1)
+
+
+def sphere_markers(centre, radius, dh, n=None):
+    """Body<IBMarker> Fibonacci sphere (Body.h:215-228): n = floor(4 pi r^2 / dh^2) unless given."""
+    if n is None:
+        n = int(math.floor(4.0 * math.pi * radius ** 2 / dh ** 2))
+    inc = math.pi * (3 - math.sqrt(5))
+    off = 2.0 / float(np.float32(n))
+    pts = []
+    for k in range(n):
+        y = k * off - 1 + (off / 2)
+        r = math.sqrt(1 - y * y)
+        phi = k * inc
+        pts.append((centre[0] + (math.cos(phi) * r * radius), y * radius + centre[1],
+                    centre[2] + (math.sin(phi) * r * radius)))
+    return np.array(pts)
+
+
+def filament_markers(start, length, angle_deg, dh):
+    """Body<IBMarker> filament (Body.h:144-187), 2D: floor(L/dh) + 1 markers along angle_deg."""
+    n = int(math.floor(length / dh)) + 1
+    spacing = length / (n - 1)
+    spacing_h = spacing * math.cos(angle_deg * math.pi / 180)
+    return np.array([(start[0] + i * spacing_h * math.cos(0.0), start[1] + i * spacing * math.sin(angle_deg * math.pi / 180.0),
+                      start[2] + i * spacing_h * math.sin(0.0)) for i in range(n)])
+
+
+def channel_2d(N: int, M: int, resolution: float, umax_lbm: float, re: float, regularised: bool = True,
+               outlet: bool = True, no_flow: bool = False) -> Case:
+    """D2Q9 channel as LUMA sets it up: parabolic inlet (eVelocity, L_PARABOLIC_INLET) on the left,
+    eExtrapolateRight (or periodic) on the right, eSolid top and bottom, one-cell walls.
+
+    resolution = L_RESOLUTION (cells per unit length); umax_lbm = 1.5 * L_UX0 in lattice units;
+    re = L_RE based on L_UX0 (the mean velocity) and unit length."""
+    dh = 1.0 / resolution
+    u0_lbm = umax_lbm / 1.5
+    dt = u0_lbm * dh                       # L_UX0 = 1: ud2ulbm(1) = dt/dh
+    edges = (N * dh, M * dh, 0.0)
+    thk = (dh, dh, dh, dh, dh, dh)
+    xpos = linspace(dh / 2.0, edges[0] - dh / 2.0, N)
+    ypos = linspace(dh / 2.0, edges[1] - dh / 2.0, M)
+    zpos = np.array([0.0])
+    lt = label_walls(N, M, 1, 2, xpos, ypos, zpos, edges, thk,
+                     dict(left=VELOCITY, right=EXTRAP_RIGHT if outlet else FLUID, bottom=SOLID, top=SOLID))
+
+    def inlet_fn(y, dt_, dh_):
+        # _LBM_initSetInletProfile, L_PARABOLIC_INLET (GridObj_init_grids.cpp:1334-1347)
+        b = edges[1] - thk[3]
+        p = (b + thk[2]) / 2.0
+        q = b - p
+        ux = ((1.5 * 1.0) * dt_) / dh_ * (1.0 - np.power((y - p) / q, 2.0))
+        return ux, np.zeros(len(y)), np.zeros(len(y))
+
+    ux_in, uy_in, uz_in = inlet_fn(ypos, dt, dh)
+    u = np.zeros((N, 1 * M, 1, 2))
+    if no_flow:
+        u[..., 0][lt == VELOCITY] = np.broadcast_to(ux_in[None, :, None], (N, M, 1))[lt == VELOCITY]
+    else:
+        u[..., 0] = np.broadcast_to(ux_in[None, :, None], (N, M, 1))
+    u[lt == SOLID] = 0.0
+    rho = np.ones((N, M, 1))
+    return Case(name=f"channel_{N}x{M}", dims=2, Q=9, N=N, M=M, K=1, omega=omega_from_re(re, dt, dh),
+                rho=rho, u=u, lattyp=lt, xpos=xpos, ypos=ypos, zpos=zpos, ux_in=ux_in, uy_in=uy_in,
+                uz_in=uz_in, regularised=regularised, forced=True, edges=edges, wall_thk=thk,
+                info={"re": re, "dh": dh, "dt": dt, "inlet_fn": inlet_fn, "origin": (0.0, 0.0, 0.0)})

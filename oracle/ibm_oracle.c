@@ -1,0 +1,336 @@
+/*
+ * ibm_oracle.c -- FP64 CPU restatement of LUMA v1.7.12's immersed-boundary step
+ * (ObjectManager::ibm_apply and the set-up it depends on).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker); see lbm_oracle.h.  Serial semantics: every marker
+ * and support site is on "this rank".  Each function names the reference lines it restates.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "lbm_oracle.h"
+
+#define SMAX 27
+
+typedef struct {
+    double pos[3], vel[3], vel_km1[3], force[3], mom[3];
+    double rho, eps, ds, dilation, area;
+    int ns;
+    int si[SMAX], sj[SMAX], sk[SMAX];
+    double sx[SMAX], sy[SMAX], sz[SMAX], delta[SMAX];
+} omarker;
+
+typedef struct {
+    int nm;
+    omarker *m;
+    int flexible;
+} obody;
+
+typedef struct ora_ibm {
+    ora_grid *g;
+    int nb;
+    obody b[16];
+    double dh, origin[3];
+    int err;
+} ora_ibm;
+
+static int D_of(const ora_ibm *s) { return ora_get_desc(s->g)->dims; }
+
+/* ObjectManager::ibm_deltaKernel, ObjectManager_ops_ibm.cpp:255-272 (Roma et al. 3-point) */
+double ora_ibm_delta(double radius, double dilation) {
+    double mag_r = fabs(radius) / dilation, value;
+    if (mag_r > 1.5) value = 0.0;
+    else if (mag_r > 0.5) value = (5.0 - (3.0 * mag_r) - sqrt(-3.0 * pow(1.0 - mag_r, 2) + 1.0)) / 6.0;
+    else value = (1.0 + sqrt(1.0 - 3.0 * pow(mag_r, 2))) / 3.0;
+    return value;
+}
+
+ora_ibm *ora_ibm_create(ora_grid *g, double dh, const double *origin) {
+    ora_ibm *s = (ora_ibm *)calloc(1, sizeof(ora_ibm));
+    s->g = g;
+    s->dh = dh;
+    for (int a = 0; a < 3; ++a) s->origin[a] = origin[a];
+    return s;
+}
+
+void ora_ibm_destroy(ora_ibm *s) {
+    if (!s) return;
+    for (int i = 0; i < s->nb; ++i) free(s->b[i].m);
+    free(s);
+}
+
+/* Body<IBMarker>::addMarker + IBMarker ctor (IBMarker.cpp:56-86): eps 0, area 1, dilation 1,
+ * ds 1, markerVel 0 */
+int ora_ibm_add_body(ora_ibm *s, int n, const double *pos, int flexible) {
+    if (s->nb >= 16) return -1;
+    obody *b = &s->b[s->nb];
+    b->nm = n;
+    b->flexible = flexible;
+    b->m = (omarker *)calloc(n, sizeof(omarker));
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) b->m[i].pos[a] = pos[3 * i + a];
+        b->m[i].area = 1.0;
+        b->m[i].dilation = 1.0;
+        b->m[i].ds = 1.0;
+    }
+    return s->nb++;
+}
+
+/* ibm_initialiseSupport, :462-491 */
+static void init_support(ora_ibm *s, omarker *m, const double est[3]) {
+    double dx = (est[0] - m->pos[0]) / s->dh;
+    double dy = (est[1] - m->pos[1]) / s->dh;
+    double val = ora_ibm_delta(dx, m->dilation) * ora_ibm_delta(dy, m->dilation);
+    if (D_of(s) == 3) {
+        double dz = (est[2] - m->pos[2]) / s->dh;
+        val = ora_ibm_delta(dx, m->dilation) * ora_ibm_delta(dy, m->dilation) * ora_ibm_delta(dz, m->dilation);
+    }
+    m->delta[m->ns] = val;
+}
+
+static int within_domain(const ora_ibm *s, const double p[3]) {
+    /* GridUtils::isWithinDomain (GridUtils.cpp:1311-1330), level-0 edges from 0 */
+    const ora_desc *d = ora_get_desc(s->g);
+    if (p[0] < 0.0 || p[0] > d->xmax || p[1] < 0.0 || p[1] > d->ymax) return 0;
+    if (d->dims == 3 && (p[2] < 0.0 || p[2] > d->zmax)) return 0;
+    return 1;
+}
+
+static int enclosing(const ora_ibm *s, double x, int axis) {
+    /* GridUtils::getEnclosingVoxel, serial (GridUtils.cpp:1630-1719) */
+    return (int)floor((x - s->origin[axis]) / s->dh);
+}
+
+/* ibm_findSupport, ObjectManager_ops_ibm.cpp:279-453 */
+static int find_support(ora_ibm *s, obody *b) {
+    const ora_desc *d = ora_get_desc(s->g);
+    const int D = d->dims;
+    const double *X = ora_pos(s->g, 0), *Y = ora_pos(s->g, 1), *Z = ora_pos(s->g, 2);
+    for (int mi = 0; mi < b->nm; ++mi) {
+        omarker *m = &b->m[mi];
+        m->ns = 0;
+        int inear = enclosing(s, m->pos[0], 0), jnear = enclosing(s, m->pos[1], 1);
+        int knear = (D == 3) ? enclosing(s, m->pos[2], 2) : 0;
+        if (inear < 0 || jnear < 0 || knear < 0 || inear >= d->N || jnear >= d->M || knear >= d->K) return -1;
+        double nearpos[3] = {X[inear], Y[jnear], (D == 3) ? Z[knear] : 0.0};
+        m->si[0] = inear;
+        m->sj[0] = jnear;
+        m->sk[0] = knear;
+        m->sx[0] = nearpos[0];
+        m->sy[0] = nearpos[1];
+        m->sz[0] = nearpos[2];
+        init_support(s, m, nearpos);
+        m->ns = 1;
+        for (int i = inear - 5; i <= inear + 5; i++)
+            for (int j = jnear - 5; j <= jnear + 5; j++)
+                for (int k = (D == 3 ? knear - 5 : 0); k <= (D == 3 ? knear + 5 : 0); k++) {
+                    double est[3] = {0.0, 0.0, 0.0};
+                    est[0] = nearpos[0] + (i - inear) * s->dh;
+                    est[1] = nearpos[1] + (j - jnear) * s->dh;
+                    if (D == 3) est[2] = nearpos[2] + (k - knear) * s->dh;
+                    int in = (fabs(m->pos[0] - est[0]) / s->dh < 1.5 * m->dilation) &&
+                             (fabs(m->pos[1] - est[1]) / s->dh < 1.5 * m->dilation);
+                    if (D == 3) in = in && (fabs(m->pos[2] - est[2]) / s->dh < 1.5 * m->dilation);
+                    if (in && within_domain(s, est)) {
+                        if (i != inear || j != jnear || (D == 3 && k != knear)) {
+                            if (m->ns >= SMAX) return -2;
+                            if (i < 0 || j < 0 || k < 0 || i >= d->N || j >= d->M || k >= d->K) return -3;
+                            m->si[m->ns] = i;
+                            m->sj[m->ns] = j;
+                            m->sk[m->ns] = k;
+                            m->sx[m->ns] = est[0];
+                            m->sy[m->ns] = est[1];
+                            m->sz[m->ns] = est[2];
+                            init_support(s, m, est);
+                            m->ns++;
+                        }
+                    }
+                }
+    }
+    return 0;
+}
+
+/* ibm_computeDs, :927-981 (GridUtils::vecnorm = sqrt of the sum of pow(d, 2)) */
+static void compute_ds(ora_ibm *s, obody *b) {
+    for (int mi = 0; mi < b->nm; ++mi) {
+        double ds = 10.0;
+        for (int n = 0; n < b->nm; ++n) {
+            if (n == mi) continue;
+            double acc = 0.0;
+            for (int a = 0; a < 3; ++a) acc += pow(b->m[mi].pos[a] - b->m[n].pos[a], 2);
+            double dist = sqrt(acc) / s->dh;
+            if (dist < ds) ds = dist;
+        }
+        b->m[mi].ds = ds;
+    }
+}
+
+/* Dense solve A x = b by LU with partial pivoting.  GridUtils::solveLinearSystem
+ * (GridUtils.cpp:356-395) hands LAPACK dgetrf_/dgetrs_('T') the row-major A, i.e. factors
+ * A^T = P L U with row pivoting and solves through the transpose.  Restated here as LAPACK's
+ * unblocked dgetf2 + dgetrs('T') sequence. */
+int ora_lu_solve_t(int n, double *a /* row-major A, destroyed */, double *b) {
+    /* column-major view: M(i,j) = a[j*n + i] is A^T */
+#define MM(i, j) a[(size_t)(j) * n + (i)]
+    int *piv = (int *)malloc(sizeof(int) * n);
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        double best = fabs(MM(j, j));
+        for (int i = j + 1; i < n; ++i)
+            if (fabs(MM(i, j)) > best) { best = fabs(MM(i, j)); p = i; }
+        piv[j] = p;
+        if (MM(p, j) == 0.0) { free(piv); return -1; }
+        if (p != j)
+            for (int k = 0; k < n; ++k) { double t = MM(j, k); MM(j, k) = MM(p, k); MM(p, k) = t; }
+        double r = 1.0 / MM(j, j);
+        for (int i = j + 1; i < n; ++i) MM(i, j) *= r;
+        for (int k = j + 1; k < n; ++k) {
+            double t = MM(j, k);
+            if (t != 0.0)
+                for (int i = j + 1; i < n; ++i) MM(i, k) -= MM(i, j) * t;
+        }
+    }
+    /* solve M^T x = b with M = P L U: U^T y = b, L^T z = y, x = P z */
+    for (int j = 0; j < n; ++j) {
+        double t = b[j];
+        for (int i = 0; i < j; ++i) t -= MM(i, j) * b[i];
+        b[j] = t / MM(j, j);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        double t = b[j];
+        for (int i = j + 1; i < n; ++i) t -= MM(i, j) * b[i];
+        b[j] = t;
+    }
+    for (int j = n - 1; j >= 0; --j)
+        if (piv[j] != j) { double t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    free(piv);
+    return 0;
+#undef MM
+}
+
+/* ibm_findEpsilon, :773-920 */
+static int find_epsilon(ora_ibm *s, obody *b) {
+    const int n = b->nm, D = D_of(s);
+    double *A = (double *)calloc((size_t)n * n, sizeof(double));
+    double *rhs = (double *)malloc(sizeof(double) * n);
+    for (int I = 0; I < n; ++I) {
+        omarker *mI = &b->m[I];
+        for (int J = 0; J < n; ++J) {
+            omarker *mJ = &b->m[J];
+            double acc = 0.0;
+            for (int k = 0; k < mI->ns; ++k) {
+                double dI = mI->delta[k], dJ;
+                if (D == 3)
+                    dJ = ora_ibm_delta((mJ->pos[0] - mI->sx[k]) / s->dh, mJ->dilation) *
+                         ora_ibm_delta((mJ->pos[1] - mI->sy[k]) / s->dh, mJ->dilation) *
+                         ora_ibm_delta((mJ->pos[2] - mI->sz[k]) / s->dh, mJ->dilation);
+                else
+                    dJ = ora_ibm_delta((mJ->pos[0] - mI->sx[k]) / s->dh, mJ->dilation) *
+                         ora_ibm_delta((mJ->pos[1] - mI->sy[k]) / s->dh, mJ->dilation);
+                acc += dI * dJ * mI->area;
+            }
+            A[(size_t)I * n + J] = acc * mJ->ds;
+        }
+    }
+    for (int i = 0; i < n; ++i) rhs[i] = 1.0;
+    int r = ora_lu_solve_t(n, A, rhs);
+    for (int i = 0; i < n && r == 0; ++i) b->m[i].eps = rhs[i];
+    free(A);
+    free(rhs);
+    return r;
+}
+
+/* ObjectManager::ibm_initialise, :190-246 */
+int ora_ibm_initialise(ora_ibm *s) {
+    for (int i = 0; i < s->nb; ++i)
+        if (find_support(s, &s->b[i])) return -1;
+    for (int i = 0; i < s->nb; ++i) compute_ds(s, &s->b[i]);
+    for (int i = 0; i < s->nb; ++i)
+        if (find_epsilon(s, &s->b[i])) return -2;
+    return 0;
+}
+
+static size_t site_id(const ora_desc *d, int i, int j, int k) { return (size_t)k + (size_t)j * d->K + (size_t)i * d->K * d->M; }
+
+/* ibm_interpolate (:498-591), ibm_computeForce (:598-614), ibm_spread (:621-689),
+ * ibm_updateMacroscopic (:696-766) */
+static void ibm_apply(ora_grid *g, void *ctx) {
+    ora_ibm *s = (ora_ibm *)ctx;
+    const ora_desc *d = ora_get_desc(g);
+    const int D = d->dims;
+    double *rho = ora_rho(g), *u = ora_u(g), *F = ora_force(g);
+    for (int ib = 0; ib < s->nb; ++ib) {
+        obody *b = &s->b[ib];
+        for (int mi = 0; mi < b->nm; ++mi) {
+            omarker *m = &b->m[mi];
+            m->mom[0] = m->mom[1] = m->mom[2] = 0.0;
+            m->rho = 0.0;
+            for (int k = 0; k < m->ns; ++k) {
+                size_t id = site_id(d, m->si[k], m->sj[k], m->sk[k]);
+                m->rho += rho[id] * m->delta[k] * m->area;
+                for (int dir = 0; dir < D; ++dir) m->mom[dir] += rho[id] * u[dir + D * id] * m->delta[k] * m->area;
+            }
+        }
+    }
+    for (int ib = 0; ib < s->nb; ++ib)
+        for (int mi = 0; mi < s->b[ib].nm; ++mi) {
+            omarker *m = &s->b[ib].m[mi];
+            for (int dir = 0; dir < D; ++dir) m->force[dir] = 2.0 * (m->mom[dir] - m->rho * m->vel[dir]) / 1.0;
+        }
+    for (int ib = 0; ib < s->nb; ++ib)
+        for (int mi = 0; mi < s->b[ib].nm; ++mi) {
+            omarker *m = &s->b[ib].m[mi];
+            for (int k = 0; k < m->ns; ++k) {
+                double volWidth = m->eps, volDepth = (D == 3) ? m->ds : 1.0;
+                size_t id = site_id(d, m->si[k], m->sj[k], m->sk[k]);
+                for (int dir = 0; dir < D; ++dir)
+                    F[dir + D * id] -= m->delta[k] * m->force[dir] * volWidth * volDepth * m->ds;
+            }
+        }
+    for (int ib = 0; ib < s->nb; ++ib)
+        for (int mi = 0; mi < s->b[ib].nm; ++mi) {
+            omarker *m = &s->b[ib].m[mi];
+            for (int k = 0; k < m->ns; ++k) ora_macro_site(g, m->si[k], m->sj[k], m->sk[k]);
+        }
+}
+
+void ora_ibm_attach(ora_ibm *s) { ora_set_ibm_hook(s->g, ibm_apply, s); }
+
+void ora_ibm_set_velocity(ora_ibm *s, int ib, const double *vel) {
+    for (int i = 0; i < s->b[ib].nm; ++i)
+        for (int a = 0; a < 3; ++a) s->b[ib].m[i].vel[a] = vel[3 * i + a];
+}
+void ora_ibm_set_epsilon(ora_ibm *s, int ib, const double *eps) {
+    for (int i = 0; i < s->b[ib].nm; ++i) s->b[ib].m[i].eps = eps[i];
+}
+
+/* per-marker record: pos[3] vel[3] force[3] mom[3] rho eps ds ns = 16 doubles */
+void ora_ibm_get_markers(const ora_ibm *s, int ib, double *out) {
+    for (int i = 0; i < s->b[ib].nm; ++i) {
+        const omarker *m = &s->b[ib].m[i];
+        double *o = out + 16 * (size_t)i;
+        for (int a = 0; a < 3; ++a) {
+            o[a] = m->pos[a];
+            o[3 + a] = m->vel[a];
+            o[6 + a] = m->force[a];
+            o[9 + a] = m->mom[a];
+        }
+        o[12] = m->rho;
+        o[13] = m->eps;
+        o[14] = m->ds;
+        o[15] = m->ns;
+    }
+}
+
+/* support list of a marker: (i, j, k) triples and delta values; returns the count */
+int ora_ibm_get_support(const ora_ibm *s, int ib, int mi, int *ijk, double *delta) {
+    const omarker *m = &s->b[ib].m[mi];
+    for (int k = 0; k < m->ns; ++k) {
+        ijk[3 * k] = m->si[k];
+        ijk[3 * k + 1] = m->sj[k];
+        ijk[3 * k + 2] = m->sk[k];
+        delta[k] = m->delta[k];
+    }
+    return m->ns;
+}

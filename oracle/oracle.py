@@ -51,6 +51,15 @@ def _load():
             "ora_reset_forces": ([_vp], None), "ora_run": ([_vp, C.c_int, C.c_int], None),
             "ora_equilibrium": ([_vp, C.c_int, C.c_int], C.c_double),
             "ora_lattice": ([C.c_int, _ip, _dp, _ip], None), "ora_cs": ([], C.c_double),
+            "ora_ibm_create": ([_vp, C.c_double, _dp], _vp), "ora_ibm_destroy": ([_vp], None),
+            "ora_ibm_add_body": ([_vp, C.c_int, _dp, C.c_int], C.c_int),
+            "ora_ibm_initialise": ([_vp], C.c_int), "ora_ibm_attach": ([_vp], None),
+            "ora_ibm_set_velocity": ([_vp, C.c_int, _dp], None),
+            "ora_ibm_set_epsilon": ([_vp, C.c_int, _dp], None),
+            "ora_ibm_get_markers": ([_vp, C.c_int, _dp], None),
+            "ora_ibm_get_support": ([_vp, C.c_int, C.c_int, _ip, _dp], C.c_int),
+            "ora_ibm_delta": ([C.c_double, C.c_double], C.c_double),
+            "ora_lu_solve_t": ([C.c_int, _dp, _dp], C.c_int),
         }
         for name, (args, res) in protos.items():
             fn = getattr(lib, name)
@@ -150,6 +159,12 @@ class OracleGrid:
         self.lib.ora_get_macro(self.h, rho.ctypes.data_as(_dp), u.ctypes.data_as(_dp))
         return rho, u
 
+    def get_force(self):
+        N, M, K = self.shape
+        F = np.empty((N, M, K, self.dims))
+        self.lib.ora_get_force(self.h, F.ctypes.data_as(_dp))
+        return F
+
     def step(self, subcycle=0):
         self.lib.ora_multi_opt(self.h, subcycle)
 
@@ -162,3 +177,64 @@ class OracleGrid:
 
     def equilibrium(self, id_, v):
         return self.lib.ora_equilibrium(self.h, id_, v)
+
+
+class OracleIBM:
+    """ObjectManager's IB bodies on one oracle grid (serial LUMA semantics)."""
+
+    def __init__(self, grid: OracleGrid, dh: float, origin=(0.0, 0.0, 0.0)):
+        self.lib = _load()
+        self.grid = grid
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        self.h = self.lib.ora_ibm_create(grid.h, dh, o.ctypes.data_as(_dp))
+        self.nm = []
+
+    def add_body(self, pos, flexible=False):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.nm.append(len(p))
+        return self.lib.ora_ibm_add_body(self.h, len(p), p.ctypes.data_as(_dp), int(flexible))
+
+    def initialise(self):
+        rc = self.lib.ora_ibm_initialise(self.h)
+        if rc != 0:
+            raise RuntimeError(f"oracle ibm_initialise failed ({rc})")
+        self.lib.ora_ibm_attach(self.h)
+
+    def set_velocity(self, ib, vel):
+        v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        self.lib.ora_ibm_set_velocity(self.h, ib, v.ctypes.data_as(_dp))
+
+    def set_epsilon(self, ib, eps):
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        self.lib.ora_ibm_set_epsilon(self.h, ib, e.ctypes.data_as(_dp))
+
+    def markers(self, ib):
+        out = np.empty((self.nm[ib], 16))
+        self.lib.ora_ibm_get_markers(self.h, ib, out.ctypes.data_as(_dp))
+        return {"pos": out[:, 0:3], "vel": out[:, 3:6], "force": out[:, 6:9], "mom": out[:, 9:12],
+                "rho": out[:, 12], "eps": out[:, 13], "ds": out[:, 14], "ns": out[:, 15].astype(int)}
+
+    def support(self, ib, m):
+        ijk = np.empty(27 * 3, np.int32)
+        d = np.empty(27)
+        n = self.lib.ora_ibm_get_support(self.h, ib, m, ijk.ctypes.data_as(_ip), d.ctypes.data_as(_dp))
+        return ijk[:3 * n].reshape(n, 3), d[:n]
+
+    def close(self):
+        if self.h:
+            self.lib.ora_ibm_destroy(self.h)
+            self.h = None
+
+
+def delta_kernel(r, dilation=1.0):
+    return _load().ora_ibm_delta(float(r), float(dilation))
+
+
+def lu_solve(A, b):
+    """A x = b through the oracle's LU (row-major A, as GridUtils::solveLinearSystem takes it)."""
+    a = np.ascontiguousarray(A, dtype=np.float64).copy()
+    x = np.ascontiguousarray(b, dtype=np.float64).copy()
+    rc = _load().ora_lu_solve_t(len(x), a.ctypes.data_as(_dp), x.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError("singular matrix")
+    return x

@@ -1,0 +1,165 @@
+"""Immersed-boundary step (ObjectManager::ibm_apply, ObjectManager_ops_ibm.cpp:33-54) on the GPU
+against the FP64 oracle: rigid and velocity-driven bodies in 2D and 3D, on one grid and on the
+fine level of a two-level hierarchy."""
+import numpy as np
+import pytest
+
+from luma_amd import setups
+from luma_amd.grid import GpuGrid
+from oracle.oracle import OracleGrid, OracleIBM
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_grid(case):
+    g = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega, regularised=case.regularised,
+                   forced=case.forced, fz_compat=case.fz_compat, rho_in=case.rho_in, edges=case.edges,
+                   wall_thk=case.wall_thk, level=case.info.get("level", 0),
+                   coarse_lims=case.info.get("coarse_lims", (0,) * 6))
+    if case.lattyp is not None:
+        g.set_labels(case.lattyp)
+    g.set_positions(case.xpos, case.ypos, case.zpos)
+    if case.ux_in is not None:
+        g.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+    g.init_feq(case.rho, case.u)
+    return g
+
+
+def gpu_grid(case, prec="fp64"):
+    g = GpuGrid.from_case(case, prec, store_macro=True)
+    if case.lattyp is None:
+        g.set_positions(case.xpos, case.ypos, case.zpos)
+    return g
+
+
+def compare_grids(case, gg, og, tag):
+    m = np.ones((case.N, case.M, case.K), bool) if case.lattyp is None else \
+        ~np.isin(case.lattyp, [setups.SOLID, setups.REFINED, setups.EXTRAP_RIGHT])
+    d = np.abs(gg.download_f()[m] - og.get_f()[m]).max()
+    assert d == 0.0, f"{tag}: f differs by {d}"
+    (rg, ug), (ro, uo) = gg.download_macro(), og.get_macro()
+    assert np.array_equal(rg[m], ro[m]) and np.array_equal(ug[m], uo[m]), tag
+
+
+def compare_markers(gg, ib, body=0):
+    a, b = gg.ibm_markers(body), ib.markers(body)
+    for key in ("eps", "ds", "ns"):
+        assert np.array_equal(a[key], b[key]), key
+    for key in ("rho", "mom", "force"):
+        assert np.array_equal(a[key], b[key]), (key, np.abs(a[key] - b[key]).max())
+
+
+def periodic_box(N, M, K, u, tau, dims=3):
+    dh = 1.0
+    c = setups.taylor_green((N, M, K), dims=dims, tau=tau, u0=0.0)
+    c.u[...] = 0.0
+    c.u[..., 0] = u
+    c.rho[...] = 1.0
+    c.forced = True
+    c.xpos = dh / 2 + dh * np.arange(N)
+    c.ypos = dh / 2 + dh * np.arange(M)
+    c.zpos = dh / 2 + dh * np.arange(K) if dims == 3 else np.array([0.0])
+    c.edges = (N * dh, M * dh, K * dh if dims == 3 else 0.0)
+    c.info.update({"dh": dh, "origin": (0.0, 0.0, 0.0)})
+    return c
+
+
+def test_ib_cylinder_channel_fp64_bitwise():
+    c = setups.channel_2d(80, 40, 10.0, 0.04, 100.0)
+    pos = setups.circle_markers((2.05, 2.03, 0.0), 0.5, c.info["dh"])
+    print(f"[ibm] cylinder channel {c.N}x{c.M}, {len(pos)} markers", flush=True)
+    og = oracle_grid(c)
+    ib = OracleIBM(og, c.info["dh"])
+    ib.add_body(pos)
+    ib.initialise()
+    with gpu_grid(c) as gg:
+        gg.ibm_add_body(pos)
+        gg.ibm_initialise()
+        for _ in range(150):
+            og.step()
+            gg.LBM_multi_opt()
+        compare_markers(gg, ib)
+        compare_grids(c, gg, og, "channel")
+        mk = ib.markers(0)
+        ref = (mk["force"] * (mk["eps"] * mk["ds"])[:, None]).sum(0)
+        assert np.allclose(gg.ibm_body_force(0), ref, rtol=1e-13, atol=1e-16)
+
+
+def test_ib_sphere_3d_fp64_bitwise():
+    c = periodic_box(20, 16, 16, 0.03, 0.6)
+    pos = setups.sphere_markers((8.3, 8.1, 7.9), 3.0, 1.0)
+    print(f"[ibm] sphere 20x16x16, {len(pos)} markers", flush=True)
+    og = oracle_grid(c)
+    ib = OracleIBM(og, 1.0)
+    ib.add_body(pos)
+    ib.initialise()
+    with gpu_grid(c) as gg:
+        gg.ibm_add_body(pos)
+        gg.ibm_initialise()
+        for _ in range(25):
+            og.step()
+            gg.LBM_multi_opt()
+        compare_markers(gg, ib)
+        compare_grids(c, gg, og, "sphere")
+
+
+def test_ib_moving_markers_fp64_bitwise():
+    # rigid positions with a prescribed surface velocity (rotating cylinder)
+    c = periodic_box(40, 32, 1, 0.0, 0.7, dims=2)
+    pos = setups.circle_markers((20.2, 16.1, 0.0), 5.0, 1.0, luma_skip_first=False)
+    vel = np.zeros_like(pos)
+    vel[:, 0] = -0.01 * (pos[:, 1] - 16.1) / 5.0
+    vel[:, 1] = 0.01 * (pos[:, 0] - 20.2) / 5.0
+    og = oracle_grid(c)
+    ib = OracleIBM(og, 1.0)
+    ib.add_body(pos)
+    ib.initialise()
+    ib.set_velocity(0, vel)
+    with gpu_grid(c) as gg:
+        gg.ibm_add_body(pos)
+        gg.ibm_initialise()
+        gg.ibm_set_velocity(0, vel)
+        for _ in range(100):
+            og.step()
+            gg.LBM_multi_opt()
+        compare_markers(gg, ib)
+        compare_grids(c, gg, og, "rotating")
+
+
+def test_ib_cylinder_on_refined_level_fp64_bitwise():
+    parent = setups.channel_2d(64, 32, 8.0, 0.04, 80.0)
+    child = setups.add_refined_region(parent, (8, 31, 6, 25), parent.info["dh"], parent.info["dt"])
+    cdh = child.info["dh"]
+    pos = setups.circle_markers((2.5 + 0.01, 2.0 + 0.13, 0.0), 0.5, cdh)
+    print(f"[ibm] refined cylinder, child {child.N}x{child.M}, {len(pos)} markers", flush=True)
+    op, oc = oracle_grid(parent), oracle_grid(child)
+    op.add_child(oc)
+    ib = OracleIBM(oc, cdh, child.info["origin"])
+    ib.add_body(pos)
+    ib.initialise()
+    with gpu_grid(parent) as gp, gpu_grid(child) as gc:
+        gp.add_child(gc)
+        gc.ibm_add_body(pos)
+        gc.ibm_initialise()
+        for _ in range(40):
+            op.step()
+            gp.LBM_multi_opt()
+        compare_markers(gc, ib)
+        compare_grids(parent, gp, op, "parent")
+        compare_grids(child, gc, oc, "child")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32s", 2e-4), ("fp16s", 5e-2)])
+def test_ib_body_force_shifted_forms_track_fp64(prec, tol):
+    c = setups.channel_2d(80, 40, 10.0, 0.04, 100.0)
+    pos = setups.circle_markers((2.05, 2.03, 0.0), 0.5, c.info["dh"])
+    res = {}
+    for pr in ("fp64", prec):
+        with gpu_grid(c, pr) as gg:
+            gg.ibm_add_body(pos)
+            gg.ibm_initialise()
+            gg.run(400)
+            res[pr] = gg.ibm_body_force(0)
+    err = np.abs(res[prec] - res["fp64"]).max() / np.abs(res["fp64"]).max()
+    print(prec, "body force", res[prec], "fp64", res["fp64"], "rel err", err)
+    assert err < tol
