@@ -13,7 +13,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 OUT = HERE / "_build" / "liboracle.so"
-SOURCES = ["lbm_oracle.c", "ibm_oracle.c"]
+SOURCES = ["lbm_oracle.c", "ibm_oracle.c", "fem_oracle.c"]
 
 
 def build_oracle(verbose: bool = False) -> Path:

@@ -9,12 +9,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "fem_oracle.h"
 #include "lbm_oracle.h"
 
 #define SMAX 27
 
 typedef struct {
-    double pos[3], vel[3], vel_km1[3], force[3], mom[3];
+    double pos[3], pos0[3], vel[3], vel_km1[3], force[3], mom[3];
     double rho, eps, ds, dilation, area;
     int ns;
     int si[SMAX], sj[SMAX], sk[SMAX];
@@ -25,6 +26,7 @@ typedef struct {
     int nm;
     omarker *m;
     int flexible;
+    ofem *fem;
 } obody;
 
 typedef struct ora_ibm {
@@ -32,7 +34,10 @@ typedef struct ora_ibm {
     int nb;
     obody b[16];
     double dh, origin[3];
+    double dt, ux0;   /* grid dt and L_UX0 for the sub-iteration residual */
     int err;
+    int sub_it;       /* sub-iterations of the last step */
+    double sub_res;
 } ora_ibm;
 
 static int D_of(const ora_ibm *s) { return ora_get_desc(s->g)->dims; }
@@ -56,7 +61,10 @@ ora_ibm *ora_ibm_create(ora_grid *g, double dh, const double *origin) {
 
 void ora_ibm_destroy(ora_ibm *s) {
     if (!s) return;
-    for (int i = 0; i < s->nb; ++i) free(s->b[i].m);
+    for (int i = 0; i < s->nb; ++i) {
+        free(s->b[i].m);
+        ora_fem_destroy(s->b[i].fem);
+    }
     free(s);
 }
 
@@ -69,12 +77,35 @@ int ora_ibm_add_body(ora_ibm *s, int n, const double *pos, int flexible) {
     b->flexible = flexible;
     b->m = (omarker *)calloc(n, sizeof(omarker));
     for (int i = 0; i < n; ++i) {
-        for (int a = 0; a < 3; ++a) b->m[i].pos[a] = pos[3 * i + a];
+        for (int a = 0; a < 3; ++a) b->m[i].pos[a] = b->m[i].pos0[a] = pos[3 * i + a];
         b->m[i].area = 1.0;
         b->m[i].dilation = 1.0;
         b->m[i].ds = 1.0;
     }
     return s->nb++;
+}
+
+/* IBBody filament ctor with an FEM body (IBBody.cpp:240-258, Body.h:144-187, 2D): markers at
+ * floor(L/dh) + 1 points along the filament, FEMBody of nEl corotational beam elements */
+int ora_ibm_add_filament(ora_ibm *s, const double *start, double length, double height, double depth,
+                         double angle_deg, int nEl, int clamped, double density, double E, double dt, double dm) {
+    const double dh = s->dh;
+    const int n = (int)floor(length / dh) + 1;
+    const double spacing = length / (n - 1);
+    const double spacing_h = spacing * cos(angle_deg * 3.14159265358979323846 / 180);
+    double *pos = (double *)malloc(sizeof(double) * 3 * n);
+    for (int i = 0; i < n; ++i) {
+        pos[3 * i] = start[0] + i * spacing_h * cos(0.0 * 3.14159265358979323846 / 180.0);
+        pos[3 * i + 1] = start[1] + i * spacing * sin(angle_deg * 3.14159265358979323846 / 180.0);
+        pos[3 * i + 2] = start[2] + i * spacing_h * sin(0.0 * 3.14159265358979323846 / 180.0);
+    }
+    int ib = ora_ibm_add_body(s, n, pos, 1);
+    free(pos);
+    if (ib < 0) return ib;
+    s->b[ib].fem = ora_fem_create(start, length, height, depth, angle_deg, nEl, clamped, density, E, dh, dt, dm);
+    s->dt = dt;
+    s->ux0 = 1.0;
+    return ib;
 }
 
 /* ibm_initialiseSupport, :462-491 */
@@ -255,8 +286,91 @@ static size_t site_id(const ora_desc *d, int i, int j, int k) { return (size_t)k
 
 /* ibm_interpolate (:498-591), ibm_computeForce (:598-614), ibm_spread (:621-689),
  * ibm_updateMacroscopic (:696-766) */
-static void ibm_apply(ora_grid *g, void *ctx) {
-    ora_ibm *s = (ora_ibm *)ctx;
+static void ibm_apply_core(ora_grid *g, ora_ibm *s);
+static int has_flexible(const ora_ibm *s) {
+    for (int i = 0; i < s->nb; ++i)
+        if (s->b[i].flexible) return 1;
+    return 0;
+}
+
+/* ibm_moveBodies, ObjectManager_ops_ibm.cpp:61-104 (serial) */
+static void move_bodies(ora_ibm *s) {
+    for (int ib = 0; ib < s->nb; ++ib) {
+        obody *b = &s->b[ib];
+        if (!b->flexible || !b->fem) continue;
+        const int n = b->nm;
+        double *force = (double *)malloc(sizeof(double) * 3 * n), *eps = (double *)malloc(sizeof(double) * n);
+        double *pos = (double *)malloc(sizeof(double) * 3 * n), *pos0 = (double *)malloc(sizeof(double) * 3 * n);
+        double *vel = (double *)malloc(sizeof(double) * 3 * n), *vkm1 = (double *)malloc(sizeof(double) * 3 * n);
+        for (int i = 0; i < n; ++i) {
+            eps[i] = b->m[i].eps;
+            for (int a = 0; a < 3; ++a) {
+                force[3 * i + a] = b->m[i].force[a];
+                pos[3 * i + a] = b->m[i].pos[a];
+                pos0[3 * i + a] = b->m[i].pos0[a];
+                vel[3 * i + a] = b->m[i].vel[a];
+                vkm1[3 * i + a] = b->m[i].vel_km1[a];
+            }
+        }
+        if (ora_fem_dynamic(b->fem, force, eps, pos, pos0, vel, vkm1)) s->err = 1;
+        for (int i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b->m[i].pos[a] = pos[3 * i + a];
+                b->m[i].vel[a] = vel[3 * i + a];
+                b->m[i].vel_km1[a] = vkm1[3 * i + a];
+            }
+        free(force); free(eps); free(pos); free(pos0); free(vel); free(vkm1);
+    }
+    for (int ib = 0; ib < s->nb; ++ib)
+        if (s->b[ib].flexible && find_support(s, &s->b[ib])) s->err = 2;
+    for (int ib = 0; ib < s->nb; ++ib) compute_ds(s, &s->b[ib]);
+    for (int ib = 0; ib < s->nb; ++ib)
+        if (find_epsilon(s, &s->b[ib])) s->err = 3;
+}
+
+/* ibm_checkVelDiff, :988-1032 */
+static double vel_diff(const ora_ibm *s) {
+    double res = 0.0;
+    for (int ib = 0; ib < s->nb; ++ib) {
+        const obody *b = &s->b[ib];
+        if (!b->flexible) continue;
+        for (int i = 0; i < b->nm; ++i) {
+            double acc = 0.0;
+            for (int a = 0; a < 3; ++a) acc += pow(b->m[i].vel[a] - b->m[i].vel_km1[a], 2);
+            double v = sqrt(acc);
+            v = v / (s->ux0 * s->dt / s->dh);
+            if (fabs(v) > res) res = fabs(v);
+        }
+    }
+    return res;
+}
+
+/* ObjectManager::ibm_apply (:33-54) and ibm_subIterate (:111-185) */
+static void ibm_apply_flag(ora_grid *g, ora_ibm *s, int doSub) {
+    ibm_apply_core(g, s);
+    if (!has_flexible(s)) return;
+    move_bodies(s);
+    if (!doSub) return;
+    const ora_desc *d = ora_get_desc(g);
+    const size_t nu = (size_t)d->N * d->M * d->K * d->dims;
+    double res;
+    int it = 0;
+    do {
+        memcpy(ora_u(g), ora_u_n(g), nu * sizeof(double));
+        ora_reset_forces(g);
+        ibm_apply_flag(g, s, 0);
+        res = vel_diff(s);
+        it++;
+    } while (res > 1e-4 && it < 10);
+    s->sub_it = it;
+    s->sub_res = res;
+    for (int ib = 0; ib < s->nb; ++ib)
+        if (s->b[ib].flexible && s->b[ib].fem) ora_fem_commit(s->b[ib].fem);
+}
+
+static void ibm_apply(ora_grid *g, void *ctx) { ibm_apply_flag(g, (ora_ibm *)ctx, 1); }
+
+static void ibm_apply_core(ora_grid *g, ora_ibm *s) {
     const ora_desc *d = ora_get_desc(g);
     const int D = d->dims;
     double *rho = ora_rho(g), *u = ora_u(g), *F = ora_force(g);
@@ -295,7 +409,20 @@ static void ibm_apply(ora_grid *g, void *ctx) {
         }
 }
 
-void ora_ibm_attach(ora_ibm *s) { ora_set_ibm_hook(s->g, ibm_apply, s); }
+void ora_ibm_attach(ora_ibm *s) {
+    ora_set_ibm_hook(s->g, ibm_apply, s);
+    if (has_flexible(s)) ora_set_u_n_copy(s->g, 1);
+}
+int ora_ibm_error(const ora_ibm *s) { return s->err; }
+void ora_ibm_subiter(const ora_ibm *s, int *it, double *res) {
+    *it = s->sub_it;
+    *res = s->sub_res;
+}
+int ora_ibm_fem_state(const ora_ibm *s, int ib, double *U, double *Udot, int *it, double *res) {
+    if (!s->b[ib].fem) return 0;
+    ora_fem_state(s->b[ib].fem, U, Udot, it, res);
+    return ora_fem_ndof(s->b[ib].fem);
+}
 
 void ora_ibm_set_velocity(ora_ibm *s, int ib, const double *vel) {
     for (int i = 0; i < s->b[ib].nm; ++i)

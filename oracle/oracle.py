@@ -60,6 +60,18 @@ def _load():
             "ora_ibm_get_support": ([_vp, C.c_int, C.c_int, _ip, _dp], C.c_int),
             "ora_ibm_delta": ([C.c_double, C.c_double], C.c_double),
             "ora_lu_solve_t": ([C.c_int, _dp, _dp], C.c_int),
+            "ora_ibm_add_filament": ([_vp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
+            "ora_ibm_error": ([_vp], C.c_int),
+            "ora_fem_create": ([_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
+                                C.c_double, C.c_double, C.c_double, C.c_double], _vp),
+            "ora_fem_destroy": ([_vp], None), "ora_fem_ndof": ([_vp], C.c_int),
+            "ora_fem_assemble": ([_vp, _dp, _dp], None),
+            "ora_fem_set_state": ([_vp, _dp, _dp, _dp], None),
+            "ora_fem_step_load": ([_vp, _dp], C.c_int),
+            "ora_fem_state": ([_vp, _dp, _dp, _ip, _dp], None),
+            "ora_ibm_subiter": ([_vp, _ip, _dp], None),
+            "ora_ibm_fem_state": ([_vp, C.c_int, _dp, _dp, _ip, _dp], C.c_int),
         }
         for name, (args, res) in protos.items():
             fn = getattr(lib, name)
@@ -188,11 +200,34 @@ class OracleIBM:
         o = np.ascontiguousarray(origin, dtype=np.float64)
         self.h = self.lib.ora_ibm_create(grid.h, dh, o.ctypes.data_as(_dp))
         self.nm = []
+        self.dh = dh
 
     def add_body(self, pos, flexible=False):
         p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         self.nm.append(len(p))
         return self.lib.ora_ibm_add_body(self.h, len(p), p.ctypes.data_as(_dp), int(flexible))
+
+    def add_filament(self, start, length, height, depth, angle_deg, n_el, clamped, density, E, dt, dm):
+        st = np.ascontiguousarray(start, dtype=np.float64)
+        ib = self.lib.ora_ibm_add_filament(self.h, st.ctypes.data_as(_dp), length, height, depth, angle_deg, n_el,
+                                           int(clamped), density, E, dt, dm)
+        self.nm.append(int(np.floor(length / self.dh)) + 1)
+        return ib
+
+    def error(self):
+        return self.lib.ora_ibm_error(self.h)
+
+    def subiterations(self):
+        it, res = C.c_int(), C.c_double()
+        self.lib.ora_ibm_subiter(self.h, C.byref(it), C.byref(res))
+        return it.value, res.value
+
+    def fem_state(self, ib, ndof):
+        U = np.empty(ndof)
+        Ud = np.empty(ndof)
+        it, res = C.c_int(), C.c_double()
+        self.lib.ora_ibm_fem_state(self.h, ib, U.ctypes.data_as(_dp), Ud.ctypes.data_as(_dp), C.byref(it), C.byref(res))
+        return U, Ud, it.value, res.value
 
     def initialise(self):
         rc = self.lib.ora_ibm_initialise(self.h)
@@ -238,3 +273,40 @@ def lu_solve(A, b):
     if rc != 0:
         raise RuntimeError("singular matrix")
     return x
+
+
+class OracleFEM:
+    """Stand-alone FEMBody (2D corotational beam) for pinning tests."""
+
+    def __init__(self, start, length, height, angle_deg, n_el, clamped, density, E, dh=1.0, dt=1.0, dm=1.0):
+        self.lib = _load()
+        st = np.ascontiguousarray(start, dtype=np.float64)
+        self.h = self.lib.ora_fem_create(st.ctypes.data_as(_dp), length, height, dh, angle_deg, n_el, int(clamped),
+                                         density, E, dh, dt, dm)
+        self.n = self.lib.ora_fem_ndof(self.h)
+
+    def assemble(self):
+        K = np.empty((self.n, self.n))
+        M = np.empty((self.n, self.n))
+        self.lib.ora_fem_assemble(self.h, K.ctypes.data_as(_dp), M.ctypes.data_as(_dp))
+        return K, M
+
+    def set_state(self, U, Ud=None, Udd=None):
+        z = np.zeros(self.n)
+        a, b, c = [np.ascontiguousarray(x if x is not None else z, dtype=np.float64) for x in (U, Ud, Udd)]
+        self.lib.ora_fem_set_state(self.h, a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), c.ctypes.data_as(_dp))
+
+    def step(self, R=None):
+        r = np.ascontiguousarray(R if R is not None else np.zeros(self.n), dtype=np.float64)
+        return self.lib.ora_fem_step_load(self.h, r.ctypes.data_as(_dp))
+
+    def state(self):
+        U, Ud = np.empty(self.n), np.empty(self.n)
+        it, res = C.c_int(), C.c_double()
+        self.lib.ora_fem_state(self.h, U.ctypes.data_as(_dp), Ud.ctypes.data_as(_dp), C.byref(it), C.byref(res))
+        return U, Ud
+
+    def close(self):
+        if self.h:
+            self.lib.ora_fem_destroy(self.h)
+            self.h = None
