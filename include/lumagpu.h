@@ -67,6 +67,8 @@ typedef struct lumagpu_grid_desc {
     int coarse_lims[6];        /* CoarseLimsX/Y/Z[min,max] on the parent (level > 0) */
     double dh;                 /* GridObj::dh (IB support search and distances) */
     double origin[3];          /* global_edges[e?Min] of this grid's level and region */
+    double dt;                 /* GridObj::dt (FEM time step, sub-iteration residual scale) */
+    double dm;                 /* GridObj::dm (FEM load scaling dm/dt^2) */
 } lumagpu_grid_desc;
 
 /* ---- lifetime ------------------------------------------------------------------------ */
@@ -124,6 +126,15 @@ int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]);
 /* Body<IBMarker>::addMarker (Body.h:144-255) + IBMarker ctor (IBMarker.cpp:56-86):
  * a body of n markers at pos[3*m + a]; epsilon 0, ds 1, markerVel 0 until initialised. */
 int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int flexible, int *body);
+/* IBBody filament with an FEMBody (IBBody.cpp:240-258, FEMBody.cpp:62-131), 2D: floor(L/dh) + 1
+ * markers from `start` at angle_deg, n_elements corotational beams, clamped or supported base,
+ * density and Young's modulus as LUMA's geometry config takes them (ObjectManager_ops_io.cpp:455-567).
+ * Such a body makes every step run LUMA's FSI sub-iterations (ObjectManager_ops_ibm.cpp:111-185). */
+int lumagpu_ibm_add_filament(lumagpu_grid *g, const double *start, double length, double height, double depth,
+                             double angle_deg, int n_elements, int clamped, double density, double youngs,
+                             int *body);
+/* sub-iterations and residual of the last step's FSI loop (ObjectManager::timeav_subResidual) */
+int lumagpu_ibm_subiterations(const lumagpu_grid *g, int *iterations, double *residual);
 /* ObjectManager::ibm_initialise (:190-246): support stencils, ds and epsilon of every body */
 int lumagpu_ibm_initialise(lumagpu_grid *g);
 /* IBMarker::markerVel for every marker of a body (vel[3*m + a]) */

@@ -31,7 +31,7 @@ class GridDesc(C.Structure):
         ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
         ("wall_thk", C.c_double * 6),
         ("coarse_lims", C.c_int * 6),
-        ("dh", C.c_double), ("origin", C.c_double * 3),
+        ("dh", C.c_double), ("origin", C.c_double * 3), ("dt", C.c_double), ("dm", C.c_double),
     ]
 
 
@@ -68,6 +68,9 @@ _PROTOS = {
     "lumagpu_set_positions": [_vp, _dp, _dp, _dp],
     "lumagpu_ibm_add_body": [_vp, C.c_int, _dp, C.c_int, _ip],
     "lumagpu_ibm_initialise": [_vp],
+    "lumagpu_ibm_add_filament": [_vp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, _ip],
+    "lumagpu_ibm_subiterations": [_vp, _ip, _dp],
     "lumagpu_ibm_set_velocity": [_vp, C.c_int, _dp],
     "lumagpu_ibm_set_epsilon": [_vp, C.c_int, _dp],
     "lumagpu_ibm_get_markers": [_vp, C.c_int, _dp],

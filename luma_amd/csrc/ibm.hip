@@ -13,6 +13,9 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
+#include "fem.hpp"
 #include "ibm_kernels.hpp"
 #include "internal.hpp"
 
@@ -23,7 +26,7 @@ namespace {
 constexpr int kSmax = 27;
 
 struct Marker {
-    double pos[3] = {0, 0, 0}, vel[3] = {0, 0, 0};
+    double pos[3] = {0, 0, 0}, pos0[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, vel_km1[3] = {0, 0, 0};
     double eps = 0.0, ds = 1.0, dilation = 1.0, area = 1.0;
     int ns = 0;
     int sijk[kSmax][3];
@@ -34,18 +37,27 @@ struct Marker {
 struct Body {
     std::vector<Marker> m;
     int flexible = 0;
-    int first = 0;   // index of the first marker in the grid-wide marker arrays
+    int first = 0;                  // index of the first marker in the grid-wide marker arrays
+    std::unique_ptr<FemBeam> fem;   // flexible filaments only
 };
 
+// Device buffer of n elements, reallocated only when n changes.  A fresh buffer is zeroed.
 template <class T>
-int dev_upload(T **d, const std::vector<T> &h, hipStream_t s) {
-    if (*d) {
-        (void)hipFree(*d);
-        *d = nullptr;
-    }
-    if (h.empty()) return 0;
-    if (hipMalloc(d, h.size() * sizeof(T)) != hipSuccess) return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
-    if (hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+int dev_resize(T **d, size_t *cap, size_t n, hipStream_t s) {
+    if (*d && *cap == n) return 0;
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    *cap = 0;
+    if (n == 0) return 0;
+    if (hipMalloc(d, n * sizeof(T)) != hipSuccess) return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
+    *cap = n;
+    if (hipMemsetAsync(*d, 0, n * sizeof(T), s) != hipSuccess) return fail(LUMAGPU_EHIP, "IB buffer clear failed");
+    return 0;
+}
+template <class T>
+int dev_upload(T **d, size_t *cap, const std::vector<T> &h, hipStream_t s) {
+    if (int r = dev_resize(d, cap, h.size(), s)) return r;
+    if (!h.empty() && hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
         return fail(LUMAGPU_EHIP, "IB buffer upload failed");
     return 0;
 }
@@ -69,8 +81,12 @@ struct IbmSystem {
     int *d_scell = nullptr, *d_ns = nullptr, *d_sites = nullptr, *d_sptr = nullptr, *d_sent = nullptr;
     double *d_sdelta = nullptr, *d_vel = nullptr, *d_eps = nullptr, *d_ds = nullptr;
     double *d_interp = nullptr, *d_force = nullptr, *d_out = nullptr;
+    size_t cap[11] = {};        // element counts of the device buffers above, in declaration order
     int nsites = 0;
-    std::vector<int> sites;   // host copy of the distinct support cells
+    std::vector<int> sites;     // host copy of the distinct support cells
+    std::vector<int> flagged;   // cells whose force and flag the device currently holds
+    int sub_it = 0;             // sub-iterations of the last step
+    double sub_res = 0.0;
 
     ~IbmSystem() {
         for (void *p : {(void *)d_scell, (void *)d_ns, (void *)d_sites, (void *)d_sptr, (void *)d_sent,
@@ -298,33 +314,20 @@ int upload(lumagpu_grid *g) {
         sptr[i + 1] = sptr[i] + (int)entries[i].size();
         sent.insert(sent.end(), entries[i].begin(), entries[i].end());
     }
-    // sites that were supports before and are not now lose their force and flag
-    std::vector<int> gone;
-    for (int c : s->sites)
-        if (site_of[c] < 0) gone.push_back(c);
-    int *d_gone = nullptr;
-    if (int r = dev_upload(&d_gone, gone, g->stream)) return r;
-    if (!gone.empty()) {
-        IbmArgs b = ibm_args(g);
-        k_ibm_clear<<<((int)gone.size() + 127) / 128, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
-        HIP_TRY(hipGetLastError());
-    }
-    for (int c : order) g->hflags[c] |= kForcedBit;
-    for (int c : gone) g->hflags[c] &= (uint8_t)~kForcedBit;
-    std::vector<double> zeros((size_t)nm * 4, 0.0);
+    // Interpolated values and marker forces persist across re-uploads (a marker keeps them
+    // when its body moves); they start at zero when the marker set changes size.
     int r = 0;
-    if ((r = dev_upload(&s->d_scell, scell, g->stream)) || (r = dev_upload(&s->d_ns, ns, g->stream)) ||
-        (r = dev_upload(&s->d_sdelta, sdelta, g->stream)) || (r = dev_upload(&s->d_vel, vel, g->stream)) ||
-        (r = dev_upload(&s->d_eps, eps, g->stream)) || (r = dev_upload(&s->d_ds, ds, g->stream)) ||
-        (r = dev_upload(&s->d_sites, order, g->stream)) || (r = dev_upload(&s->d_sptr, sptr, g->stream)) ||
-        (r = dev_upload(&s->d_sent, sent, g->stream)) || (r = dev_upload(&s->d_interp, zeros, g->stream)))
+    size_t *c = s->cap;
+    if ((r = dev_upload(&s->d_scell, c + 0, scell, g->stream)) || (r = dev_upload(&s->d_ns, c + 1, ns, g->stream)) ||
+        (r = dev_upload(&s->d_sites, c + 2, order, g->stream)) || (r = dev_upload(&s->d_sptr, c + 3, sptr, g->stream)) ||
+        (r = dev_upload(&s->d_sent, c + 4, sent, g->stream)) || (r = dev_upload(&s->d_sdelta, c + 5, sdelta, g->stream)) ||
+        (r = dev_upload(&s->d_vel, c + 6, vel, g->stream)) || (r = dev_upload(&s->d_eps, c + 7, eps, g->stream)) ||
+        (r = dev_upload(&s->d_ds, c + 8, ds, g->stream)) || (r = dev_resize(&s->d_interp, c + 9, (size_t)nm * 4, g->stream)) ||
+        (r = dev_resize(&s->d_force, c + 10, (size_t)nm * 3, g->stream)))
         return r;
-    std::vector<double> fz((size_t)nm * 3, 0.0);
-    if ((r = dev_upload(&s->d_force, fz, g->stream))) return r;
     if (!s->d_out && hipMalloc(&s->d_out, 3 * sizeof(double)) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
     HIP_TRY(hipStreamSynchronize(g->stream));
-    (void)hipFree(d_gone);
     s->sites = order;
     s->nsites = (int)order.size();
     return 0;
@@ -366,14 +369,32 @@ void ibm_destroy(lumagpu_grid *g) {
     g->ibm = nullptr;
 }
 
-// ObjectManager::ibm_apply (:33-54) for rigid bodies: the sweep-1 macroscopic values at the
-// support sites, interpolation and marker force, spread into force_xyz.  The macroscopic
-// update at the support sites (ibm_updateMacroscopic, :696-766) happens in the step kernel,
-// which recomputes u with +F/2 wherever the force bit is set.
-int ibm_apply(lumagpu_grid *g) {
+namespace {
+
+// Interpolate, force and spread at the current supports.  Sites flagged by the previous
+// application and no longer supports are reset first (_LBM_resetForces before every
+// application, ObjectManager_ops_ibm.cpp:120-128 and GridObj_ops_lbm_optimised.cpp:55-58);
+// current supports are overwritten by the spread, which starts from zero.
+int apply_core(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
-    if (!s || s->nm == 0) return 0;
-    if (!s->initialised) return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
+    if (!s->flagged.empty()) {
+        std::vector<char> now(g->ncell, 0);
+        for (int c : s->sites) now[c] = 1;
+        std::vector<int> gone;
+        for (int c : s->flagged)
+            if (!now[c]) gone.push_back(c);
+        if (!gone.empty()) {
+            int *d_gone = nullptr;
+            size_t n_gone = 0;
+            if (int r = dev_upload(&d_gone, &n_gone, gone, g->stream)) return r;
+            IbmArgs b = ibm_args(g);
+            k_ibm_clear<<<((int)gone.size() + 127) / 128, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            (void)hipFree(d_gone);
+            for (int c : gone) g->hflags[c] &= (uint8_t)~kForcedBit;
+        }
+    }
     StepArgs a = make_args(g);
     hipError_t e = launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB site macro launch: ") + hipGetErrorString(e));
@@ -382,6 +403,94 @@ int ibm_apply(lumagpu_grid *g) {
     HIP_TRY(hipGetLastError());
     k_ibm_spread<<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
     HIP_TRY(hipGetLastError());
+    for (int c : s->sites) g->hflags[c] |= kForcedBit;
+    s->flagged = s->sites;
+    return 0;
+}
+
+// ibm_moveBodies (:61-104): structural step of every flexible body from the marker forces,
+// new supports for flexible bodies, then ds and epsilon of all bodies on the grid.
+int move_bodies(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    std::vector<double> force((size_t)s->nm * 3);
+    HIP_TRY(hipMemcpyAsync(force.data(), s->d_force, force.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    const FemBeam::Solver solver = [](int n, std::vector<double> &a, std::vector<double> &x) { return lu_solve(n, a, x); };
+    for (auto &b : s->bodies) {
+        if (!b.flexible || !b.fem) continue;
+        const size_t n = b.m.size();
+        std::vector<double> eps(n), pos(3 * n), pos0(3 * n), vel(3 * n), vkm1(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            eps[i] = b.m[i].eps;
+            for (int a = 0; a < 3; ++a) {
+                pos[3 * i + a] = b.m[i].pos[a];
+                pos0[3 * i + a] = b.m[i].pos0[a];
+                vel[3 * i + a] = b.m[i].vel[a];
+                vkm1[3 * i + a] = b.m[i].vel_km1[a];
+            }
+        }
+        if (int r = b.fem->dynamic(force.data() + 3 * (size_t)b.first, eps.data(), pos.data(), pos0.data(), vel.data(),
+                                   vkm1.data(), solver))
+            return r;
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b.m[i].pos[a] = pos[3 * i + a];
+                b.m[i].vel[a] = vel[3 * i + a];
+                b.m[i].vel_km1[a] = vkm1[3 * i + a];
+            }
+    }
+    for (size_t i = 0; i < s->bodies.size(); ++i)
+        if (s->bodies[i].flexible)
+            if (int r = find_support(g, s->bodies[i], (int)i)) return r;
+    for (auto &b : s->bodies) compute_ds(g, b);
+    for (auto &b : s->bodies)
+        if (int r = find_epsilon(g, b)) return r;
+    return upload(g);
+}
+
+// ibm_checkVelDiff (:988-1032), normalised by L_UX0 dt/dh with L_UX0 = 1
+double vel_diff(const lumagpu_grid *g) {
+    double res = 0.0;
+    for (const auto &b : g->ibm->bodies) {
+        if (!b.flexible) continue;
+        for (const auto &m : b.m) {
+            double acc = 0.0;
+            for (int a = 0; a < 3; ++a) acc += std::pow(m.vel[a] - m.vel_km1[a], 2);
+            const double v = std::sqrt(acc) / (1.0 * g->d.dt / g->d.dh);
+            if (std::fabs(v) > res) res = std::fabs(v);
+        }
+    }
+    return res;
+}
+
+}  // namespace
+
+// ObjectManager::ibm_apply (:33-54) with ibm_subIterate (:111-185) for flexible bodies.  The
+// macroscopic update at the support sites (ibm_updateMacroscopic, :696-766) happens in the
+// step kernel, which recomputes u with +F/2 wherever the force bit is set; every application
+// recomputes the sweep-1 rho, u at its supports from the populations, which is LUMA's
+// u = u_n reset.
+int ibm_apply(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    if (!s || s->nm == 0) return 0;
+    if (!s->initialised) return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
+    if (int r = apply_core(g)) return r;
+    bool flexible = false;
+    for (const auto &b : s->bodies) flexible = flexible || b.flexible;
+    if (!flexible) return 0;
+    if (int r = move_bodies(g)) return r;
+    int it = 0;
+    double res = 0.0;
+    do {
+        if (int r = apply_core(g)) return r;
+        if (int r = move_bodies(g)) return r;
+        res = vel_diff(g);
+        it++;
+    } while (res > 1e-4 && it < 10);
+    s->sub_it = it;
+    s->sub_res = res;
+    for (auto &b : s->bodies)
+        if (b.flexible && b.fem) b.fem->commit();
     return 0;
 }
 
@@ -404,16 +513,59 @@ int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int f
     if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
     if (!g->d.forced) return fail(LUMAGPU_ESTATE, "IB bodies need a grid created with desc.forced = 1");
     if (nmarkers < 2 || !pos) return fail(LUMAGPU_EINVAL, "an IB body needs at least two markers");
-    if (flexible) return fail(LUMAGPU_EUNSUPPORTED, "flexible IB bodies are not supported by this entry point");
+    if (flexible) return fail(LUMAGPU_EUNSUPPORTED, "flexible bodies are built with lumagpu_ibm_add_filament");
     IbmSystem *s = system_of(g);
     Body b;
     b.flexible = flexible;
     b.m.resize(nmarkers);
     for (int i = 0; i < nmarkers; ++i)
-        for (int a = 0; a < 3; ++a) b.m[i].pos[a] = pos[3 * i + a];
+        for (int a = 0; a < 3; ++a) b.m[i].pos[a] = b.m[i].pos0[a] = pos[3 * i + a];
     s->bodies.push_back(std::move(b));
     s->initialised = 0;
     if (body) *body = (int)s->bodies.size() - 1;
+    return 0;
+}
+
+int lumagpu_ibm_add_filament(lumagpu_grid *g, const double *start, double length, double height, double depth,
+                             double angle_deg, int n_elements, int clamped, double density, double youngs,
+                             int *body) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->d.forced) return fail(LUMAGPU_ESTATE, "IB bodies need a grid created with desc.forced = 1");
+    if (g->D != 2) return fail(LUMAGPU_EUNSUPPORTED, "flexible filaments are two-dimensional");
+    if (!start || !(length > 0.0) || n_elements < 1 || !(g->d.dh > 0.0) || !(g->d.dt > 0.0))
+        return fail(LUMAGPU_EINVAL, "filament needs start, length > 0, elements >= 1 and desc.dh, desc.dt > 0");
+    (void)depth;   // 2D: one lattice spacing deep (FEMBody.cpp:103-105)
+    const double pi = 3.14159265358979323846, dh = g->d.dh;
+    // Body filament constructor (Body.h:144-187): floor(L/dh) + 1 markers
+    const int n = (int)std::floor(length / dh) + 1;
+    const double spacing = length / (n - 1);
+    const double spacing_h = spacing * std::cos(angle_deg * pi / 180);
+    std::vector<double> pos(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        pos[3 * i] = start[0] + i * spacing_h * std::cos(0.0 * pi / 180.0);
+        pos[3 * i + 1] = start[1] + i * spacing * std::sin(angle_deg * pi / 180.0);
+        pos[3 * i + 2] = start[2] + i * spacing_h * std::sin(0.0 * pi / 180.0);
+    }
+    IbmSystem *s = system_of(g);
+    Body b;
+    b.flexible = 1;
+    b.m.resize(n);
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) b.m[i].pos[a] = b.m[i].pos0[a] = pos[3 * i + a];
+    b.fem.reset(new FemBeam(start, length, height, angle_deg, n_elements, clamped != 0, density, youngs, dh,
+                            g->d.dt, g->d.dm));
+    if (b.fem->ib_nodes() != n) return fail(LUMAGPU_EINVAL, "filament IB-FEM node mapping mismatch");
+    s->bodies.push_back(std::move(b));
+    s->initialised = 0;
+    if (body) *body = (int)s->bodies.size() - 1;
+    return 0;
+}
+
+int lumagpu_ibm_subiterations(const lumagpu_grid *g, int *iterations, double *residual) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->ibm) return fail(LUMAGPU_ESTATE, "no IB bodies on this grid");
+    if (iterations) *iterations = g->ibm->sub_it;
+    if (residual) *residual = g->ibm->sub_res;
     return 0;
 }
 

@@ -27,7 +27,7 @@ def _f64(a):
 class GpuGrid:
     def __init__(self, dims, Q, N, M, K=1, *, precision="fp64", omega=1.0, regularised=True,
                  forced=False, fz_compat=True, rho_in=1.0, rho0=1.0, edges=(1.0, 1.0, 1.0),
-                 wall_thk=(0.0,) * 6, level=0, coarse_lims=(0,) * 6, dh=1.0, origin=(0.0, 0.0, 0.0)):
+                 wall_thk=(0.0,) * 6, level=0, coarse_lims=(0,) * 6, dh=1.0, origin=(0.0, 0.0, 0.0), dt=1.0, dm=1.0):
         self.lib = _abi.load()
         d = GridDesc()
         d.dims, d.Q, d.N, d.M, d.K = dims, Q, N, M, K
@@ -38,7 +38,7 @@ class GpuGrid:
         for i in range(6):
             d.wall_thk[i] = wall_thk[i]
             d.coarse_lims[i] = coarse_lims[i]
-        d.dh = dh
+        d.dh, d.dt, d.dm = dh, dt, dm
         for i in range(3):
             d.origin[i] = origin[i]
         self.desc = d
@@ -54,7 +54,8 @@ class GpuGrid:
                 regularised=case.regularised, forced=case.forced, fz_compat=case.fz_compat,
                 rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk,
                 level=case.info.get("level", 0), coarse_lims=case.info.get("coarse_lims", (0,) * 6),
-                dh=case.info.get("dh", 1.0), origin=case.info.get("origin", (0.0, 0.0, 0.0)))
+                dh=case.info.get("dh", 1.0), origin=case.info.get("origin", (0.0, 0.0, 0.0)),
+                dt=case.info.get("dt", 1.0), dm=case.info.get("dm", 1.0))
         if case.lattyp is not None:
             g.set_labels(case.lattyp, case.xpos, case.ypos, case.zpos)
         if case.ux_in is not None:
@@ -172,6 +173,19 @@ class GpuGrid:
         call("lumagpu_ibm_add_body", self.h, len(p), _dptr(p), int(flexible), C.byref(body))
         self.nmarkers.append(len(p))
         return body.value
+
+    def ibm_add_filament(self, start, length, height, angle_deg, n_elements, clamped, density, youngs, depth=0.0):
+        st = np.ascontiguousarray(start, dtype=np.float64)
+        body = C.c_int()
+        call("lumagpu_ibm_add_filament", self.h, _dptr(st), length, height, depth, angle_deg, int(n_elements),
+             int(clamped), density, youngs, C.byref(body))
+        self.nmarkers.append(int(np.floor(length / self.desc.dh)) + 1)
+        return body.value
+
+    def ibm_subiterations(self):
+        it, res = C.c_int(), C.c_double()
+        call("lumagpu_ibm_subiterations", self.h, C.byref(it), C.byref(res))
+        return it.value, res.value
 
     def ibm_initialise(self):
         call("lumagpu_ibm_initialise", self.h)

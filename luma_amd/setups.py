@@ -259,7 +259,8 @@ def add_refined_region(parent: Case, lims, dh_parent: float, dt_parent: float, p
                  xpos=xpos, ypos=ypos, zpos=zpos, regularised=parent.regularised, forced=parent.forced,
                  fz_compat=parent.fz_compat, rho_in=parent.rho_in, edges=parent.edges, wall_thk=parent.wall_thk,
                  info={"level": level, "coarse_lims": (i0, i1, j0, j1, k0, k1), "dh": dh, "dt": dt,
-                       "region_edges": edges, "origin": (edges[0], edges[2], edges[4])})
+                       "region_edges": edges, "origin": (edges[0], edges[2], edges[4]),
+                       "dm": parent.info.get("dm", 1.0) / 8.0})
     if "inlet_fn" in parent.info:
         # _LBM_initSetInletProfile evaluated on the child's positions, dt and dh
         child.ux_in, child.uy_in, child.uz_in = parent.info["inlet_fn"](ypos, dt, dh)
@@ -354,4 +355,14 @@ def channel_2d(N: int, M: int, resolution: float, umax_lbm: float, re: float, re
     return Case(name=f"channel_{N}x{M}", dims=2, Q=9, N=N, M=M, K=1, omega=omega_from_re(re, dt, dh),
                 rho=rho, u=u, lattyp=lt, xpos=xpos, ypos=ypos, zpos=zpos, ux_in=ux_in, uy_in=uy_in,
                 uz_in=uz_in, regularised=regularised, forced=True, edges=edges, wall_thk=thk,
-                info={"re": re, "dh": dh, "dt": dt, "inlet_fn": inlet_fn, "origin": (0.0, 0.0, 0.0)})
+                info={"re": re, "dh": dh, "dt": dt, "inlet_fn": inlet_fn, "origin": (0.0, 0.0, 0.0),
+                      "dm": (1000.0 / 1.0) * dh * dh * dh})
+
+
+def filament_material(length: float, height: float, dh: float, mass_ratio: float = 10.0,
+                      bending: float = 1e-3, rho_f: float = 1000.0, u_ref: float = 1.0):
+    """Density and Young's modulus of a 2D filament for LUMA's FEM from dimensionless groups:
+    density = mass_ratio * rho_f (L_PHYSICAL_RHO), EI = bending * rho_f u_ref^2 L^3 with the
+    element's I = depth h^3 / 12 and depth = dh (FEMElement.cpp:79, FEMBody.cpp:103-105)."""
+    I = dh * height ** 3 / 12.0
+    return mass_ratio * rho_f, bending * rho_f * u_ref ** 2 * length ** 3 / I

@@ -163,3 +163,33 @@ def test_ib_body_force_shifted_forms_track_fp64(prec, tol):
     err = np.abs(res[prec] - res["fp64"]).max() / np.abs(res["fp64"]).max()
     print(prec, "body force", res[prec], "fp64", res["fp64"], "rel err", err)
     assert err < tol
+
+
+def test_flexible_filament_fsi_fp64_bitwise():
+    # clamped FEM filament in a channel: FEM Newton solve, marker moves and FSI sub-iterations
+    c = setups.channel_2d(64, 32, 8.0, 0.04, 100.0)
+    dh, dt, dm = c.info["dh"], c.info["dt"], c.info["dm"]
+    length, height = 1.0, 0.05
+    density, E = setups.filament_material(length, height, dh, mass_ratio=10.0, bending=1e-2)
+    start = (1.5, 2.0 + 0.013, 0.0)
+    print(f"[ibm] filament {c.N}x{c.M}, density {density}, E {E:.4g}", flush=True)
+    og = oracle_grid(c)
+    ib = OracleIBM(og, dh)
+    ib.add_filament(start, length, height, dh, 8.0, 8, True, density, E, dt, dm)
+    ib.initialise()
+    with gpu_grid(c) as gg:
+        gg.ibm_add_filament(start, length, height, 8.0, 8, True, density, E)
+        gg.ibm_initialise()
+        its = []
+        for _ in range(30):
+            og.step()
+            gg.LBM_multi_opt()
+            its.append(gg.ibm_subiterations()[0])
+            assert gg.ibm_subiterations()[0] == ib.subiterations()[0]
+        assert ib.error() == 0
+        a, b = gg.ibm_markers(0), ib.markers(0)
+        for key in ("pos", "vel", "force", "eps", "ds"):
+            assert np.array_equal(a[key], b[key]), (key, np.abs(a[key] - b[key]).max())
+        assert np.abs(a["pos"] - setups.filament_markers(start, length, 8.0, dh)).max() > 0.0
+        compare_grids(c, gg, og, "filament")
+        print("sub-iterations per step", its)
