@@ -270,6 +270,55 @@ __device__ __forceinline__ void regularised_face(T *fa, int ncode, const T *vel_
     }
 }
 
+// _LBM_regularised_opt for a velocity site on an edge or corner (edgeCount > 1), :346-364 and
+// :432-508: rho is the first-order extrapolation from the two inward neighbours along the
+// combined normal n (GridUtils::extrapolate, GridUtils.h:231-280, done by the caller); every
+// population sharing a normal component with n is unknown -- buried links (c.n = 0, |c| > 1)
+// take feq, the others non-equilibrium bounce-back -- updated in place in v order, so a later
+// v may read an opposite that an earlier v already replaced, as LUMA's loop does.
+template <int Q, class T>
+__device__ __forceinline__ void regularised_edge(T *fa, const int *n, const T *vel_in, T dens,
+                                                 T cs2, T den2, T &rho, T *u) {
+    using L = Lattice<Q>;
+    rho = dens;
+    u[0] = vel_in[0];
+    u[1] = vel_in[1];
+    u[2] = (L::D == 3) ? vel_in[2] : T(0);
+    T Sxx = 0, Syy = 0, Sxy = 0, Szz = 0, Sxz = 0, Syz = 0;
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int c0 = L::c(v, 0), c1 = L::c(v, 1), c2 = L::c(v, 2);
+        const int o = L::opp(v);
+        const bool unknown = c0 == n[0] || c1 == n[1] || (L::D == 3 && c2 == n[2]);
+        if (unknown) {
+            const int dp = c0 * n[0] + c1 * n[1] + (L::D == 3 ? c2 * n[2] : 0);
+            const int mag2 = c0 * c0 + c1 * c1 + (L::D == 3 ? c2 * c2 : 0);
+            if (dp == 0 && mag2 > 1)
+                fa[v] = feq_exact<Q, T>(v, rho, u, cs2, den2);
+            else
+                fa[v] = feq_exact<Q, T>(v, rho, u, cs2, den2) + (fa[o] - feq_exact<Q, T>(o, rho, u, cs2, den2));
+        }
+        const T fneq = fa[v] - feq_exact<Q, T>(v, rho, u, cs2, den2);
+        Sxx += T(c0 * c0) * fneq;
+        Syy += T(c1 * c1) * fneq;
+        Sxy += T(c0 * c1) * fneq;
+        if constexpr (L::D == 3) {
+            Szz += T(c2 * c2) * fneq;
+            Sxz += T(c0 * c2) * fneq;
+            Syz += T(c1 * c2) * fneq;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int c0 = L::c(v, 0), c1 = L::c(v, 1), c2 = L::c(v, 2);
+        fa[v] = feq_exact<Q, T>(v, rho, u, cs2, den2) +
+                (T(L::w(v)) / den2) *
+                    (((T(c0 * c0) - cs2) * Sxx) + ((T(c1 * c1) - cs2) * Syy) +
+                     ((T(c2 * c2) - cs2) * Szz) + (T(2.0) * T(c0) * T(c1) * Sxy) +
+                     (T(2.0) * T(c0) * T(c2) * Sxz) + (T(2.0) * T(c1) * T(c2) * Syz));
+    }
+}
+
 // ----------------------------------------------------------------------------------------
 // shifted (deviation) per-site arithmetic, FP32 compute
 
@@ -554,6 +603,8 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     const int ty = fl & kTypeMask;
     // sites LUMA's sweep 1 skips (:91-95); outlet sites are not read by any other site
     if (ty == kSolid || ty == kRefined || ty == kExtrapRight || (!a.regularised && ty == kVelocity)) return;
+    // edge and corner velocity sites run in k_edge_sites after this kernel
+    if (ty == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge) return;
     S *__restrict__ f = static_cast<S *>(a.f);
     const size_t ps = a.ps;
     Nbr nb;
@@ -633,7 +684,7 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
     const bool macro_type = ty == kFluid || ty == kTL2F;
-    const bool vel_type = a.regularised && ty == kVelocity;
+    const bool vel_type = a.regularised && ty == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
     if (!macro_type && !vel_type) return;
     Nbr nb;
     nb.init(x, y, z, a.nx, a.ny, a.nz);
@@ -652,6 +703,100 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
         site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);
     }
     store_macro<L::D>(a, cell, rho, u);
+}
+
+// Regularised velocity sites on edges and corners.  Each entry e holds the site's cell, its
+// packed inward normal (n_a + 1 in bits 2a..2a+1) and the cells of the two inward neighbours
+// x + n and x + 2n, which are fluid or TL2F sites.
+struct EdgeArgs {
+    int n;
+    const int *cell, *nrm, *n1, *n2;
+    double *erho;   // [2n]: sweep-1 rho of the two neighbours
+};
+
+// Sweep-1 density of the extrapolation neighbours, read from the layout the step kernel is
+// about to consume.  It equals LUMA's rho[id1], rho[id2] whether the neighbour was visited
+// before the edge site or is streamed and macro-updated on the fly (:1376-1404): both give
+// the neighbour's own stream + macro of this step.
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(256) void k_edge_rho(StepArgs a, EdgeArgs e) {
+    using T = typename P::T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * e.n) return;
+    const size_t cell = (size_t)((i & 1) ? e.n2[i >> 1] : e.n1[i >> 1]);
+    const int x = (int)(cell % a.nx), y = (int)((cell / a.nx) % a.ny), z = (int)(cell / ((size_t)a.nx * a.ny));
+    const int ty = a.flags[cell] & kTypeMask;
+    Nbr nb;
+    nb.init(x, y, z, a.nx, a.ny, a.nz);
+    T fin[Q], fout[Q];
+    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
+    double rho, u[3];
+    site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);
+    e.erho[i] = rho;
+}
+
+// Edge/corner velocity sites: stream, regularise with the extrapolated density
+// rho = 2 rho(x+n) - rho(x+2n), Guo force and BGK collision, then the same in-place AA store
+// as the step kernel.  A site's AA slots are its own, so running after the step kernel reads
+// and writes exactly what running inside it would.
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int store) {
+    using L = Lattice<Q>;
+    using S = typename P::S;
+    using T = typename P::T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e.n) return;
+    const size_t cell = (size_t)e.cell[i];
+    const int nx = a.nx, ny = a.ny;
+    const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / ((size_t)nx * ny));
+    const unsigned fl = a.flags[cell];
+    const int pk = e.nrm[i];
+    const int n[3] = {(pk & 3) - 1, ((pk >> 2) & 3) - 1, ((pk >> 4) & 3) - 1};
+    S *__restrict__ f = static_cast<S *>(a.f);
+    const size_t ps = a.ps;
+    Nbr nb;
+    nb.init(x, y, z, nx, ny, a.nz);
+    T fin[Q], fout[Q];
+    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kVelocity, fin);
+
+    const double *F = nullptr;
+    double Fv[3];
+    if (fl & kForcedBit) {
+        Fv[0] = a.force[cell];
+        Fv[1] = a.force[a.ncell + cell];
+        Fv[2] = (L::D == 3) ? a.force[2 * a.ncell + cell] : 0.0;
+        F = Fv;
+    }
+    const T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
+    const T dens = T(2.0) * T(e.erho[2 * i]) - T(e.erho[2 * i + 1]);
+    T fa[Q], rr, uu[3];
+#pragma unroll
+    for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
+    regularised_edge<Q, T>(fa, n, vin, dens, T(a.cs2), T(a.den2), rr, uu);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) fin[v] = P::kShift ? fa[v] - T(L::w(v) * a.rho0) : fa[v];
+    if constexpr (!P::kShift) {
+        collide_exact<Q, T>(fin, rr, uu, F, T(a.omega), T(a.cs2), T(a.den2), fout);
+    } else {
+        collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
+    }
+    if (store) {
+        const double u[3] = {double(uu[0]), double(uu[1]), double(uu[2])};
+        store_macro<L::D>(a, cell, double(rr), u);
+    }
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+        if constexpr (ODD) {
+            const size_t dst = nb.at(cx, cy, cz, nx, ny);
+            if ((a.flags[dst] & kTypeMask) == kSolid)
+                st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
+            else
+                st<P>(f + (size_t)v * ps + dst, fout[v]);
+        } else {
+            st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
+        }
+    });
 }
 
 // Side buffer for ExtrapolateRight sources (:246-251): xr[v][y, z] = f*_v(N-3, y, z), taken

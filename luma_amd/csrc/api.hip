@@ -65,19 +65,40 @@ int check_grid(const lumagpu_grid *g) {
 
 // GridUtils::isWithinDomainWall, LUMA/src/GridUtils.cpp:1369-1430: returns edge count and
 // the code of the last inward normal found (1..6 = +x,-x,+y,-y,+z,-z).
-int wall_normal(const lumagpu_grid_desc &d, double px, double py, double pz, int *code) {
+// GridUtils::isWithinDomainWall (GridUtils.cpp:1369-1430): the number of walls the position
+// lies in, the code of the last one (1..6 = +x,-x,+y,-y,+z,-z inward normal) and the combined
+// inward vector n.
+int wall_normal(const lumagpu_grid_desc &d, double px, double py, double pz, int *code, int n[3]) {
     const double *th = d.wall_thk;
     int cnt = 0;
     *code = 0;
-    if (px > 0.0 && px < th[0]) { *code = 1; cnt++; }
-    if (px < d.xmax && px > d.xmax - th[1]) { *code = 2; cnt++; }
-    if (py > 0.0 && py < th[2]) { *code = 3; cnt++; }
-    if (py < d.ymax && py > d.ymax - th[3]) { *code = 4; cnt++; }
+    n[0] = n[1] = n[2] = 0;
+    if (px > 0.0 && px < th[0]) { *code = 1; n[0] = 1; cnt++; }
+    if (px < d.xmax && px > d.xmax - th[1]) { *code = 2; n[0] = -1; cnt++; }
+    if (py > 0.0 && py < th[2]) { *code = 3; n[1] = 1; cnt++; }
+    if (py < d.ymax && py > d.ymax - th[3]) { *code = 4; n[1] = -1; cnt++; }
     if (d.dims == 3) {
-        if (pz > 0.0 && pz < th[4]) { *code = 5; cnt++; }
-        if (pz < d.zmax && pz > d.zmax - th[5]) { *code = 6; cnt++; }
+        if (pz > 0.0 && pz < th[4]) { *code = 5; n[2] = 1; cnt++; }
+        if (pz < d.zmax && pz > d.zmax - th[5]) { *code = 6; n[2] = -1; cnt++; }
     }
     return cnt;
+}
+
+hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
+    EdgeArgs e{};
+    e.n = g->nedge;
+    e.cell = g->edge;
+    e.nrm = g->edge + g->nedge;
+    e.n1 = g->edge + 2 * g->nedge;
+    e.n2 = g->edge + 3 * g->nedge;
+    e.erho = g->erho;
+    const int store = (g->store_macro && a.rho_out) ? 1 : 0;
+    const bool odd = g->parity == 0;
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_edges_f64(g->Q, a, e, odd, phase, store, g->stream);
+        case LUMAGPU_FP32S: return launch_edges_f32s(g->Q, a, e, odd, phase, store, g->stream);
+        default: return launch_edges_f16s(g->Q, a, e, odd, phase, store, g->stream);
+    }
 }
 
 hipError_t launch_extrap(const lumagpu_grid *g) {
@@ -122,10 +143,18 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0};
     StepArgs a = make_args(g);
     a.subcycle = subcycle;
+    if (g->nedge) {
+        hipError_t e = launch_edges(g, a, 0);
+        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("edge launch: ") + hipGetErrorString(e));
+    }
     if (e0) HIP_TRY(hipEventRecord(e0, g->stream));
     hipError_t e = launch_step(g, ls, a);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("step launch: ") + hipGetErrorString(e));
     if (e1) HIP_TRY(hipEventRecord(e1, g->stream));
+    if (g->nedge) {
+        e = launch_edges(g, a, 1);
+        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("edge launch: ") + hipGetErrorString(e));
+    }
     g->parity ^= 1;
     g->t += 1;
     return 0;
@@ -201,6 +230,8 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->stage);
     (void)hipFree(g->moments_out);
     (void)hipFree(g->xr);
+    (void)hipFree(g->edge);
+    (void)hipFree(g->erho);
     ibm_destroy(g);
     if (g->parent && g->parent->child == g) g->parent->child = nullptr;
     if (g->child && g->child->parent == g) g->child->parent = nullptr;
@@ -224,6 +255,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     if (prev.size() != g->ncell) prev.assign(g->ncell, 0);
     g->hflags.assign(g->ncell, 0);
     int any = 0, outlet = 0;
+    std::vector<int> ecell, enrm;   // edge/corner velocity sites (device cell, packed normal)
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < M; ++j)
             for (int k = 0; k < K; ++k) {
@@ -238,14 +270,16 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                         if (g->d.regularised) {
                             if (!xpos || !ypos || !zpos)
                                 return fail(LUMAGPU_EINVAL, "positions are required for regularised velocity sites");
-                            int code = 0;
-                            const int cnt = wall_normal(g->d, xpos[i], ypos[j], zpos[k], &code);
+                            int code = 0, n[3];
+                            const int cnt = wall_normal(g->d, xpos[i], ypos[j], zpos[k], &code, n);
                             if (cnt == 0)
                                 return fail(LUMAGPU_EINVAL,
                                             "Velocity site not outside domain walls is currently not supported.");
-                            if (cnt > 1)
-                                return fail(LUMAGPU_EUNSUPPORTED,
-                                            "regularised velocity site on an edge or corner is not supported");
+                            if (cnt > 1) {
+                                code = kNormalEdge;
+                                ecell.push_back(i + N * (j + M * k));
+                                enrm.push_back((n[0] + 1) | ((n[1] + 1) << 2) | ((n[2] + 1) << 4));
+                            }
                             fl |= (uint8_t)(code << kNormalShift);
                         }
                         break;
@@ -269,6 +303,43 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                 const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
                 g->hflags[cell] = (uint8_t)(fl | (prev[cell] & kForcedBit));
             }
+    // Edge/corner sites extrapolate rho from x + n and x + 2n (_LBM_updateAndExtrapolate,
+    // :1353-1412).  Those neighbours must be sites whose sweep 1 computes rho (fluid, TL2F);
+    // for any other type LUMA reads a value left from an earlier step or its initial state.
+    const int ne = (int)ecell.size();
+    std::vector<int> etab(4 * (size_t)ne);
+    for (int e = 0; e < ne; ++e) {
+        const int c = ecell[e];
+        const int i = c % N, j = (c / N) % M, k = c / (N * M);
+        const int n[3] = {(enrm[e] & 3) - 1, ((enrm[e] >> 2) & 3) - 1, ((enrm[e] >> 4) & 3) - 1};
+        int nb[2];
+        for (int s = 1; s <= 2; ++s) {
+            const int ii = i + s * n[0], jj = j + s * n[1], kk = k + s * n[2];
+            if (ii < 0 || ii >= N || jj < 0 || jj >= M || kk < 0 || kk >= K)
+                return fail(LUMAGPU_EINVAL, "Extrapolation does not have enough available cells.");
+            nb[s - 1] = ii + N * (jj + M * kk);
+            const int nty = g->hflags[nb[s - 1]] & kTypeMask;
+            if (nty != kFluid && nty != kTL2F)
+                return fail(LUMAGPU_EUNSUPPORTED,
+                            "edge/corner velocity site whose extrapolation neighbours are not fluid is not supported");
+        }
+        etab[e] = c;
+        etab[ne + e] = enrm[e];
+        etab[2 * ne + e] = nb[0];
+        etab[3 * ne + e] = nb[1];
+    }
+    (void)hipFree(g->edge);
+    (void)hipFree(g->erho);
+    g->edge = nullptr;
+    g->erho = nullptr;
+    g->nedge = 0;
+    if (ne) {
+        if (hipMalloc(&g->edge, etab.size() * sizeof(int)) != hipSuccess ||
+            hipMalloc(&g->erho, 2 * (size_t)ne * sizeof(double)) != hipSuccess)
+            return fail(LUMAGPU_ENOMEM, "edge site table allocation failed");
+        HIP_TRY(hipMemcpyAsync(g->edge, etab.data(), etab.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
+        g->nedge = ne;
+    }
     if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
     HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));

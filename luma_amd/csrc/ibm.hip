@@ -302,6 +302,9 @@ int upload(lumagpu_grid *g) {
                 scell[(size_t)gm * S + k] = cell;
                 sdelta[(size_t)gm * S + k] = m.delta[k];
                 if (site_of[cell] < 0) {
+                    const unsigned fl = g->hflags.empty() ? 0u : g->hflags[cell];
+                    if ((fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)
+                        return fail(LUMAGPU_EUNSUPPORTED, "IB support on an edge/corner velocity site is not supported");
                     site_of[cell] = (int)order.size();
                     order.push_back(cell);
                     entries.emplace_back();

@@ -32,6 +32,9 @@ struct lumagpu_grid {
     lumagpu_grid *child = nullptr;    // GridObj::subGrid[0] (non-owning; one region per level)
     double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
     int has_outlet = 0;
+    int nedge = 0;                    // regularised velocity sites on edges and corners
+    int *edge = nullptr;              // [4*nedge]: cells, packed normals, neighbour x+n, x+2n
+    double *erho = nullptr;           // [2*nedge] neighbour densities of the current step
     int masked = 0;
     int has_state = 0;
     int store_macro = 0;

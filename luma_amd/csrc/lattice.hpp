@@ -75,5 +75,8 @@ enum SiteType : int {
 constexpr unsigned kTypeMask = 0x0F;
 constexpr unsigned kForcedBit = 0x10;
 constexpr int kNormalShift = 5;
+// wall-normal code 7: a regularised velocity site on an edge or corner, updated by its own
+// kernel after the step kernel (the density comes from two inward neighbours)
+constexpr int kNormalEdge = 7;
 
 }  // namespace lumagpu

@@ -48,6 +48,30 @@ hipError_t launch_sites_p(int Q, const StepArgs &a, bool odd, int n, const int *
     return hipGetLastError();
 }
 
+// Edge/corner regularised velocity sites: phase 0 = neighbour densities (before the step
+// kernel), phase 1 = the sites themselves (after it)
+hipError_t launch_edges_f64(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
+hipError_t launch_edges_f32s(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
+hipError_t launch_edges_f16s(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
+
+template <int Q, class P, bool ODD>
+void launch_edges_qo(const StepArgs &a, const EdgeArgs &e, int phase, int store, hipStream_t s) {
+    if (phase == 0) k_edge_rho<Q, P, ODD><<<(2 * e.n + 127) / 128, 128, 0, s>>>(a, e);
+    else k_edge_sites<Q, P, ODD><<<(e.n + 127) / 128, 128, 0, s>>>(a, e, store);
+}
+template <class P>
+hipError_t launch_edges_p(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s) {
+    if (e.n <= 0) return hipSuccess;
+    if (Q == 9) {
+        if (odd) launch_edges_qo<9, P, true>(a, e, phase, store, s);
+        else launch_edges_qo<9, P, false>(a, e, phase, store, s);
+    } else {
+        if (odd) launch_edges_qo<19, P, true>(a, e, phase, store, s);
+        else launch_edges_qo<19, P, false>(a, e, phase, store, s);
+    }
+    return hipGetLastError();
+}
+
 // aux ops
 enum AuxOp : int {
     kAuxInitFeq = 0,     // p0 = rho (LUMA id), p1 = u (LUMA id)

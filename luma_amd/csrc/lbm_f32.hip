@@ -24,5 +24,11 @@ hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const in
 hipError_t launch_sites_f16s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
     return launch_sites_p<P16S>(Q, a, odd, n, cells, s);
 }
+hipError_t launch_edges_f32s(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s) {
+    return launch_edges_p<P32S>(Q, a, e, odd, phase, store, s);
+}
+hipError_t launch_edges_f16s(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s) {
+    return launch_edges_p<P16S>(Q, a, e, odd, phase, store, s);
+}
 
 }  // namespace lumagpu

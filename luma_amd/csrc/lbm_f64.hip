@@ -17,4 +17,8 @@ hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int
     return launch_sites_p<P64>(Q, a, odd, n, cells, s);
 }
 
+hipError_t launch_edges_f64(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s) {
+    return launch_edges_p<P64>(Q, a, e, odd, phase, store, s);
+}
+
 }  // namespace lumagpu

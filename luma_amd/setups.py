@@ -175,6 +175,34 @@ def lid_driven_cavity(n: int = 256, re: float = 1280.0, u_lid_lbm: float = 0.1,
                 info={"re": re, "dh": dh, "dt": dt, "u_lid": u_lbm, "inlet_fn": inlet_fn})
 
 
+def velocity_corner_box(N: int, M: int, K: int = 1, dims: int = 2, u_in: float = 0.02, u_lid: float = 0.05,
+                        tau: float = 0.6) -> Case:
+    """Box whose left, right and top walls (and front wall in 3D) are regularised velocity
+    walls, so velocity walls meet at edges and corners (GridObj_ops_lbm_optimised.cpp:346-364);
+    bottom (and back) walls are solid, which wins at the corners it shares (:1374).
+    ux_in[j] = u_in below the lid row and u_lid on it; one-cell walls, cell-centred positions."""
+    Q = 9 if dims == 2 else 19
+    dh = 1.0 / M
+    edges = (N * dh, M * dh, K * dh if dims == 3 else 0.0)
+    thk = (dh,) * 6
+    xpos = linspace(dh / 2.0, edges[0] - dh / 2.0, N)
+    ypos = linspace(dh / 2.0, edges[1] - dh / 2.0, M)
+    zpos = linspace(dh / 2.0, edges[2] - dh / 2.0, K) if dims == 3 else np.array([0.0])
+    walls = dict(left=VELOCITY, right=VELOCITY, bottom=SOLID, top=VELOCITY, front=VELOCITY, back=SOLID)
+    lt = label_walls(N, M, K, dims, xpos, ypos, zpos, edges, thk, walls)
+    ux_in = np.full(M, u_in)
+    ux_in[M - 1] = u_lid
+    uy_in = np.zeros(M)
+    uz_in = np.zeros(M)
+    u = np.zeros((N, M, K, dims))
+    vel = lt == VELOCITY
+    u[..., 0][vel] = np.broadcast_to(ux_in[None, :, None], (N, M, K))[vel]
+    rho = np.ones((N, M, K))
+    return Case(name=f"velocity_corners{dims}d_{N}x{M}x{K}", dims=dims, Q=Q, N=N, M=M, K=K, omega=1.0 / tau,
+                rho=rho, u=u, lattyp=lt, xpos=xpos, ypos=ypos, zpos=zpos, ux_in=ux_in, uy_in=uy_in,
+                uz_in=uz_in, regularised=True, edges=edges, wall_thk=thk, info={"dh": dh})
+
+
 def body_force_channel(N: int, M: int, K: int = 1, dims: int = 2, tau: float = 0.8,
                        g: float = 1e-6) -> Case:
     """Channel with solid walls at j = 0 and j = M-1, periodic in x (and z), driven by a

@@ -75,6 +75,20 @@ def test_cavity_forced_equilibrium_fp64_bitwise():
     assert_bitwise(c, 300)
 
 
+def test_regularised_edges_and_corners_2d_fp64_bitwise():
+    # velocity walls meet at two corners: rho extrapolated from the two inward neighbours
+    c = setups.velocity_corner_box(20, 16, dims=2)
+    assert (c.lattyp == setups.VELOCITY).sum() > 0
+    assert_bitwise(c, 150)
+
+
+def test_regularised_edges_and_corners_3d_fp64_bitwise():
+    # edges with one zero normal component (whose whole c_z = 0 plane is "unknown") and
+    # corners where three velocity walls meet
+    c = setups.velocity_corner_box(14, 12, 10, dims=3)
+    assert_bitwise(c, 80)
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_body_force_channel_fp64_bitwise(dims):
     c = setups.body_force_channel(9, 12, 5 if dims == 3 else 1, dims=dims, tau=0.7, g=2e-5)
@@ -143,6 +157,14 @@ def test_fp16_shifted_tgv3d_velocity_error():
 def test_fp32_shifted_cavity_velocity_error():
     err = _u_error(setups.lid_driven_cavity(48, re=1280.0 * 48 / 256), "fp32s", 400)
     print("fp32s cavity rel L2 u error", err)
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_fp32_shifted_edges_and_corners_velocity_error(dims):
+    c = setups.velocity_corner_box(20, 16, dims=2) if dims == 2 else setups.velocity_corner_box(14, 12, 10, dims=3)
+    err = _u_error(c, "fp32s", 150)
+    print(f"fp32s velocity-corner box {dims}D rel L2 u error", err)
     assert err < 1e-4
 
 
