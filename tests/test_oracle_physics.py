@@ -149,3 +149,71 @@ def test_regularised_couette_is_linear():
     ref = U * (j - 0.5) / (M - 1 - 0.5)
     assert np.allclose(u[2, 1:, 0, 0], ref, rtol=1e-8, atol=1e-12)
     assert np.allclose(rho[:, 1:], 1.0, atol=1e-10)
+
+
+def _corner_oracle(case):
+    g = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega, regularised=True,
+                   edges=case.edges, wall_thk=case.wall_thk)
+    g.set_labels(case.lattyp)
+    g.set_positions(case.xpos, case.ypos, case.zpos)
+    g.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+    return g
+
+
+def _edge_sites(case):
+    """Velocity sites inside two or more one-cell walls, with their inward vector."""
+    out = []
+    N, M, K = case.N, case.M, case.K
+    for i, j, k in zip(*np.nonzero(case.lattyp == setups.VELOCITY)):
+        n = [int(i == 0) - int(i == N - 1), int(j == 0) - int(j == M - 1),
+             (int(k == 0) - int(k == K - 1)) if case.dims == 3 else 0]
+        if sum(a != 0 for a in n) > 1:
+            out.append(((i, j, k), n))
+    return out
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_regularised_edges_extrapolate_density_and_keep_moments(dims):
+    # GridObj_ops_lbm_optimised.cpp:346-364 with GridUtils::extrapolate (GridUtils.h:231-280):
+    # rho(x) = 2 rho(x+n) - rho(x+2n) along the combined inward normal n; after the
+    # regularised reconstruction and BGK (mass and momentum conserving) the site's populations
+    # carry exactly rho(x) and rho(x) u_in.
+    case = (setups.velocity_corner_box(12, 10, dims=2) if dims == 2
+            else setups.velocity_corner_box(9, 8, 7, dims=3))
+    rng = np.random.default_rng(3)
+    g = _corner_oracle(case)
+    rho0 = 1 + 0.01 * rng.standard_normal((case.N, case.M, case.K))
+    u0 = 0.02 * rng.standard_normal((case.N, case.M, case.K, dims))
+    g.init_feq(rho0, u0)
+    g.run(3)
+    f = g.get_f()
+    rho, u = g.get_macro()
+    c, _, _ = lattice(case.Q)
+    sites = _edge_sites(case)
+    assert len(sites) >= (2 if dims == 2 else 8)
+    for (i, j, k), n in sites:
+        i1, j1, k1 = i + n[0], j + n[1], k + n[2]
+        i2, j2, k2 = i1 + n[0], j1 + n[1], k1 + n[2]
+        assert rho[i, j, k] == 2.0 * rho[i1, j1, k1] - rho[i2, j2, k2]
+        uin = np.array([case.ux_in[j], case.uy_in[j], case.uz_in[j]])[:dims]
+        assert np.array_equal(u[i, j, k], uin)
+        assert abs(f[i, j, k].sum() - rho[i, j, k]) < 1e-14
+        mom = f[i, j, k] @ c[:, :dims].astype(float)
+        assert np.allclose(mom, rho[i, j, k] * uin, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_uniform_flow_is_stationary_through_velocity_edges(dims):
+    # feq(rho0, U) everywhere with U imposed on every velocity wall is a fixed point of the
+    # stream / regularise / collide step, edges and corners included
+    case = (setups.velocity_corner_box(12, 10, dims=2) if dims == 2
+            else setups.velocity_corner_box(9, 8, 7, dims=3))
+    case.lattyp[case.lattyp == setups.SOLID] = setups.VELOCITY
+    U = np.array([0.03, -0.01, 0.02])[:dims]
+    case.ux_in[:], case.uy_in[:] = U[0], U[1]
+    case.uz_in[:] = U[2] if dims == 3 else 0.0
+    g = _corner_oracle(case)
+    g.init_feq(np.ones((case.N, case.M, case.K)), np.broadcast_to(U, (case.N, case.M, case.K, dims)).copy())
+    f0 = g.get_f()
+    g.run(5)
+    assert np.abs(g.get_f() - f0).max() < 1e-15
