@@ -65,6 +65,35 @@ def time_form(case, form, steps, warmup):
     }
 
 
+def slab_share(n, nz, steps, warmup, form="fp32s"):
+    """One GPU's share of an 8-way z-sharded n^3 box: an n x n x nz slab plus two halo planes,
+    the five face-crossing populations per face exchanged around every odd step (locally, the
+    slab being its own neighbour), against the same slab stepped as a periodic box."""
+    import dataclasses
+
+    import numpy as np
+
+    from luma_amd import setups
+    from luma_amd.grid import GpuGrid
+
+    box = setups.taylor_green((n, n, nz), u0=0.05, tau=0.55, dims=3)
+    pad = lambda a: np.concatenate([a[:, :, -1:], a, a[:, :, :1]], axis=2)
+    slab = dataclasses.replace(box, K=nz + 2, rho=pad(box.rho), u=pad(box.u))
+    out = {"shape": [n, n, nz], "form": form}
+    for key, case, zs in (("periodic", box, False), ("slab", slab, True)):
+        with GpuGrid.from_case(case, form) as g:
+            if zs:
+                g.set_zslab(True)
+            g.run(warmup)
+            g.sync()
+            tot_ms, ker_ms = g.run_timed(steps)
+        out[key] = {"step_us": 1e3 * tot_ms / steps, "kernel_us": 1e3 * ker_ms}
+    out["halo_us_per_step"] = out["slab"]["step_us"] - out["slab"]["kernel_us"]
+    out["face_bytes_per_step"] = 5 * n * n * ELEM[form]
+    out["mlups"] = n * n * nz / out["slab"]["step_us"]
+    return out
+
+
 def error_run(case, nsteps):
     """Run every form nsteps from the same state; u error and KE of shifted forms vs FP64."""
     import numpy as np
@@ -125,6 +154,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="cube edge (cells)")
     ap.add_argument("--error-steps", type=int, default=5000, help="0 skips the error run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--slab-steps", type=int, default=200, help="0 skips the per-GPU slab measurement")
     ap.add_argument("--forms", default="fp64,fp32s,fp16s")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -147,6 +177,11 @@ def main():
         log(f"[bench] error run: {args.error_steps} steps per form")
         err = error_run(case, args.error_steps)
         log(f"[bench] errors: {json.dumps(err)}")
+    slab = None
+    if args.slab_steps > 0:
+        log(f"[bench] one GPU's share of an 8-way z-sharded {args.n}^3 box")
+        slab = slab_share(args.n, args.n // 8, args.slab_steps, args.warmup)
+        log(f"[bench] slab: {json.dumps(slab)}")
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
@@ -186,6 +221,8 @@ def main():
         line["fp32s_over_fp64"] = round(res["fp32s"]["mlups"] / res["fp64"]["mlups"], 3)
     if err is not None:
         line["error"] = {"steps": args.error_steps, **err}
+    if slab is not None:
+        line["slab_share"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in slab.items()}
     print(json.dumps(line), flush=True)
 
 

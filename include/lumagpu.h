@@ -150,6 +150,15 @@ int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]);
 /* XPos/YPos/ZPos of a grid that has no labels to set */
 int lumagpu_set_positions(lumagpu_grid *g, const double *xpos, const double *ypos, const double *zpos);
 
+/* ---- one GPU's share of a sharded box ---------------------------------------------------
+ * MpiManager's Cartesian decomposition (MpiManager.cpp:103-226) with its exchange after every
+ * grid step (GridObj_ops_lbm_optimised.cpp:186-191), for a z-sharded D3Q19 periodic box: with
+ * on = 1 the grid's first and last z planes become halo planes, the step updates planes
+ * 1..K-2 only, and around every odd (neighbour-access) step the five populations that cross
+ * each face are exchanged -- here with the grid itself as both neighbours, the copy a rank
+ * would replace with RCCL send/receive of the same contiguous planes. */
+int lumagpu_set_zslab(lumagpu_grid *g, int on);
+
 /* ---- diagnostics --------------------------------------------------------------------- */
 const char *lumagpu_last_error(void);   /* replaces L_ERROR's log line (stdafx.h:119-149) */
 int lumagpu_abi_version(void);

@@ -62,6 +62,7 @@ _PROTOS = {
     "lumagpu_run": [_vp, C.c_int],
     "lumagpu_run_timed": [_vp, C.c_int, _dp, _dp],
     "lumagpu_set_store_macro": [_vp, C.c_int],
+    "lumagpu_set_zslab": [_vp, C.c_int],
     "lumagpu_get_time": [_vp, _ip],
     "lumagpu_sync": [_vp],
     "lumagpu_moments": [_vp, _dp],

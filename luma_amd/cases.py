@@ -1,0 +1,182 @@
+"""The reference's flow configurations at full size, driven through the C ABI.
+
+Each builder returns the LUMA-layout set-up (labels, inlet, initial state, refined region,
+markers) as LUMA's initialisation would produce it; each runner steps it with
+``LBM_multi_opt`` and reports what the configuration is judged on (fields, drag, lift,
+Strouhal number, throughput).  Command line::
+
+    python -m luma_amd.cases c1 --precision fp64 --steps 20000
+    python -m luma_amd.cases c3 --precision fp64,fp32s --steps 100000 --out c3.json
+
+Lattice units throughout.  Drag and lift coefficients use the mean inlet velocity
+U0 = Umax / 1.5 (LUMA's L_UX0, on which L_RE is based) and the diameter in cells of the level
+carrying the body:  C = 2 F / (rho0 U0^2 D).  The body force is the FP64 sum LUMA writes to
+Body_<id>_LD (ObjectManager_ops_io.cpp:98-121).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import setups
+from .grid import GpuGrid
+
+
+# ---- C1: D2Q9 lid-driven cavity 256^2, tau 0.56, lid 0.1, HWBB walls --------------------
+
+def c1_cavity(n: int = 256) -> setups.Case:
+    """L_RESOLUTION 256, L_RE 1280, lid u = 0.1: tau = 0.56 (GridObj_init_grids.cpp:339-344)."""
+    return setups.lid_driven_cavity(n, re=1280.0 * n / 256, u_lid_lbm=0.1)
+
+
+def run_c1(precision: str, steps: int, n: int = 256):
+    case = c1_cavity(n)
+    with GpuGrid.from_case(case, precision, store_macro=True) as g:
+        t0 = time.perf_counter()
+        g.run(steps)
+        g.sync()
+        wall = time.perf_counter() - t0
+        rho, u = g.download_macro()
+    return {"config": "c1_cavity", "precision": precision, "steps": steps, "N": n,
+            "wall_s": wall, "mlups": case.N * case.M * steps / wall / 1e6}, (rho, u, case)
+
+
+# ---- C3: D2Q9 channel, IB cylinder Re 100, one refined level -----------------------------
+
+C3_RES = 40.0                     # coarse cells per cylinder diameter (LUMA's unit length)
+C3_UMAX = 0.04                    # parabolic inlet maximum, lattice units
+C3_RE = 100.0                     # on the mean inlet velocity and the diameter
+C3_CENTRE = (256.0, 130.0)        # coarse cells: x = 256, y = 128 + 2 (the seeded offset)
+C3_PATCH = (208, 400, 90, 170)    # coarse cells covered by the fine level
+C3_MARKERS = 252
+
+
+def c3_cylinder(nx: int = 1024, ny: int = 256, patch=C3_PATCH, centre=C3_CENTRE, nmark: int = C3_MARKERS):
+    """Coarse 1024 x 256 channel (parabolic inlet, ExtrapolateRight outlet, solid walls) with a
+    2:1 fine patch around a 40-cell cylinder of ``nmark`` equally spaced markers on the fine level."""
+    parent = setups.channel_2d(nx, ny, C3_RES, C3_UMAX, C3_RE)
+    dh, dt = parent.info["dh"], parent.info["dt"]
+    child = setups.add_refined_region(parent, patch, dh, dt)
+    setups.velocity_field_on(child, parent)
+    pos = setups.circle_markers((centre[0] * dh, centre[1] * dh, 0.0), 0.5 * C3_RES * dh, child.info["dh"],
+                                luma_skip_first=False, n=nmark)
+    return parent, child, pos
+
+
+def strouhal(cl: np.ndarray, dstep: float, diameter: float, u0: float):
+    """St = f D / U0 from the upward zero crossings of the lift signal (second half of it)."""
+    x = cl[len(cl) // 2:]
+    x = x - x.mean()
+    up = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+    if len(up) < 3:
+        return None, None
+    # linear interpolation of each crossing
+    tc = up + (-x[up]) / (x[up + 1] - x[up])
+    period = np.diff(tc).mean() * dstep
+    return diameter / (period * u0), period
+
+
+def run_c3(precision: str, steps: int, sample: int = 10, fields_at=(), geometry=None):
+    parent, child, pos = geometry if geometry is not None else c3_cylinder()
+    u0 = C3_UMAX / 1.5
+    d_fine = C3_RES * 2.0
+    snaps = {}
+    cd, cl = [], []
+    with GpuGrid.from_case(parent, precision, store_macro=True) as gp, \
+            GpuGrid.from_case(child, precision, store_macro=True) as gc:
+        gp.add_child(gc)
+        body = gc.ibm_add_body(pos)
+        gc.ibm_initialise()
+        t0 = time.perf_counter()
+        for s in range(1, steps + 1):
+            gp.LBM_multi_opt()
+            if s % sample == 0:
+                F = gc.ibm_body_force(body)
+                cd.append(2.0 * F[0] / (u0 * u0 * d_fine))
+                cl.append(2.0 * F[1] / (u0 * u0 * d_fine))
+            if s in fields_at:
+                snaps[s] = (gp.download_macro(), gc.download_macro())
+            if s % 10000 == 0:
+                print(f"[c3 {precision}] step {s} Cd {cd[-1]:.5f} Cl {cl[-1]:+.5f}", flush=True)
+        gp.sync()
+        wall = time.perf_counter() - t0
+    cd, cl = np.array(cd), np.array(cl)
+    st, period = strouhal(cl, sample, C3_RES, u0)
+    half = cd[len(cd) // 2:]
+    cells = parent.N * parent.M + 2 * child.N * child.M
+    out = {"config": "c3_cylinder", "precision": precision, "steps": steps, "markers": len(pos),
+           "fine_patch": [child.N, child.M], "tau_coarse": 1.0 / parent.omega, "tau_fine": 1.0 / child.omega,
+           "Cd_mean": float(half.mean()), "Cd_amp": float((half.max() - half.min()) / 2),
+           "Cl_amp": float((cl[len(cl) // 2:].max() - cl[len(cl) // 2:].min()) / 2),
+           "St": st, "period_steps": period, "wall_s": wall, "mlups": cells * steps / wall / 1e6}
+    return out, {"cd": cd, "cl": cl, "snaps": snaps, "parent": parent, "child": child}
+
+
+def rel_l2(a, b, mask):
+    d = (a - b)[mask]
+    return float(np.sqrt((d * d).sum() / (b[mask] * b[mask]).sum()))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("config", choices=["c1", "c3"])
+    ap.add_argument("--precision", default="fp64,fp32s")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--sample", type=int, default=10)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    precs = a.precision.split(",")
+    report = {"runs": []}
+    if a.config == "c1":
+        steps = a.steps or 20000
+        fields = {}
+        for p in precs:
+            r, (rho, u, case) = run_c1(p, steps)
+            fields[p] = (rho, u)
+            report["runs"].append(r)
+            print(json.dumps(r), flush=True)
+        if "fp64" in fields:
+            m = case.lattyp != setups.SOLID
+            for p in precs:
+                if p != "fp64":
+                    report[f"u_rel_l2_{p}_vs_fp64"] = rel_l2(fields[p][1], fields["fp64"][1], m)
+    else:
+        steps = a.steps or 100000
+        marks = tuple(s for s in (1000, 10000, steps) if s <= steps)
+        geo = c3_cylinder()
+        res = {}
+        for p in precs:
+            r, extra = run_c3(p, steps, a.sample, fields_at=marks, geometry=geo)
+            res[p] = extra
+            report["runs"].append(r)
+            print(json.dumps(r), flush=True)
+        if "fp64" in res:
+            parent, child = geo[0], geo[1]
+            mp = ~np.isin(parent.lattyp, [setups.SOLID, setups.REFINED, setups.EXTRAP_RIGHT])
+            mc = child.lattyp != setups.SOLID
+            for p in precs:
+                if p == "fp64":
+                    continue
+                errs = {}
+                for s in marks:
+                    (pr, pu), (cr, cu) = res[p]["snaps"][s]
+                    (qr, qu), (dr, du) = res["fp64"]["snaps"][s]
+                    errs[str(s)] = {"coarse": rel_l2(pu, qu, mp), "fine": rel_l2(cu, du, mc)}
+                report[f"u_rel_l2_{p}_vs_fp64"] = errs
+                n = min(len(res[p]["cd"]), len(res["fp64"]["cd"]))
+                report[f"cd_max_abs_diff_{p}_vs_fp64_first_1000_steps"] = float(
+                    np.abs(res[p]["cd"][:1000 // a.sample] - res["fp64"]["cd"][:1000 // a.sample]).max())
+                report[f"cd_series_len"] = n
+    print(json.dumps({k: v for k, v in report.items() if k != "runs"}), flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(report, fh, indent=1)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -81,6 +81,8 @@ struct StepArgs {
     int subcycle;
     // --- ExtrapolateRight sources: xr[v*(ny*nz) + y + ny*z] = f*_v two cells left of (N-1, y, z)
     const double *xr;
+    // --- z-slab with halo planes: the step updates planes zoff .. nz-1-zoff only ------------
+    int zoff;
 };
 
 // ----------------------------------------------------------------------------------------
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
     using S = typename P::S;
     using T = typename P::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
+    const int y = blockIdx.y, z = blockIdx.z + a.zoff;
     if (x >= a.nx) return;
     S *__restrict__ f = static_cast<S *>(a.f);
     const size_t ps = a.ps;
@@ -492,6 +494,46 @@ __global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
         else
             st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
     });
+}
+
+// Halo exchange of a z-slab (planes 1..n interior, 0 and n+1 halo; n = nz - 2) with itself as
+// both neighbours, i.e. what one rank of a z-sharded periodic box does with RCCL, done as a
+// local copy.  Only the populations with c_z != 0 cross a face (5 of 19 each way), and each one
+// is a contiguous nx*ny block of one population plane, so nothing is packed.
+//   phase 0, before the odd step: halo slots the odd gather reads
+//     A[opp v][0]   <- A[opp v][n]   (c_z(v) = +1: pulled from the lower neighbour)
+//     A[opp v][n+1] <- A[opp v][1]   (c_z(v) = -1: pulled from the upper neighbour)
+//   phase 1, after the odd step: halo slots the odd scatter wrote
+//     A[v][1] <- A[v][n+1]           (c_z(v) = +1: pushed to the upper neighbour)
+//     A[v][n] <- A[v][0]             (c_z(v) = -1: pushed to the lower neighbour)
+template <class P>
+__global__ __launch_bounds__(256) void k_zslab_exchange(StepArgs a, int phase) {
+    using L = Lattice<19>;
+    using S = typename P::S;
+    const size_t plane = (size_t)a.nx * a.ny;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane) return;
+    const int n = a.nz - 2;
+    const int k = blockIdx.y;           // 0..9: five velocities with c_z = +1, then five with -1
+    int cnt = 0, v = 0;
+    const int want = (k < 5) ? 1 : -1;
+    for (int w = 0; w < 19; ++w)
+        if (L::c(w, 2) == want) {
+            if (cnt == (k % 5)) v = w;
+            ++cnt;
+        }
+    int slot, src, dst;
+    if (phase == 0) {
+        slot = L::opp(v);
+        src = (want == 1) ? n : 1;
+        dst = (want == 1) ? 0 : n + 1;
+    } else {
+        slot = v;
+        src = (want == 1) ? n + 1 : 0;
+        dst = (want == 1) ? 1 : n;
+    }
+    S *f = static_cast<S *>(a.f);
+    f[(size_t)slot * a.ps + (size_t)dst * plane + i] = f[(size_t)slot * a.ps + (size_t)src * plane + i];
 }
 
 // ----------------------------------------------------------------------------------------

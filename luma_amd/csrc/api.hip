@@ -84,6 +84,14 @@ int wall_normal(const lumagpu_grid_desc &d, double px, double py, double pz, int
     return cnt;
 }
 
+hipError_t launch_zslab(const lumagpu_grid *g, const StepArgs &a, int phase) {
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_zslab_f64(a, phase, g->stream);
+        case LUMAGPU_FP32S: return launch_zslab_f32s(a, phase, g->stream);
+        default: return launch_zslab_f16s(a, phase, g->stream);
+    }
+}
+
 hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
     EdgeArgs e{};
     e.n = g->nedge;
@@ -147,10 +155,19 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
         hipError_t e = launch_edges(g, a, 0);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("edge launch: ") + hipGetErrorString(e));
     }
+    const bool halo = g->zslab && g->parity == 0;
+    if (halo) {
+        hipError_t e = launch_zslab(g, a, 0);
+        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("halo launch: ") + hipGetErrorString(e));
+    }
     if (e0) HIP_TRY(hipEventRecord(e0, g->stream));
     hipError_t e = launch_step(g, ls, a);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("step launch: ") + hipGetErrorString(e));
     if (e1) HIP_TRY(hipEventRecord(e1, g->stream));
+    if (halo) {
+        e = launch_zslab(g, a, 1);
+        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("halo launch: ") + hipGetErrorString(e));
+    }
     if (g->nedge) {
         e = launch_edges(g, a, 1);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("edge launch: ") + hipGetErrorString(e));
@@ -349,6 +366,21 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->has_outlet = outlet;
     g->masked = any || g->d.forced;
+    return 0;
+}
+
+int lumagpu_set_zslab(lumagpu_grid *g, int on) {
+    if (int r = check_grid(g)) return r;
+    if (!on) {
+        g->zslab = 0;
+        return 0;
+    }
+    if (g->d.dims != 3 || g->Q != 19 || g->d.K < 3)
+        return fail(LUMAGPU_EINVAL, "a z-slab needs a D3Q19 grid with at least one interior and two halo planes");
+    if (g->masked || g->d.forced || g->parent || g->child || g->ibm)
+        return fail(LUMAGPU_EUNSUPPORTED, "z-slab halo exchange is implemented for unlabelled, unforced single-level grids");
+    if (g->parity != 0) return fail(LUMAGPU_ESTATE, "switch to a z-slab between step pairs");
+    g->zslab = 1;
     return 0;
 }
 

@@ -36,6 +36,7 @@ struct lumagpu_grid {
     int *edge = nullptr;              // [4*nedge]: cells, packed normals, neighbour x+n, x+2n
     double *erho = nullptr;           // [2*nedge] neighbour densities of the current step
     int masked = 0;
+    int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
     int store_macro = 0;
     int parity = 0;
@@ -78,6 +79,7 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.den2 = 2.0 * (cs * cs) * (cs * cs);
     a.om_hi = float(g->d.omega);
     a.om_lo = float(g->d.omega - double(a.om_hi));
+    a.zoff = g->zslab ? 1 : 0;
     a.regularised = g->d.regularised;
     a.fz_compat = g->d.fz_compat;
     if (g->parent) {

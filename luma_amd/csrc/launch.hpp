@@ -72,6 +72,18 @@ hipError_t launch_edges_p(int Q, const StepArgs &a, const EdgeArgs &e, bool odd,
     return hipGetLastError();
 }
 
+// z-slab halo exchange (D3Q19 periodic grids with halo planes), phase 0 before / 1 after the odd step
+hipError_t launch_zslab_f64(const StepArgs &a, int phase, hipStream_t s);
+hipError_t launch_zslab_f32s(const StepArgs &a, int phase, hipStream_t s);
+hipError_t launch_zslab_f16s(const StepArgs &a, int phase, hipStream_t s);
+
+template <class P>
+hipError_t launch_zslab_p(const StepArgs &a, int phase, hipStream_t s) {
+    const size_t plane = (size_t)a.nx * a.ny;
+    k_zslab_exchange<P><<<dim3((unsigned)((plane + 255) / 256), 10), 256, 0, s>>>(a, phase);
+    return hipGetLastError();
+}
+
 // aux ops
 enum AuxOp : int {
     kAuxInitFeq = 0,     // p0 = rho (LUMA id), p1 = u (LUMA id)
@@ -92,7 +104,7 @@ inline dim3 cell_grid(int nx, int ny, int nz) {
 
 template <int Q, class P>
 hipError_t launch_step_q(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
-    const dim3 b = cell_block(a.nx), g = cell_grid(a.nx, a.ny, a.nz);
+    const dim3 b = cell_block(a.nx), g = cell_grid(a.nx, a.ny, a.nz - 2 * a.zoff);
     if (ls.masked) {
         if (ls.odd) {
             if (ls.store) k_aa_masked<Q, P, true, true><<<g, b, 0, s>>>(a);

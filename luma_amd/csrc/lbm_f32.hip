@@ -31,4 +31,7 @@ hipError_t launch_edges_f16s(int Q, const StepArgs &a, const EdgeArgs &e, bool o
     return launch_edges_p<P16S>(Q, a, e, odd, phase, store, s);
 }
 
+hipError_t launch_zslab_f32s(const StepArgs &a, int phase, hipStream_t s) { return launch_zslab_p<P32S>(a, phase, s); }
+hipError_t launch_zslab_f16s(const StepArgs &a, int phase, hipStream_t s) { return launch_zslab_p<P16S>(a, phase, s); }
+
 }  // namespace lumagpu

@@ -21,4 +21,6 @@ hipError_t launch_edges_f64(int Q, const StepArgs &a, const EdgeArgs &e, bool od
     return launch_edges_p<P64>(Q, a, e, odd, phase, store, s);
 }
 
+hipError_t launch_zslab_f64(const StepArgs &a, int phase, hipStream_t s) { return launch_zslab_p<P64>(a, phase, s); }
+
 }  // namespace lumagpu

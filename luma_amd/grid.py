@@ -132,6 +132,11 @@ class GpuGrid:
         call("lumagpu_set_store_macro", self.h, int(bool(on)))
 
     # stepping -------------------------------------------------------------------------
+    def set_zslab(self, on=True):
+        """z planes 0 and K-1 become halo planes exchanged around every odd step (one rank of a
+        z-sharded periodic box, its own neighbour on both faces)."""
+        call("lumagpu_set_zslab", self.h, int(on))
+
     def LBM_multi_opt(self, subcycle=0):
         call("lumagpu_step", self.h, int(subcycle))
 
