@@ -68,16 +68,20 @@ def c3_cylinder(nx: int = 1024, ny: int = 256, patch=C3_PATCH, centre=C3_CENTRE,
     return parent, child, pos
 
 
-def strouhal(cl: np.ndarray, dstep: float, diameter: float, u0: float):
-    """St = f D / U0 from the upward zero crossings of the lift signal (second half of it)."""
-    x = cl[len(cl) // 2:]
-    x = x - x.mean()
-    up = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
-    if len(up) < 3:
+def strouhal(cl: np.ndarray, dstep: float, diameter: float, u0: float, tail: float = 0.5):
+    """St = f D / U0 with f the dominant frequency of the lift signal over its last ``tail``
+    fraction (FFT peak of the mean-removed, Hann-windowed signal, refined by a parabola
+    through the peak and its two neighbours).  Returns (St, period in steps)."""
+    x = cl[int(len(cl) * (1.0 - tail)):]
+    x = (x - x.mean()) * np.hanning(len(x))
+    if len(x) < 16 or not np.any(x):
         return None, None
-    # linear interpolation of each crossing
-    tc = up + (-x[up]) / (x[up + 1] - x[up])
-    period = np.diff(tc).mean() * dstep
+    p = np.abs(np.fft.rfft(x)) ** 2
+    k = int(np.argmax(p[1:])) + 1
+    if 1 <= k < len(p) - 1:
+        a, b, c = np.log(p[k - 1] + 1e-300), np.log(p[k]), np.log(p[k + 1] + 1e-300)
+        k = k + 0.5 * (a - c) / (a - 2 * b + c)
+    period = len(x) * dstep / k
     return diameter / (period * u0), period
 
 
@@ -168,6 +172,10 @@ def main(argv=None):
                     (qr, qu), (dr, du) = res["fp64"]["snaps"][s]
                     errs[str(s)] = {"coarse": rel_l2(pu, qu, mp), "fine": rel_l2(cu, du, mc)}
                 report[f"u_rel_l2_{p}_vs_fp64"] = errs
+                a64 = next(r for r in report["runs"] if r["precision"] == "fp64")
+                ap_ = next(r for r in report["runs"] if r["precision"] == p)
+                report[f"{p}_vs_fp64_rel"] = {k: (abs(ap_[k] - a64[k]) / abs(a64[k]) if a64[k] else None)
+                                              for k in ("Cd_mean", "Cl_amp", "St")}
                 n = min(len(res[p]["cd"]), len(res["fp64"]["cd"]))
                 report[f"cd_max_abs_diff_{p}_vs_fp64_first_1000_steps"] = float(
                     np.abs(res[p]["cd"][:1000 // a.sample] - res["fp64"]["cd"][:1000 // a.sample]).max())
