@@ -121,9 +121,11 @@ def error_run(case, nsteps):
                      "ke_ratio": m["ke"] / ke0,
                      "ke_rel_vs_fp64": m["ke"] / m64["ke"] - 1.0}
         del u
+    # linear (Stokes) decay of the TGV mode, |k|^2 = 3 k^2; the 3D vortex at this Reynolds
+    # number also cascades energy to small scales, so the measured ratio lies below it
     nu = (0.55 - 0.5) / 3.0
-    k2 = 2 * (2 * math.pi / case.N) ** 2
-    out["ke_ratio_viscous_decay_estimate"] = math.exp(-2 * nu * k2 * nsteps)
+    k2 = 3 * (2 * math.pi / case.N) ** 2
+    out["ke_ratio_stokes_decay"] = math.exp(-2 * nu * k2 * nsteps)
     return out
 
 

@@ -159,6 +159,29 @@ int lumagpu_set_positions(lumagpu_grid *g, const double *xpos, const double *ypo
  * would replace with RCCL send/receive of the same contiguous planes. */
 int lumagpu_set_zslab(lumagpu_grid *g, int on);
 
+/* ---- discrete adjoint (the differentiation form) ------------------------------------------
+ * No LUMA counterpart.  Objective: J = (1/T) sum over the next T steps of dir . (the IB body
+ * force, summed over all bodies of the grid: the per-marker terms LUMA writes to
+ * Body_<id>_LD, ObjectManager_ops_io.cpp:98-121).  lumagpu_adj_run advances the grid T steps
+ * (as T lumagpu_step calls; it is left at the end of them), keeping a copy of the populations
+ * every checkpoint_every steps; with backward = 1 it then runs the discrete adjoint of those
+ * steps, recomputing each segment from its checkpoint, and returns
+ *   res->J, res->dJ_domega (relaxation frequency), res->dJ_dinlet (derivative with respect to
+ *   a common scale s of the inlet profile, u_in -> s u_in, at s = 1),
+ *   dJ_dpos[3*m + a] (marker positions of all bodies in marker order, through the delta
+ *   weights, epsilon and ds; NULL to skip) and lambda0 (dJ/df of the initial post-collision
+ *   populations in LUMA's layout f[v + Q*id]; NULL to skip).
+ * Covered: FP64 and FP32-shifted forms, one grid level, fluid / solid / forced-equilibrium
+ * velocity / ExtrapolateRight sites, body force, rigid IB bodies.  Memory: ceil(T/K) + K + 1
+ * population arrays plus three adjoint arrays. */
+typedef struct lumagpu_adj_result {
+    double J;
+    double dJ_domega;
+    double dJ_dinlet;
+} lumagpu_adj_result;
+int lumagpu_adj_run(lumagpu_grid *g, int nsteps, int checkpoint_every, const double *dir, int backward,
+                    lumagpu_adj_result *res, double *dJ_dpos, double *lambda0);
+
 /* ---- diagnostics --------------------------------------------------------------------- */
 const char *lumagpu_last_error(void);   /* replaces L_ERROR's log line (stdafx.h:119-149) */
 int lumagpu_abi_version(void);

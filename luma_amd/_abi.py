@@ -35,6 +35,10 @@ class GridDesc(C.Structure):
     ]
 
 
+class AdjResult(C.Structure):
+    _fields_ = [("J", C.c_double), ("dJ_domega", C.c_double), ("dJ_dinlet", C.c_double)]
+
+
 class LumaGpuError(RuntimeError):
     def __init__(self, code: int, fn: str, msg: str):
         super().__init__(f"{fn} failed with {ERROR_NAMES.get(code, code)}: {msg}")
@@ -76,6 +80,7 @@ _PROTOS = {
     "lumagpu_ibm_set_epsilon": [_vp, C.c_int, _dp],
     "lumagpu_ibm_get_markers": [_vp, C.c_int, _dp],
     "lumagpu_ibm_body_force": [_vp, C.c_int, _dp],
+    "lumagpu_adj_run": [_vp, C.c_int, C.c_int, _dp, C.c_int, C.POINTER(AdjResult), _dp, _dp],
     "lumagpu_last_error": [],
     "lumagpu_abi_version": [],
     "lumagpu_footprint": [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],

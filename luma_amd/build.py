@@ -34,6 +34,7 @@ UNITS = [
     ("lbm_f32.hip", ["-ffp-contract=fast"]),
     ("api.hip", ["-ffp-contract=off"]),
     ("ibm.hip", ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"]),
+    ("adjoint.hip", ["-ffp-contract=off"]),
 ]
 
 

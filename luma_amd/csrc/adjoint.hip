@@ -1,0 +1,371 @@
+// adjoint.hip -- discrete adjoint of the time step with checkpointing (the differentiation
+// form): gradients of the time-averaged IB body force with respect to the relaxation
+// frequency, the inlet velocity scale, the marker positions and the initial populations.
+//
+// Forward: T steps of the ordinary step (lumagpu_step semantics), the population array copied
+// to a checkpoint every K steps and the objective accumulated on the device.  Backward: for
+// each segment, newest first, the segment's K states are recomputed from its checkpoint and
+// kept in HBM; then every step of the segment is reversed from its stored input state (the
+// outlet side buffer and the IB forces are recomputed from that state, then the adjoint
+// kernels of adjoint_kernels.hpp run).  The marker-position gradient adds, on the host, the
+// dependence of epsilon (the RKPM system A eps = 1, ObjectManager_ops_ibm.cpp:773-920) and of
+// ds (nearest-marker spacing, :927-981) on the positions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "adjoint_kernels.hpp"
+#include "ibm_system.hpp"
+#include "launch.hpp"
+
+namespace lumagpu {
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return fail(LUMAGPU_ENOMEM, "adjoint buffer allocation failed (" + std::to_string(bytes) + " bytes)");
+        }
+        return 0;
+    }
+};
+
+template <class P>
+hipError_t launch_adj_collide(const lumagpu_grid *g, const StepArgs &a, const AdjArgs<typename P::T> &b) {
+    const dim3 bl(256), gr((a.nx + 255) / 256, a.ny, a.nz);
+    const bool odd = g->parity == 0;
+    if (g->Q == 9) {
+        if (odd) k_adj_collide<9, P, true><<<gr, bl, 0, g->stream>>>(a, b);
+        else k_adj_collide<9, P, false><<<gr, bl, 0, g->stream>>>(a, b);
+    } else {
+        if (odd) k_adj_collide<19, P, true><<<gr, bl, 0, g->stream>>>(a, b);
+        else k_adj_collide<19, P, false><<<gr, bl, 0, g->stream>>>(a, b);
+    }
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_adj_stream(const lumagpu_grid *g, const StepArgs &a, const AdjArgs<typename P::T> &b) {
+    const dim3 bl(256), gr((a.nx + 255) / 256, a.ny, a.nz);
+    if (g->Q == 9) k_adj_stream<9, P><<<gr, bl, 0, g->stream>>>(a, b);
+    else k_adj_stream<19, P><<<gr, bl, 0, g->stream>>>(a, b);
+    return hipGetLastError();
+}
+
+int check_adjoint_grid(const lumagpu_grid *g) {
+    if (g->d.precision != LUMAGPU_FP64 && g->d.precision != LUMAGPU_FP32S)
+        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs in the FP64 and FP32-shifted forms");
+    if (g->parent || g->child) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs on a single grid level");
+    if (g->zslab || g->nedge) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs or edge sites");
+    for (size_t c = 0; c < g->hflags.size(); ++c) {
+        const int ty = g->hflags[c] & kTypeMask;
+        if (ty == kFluid || ty == kSolid || ty == kExtrapRight) continue;
+        if (ty == kVelocity && !g->d.regularised) continue;
+        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint covers fluid, solid, forced-equilibrium velocity and "
+                                          "ExtrapolateRight sites only");
+    }
+    if (g->ibm)
+        for (const auto &b : g->ibm->bodies)
+            if (b.flexible) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint covers rigid IB bodies only");
+    return 0;
+}
+
+// grad delta(x; s) = d/dx of prod_a phi((x_a - s_a)/dh)
+void delta_grad(const double *x, const double *s, double dh, double dil, int D, double out[3]) {
+    double ph[3], dph[3];
+    for (int a = 0; a < 3; ++a) {
+        const double r = (a < D) ? (x[a] - s[a]) / dh : 0.0;
+        ph[a] = (a < D) ? delta_kernel(r, dil) : 1.0;
+        dph[a] = (a < D) ? delta_kernel_deriv(r, dil) / dh : 0.0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        out[a] = 0.0;
+        if (a >= D) continue;
+        double v = dph[a];
+        for (int b = 0; b < D; ++b)
+            if (b != a) v *= ph[b];
+        out[a] = v;
+    }
+}
+double delta_at(const double *x, const double *s, double dh, double dil, int D) {
+    double v = 1.0;
+    for (int a = 0; a < D; ++a) v *= delta_kernel((x[a] - s[a]) / dh, dil);
+    return v;
+}
+
+// epsilon and ds chains of one body: xg += dJ/dx through eps (A eps = 1) and ds
+int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsbar, const double *dsbar_dev,
+                        double *xg) {
+    const int n = (int)b.m.size(), D = g->D;
+    const double dh = g->d.dh;
+    std::vector<double> dsbar(dsbar_dev, dsbar_dev + n);
+    // A_IJ = ds_J sum_k delta(x_I; s^I_k) delta(x_J; s^I_k) area_I; solve A^T mu = epsbar
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int I = 0; I < n; ++I)
+        for (int J = 0; J < n; ++J) {
+            double acc = 0.0;
+            for (int k = 0; k < b.m[I].ns; ++k)
+                acc += b.m[I].delta[k] * delta_at(b.m[J].pos, b.m[I].sxyz[k], dh, b.m[J].dilation, D) * b.m[I].area;
+            A[(size_t)I * n + J] = acc * b.m[J].ds;
+        }
+    std::vector<double> At((size_t)n * n), mu(epsbar, epsbar + n);
+    for (int I = 0; I < n; ++I)
+        for (int J = 0; J < n; ++J) At[(size_t)J * n + I] = A[(size_t)I * n + J];
+    if (int r = lu_solve(n, At, mu)) return r;
+    for (int I = 0; I < n; ++I) {
+        const Marker &mI = b.m[I];
+        for (int J = 0; J < n; ++J) {
+            const Marker &mJ = b.m[J];
+            const double Abar = -mu[I] * mJ.eps;
+            if (Abar == 0.0) continue;
+            dsbar[J] += Abar * A[(size_t)I * n + J] / mJ.ds;
+            for (int k = 0; k < mI.ns; ++k) {
+                double gI[3], gJ[3];
+                delta_grad(mI.pos, mI.sxyz[k], dh, mI.dilation, D, gI);
+                delta_grad(mJ.pos, mI.sxyz[k], dh, mJ.dilation, D, gJ);
+                const double dJk = delta_at(mJ.pos, mI.sxyz[k], dh, mJ.dilation, D);
+                for (int a = 0; a < D; ++a) {
+                    xg[3 * I + a] += Abar * mJ.ds * gI[a] * dJk * mI.area;
+                    xg[3 * J + a] += Abar * mJ.ds * mI.delta[k] * gJ[a] * mI.area;
+                }
+            }
+        }
+    }
+    // ds_m = min(10, min_n |x_m - x_n| / dh)
+    for (int m = 0; m < n; ++m) {
+        double best = 10.0;
+        int arg = -1;
+        for (int q = 0; q < n; ++q) {
+            if (q == m) continue;
+            double acc = 0.0;
+            for (int a = 0; a < 3; ++a) acc += std::pow(b.m[m].pos[a] - b.m[q].pos[a], 2);
+            const double dist = std::sqrt(acc) / dh;
+            if (dist < best) {
+                best = dist;
+                arg = q;
+            }
+        }
+        if (arg < 0 || best <= 0.0) continue;
+        for (int a = 0; a < D; ++a) {
+            const double gdir = (b.m[m].pos[a] - b.m[arg].pos[a]) / (best * dh * dh);
+            xg[3 * m + a] += dsbar[m] * gdir;
+            xg[3 * arg + a] -= dsbar[m] * gdir;
+        }
+    }
+    return 0;
+}
+
+template <class P>
+int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int backward, lumagpu_adj_result *res,
+            double *dpos, double *lambda0) {
+    using T = typename P::T;
+    IbmSystem *s = g->ibm;
+    const bool ib = s && s->nm > 0;
+    const size_t fbytes = (size_t)g->elem * g->Q * g->ps;
+    const double invT = 1.0 / nsteps;
+    const int nck = (nsteps + every - 1) / every;
+
+    std::vector<DevBuf> ck(nck);
+    std::vector<int> ckpar(nck);
+    for (auto &b : ck)
+        if (int r = b.alloc(fbytes)) return r;
+    DevBuf fin, accJ;
+    if (int r = fin.alloc(fbytes)) return r;
+    if (int r = accJ.alloc(4 * sizeof(double))) return r;
+    HIP_TRY(hipMemsetAsync(accJ.p, 0, 4 * sizeof(double), g->stream));
+    const int t_start = g->t;
+
+    // forward with checkpoints
+    for (int t = 0; t < nsteps; ++t) {
+        if (t % every == 0) {
+            HIP_TRY(hipMemcpyAsync(ck[t / every].p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
+            ckpar[t / every] = g->parity;
+        }
+        if (int r = step_grid(g, 0)) return r;
+        if (ib) {
+            k_adj_objective<<<1, 64, 0, g->stream>>>(ibm_args(g), dir[0], dir[1], dir[2], invT,
+                                                     static_cast<double *>(accJ.p));
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(fin.p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
+    const int fin_par = g->parity, fin_t = g->t;
+    double hacc[4] = {0, 0, 0, 0};
+    if (!backward) {
+        HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (res) *res = lumagpu_adj_result{hacc[0], 0.0, 0.0};
+        return 0;
+    }
+
+    // backward buffers
+    const size_t nc = g->ncell;
+    const int nblk = ((g->d.N + 255) / 256) * g->d.M * g->d.K;
+    std::vector<DevBuf> tape(every);
+    std::vector<int> tpar(every);
+    for (auto &b : tape)
+        if (int r = b.alloc(fbytes)) return r;
+    DevBuf lam, gbar, fbar, part;
+    if (int r = lam.alloc((size_t)g->Q * nc * sizeof(T))) return r;
+    if (int r = gbar.alloc((size_t)g->Q * nc * sizeof(T))) return r;
+    if (int r = fbar.alloc(3 * nc * sizeof(double))) return r;
+    if (int r = part.alloc(2 * (size_t)nblk * sizeof(double))) return r;
+    HIP_TRY(hipMemsetAsync(lam.p, 0, (size_t)g->Q * nc * sizeof(T), g->stream));
+    HIP_TRY(hipMemsetAsync(fbar.p, 0, 3 * nc * sizeof(double), g->stream));
+    AdjArgs<T> ab{static_cast<T *>(lam.p), static_cast<T *>(gbar.p), static_cast<double *>(fbar.p),
+                  static_cast<double *>(part.p), nblk};
+    DevBuf mbar, xbar, epsbar, dsbar, sdd;
+    AdjIbm aj{};
+    if (ib) {
+        const size_t nm = (size_t)s->nm;
+        if (int r = mbar.alloc(4 * nm * sizeof(double))) return r;
+        if (int r = xbar.alloc(3 * nm * sizeof(double))) return r;
+        if (int r = epsbar.alloc(nm * sizeof(double))) return r;
+        if (int r = dsbar.alloc(nm * sizeof(double))) return r;
+        if (int r = sdd.alloc(nm * s->S * 3 * sizeof(double))) return r;
+        HIP_TRY(hipMemsetAsync(xbar.p, 0, 3 * nm * sizeof(double), g->stream));
+        HIP_TRY(hipMemsetAsync(epsbar.p, 0, nm * sizeof(double), g->stream));
+        HIP_TRY(hipMemsetAsync(dsbar.p, 0, nm * sizeof(double), g->stream));
+        std::vector<double> h(nm * s->S * 3, 0.0);
+        for (const auto &b : s->bodies)
+            for (size_t i = 0; i < b.m.size(); ++i) {
+                const Marker &m = b.m[i];
+                for (int k = 0; k < m.ns; ++k)
+                    delta_grad(m.pos, m.sxyz[k], g->d.dh, m.dilation, g->D,
+                               &h[((size_t)(b.first + i) * s->S + k) * 3]);
+            }
+        HIP_TRY(hipMemcpyAsync(sdd.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        aj.fbar = static_cast<double *>(fbar.p);
+        aj.sddelta = static_cast<double *>(sdd.p);
+        aj.mbar = static_cast<double *>(mbar.p);
+        aj.xbar = static_cast<double *>(xbar.p);
+        aj.epsbar = static_cast<double *>(epsbar.p);
+        aj.dsbar = static_cast<double *>(dsbar.p);
+        for (int d = 0; d < 3; ++d) aj.dir[d] = dir[d];
+        aj.invT = invT;
+    }
+    double *acc = static_cast<double *>(accJ.p);
+
+    void *const f_own = g->f;
+    int rc = 0;
+    for (int seg = nck - 1; seg >= 0 && !rc; --seg) {
+        const int t0 = seg * every, t1 = std::min(nsteps, t0 + every);
+        HIP_TRY(hipMemcpyAsync(g->f, ck[seg].p, fbytes, hipMemcpyDeviceToDevice, g->stream));
+        g->parity = ckpar[seg];
+        for (int t = t0; t < t1 && !rc; ++t) {
+            HIP_TRY(hipMemcpyAsync(tape[t - t0].p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
+            tpar[t - t0] = g->parity;
+            if (t + 1 < t1) rc = step_grid(g, 0);
+        }
+        for (int t = t1 - 1; t >= t0 && !rc; --t) {
+            // the step's input state stands in for the grid's populations while it is reversed
+            g->f = tape[t - t0].p;
+            g->parity = tpar[t - t0];
+            if (g->has_outlet) {
+                hipError_t e = outlet_prepass(g);
+                if (e != hipSuccess) rc = fail(LUMAGPU_EHIP, std::string("adjoint outlet: ") + hipGetErrorString(e));
+            }
+            if (!rc && ib) rc = apply_core(g);
+            const StepArgs a = make_args(g);
+            if (!rc) {
+                hipError_t e = launch_adj_collide<P>(g, a, ab);
+                if (e == hipSuccess && ib) {
+                    IbmArgs ia = ibm_args(g);
+                    k_adj_ibm_marker<<<(s->nm + 127) / 128, 128, 0, g->stream>>>(ia, aj);
+                    e = hipGetLastError();
+                    if (e == hipSuccess) {
+                        if (g->Q == 9)
+                            k_adj_ibm_site<9, T><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(
+                                ia, aj.mbar, g->flags, ab.gbar);
+                        else
+                            k_adj_ibm_site<19, T><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(
+                                ia, aj.mbar, g->flags, ab.gbar);
+                        e = hipGetLastError();
+                    }
+                }
+                if (e == hipSuccess) e = launch_adj_stream<P>(g, a, ab);
+                if (e == hipSuccess) {
+                    k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ab.part, nblk, 2, acc + 1);
+                    e = hipGetLastError();
+                }
+                if (e != hipSuccess) rc = fail(LUMAGPU_EHIP, std::string("adjoint launch: ") + hipGetErrorString(e));
+            }
+            g->f = f_own;
+        }
+    }
+    // the grid returns to the end of the forward run
+    HIP_TRY(hipMemcpyAsync(g->f, fin.p, fbytes, hipMemcpyDeviceToDevice, g->stream));
+    g->parity = fin_par;
+    g->t = fin_t;
+    (void)t_start;
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
+    std::vector<T> hl;
+    if (lambda0) {
+        hl.resize((size_t)g->Q * nc);
+        HIP_TRY(hipMemcpyAsync(hl.data(), lam.p, hl.size() * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+    }
+    std::vector<double> hx, he, hd;
+    if (ib && dpos) {
+        hx.resize(3 * (size_t)s->nm);
+        he.resize(s->nm);
+        hd.resize(s->nm);
+        HIP_TRY(hipMemcpyAsync(hx.data(), xbar.p, hx.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(he.data(), epsbar.p, he.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(hd.data(), dsbar.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1], hacc[2]};
+    if (lambda0) {
+        // LUMA layout f[v + Q*id], id = k + j*K + i*K*M
+        const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < M; ++j)
+                for (int k = 0; k < K; ++k) {
+                    const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+                    const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
+                    for (int v = 0; v < Q; ++v) lambda0[v + Q * id] = double(hl[(size_t)v * nc + cell]);
+                }
+    }
+    if (ib && dpos) {
+        for (size_t i = 0; i < hx.size(); ++i) dpos[i] = hx[i];
+        for (const auto &b : s->bodies)
+            if (int r = body_position_chain(g, b, he.data() + b.first, hd.data() + b.first, dpos + 3 * (size_t)b.first))
+                return r;
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace lumagpu
+
+using namespace lumagpu;
+
+extern "C" int lumagpu_adj_run(lumagpu_grid *g, int nsteps, int checkpoint_every, const double *dir, int backward,
+                               lumagpu_adj_result *res, double *dJ_dpos, double *lambda0) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "adjoint before the populations were initialised");
+    if (nsteps <= 0 || checkpoint_every <= 0) return fail(LUMAGPU_EINVAL, "step count and checkpoint interval must be positive");
+    if (!dir) return fail(LUMAGPU_EINVAL, "null objective direction");
+    if (int r = check_adjoint_grid(g)) return r;
+    if (g->ibm && !g->ibm->initialised && g->ibm->nm > 0)
+        return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
+    if (!g->flags) {
+        if (int r = ensure_flags(g)) return r;
+        HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    }
+    if (g->d.precision == LUMAGPU_FP64)
+        return adj_run<P64>(g, nsteps, checkpoint_every, dir, backward, res, dJ_dpos, lambda0);
+    return adj_run<P32S>(g, nsteps, checkpoint_every, dir, backward, res, dJ_dpos, lambda0);
+}

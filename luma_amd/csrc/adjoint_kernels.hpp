@@ -1,0 +1,306 @@
+// adjoint_kernels.hpp -- discrete adjoint of one fused stream / IB / collide step.
+//
+// Forward step t (post-collision f of step t-1  ->  post-collision f' of step t):
+//   g_v(x)  = stream of f (periodic pull, halfway bounce-back, ExtrapolateRight source,
+//             forced-equilibrium velocity source), _LBM_stream_opt :206-297;
+//   IB      : rho_s, j_s at support sites from g; (rho u)_m = sum_s j_s d_ms, rho_m = sum_s
+//             rho_s d_ms; F_m = 2((rho u)_m - rho_m U_m); F_s = -sum_m d_ms F_m eps_m ds_m vd_m
+//             (ObjectManager_ops_ibm.cpp:498-689);
+//   collide : rho = sum g, u = (sum c g + B F/2)/rho, f'_v = g_v + omega (feq_v - g_v) + Guo_v
+//             (:765-990; B adds F_z to the x momentum under the :833 quirk).
+// Objective: J = (1/T) sum_t dir . sum_m F_m eps_m ds_m vd_m (the body force LUMA writes to
+// Body_<id>_LD, ObjectManager_ops_io.cpp:98-121, averaged over the T steps).
+//
+// The adjoint runs the step backwards: the collision's transposed Jacobian per site
+// (k_adj_collide), then spread and interpolation swapped (the spread's adjoint gathers the
+// site force adjoint to the markers, k_adj_ibm_marker; the interpolation's adjoint spreads the
+// marker adjoint to the sites through the same per-site gather lists, k_adj_ibm_site), then
+// reverse streaming (k_adj_stream: every population adjoint is gathered from the site that
+// pulled it).  Adjoint arrays are plain [v][cell] (no AA layout): lam = dJ/df, gbar = dJ/dg.
+#pragma once
+
+#include "aa_kernels.hpp"
+#include "ibm_kernels.hpp"
+
+namespace lumagpu {
+
+template <class T>
+struct AdjArgs {
+    T *lam;          // [Q][ncell] adjoint of the post-collision populations
+    T *gbar;         // [Q][ncell] adjoint of the streamed populations
+    double *fbar;    // [3][ncell] adjoint of the body-force field (flagged sites)
+    double *part;    // per-block partial sums: omega [0, nblk), inlet scale [nblk, 2 nblk)
+    int nblk;
+};
+
+__device__ __forceinline__ int block_linear() {
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+// deterministic block sum (256 threads)
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// Transposed collision Jacobian at every fluid site: gbar = (d f'/d g)^T lam, fbar =
+// (d f'/d F)^T lam at forced sites, and the omega derivative sum_v lam_v d f'_v / d omega.
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typename P::T> b) {
+    using L = Lattice<Q>;
+    using T = typename P::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    double om_site = 0.0;
+    if (x < a.nx) {
+        const size_t cell = cell_of(x, y, z, a.nx, a.ny);
+        const unsigned fl = a.flags[cell];
+        if ((fl & kTypeMask) == kFluid) {
+            Nbr nb;
+            nb.init(x, y, z, a.nx, a.ny, a.nz);
+            T fin[Q];
+            gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kFluid, fin);
+            T g[Q];
+#pragma unroll
+            for (int v = 0; v < Q; ++v) g[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
+            const bool forced = (fl & kForcedBit) != 0;
+            T F[3] = {T(0), T(0), T(0)};
+            if (forced) {
+                F[0] = T(a.force[cell]);
+                F[1] = T(a.force[a.ncell + cell]);
+                if constexpr (L::D == 3) F[2] = T(a.force[2 * a.ncell + cell]);
+            }
+            T rho = T(0), j[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int v = 0; v < Q; ++v) {
+                rho += g[v];
+                j[0] += T(L::c(v, 0)) * g[v];
+                j[1] += T(L::c(v, 1)) * g[v];
+                j[2] += T(L::c(v, 2)) * g[v];
+            }
+            j[0] += T(0.5) * F[0];
+            j[1] += T(0.5) * F[1];
+            if constexpr (L::D == 3) {
+                if (a.fz_compat) j[0] += T(0.5) * F[2];
+                else j[2] += T(0.5) * F[2];
+            }
+            const T u[3] = {j[0] / rho, j[1] / rho, j[2] / rho};
+            const T usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            const T uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+            const T om = T(a.omega), guo = T(1) - T(0.5) * om;
+            T lam[Q];
+            T S0 = T(0), Ua[3] = {T(0), T(0), T(0)}, Ga[3] = {T(0), T(0), T(0)}, Fb[3] = {T(0), T(0), T(0)};
+            T omb = T(0);
+#pragma unroll
+            for (int v = 0; v < Q; ++v) {
+                const T cx = T(L::c(v, 0)), cy = T(L::c(v, 1)), cz = T(L::c(v, 2)), w = T(L::w(v));
+                lam[v] = b.lam[(size_t)v * a.ncell + cell];
+                const T cu = cx * u[0] + cy * u[1] + cz * u[2];
+                const T cF = cx * F[0] + cy * F[1] + cz * F[2];
+                const T e = T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq;
+                const T lw = lam[v] * w;
+                S0 += lw * e;
+                Ua[0] += lw * (T(3) * cx + T(9) * cu * cx - T(3) * u[0]);
+                Ua[1] += lw * (T(3) * cy + T(9) * cu * cy - T(3) * u[1]);
+                Ua[2] += lw * (T(3) * cz + T(9) * cu * cz - T(3) * u[2]);
+                T dom = w * rho * e - g[v];
+                if (forced) {
+                    Ga[0] += lw * (T(-3) * F[0] + T(9) * cx * cF);
+                    Ga[1] += lw * (T(-3) * F[1] + T(9) * cy * cF);
+                    Ga[2] += lw * (T(-3) * F[2] + T(9) * cz * cF);
+                    Fb[0] += lw * (T(3) * (cx - u[0]) + T(9) * cu * cx);
+                    Fb[1] += lw * (T(3) * (cy - u[1]) + T(9) * cu * cy);
+                    Fb[2] += lw * (T(3) * (cz - u[2]) + T(9) * cu * cz);
+                    dom -= T(0.5) * w * (T(3) * (cF - uF) + T(9) * cu * cF);
+                }
+                omb += lam[v] * dom;
+            }
+            T ub[3], jb[3];
+            for (int d = 0; d < 3; ++d) {
+                ub[d] = om * rho * Ua[d] + guo * Ga[d];
+                jb[d] = ub[d] / rho;
+            }
+            if constexpr (L::D == 2) jb[2] = T(0);
+            const T rb = om * S0 - (ub[0] * u[0] + ub[1] * u[1] + ub[2] * u[2]) / rho;
+#pragma unroll
+            for (int v = 0; v < Q; ++v)
+                b.gbar[(size_t)v * a.ncell + cell] = (T(1) - om) * lam[v] + rb + T(L::c(v, 0)) * jb[0] +
+                                                    T(L::c(v, 1)) * jb[1] + T(L::c(v, 2)) * jb[2];
+            if (forced) {
+                double Fbar[3] = {double(guo * Fb[0] + T(0.5) * jb[0]), double(guo * Fb[1] + T(0.5) * jb[1]),
+                                  double(guo * Fb[2])};
+                if constexpr (L::D == 3) Fbar[2] += double(T(0.5) * (a.fz_compat ? jb[0] : jb[2]));
+                for (int d = 0; d < L::D; ++d) b.fbar[(size_t)d * a.ncell + cell] = Fbar[d];
+            }
+            om_site = double(omb);
+        }
+    }
+    const double s = block_sum(om_site);
+    if (threadIdx.x == 0) b.part[block_linear()] = s;
+}
+
+// Reverse streaming: lam_w(x) gathers the adjoint of every streamed population that read
+// f_w(x) -- the pull by x + c_w, the bounce-back at x itself when x + c_w is solid, and the
+// pull through an ExtrapolateRight source two cells to the right.  Also the derivative with
+// respect to a scale s of the inlet profile (u_in -> s u_in) of the forced-equilibrium
+// velocity sources, d feq_v(rho_in, s u_in) / ds at s = 1.
+template <int Q, class P>
+__global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename P::T> b) {
+    using L = Lattice<Q>;
+    using T = typename P::T;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    double in_site = 0.0;
+    if (x < a.nx) {
+        const int nx = a.nx, ny = a.ny;
+        const size_t cell = cell_of(x, y, z, nx, ny);
+        const int ty = a.flags[cell] & kTypeMask;
+        if (ty == kFluid) {
+            Nbr nb;
+            nb.init(x, y, z, nx, ny, a.nz);
+            const bool xr = (x + 2 < nx) && ((a.flags[cell + 2] & kTypeMask) == kExtrapRight);
+            Nbr ns;
+            if (xr) ns.init(x + 2, y, z, nx, ny, a.nz);
+#pragma unroll
+            for (int w = 0; w < Q; ++w) {
+                const int cx = L::c(w, 0), cy = L::c(w, 1), cz = L::c(w, 2);
+                const size_t yn = nb.at(cx, cy, cz, nx, ny);
+                const int tn = a.flags[yn] & kTypeMask;
+                T acc = T(0);
+                if (tn == kFluid) acc = b.gbar[(size_t)w * a.ncell + yn];
+                else if (tn == kSolid) acc = b.gbar[(size_t)L::opp(w) * a.ncell + cell];
+                if (xr) {
+                    const size_t yp = ns.at(cx, cy, cz, nx, ny);
+                    if ((a.flags[yp] & kTypeMask) == kFluid) acc += b.gbar[(size_t)w * a.ncell + yp];
+                }
+                b.lam[(size_t)w * a.ncell + cell] = acc;
+            }
+            if (!a.regularised) {
+                for (int v = 0; v < Q; ++v) {
+                    const size_t src = nb.at(-L::c(v, 0), -L::c(v, 1), -L::c(v, 2), nx, ny);
+                    if ((a.flags[src] & kTypeMask) != kVelocity) continue;
+                    const int js = nb.ys[1 - L::c(v, 1)];
+                    const double us[3] = {a.uin[js], a.uin[ny + js], (L::D == 3) ? a.uin[2 * ny + js] : 0.0};
+                    const double cu = L::c(v, 0) * us[0] + L::c(v, 1) * us[1] + L::c(v, 2) * us[2];
+                    // d feq_v / du . u_in = w rho_in (3 c.u + 9 (c.u)^2 - 3 u.u)
+                    const double usq = us[0] * us[0] + us[1] * us[1] + us[2] * us[2];
+                    const double d = L::w(v) * a.rho_in * (3.0 * cu + 9.0 * cu * cu - 3.0 * usq);
+                    in_site += double(b.gbar[(size_t)v * a.ncell + cell]) * d;
+                }
+            }
+        } else {
+            for (int w = 0; w < Q; ++w) b.lam[(size_t)w * a.ncell + cell] = T(0);
+        }
+    }
+    const double s = block_sum(in_site);
+    if (threadIdx.x == 0) b.part[b.nblk + block_linear()] = s;
+}
+
+// IB adjoint, marker side: the spread's adjoint gathers the site force adjoint to the
+// markers; the marker's force adjoint then feeds the interpolation's adjoint (mbar) and the
+// derivatives with respect to the marker's epsilon, ds and position (through its delta
+// weights), accumulated over the steps.
+struct AdjIbm {
+    const double *fbar;      // [3][ncell]
+    const double *sddelta;   // [nm][S][3] d delta_ms / d x_m
+    double *mbar;            // [nm][4]: adjoint of (rho u)_m, rho_m
+    double *xbar, *epsbar, *dsbar;
+    double dir[3];
+    double invT;
+};
+
+static __global__ void k_adj_ibm_marker(IbmArgs ib, AdjIbm aj) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= ib.nm) return;
+    const int D = ib.D;
+    const double eps = ib.eps[m], ds = ib.ds[m];
+    const double vd = (D == 3) ? ds : 1.0;
+    const double wgt = eps * ds * vd;
+    double Fs[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < ib.ns[m]; ++k) {
+        const size_t cell = (size_t)ib.scell[m * ib.S + k];
+        const double dl = ib.sdelta[m * ib.S + k];
+        for (int d = 0; d < D; ++d) Fs[d] += dl * aj.fbar[(size_t)d * ib.ncell + cell];
+    }
+    double Fb[3], mom[3], FdotFs = 0.0, Fdir = 0.0, rb = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double Fm = (d < D) ? ib.force[3 * m + d] : 0.0;
+        Fb[d] = (d < D) ? (-wgt * Fs[d] + aj.dir[d] * wgt * aj.invT) : 0.0;
+        mom[d] = 2.0 * Fb[d];
+        rb += -2.0 * Fb[d] * ((d < D) ? ib.vel[3 * m + d] : 0.0);
+        FdotFs += Fm * Fs[d];
+        Fdir += Fm * aj.dir[d];
+    }
+    for (int d = 0; d < 3; ++d) aj.mbar[4 * m + d] = mom[d];
+    aj.mbar[4 * m + 3] = rb;
+    aj.epsbar[m] += -ds * vd * FdotFs + Fdir * ds * vd * aj.invT;
+    aj.dsbar[m] += ((D == 3) ? 2.0 * eps * ds : eps) * (-FdotFs + Fdir * aj.invT);
+    double xb[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < ib.ns[m]; ++k) {
+        const size_t cell = (size_t)ib.scell[m * ib.S + k];
+        const double r = ib.rho[cell];
+        double term = rb * r;
+        for (int d = 0; d < D; ++d) {
+            term += mom[d] * r * ib.u[(size_t)d * ib.ncell + cell];
+            term -= wgt * ib.force[3 * m + d] * aj.fbar[(size_t)d * ib.ncell + cell];
+        }
+        for (int d = 0; d < D; ++d) xb[d] += aj.sddelta[((size_t)m * ib.S + k) * 3 + d] * term;
+    }
+    for (int d = 0; d < D; ++d) aj.xbar[3 * m + d] += xb[d];
+}
+
+// IB adjoint, site side: the interpolation's adjoint spread through the per-site gather lists
+// (no atomics): gbar_v(s) += rho_bar_s + c_v . j_bar_s at every fluid support site.
+template <int Q, class T>
+__global__ void k_adj_ibm_site(IbmArgs ib, const double *mbar, const uint8_t *flags, T *gbar) {
+    using L = Lattice<Q>;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ib.nsites) return;
+    const size_t cell = (size_t)ib.sites[s];
+    if ((flags[cell] & kTypeMask) != kFluid) return;
+    double jb[3] = {0.0, 0.0, 0.0}, rb = 0.0;
+    for (int e = ib.sptr[s]; e < ib.sptr[s + 1]; ++e) {
+        const int code = ib.sent[e];
+        const int m = code / ib.S;
+        const double dl = ib.sdelta[code];
+        for (int d = 0; d < 3; ++d) jb[d] += dl * mbar[4 * m + d];
+        rb += dl * mbar[4 * m + 3];
+    }
+    for (int v = 0; v < Q; ++v)
+        gbar[(size_t)v * ib.ncell + cell] +=
+            T(rb + L::c(v, 0) * jb[0] + L::c(v, 1) * jb[1] + L::c(v, 2) * jb[2]);
+}
+
+// (1/T) dir . sum_m F_m eps_m ds_m vd_m added to acc[0]; one 64-lane wave, fixed order
+static __global__ void k_adj_objective(IbmArgs ib, double dx, double dy, double dz, double invT, double *acc) {
+    const int lane = threadIdx.x;
+    const double dir[3] = {dx, dy, dz};
+    double s = 0.0;
+    for (int m = lane; m < ib.nm; m += 64) {
+        const double w = ib.eps[m] * ((ib.D == 3) ? ib.ds[m] : 1.0) * ib.ds[m];
+        double fd = 0.0;
+        for (int d = 0; d < ib.D; ++d) fd += ib.force[3 * m + d] * dir[d];
+        s += fd * w;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) acc[0] += s * invT;
+}
+
+// acc[k] += sum of part[k*n .. k*n + n), k = 0..nk-1; one block of 256, fixed order
+static __global__ __launch_bounds__(256) void k_adj_sum_partials(const double *part, int n, int nk, double *acc) {
+    for (int k = 0; k < nk; ++k) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < n; i += 256) s += part[(size_t)k * n + i];
+        s = block_sum(s);
+        if (threadIdx.x == 0) acc[k] += s;
+        __syncthreads();
+    }
+}
+
+}  // namespace lumagpu

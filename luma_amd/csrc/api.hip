@@ -179,6 +179,11 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
 
 }  // namespace
 
+namespace lumagpu {
+int step_grid(lumagpu_grid *g, int subcycle) { return apply_step(g, subcycle, nullptr, nullptr); }
+hipError_t outlet_prepass(const lumagpu_grid *g) { return launch_extrap(g); }
+}  // namespace lumagpu
+
 extern "C" {
 
 const char *lumagpu_last_error(void) { return g_last_error.c_str(); }

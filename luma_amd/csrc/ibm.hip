@@ -15,52 +15,9 @@
 
 #include <memory>
 
-#include "fem.hpp"
-#include "ibm_kernels.hpp"
-#include "internal.hpp"
+#include "ibm_system.hpp"
 
 namespace lumagpu {
-
-namespace {
-
-constexpr int kSmax = 27;
-
-struct Marker {
-    double pos[3] = {0, 0, 0}, pos0[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, vel_km1[3] = {0, 0, 0};
-    double eps = 0.0, ds = 1.0, dilation = 1.0, area = 1.0;
-    int ns = 0;
-    int sijk[kSmax][3];
-    double sxyz[kSmax][3];
-    double delta[kSmax];
-};
-
-struct Body {
-    std::vector<Marker> m;
-    int flexible = 0;
-    int first = 0;                  // index of the first marker in the grid-wide marker arrays
-    std::unique_ptr<FemBeam> fem;   // flexible filaments only
-};
-
-// Device buffer of n elements, reallocated only when n changes.  A fresh buffer is zeroed.
-template <class T>
-int dev_resize(T **d, size_t *cap, size_t n, hipStream_t s) {
-    if (*d && *cap == n) return 0;
-    if (*d) (void)hipFree(*d);
-    *d = nullptr;
-    *cap = 0;
-    if (n == 0) return 0;
-    if (hipMalloc(d, n * sizeof(T)) != hipSuccess) return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
-    *cap = n;
-    if (hipMemsetAsync(*d, 0, n * sizeof(T), s) != hipSuccess) return fail(LUMAGPU_EHIP, "IB buffer clear failed");
-    return 0;
-}
-template <class T>
-int dev_upload(T **d, size_t *cap, const std::vector<T> &h, hipStream_t s) {
-    if (int r = dev_resize(d, cap, h.size(), s)) return r;
-    if (!h.empty() && hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
-        return fail(LUMAGPU_EHIP, "IB buffer upload failed");
-    return 0;
-}
 
 // ibm_deltaKernel, :255-272: Roma et al. three-point kernel
 double delta_kernel(double radius, double dilation) {
@@ -70,33 +27,20 @@ double delta_kernel(double radius, double dilation) {
     return (1.0 + std::sqrt(1.0 - 3.0 * std::pow(mag_r, 2))) / 3.0;
 }
 
-}  // namespace
-
-struct IbmSystem {
-    std::vector<Body> bodies;
-    int nm = 0;
-    int S = 9;
-    int initialised = 0;
-    // device copies
-    int *d_scell = nullptr, *d_ns = nullptr, *d_sites = nullptr, *d_sptr = nullptr, *d_sent = nullptr;
-    double *d_sdelta = nullptr, *d_vel = nullptr, *d_eps = nullptr, *d_ds = nullptr;
-    double *d_interp = nullptr, *d_force = nullptr, *d_out = nullptr;
-    size_t cap[11] = {};        // element counts of the device buffers above, in declaration order
-    int nsites = 0;
-    std::vector<int> sites;     // host copy of the distinct support cells
-    std::vector<int> flagged;   // cells whose force and flag the device currently holds
-    int sub_it = 0;             // sub-iterations of the last step
-    double sub_res = 0.0;
-
-    ~IbmSystem() {
-        for (void *p : {(void *)d_scell, (void *)d_ns, (void *)d_sites, (void *)d_sptr, (void *)d_sent,
-                        (void *)d_sdelta, (void *)d_vel, (void *)d_eps, (void *)d_ds, (void *)d_interp,
-                        (void *)d_force, (void *)d_out})
-            if (p) (void)hipFree(p);
+// d delta_kernel / d radius (zero outside the support and at the kinks' one-sided limits)
+double delta_kernel_deriv(double radius, double dilation) {
+    const double mag_r = std::fabs(radius) / dilation;
+    const double sgn = radius < 0.0 ? -1.0 : 1.0;
+    if (mag_r > 1.5) return 0.0;
+    double d;
+    if (mag_r > 0.5) {
+        const double q = 1.0 - mag_r;
+        d = (-3.0 - 3.0 * q / std::sqrt(1.0 - 3.0 * q * q)) / 6.0;
+    } else {
+        d = -mag_r / std::sqrt(1.0 - 3.0 * mag_r * mag_r);
     }
-};
-
-namespace {
+    return sgn * d / dilation;
+}
 
 IbmArgs ibm_args(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
@@ -123,6 +67,8 @@ IbmArgs ibm_args(lumagpu_grid *g) {
     b.sent = s->d_sent;
     return b;
 }
+
+namespace {
 
 // GridUtils::getEnclosingVoxel (serial), GridUtils.cpp:1630-1719
 int enclosing(const lumagpu_grid *g, double x, int axis) {
@@ -205,6 +151,8 @@ void compute_ds(const lumagpu_grid *g, Body &b) {
     }
 }
 
+}  // namespace
+
 // A x = rhs for row-major A: factor the column-major view (A^T) with row pivoting, then solve
 // through its transpose; the operation order of dgetf2 followed by dgetrs('T').
 int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
@@ -244,6 +192,8 @@ int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
         if (piv[j] != j) std::swap(x[j], x[piv[j]]);
     return 0;
 }
+
+namespace {
 
 // ibm_findEpsilon, :773-920: A_IJ = ds_J * sum_s delta_I(s) delta(x_J - x_s) A_I, A eps = 1
 int find_epsilon(const lumagpu_grid *g, Body &b) {
@@ -331,6 +281,7 @@ int upload(lumagpu_grid *g) {
     if (!s->d_out && hipMalloc(&s->d_out, 3 * sizeof(double)) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
     HIP_TRY(hipStreamSynchronize(g->stream));
+    if (order != s->sites) s->sites_dirty = 1;
     s->sites = order;
     s->nsites = (int)order.size();
     return 0;
@@ -372,15 +323,13 @@ void ibm_destroy(lumagpu_grid *g) {
     g->ibm = nullptr;
 }
 
-namespace {
-
 // Interpolate, force and spread at the current supports.  Sites flagged by the previous
 // application and no longer supports are reset first (_LBM_resetForces before every
 // application, ObjectManager_ops_ibm.cpp:120-128 and GridObj_ops_lbm_optimised.cpp:55-58);
 // current supports are overwritten by the spread, which starts from zero.
 int apply_core(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
-    if (!s->flagged.empty()) {
+    if (s->sites_dirty && !s->flagged.empty()) {
         std::vector<char> now(g->ncell, 0);
         for (int c : s->sites) now[c] = 1;
         std::vector<int> gone;
@@ -406,10 +355,15 @@ int apply_core(lumagpu_grid *g) {
     HIP_TRY(hipGetLastError());
     k_ibm_spread<<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
     HIP_TRY(hipGetLastError());
-    for (int c : s->sites) g->hflags[c] |= kForcedBit;
-    s->flagged = s->sites;
+    if (s->sites_dirty) {
+        for (int c : s->sites) g->hflags[c] |= kForcedBit;
+        s->flagged = s->sites;
+        s->sites_dirty = 0;
+    }
     return 0;
 }
+
+namespace {
 
 // ibm_moveBodies (:61-104): structural step of every flexible body from the marker forces,
 // new supports for flexible bodies, then ds and epsilon of all bodies on the grid.

@@ -38,7 +38,7 @@ struct IbmArgs {
 
 // ibm_interpolate (:498-591) and ibm_computeForce (:598-614); one thread per marker, sums in
 // support order.  local_area = 1 (IBMarker.cpp:61).
-__global__ void k_ibm_interp_force(IbmArgs b) {
+static __global__ void k_ibm_interp_force(IbmArgs b) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= b.nm) return;
     const double area = 1.0;
@@ -58,7 +58,7 @@ __global__ void k_ibm_interp_force(IbmArgs b) {
 // ibm_spread (:621-689) as a per-site gather; the site's force starts from the reset value 0
 // (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
 // step kernel adds it in the macro (+F/2) and the Guo term.
-__global__ void k_ibm_spread(IbmArgs b) {
+static __global__ void k_ibm_spread(IbmArgs b) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= b.nsites) return;
     const size_t cell = (size_t)b.sites[s];
@@ -76,7 +76,7 @@ __global__ void k_ibm_spread(IbmArgs b) {
 }
 
 // Clear the force and flag of sites that stop being supports.
-__global__ void k_ibm_clear(IbmArgs b, int n, const int *cells) {
+static __global__ void k_ibm_clear(IbmArgs b, int n, const int *cells) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const size_t cell = (size_t)cells[s];
@@ -87,7 +87,7 @@ __global__ void k_ibm_clear(IbmArgs b, int n, const int *cells) {
 // FP64 sum over one body's markers of force * epsilon * volDepth * ds, the per-marker values
 // LUMA writes to Body_<id>_LD (ObjectManager_ops_io.cpp:98-121).  One 64-wide wave: strided
 // lane sums, then a fixed xor-shuffle tree, so the result does not depend on timing.
-__global__ void k_ibm_body_force(IbmArgs b, int m0, int m1, double *out) {
+static __global__ void k_ibm_body_force(IbmArgs b, int m0, int m1, double *out) {
     const int lane = threadIdx.x;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int m = m0 + lane; m < m1; m += 64) {

@@ -1,0 +1,90 @@
+// ibm_system.hpp -- host-side state of a grid's immersed bodies (markers, supports, device
+// copies), shared by the IB step (ibm.hip) and the discrete adjoint (adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fem.hpp"
+#include "ibm_kernels.hpp"
+#include "internal.hpp"
+
+namespace lumagpu {
+
+constexpr int kSmax = 27;
+
+struct Marker {
+    double pos[3] = {0, 0, 0}, pos0[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, vel_km1[3] = {0, 0, 0};
+    double eps = 0.0, ds = 1.0, dilation = 1.0, area = 1.0;
+    int ns = 0;
+    int sijk[kSmax][3];
+    double sxyz[kSmax][3];
+    double delta[kSmax];
+};
+
+struct Body {
+    std::vector<Marker> m;
+    int flexible = 0;
+    int first = 0;                  // index of the first marker in the grid-wide marker arrays
+    std::unique_ptr<FemBeam> fem;   // flexible filaments only
+};
+
+// Device buffer of n elements, reallocated only when n changes.  A fresh buffer is zeroed.
+template <class T>
+int dev_resize(T **d, size_t *cap, size_t n, hipStream_t s) {
+    if (*d && *cap == n) return 0;
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    *cap = 0;
+    if (n == 0) return 0;
+    if (hipMalloc(d, n * sizeof(T)) != hipSuccess) return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
+    *cap = n;
+    if (hipMemsetAsync(*d, 0, n * sizeof(T), s) != hipSuccess) return fail(LUMAGPU_EHIP, "IB buffer clear failed");
+    return 0;
+}
+template <class T>
+int dev_upload(T **d, size_t *cap, const std::vector<T> &h, hipStream_t s) {
+    if (int r = dev_resize(d, cap, h.size(), s)) return r;
+    if (!h.empty() && hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(LUMAGPU_EHIP, "IB buffer upload failed");
+    return 0;
+}
+
+struct IbmSystem {
+    std::vector<Body> bodies;
+    int nm = 0;
+    int S = 9;
+    int initialised = 0;
+    // device copies
+    int *d_scell = nullptr, *d_ns = nullptr, *d_sites = nullptr, *d_sptr = nullptr, *d_sent = nullptr;
+    double *d_sdelta = nullptr, *d_vel = nullptr, *d_eps = nullptr, *d_ds = nullptr;
+    double *d_interp = nullptr, *d_force = nullptr, *d_out = nullptr;
+    size_t cap[11] = {};        // element counts of the device buffers above, in declaration order
+    int nsites = 0;
+    std::vector<int> sites;     // host copy of the distinct support cells
+    std::vector<int> flagged;   // cells whose force and flag the device currently holds
+    int sites_dirty = 0;        // the support sites changed since the last application
+    int sub_it = 0;             // sub-iterations of the last step
+    double sub_res = 0.0;
+
+    ~IbmSystem() {
+        for (void *p : {(void *)d_scell, (void *)d_ns, (void *)d_sites, (void *)d_sptr, (void *)d_sent,
+                        (void *)d_sdelta, (void *)d_vel, (void *)d_eps, (void *)d_ds, (void *)d_interp,
+                        (void *)d_force, (void *)d_out})
+            if (p) (void)hipFree(p);
+    }
+};
+
+// ibm_deltaKernel (ObjectManager_ops_ibm.cpp:255-272) and its derivative in the radius
+double delta_kernel(double radius, double dilation);
+double delta_kernel_deriv(double radius, double dilation);
+IbmArgs ibm_args(lumagpu_grid *g);
+// interpolate, force and spread at the current supports from the grid's current populations
+int apply_core(lumagpu_grid *g);
+// A x = rhs, row-major A, LAPACK dgetf2 + dgetrs('T') operation order
+int lu_solve(int n, std::vector<double> &a, std::vector<double> &x);
+
+}  // namespace lumagpu

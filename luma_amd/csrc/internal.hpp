@@ -118,5 +118,9 @@ inline StepArgs make_args(const lumagpu_grid *g) {
 int ibm_apply(lumagpu_grid *g);
 void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
+// one LBM_multi_opt of the grid (children first), as lumagpu_step; defined in api.hip
+int step_grid(lumagpu_grid *g, int subcycle);
+// ExtrapolateRight side buffer from the grid's current populations; defined in api.hip
+hipError_t outlet_prepass(const lumagpu_grid *g);
 
 }  // namespace lumagpu

@@ -132,6 +132,30 @@ class GpuGrid:
         call("lumagpu_set_store_macro", self.h, int(bool(on)))
 
     # stepping -------------------------------------------------------------------------
+    def adj_run(self, nsteps, checkpoint_every, direction=(1.0, 0.0, 0.0), backward=True,
+                positions=True, initial=False):
+        """Advance nsteps with checkpoints and, with backward, run the discrete adjoint of them.
+
+        Returns a dict: J (time-averaged dir . IB body force), dJ_domega, dJ_dinlet (derivative
+        with respect to a scale of the inlet profile), dJ_dpos (n_markers x 3, all bodies in
+        marker order) and lambda0 (dJ/df of the initial populations, LUMA layout
+        (N, M, K, Q)) when requested."""
+        res = _abi.AdjResult()
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        nm = sum(self.nmarkers)
+        dpos = np.zeros((nm, 3)) if (backward and positions and nm) else None
+        lam = np.zeros(self.shape + (self.Q,)) if (backward and initial) else None
+        call("lumagpu_adj_run", self.h, int(nsteps), int(checkpoint_every), _dptr(d), int(bool(backward)),
+             C.byref(res), _dptr(dpos) if dpos is not None else None, _dptr(lam) if lam is not None else None)
+        out = {"J": res.J}
+        if backward:
+            out.update(dJ_domega=res.dJ_domega, dJ_dinlet=res.dJ_dinlet)
+            if dpos is not None:
+                out["dJ_dpos"] = dpos
+            if lam is not None:
+                out["lambda0"] = lam
+        return out
+
     def set_zslab(self, on=True):
         """z planes 0 and K-1 become halo planes exchanged around every odd step (one rank of a
         z-sharded periodic box, its own neighbour on both faces)."""

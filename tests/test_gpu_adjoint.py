@@ -1,0 +1,160 @@
+"""Discrete adjoint of the step (lumagpu_adj_run) against central finite differences of the
+forward step, whose FP64 form equals the oracle bit for bit (the other GPU tests): gradients
+of the time-averaged IB drag with respect to the relaxation frequency, the inlet velocity
+scale, the marker positions and the initial populations, in 2D (channel with forced-
+equilibrium inlet, ExtrapolateRight outlet, solid walls, IB cylinder) and 3D (periodic box,
+IB sphere); checkpointing with a partial last segment; FP32-shifted adjoint against FP64.
+Bound: 1e-4 relative (central differences, step sizes in each test)."""
+import numpy as np
+import pytest
+
+from luma_amd import setups
+from luma_amd.grid import GpuGrid
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+
+
+def channel_case():
+    c = setups.channel_2d(48, 24, 8.0, 0.04, 20.0, regularised=False)
+    pos = setups.circle_markers((1.52, 1.49, 0.0), 0.3, c.info["dh"])
+    return c, pos
+
+
+def sphere_case():
+    c = setups.taylor_green((20, 16, 16), dims=3, tau=0.6, u0=0.0)
+    c.u[...] = 0.0
+    c.u[..., 0] = 0.03
+    c.rho[...] = 1.0
+    c.forced = True
+    c.xpos = 0.5 + np.arange(c.N, dtype=float)
+    c.ypos = 0.5 + np.arange(c.M, dtype=float)
+    c.zpos = 0.5 + np.arange(c.K, dtype=float)
+    c.edges = (float(c.N), float(c.M), float(c.K))
+    c.info.update({"dh": 1.0, "origin": (0.0, 0.0, 0.0)})
+    pos = setups.sphere_markers((8.3, 8.1, 7.9), 2.5, 1.0)
+    return c, pos
+
+
+def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None):
+    c = case
+    if omega is not None:
+        import dataclasses
+        c = dataclasses.replace(case, omega=omega)
+    g = GpuGrid.from_case(c, prec, store_macro=True)
+    if c.lattyp is None:
+        g.set_positions(c.xpos, c.ypos, c.zpos)
+    if c.ux_in is not None and inlet_scale != 1.0:
+        g.set_inlet(c.ux_in * inlet_scale, c.uy_in * inlet_scale, c.uz_in * inlet_scale)
+    if f0 is not None:
+        g.upload_f(f0)
+    g.ibm_add_body(pos)
+    g.ibm_initialise()
+    return g
+
+
+def J_of(case, pos, T, **kw):
+    with make_grid(case, pos, **kw) as g:
+        return g.adj_run(T, T, backward=False)["J"]
+
+
+def close(a, b, rtol=RTOL, scale=None):
+    s = max(abs(b), scale or 0.0)
+    return abs(a - b) <= rtol * s
+
+
+@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (sphere_case, 24, 10)])
+def test_adjoint_matches_central_differences(build, T, K):
+    case, pos = build()
+    print(f"[adjoint] {case.name} T={T} K={K} markers={len(pos)}", flush=True)
+    with make_grid(case, pos) as g:
+        f0 = g.download_f()
+        adj = g.adj_run(T, K, positions=True, initial=True)
+    J = adj["J"]
+    assert J != 0.0
+
+    # relaxation frequency
+    h = 1e-5 * case.omega
+    fd = (J_of(case, pos, T, omega=case.omega + h) - J_of(case, pos, T, omega=case.omega - h)) / (2 * h)
+    print("dJ/domega adjoint", adj["dJ_domega"], "fd", fd, flush=True)
+    assert close(adj["dJ_domega"], fd)
+
+    # inlet velocity scale
+    if case.ux_in is not None and np.any(case.lattyp == setups.VELOCITY):
+        h = 1e-5
+        fd = (J_of(case, pos, T, inlet_scale=1 + h) - J_of(case, pos, T, inlet_scale=1 - h)) / (2 * h)
+        print("dJ/dinlet adjoint", adj["dJ_dinlet"], "fd", fd, flush=True)
+        assert close(adj["dJ_dinlet"], fd)
+
+    # initial populations along a random direction on the fluid sites
+    rng = np.random.default_rng(11)
+    fluid = np.ones(case.N * case.M * case.K, bool) if case.lattyp is None else \
+        (case.lattyp == setups.FLUID).reshape(-1)
+    dirn = rng.standard_normal(f0.shape) * fluid.reshape(f0.shape[:3])[..., None]
+    h = 1e-6
+    fd = (J_of(case, pos, T, f0=f0 + h * dirn) - J_of(case, pos, T, f0=f0 - h * dirn)) / (2 * h)
+    pred = float((adj["lambda0"] * dirn).sum())
+    print("dJ/df0 . d adjoint", pred, "fd", fd, flush=True)
+    assert close(pred, fd)
+
+    # marker positions (through delta weights, epsilon and ds)
+    gscale = np.abs(adj["dJ_dpos"]).max()
+    dh = case.info.get("dh", 1.0)
+    for m in (0, len(pos) // 3):
+        for a in range(case.dims):
+            h = 1e-5 * dh
+            pp, pm = pos.copy(), pos.copy()
+            pp[m, a] += h
+            pm[m, a] -= h
+            fd = (J_of(case, pp, T) - J_of(case, pm, T)) / (2 * h)
+            print(f"dJ/dx[{m},{a}] adjoint", adj["dJ_dpos"][m, a], "fd", fd, flush=True)
+            assert close(adj["dJ_dpos"][m, a], fd, rtol=RTOL, scale=1e-2 * gscale)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    case, pos = channel_case()
+    res = []
+    for K in (7, 30, 60):
+        with make_grid(case, pos) as g:
+            res.append(g.adj_run(60, K))
+    for r in res[1:]:
+        assert r["J"] == res[0]["J"]
+        assert r["dJ_domega"] == res[0]["dJ_domega"]
+        assert np.array_equal(r["dJ_dpos"], res[0]["dJ_dpos"])
+
+
+def test_fp32_shifted_adjoint_tracks_fp64():
+    case, pos = channel_case()
+    out = {}
+    for prec in ("fp64", "fp32s"):
+        with make_grid(case, pos, prec=prec) as g:
+            out[prec] = g.adj_run(60, 16)
+    a, b = out["fp32s"], out["fp64"]
+    print("fp32s vs fp64 adjoint", a["dJ_domega"], b["dJ_domega"], a["dJ_dinlet"], b["dJ_dinlet"], flush=True)
+    assert close(a["J"], b["J"], rtol=1e-4)
+    assert close(a["dJ_domega"], b["dJ_domega"], rtol=1e-3)
+    assert close(a["dJ_dinlet"], b["dJ_dinlet"], rtol=1e-3)
+    assert np.abs(a["dJ_dpos"] - b["dJ_dpos"]).max() <= 1e-3 * np.abs(b["dJ_dpos"]).max()
+
+
+def test_torch_autograd_binding():
+    import torch
+
+    from luma_amd.adjoint import DragObjective
+
+    case, pos = channel_case()
+
+    def build(omega, inlet_scale, p):
+        return make_grid(case, p, omega=omega, inlet_scale=inlet_scale)
+
+    om = torch.tensor(case.omega, dtype=torch.float64, requires_grad=True)
+    s = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    p = torch.tensor(pos, dtype=torch.float64, requires_grad=True)
+    J = DragObjective.apply(om, s, p, build, 40, 16)
+    J.backward()
+    with make_grid(case, pos) as g:
+        ref = g.adj_run(40, 16)
+    assert J.item() == ref["J"]
+    assert om.grad.item() == ref["dJ_domega"] and s.grad.item() == ref["dJ_dinlet"]
+    assert np.array_equal(p.grad.numpy(), ref["dJ_dpos"])
