@@ -121,6 +121,95 @@ def run_c3(precision: str, steps: int, sample: int = 10, fields_at=(), geometry=
     return out, {"cd": cd, "cl": cl, "snaps": snaps, "parent": parent, "child": child}
 
 
+# ---- C4: D3Q19 uniform inflow past an IB sphere, adjoint of the time-averaged drag --------
+
+C4_U = 0.03
+C4_RE = 50.0                      # on U and the sphere diameter
+C4_DIAM = 24.0
+C4_CENTRE = (96.0, 64.0, 64.0)
+C4_MARKERS = 1810
+C4_SEED = 20261202
+C4_SIGMA = 0.05                   # marker position noise, cells
+
+
+def c4_markers(seed: int = C4_SEED, n: int = C4_MARKERS, centre=C4_CENTRE, diam: float = C4_DIAM):
+    """Fibonacci sphere of n markers (Body.h:215-228 with n given) with seeded N(0, 0.05 cell)
+    noise on every coordinate (numpy PCG64)."""
+    base = setups.sphere_markers(centre, diam / 2.0, 1.0, n=n)
+    return base + np.random.default_rng(seed).normal(0.0, C4_SIGMA, base.shape)
+
+
+def c4_tau(U: float = C4_U) -> float:
+    return 3.0 * U * C4_DIAM / C4_RE + 0.5
+
+
+def c4_case(U: float = C4_U, tau: float = None, shape=(384, 128, 128)):
+    return setups.uniform_inflow_3d(*shape, U, tau if tau is not None else c4_tau(C4_U))
+
+
+def c4_eval(precision, U, tau, pos, steps, every, backward, shape=(384, 128, 128)):
+    case = c4_case(U, tau, shape)
+    with GpuGrid.from_case(case, precision) as g:
+        g.ibm_add_body(pos)
+        g.ibm_initialise()
+        t0 = time.perf_counter()
+        out = g.adj_run(steps, every, backward=backward, positions=backward, initial=backward)
+        out["wall_s"] = time.perf_counter() - t0
+    if backward:
+        # U enters through the inlet (scale U/U0) and the uniform initial state feq(1, U e_x)
+        c = np.array([setups.lattice_c(19, v) for v in range(19)], dtype=float)
+        w = np.array([setups.lattice_w(19, v) for v in range(19)])
+        dfeq = w * (3.0 * c[:, 0] + 9.0 * (c[:, 0] * U) * c[:, 0] - 3.0 * U)
+        out["dJ_dU"] = out["dJ_dinlet"] / U + float((out.pop("lambda0") * dfeq).sum())
+        out["dJ_dtau"] = -out["dJ_domega"] / (tau * tau)
+    return out
+
+
+def run_c4(precisions=("fp64", "fp32s"), steps=4000, every=200, h=1e-4, fd_markers=(0, 605, 1210),
+           shape=(384, 128, 128)):
+    pos = c4_markers()
+    U, tau = C4_U, c4_tau()
+    rep = {"config": "c4_sphere_adjoint", "shape": list(shape), "markers": len(pos), "U": U, "tau": tau,
+           "steps": steps, "checkpoint_every": every, "fd_h": h}
+    for p in precisions:
+        a = c4_eval(p, U, tau, pos, steps, every, True, shape)
+        rep[p] = {"J": a["J"], "dJ_dU": a["dJ_dU"], "dJ_dtau": a["dJ_dtau"], "wall_s": a["wall_s"],
+                  "dJ_dpos_fd_markers": a["dJ_dpos"][list(fd_markers)].tolist(),
+                  "dJ_dpos_norm": float(np.linalg.norm(a["dJ_dpos"]))}
+        print(f"[c4 {p}] J {a['J']:.9e} dJ/dU {a['dJ_dU']:.9e} dJ/dtau {a['dJ_dtau']:.9e} ({a['wall_s']:.1f} s)",
+              flush=True)
+    # central differences of the FP64 forward
+    J = lambda U_, tau_, pos_: c4_eval("fp64", U_, tau_, pos_, steps, every, False, shape)["J"]
+    fd = {"dJ_dU": (J(U * (1 + h), tau, pos) - J(U * (1 - h), tau, pos)) / (2 * h * U),
+          "dJ_dtau": (J(U, tau * (1 + h), pos) - J(U, tau * (1 - h), pos)) / (2 * h * tau)}
+    print(f"[c4 fd] dJ/dU {fd['dJ_dU']:.9e} dJ/dtau {fd['dJ_dtau']:.9e}", flush=True)
+    pfd = []
+    for m in fd_markers:
+        row = []
+        for a_ in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[m, a_] += h
+            pm[m, a_] -= h
+            row.append((J(U, tau, pp) - J(U, tau, pm)) / (2 * h))
+        pfd.append(row)
+        print(f"[c4 fd] dJ/dx[{m}] {row}", flush=True)
+    fd["dJ_dpos_fd_markers"] = pfd
+    rep["fd_fp64"] = fd
+    ref = rep["fp64"] if "fp64" in rep else None
+    for p in precisions:
+        r = rep[p]
+        gp = np.array(r["dJ_dpos_fd_markers"])
+        scale = np.abs(np.array(pfd)).max()
+        r["rel_err_vs_fd"] = {"dJ_dU": abs(r["dJ_dU"] - fd["dJ_dU"]) / abs(fd["dJ_dU"]),
+                              "dJ_dtau": abs(r["dJ_dtau"] - fd["dJ_dtau"]) / abs(fd["dJ_dtau"]),
+                              "dJ_dpos_max_over_max": float(np.abs(gp - np.array(pfd)).max() / scale)}
+        if ref is not None and p != "fp64":
+            r["rel_err_vs_fp64_adjoint"] = {
+                "dJ_dU": abs(r["dJ_dU"] - ref["dJ_dU"]) / abs(ref["dJ_dU"]),
+                "dJ_dtau": abs(r["dJ_dtau"] - ref["dJ_dtau"]) / abs(ref["dJ_dtau"])}
+    return rep
+
+
 def rel_l2(a, b, mask):
     d = (a - b)[mask]
     return float(np.sqrt((d * d).sum() / (b[mask] * b[mask]).sum()))
@@ -128,7 +217,7 @@ def rel_l2(a, b, mask):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("config", choices=["c1", "c3"])
+    ap.add_argument("config", choices=["c1", "c3", "c4"])
     ap.add_argument("--precision", default="fp64,fp32s")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--sample", type=int, default=10)
@@ -136,6 +225,13 @@ def main(argv=None):
     a = ap.parse_args(argv)
     precs = a.precision.split(",")
     report = {"runs": []}
+    if a.config == "c4":
+        report = run_c4(tuple(precs), steps=a.steps or 4000)
+        print(json.dumps(report), flush=True)
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump(report, fh, indent=1)
+        return 0
     if a.config == "c1":
         steps = a.steps or 20000
         fields = {}

@@ -113,6 +113,7 @@ int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsb
     std::vector<double> A((size_t)n * n, 0.0);
     for (int I = 0; I < n; ++I)
         for (int J = 0; J < n; ++J) {
+            if (far_apart(b.m[I], b.m[J], D, dh)) continue;
             double acc = 0.0;
             for (int k = 0; k < b.m[I].ns; ++k)
                 acc += b.m[I].delta[k] * delta_at(b.m[J].pos, b.m[I].sxyz[k], dh, b.m[J].dilation, D) * b.m[I].area;
@@ -127,7 +128,7 @@ int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsb
         for (int J = 0; J < n; ++J) {
             const Marker &mJ = b.m[J];
             const double Abar = -mu[I] * mJ.eps;
-            if (Abar == 0.0) continue;
+            if (Abar == 0.0 || far_apart(mI, mJ, D, dh)) continue;
             dsbar[J] += Abar * A[(size_t)I * n + J] / mJ.ds;
             for (int k = 0; k < mI.ns; ++k) {
                 double gI[3], gJ[3];

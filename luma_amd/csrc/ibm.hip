@@ -157,7 +157,8 @@ void compute_ds(const lumagpu_grid *g, Body &b) {
 // through its transpose; the operation order of dgetf2 followed by dgetrs('T').
 int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
     auto M = [&](int i, int j) -> double & { return a[(size_t)j * n + i]; };
-    std::vector<int> piv(n);
+    std::vector<int> piv(n), rows;
+    rows.reserve(n);
     for (int j = 0; j < n; ++j) {
         int p = j;
         double best = std::fabs(M(j, j));
@@ -171,11 +172,17 @@ int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
         if (p != j)
             for (int k = 0; k < n; ++k) std::swap(M(j, k), M(p, k));
         const double r = 1.0 / M(j, j);
-        for (int i = j + 1; i < n; ++i) M(i, j) *= r;
+        rows.clear();
+        for (int i = j + 1; i < n; ++i) {
+            M(i, j) *= r;
+            if (M(i, j) != 0.0) rows.push_back(i);
+        }
+        // rank-1 update; a zero multiplier or pivot-row entry changes nothing but the sign of a
+        // zero, so only the nonzero pairs are visited (the IB systems are sparse and banded)
         for (int k = j + 1; k < n; ++k) {
             const double t = M(j, k);
             if (t != 0.0)
-                for (int i = j + 1; i < n; ++i) M(i, k) -= M(i, j) * t;
+                for (int i : rows) M(i, k) -= M(i, j) * t;
         }
     }
     for (int j = 0; j < n; ++j) {
@@ -204,6 +211,10 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
         const Marker &mI = b.m[I];
         for (int J = 0; J < n; ++J) {
             const Marker &mJ = b.m[J];
+            // every support of I lies within 1.5 dilations of I per axis, so a marker J further
+            // than 1.5 + 1.5 dilations (+ margin) away on some axis has a zero delta at all of
+            // them; skipping those exact zero terms leaves the sum unchanged
+            if (far_apart(mI, mJ, D, dh)) continue;
             double acc = 0.0;
             for (int k = 0; k < mI.ns; ++k) {
                 double dJ;

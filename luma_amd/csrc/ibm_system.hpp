@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <memory>
 #include <string>
 #include <vector>
@@ -77,6 +78,14 @@ struct IbmSystem {
             if (p) (void)hipFree(p);
     }
 };
+
+// true when no support of marker I can carry a nonzero delta weight of marker J
+inline bool far_apart(const Marker &mI, const Marker &mJ, int D, double dh) {
+    const double lim = (1.5 * mI.dilation + 1.5 * mJ.dilation + 0.01) * dh;
+    for (int a = 0; a < D; ++a)
+        if (std::fabs(mJ.pos[a] - mI.pos[a]) > lim) return true;
+    return false;
+}
 
 // ibm_deltaKernel (ObjectManager_ops_ibm.cpp:255-272) and its derivative in the radius
 double delta_kernel(double radius, double dilation);

@@ -20,6 +20,22 @@ import numpy as np
 SOLID, FLUID, REFINED, TL2C, TL2F, BFL, VELOCITY, PRESSURE, SLIP, EXTRAP_RIGHT = range(10)
 CS = 1.0 / math.sqrt(3.0)
 
+# velocity sets in LUMA's order (stdafx.cpp:81-153); rest particle last
+_C9 = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (1, 1, 0), (-1, -1, 0), (1, -1, 0), (-1, 1, 0), (0, 0, 0)]
+_W9 = [1 / 9] * 4 + [1 / 36] * 4 + [4 / 9]
+_C19 = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0), (1, -1, 0),
+        (-1, 1, 0), (0, 1, 1), (0, -1, -1), (0, 1, -1), (0, -1, 1), (1, 0, 1), (-1, 0, -1), (-1, 0, 1), (1, 0, -1),
+        (0, 0, 0)]
+_W19 = [1 / 18] * 6 + [1 / 36] * 12 + [1 / 3]
+
+
+def lattice_c(Q: int, v: int):
+    return (_C9 if Q == 9 else _C19)[v]
+
+
+def lattice_w(Q: int, v: int) -> float:
+    return (_W9 if Q == 9 else _W19)[v]
+
 
 @dataclass
 class Case:
@@ -201,6 +217,25 @@ def velocity_corner_box(N: int, M: int, K: int = 1, dims: int = 2, u_in: float =
     return Case(name=f"velocity_corners{dims}d_{N}x{M}x{K}", dims=dims, Q=Q, N=N, M=M, K=K, omega=1.0 / tau,
                 rho=rho, u=u, lattyp=lt, xpos=xpos, ypos=ypos, zpos=zpos, ux_in=ux_in, uy_in=uy_in,
                 uz_in=uz_in, regularised=True, edges=edges, wall_thk=thk, info={"dh": dh})
+
+
+def uniform_inflow_3d(N: int, M: int, K: int, U: float, tau: float, dh: float = 1.0) -> Case:
+    """D3Q19 box with a uniform inflow: the x = 0 plane is an eVelocity inlet with forced
+    equilibrium (L_REGULARISED_BOUNDARIES off, GridObj_ops_lbm_optimised.cpp:254-270) at
+    u = (U, 0, 0), the x = N-1 plane eExtrapolateRight, periodic in y and z; uniform initial
+    flow U, rho = 1; cell-centred positions with spacing dh; forced (IB) grid."""
+    lt = np.full((N, M, K), FLUID, dtype=np.int32)
+    lt[0] = VELOCITY
+    lt[N - 1] = EXTRAP_RIGHT
+    u = np.zeros((N, M, K, 3))
+    u[..., 0] = U
+    rho = np.ones((N, M, K))
+    edges = (N * dh, M * dh, K * dh)
+    return Case(name=f"inflow3d_{N}x{M}x{K}", dims=3, Q=19, N=N, M=M, K=K, omega=1.0 / tau, rho=rho, u=u,
+                lattyp=lt, xpos=(np.arange(N) + 0.5) * dh, ypos=(np.arange(M) + 0.5) * dh,
+                zpos=(np.arange(K) + 0.5) * dh, ux_in=np.full(M, U), uy_in=np.zeros(M), uz_in=np.zeros(M),
+                regularised=False, forced=True, edges=edges, wall_thk=(0.0,) * 6,
+                info={"dh": dh, "origin": (0.0, 0.0, 0.0), "U": U})
 
 
 def body_force_channel(N: int, M: int, K: int = 1, dims: int = 2, tau: float = 0.8,

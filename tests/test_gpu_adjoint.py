@@ -37,6 +37,12 @@ def sphere_case():
     return c, pos
 
 
+def inflow_case():
+    c = setups.uniform_inflow_3d(24, 14, 14, 0.03, 0.6)
+    pos = setups.sphere_markers((9.2, 7.1, 6.9), 2.5, 1.0)
+    return c, pos
+
+
 def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None):
     c = case
     if omega is not None:
@@ -64,7 +70,7 @@ def close(a, b, rtol=RTOL, scale=None):
     return abs(a - b) <= rtol * s
 
 
-@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (sphere_case, 24, 10)])
+@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (sphere_case, 24, 10), (inflow_case, 30, 12)])
 def test_adjoint_matches_central_differences(build, T, K):
     case, pos = build()
     print(f"[adjoint] {case.name} T={T} K={K} markers={len(pos)}", flush=True)

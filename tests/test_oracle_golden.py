@@ -60,3 +60,22 @@ def test_equilibrium_moments_closed_form(Q, D):
     pi = np.einsum("nv,va,vb->nab", f, c, c)
     ref = r[:, None, None] * (np.eye(D)[None] / 3.0 + uu[:, :, None] * uu[:, None, :])
     assert np.allclose(pi, ref, rtol=1e-12, atol=1e-15)
+
+
+def test_setup_lattice_tables_match_the_oracle():
+    from luma_amd import setups
+    for Q in (9, 19):
+        c, w, _ = lattice(Q)
+        for v in range(Q):
+            assert tuple(c[v]) == setups.lattice_c(Q, v)
+            assert w[v] == setups.lattice_w(Q, v)
+
+
+def test_c4_marker_fixture_is_reproduced():
+    # the committed, seeded marker positions of the sphere-adjoint configuration
+    from luma_amd.cases import c4_markers
+    fix = np.load(Path(__file__).resolve().parent / "golden" / "c4_sphere_markers.npy")
+    pos = c4_markers()
+    assert fix.shape == (1810, 3) and np.array_equal(pos, fix)
+    r = np.linalg.norm(pos - np.array([96.0, 64.0, 64.0]), axis=1)
+    assert abs(r.mean() - 12.0) < 0.01 and 0.04 < (r - 12.0).std() < 0.06
