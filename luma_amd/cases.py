@@ -210,6 +210,74 @@ def run_c4(precisions=("fp64", "fp32s"), steps=4000, every=200, h=1e-4, fd_marke
     return rep
 
 
+# ---- C5: FEM-coupled flexible filament in a D2Q9 channel with a refined patch -------------
+
+C5_RES = 100.0                    # coarse cells per filament length (LUMA's unit length)
+C5_UMAX = 0.05
+C5_RE = 200.0                     # on the mean inlet velocity and the filament length
+C5_PATCH = (450, 800, 180, 331)   # coarse cells covered by the fine level
+C5_LEAD = (500.0, 256.0)          # clamped leading end, coarse cells
+C5_HEIGHT = 0.01                  # filament thickness, units of length
+C5_ELEMENTS = 40
+C5_SEED = 20261202
+
+
+def c5_tip_deflection(seed: int = C5_SEED) -> float:
+    """Seeded initial tip deflection, U(-2, 2) fine cells."""
+    return float(np.random.default_rng(seed).uniform(-2.0, 2.0))
+
+
+def c5_filament(nx: int = 2048, ny: int = 512, patch=C5_PATCH):
+    """Coarse 2048 x 512 channel (parabolic inlet, outlet, solid walls) with a 2:1 fine patch
+    around a clamped filament of 200 fine cells (201 markers, 40 corotational elements),
+    density ratio 10, dimensionless bending stiffness 1e-3.  LUMA starts the filament straight
+    (FEMBody.cpp:95-100); the seeded initial tip deflection is given here as the filament's
+    initial angle."""
+    parent = setups.channel_2d(nx, ny, C5_RES, C5_UMAX, C5_RE)
+    dh, dt = parent.info["dh"], parent.info["dt"]
+    child = setups.add_refined_region(parent, patch, dh, dt)
+    setups.velocity_field_on(child, parent)
+    dhf = child.info["dh"]
+    length = 1.0
+    tip = c5_tip_deflection()
+    angle = math.degrees(math.asin(tip * dhf / length))
+    density, E = setups.filament_material(length, C5_HEIGHT, dhf, mass_ratio=10.0, bending=1e-3)
+    fil = {"start": (C5_LEAD[0] * dh, C5_LEAD[1] * dh + 0.0013, 0.0), "length": length, "height": C5_HEIGHT,
+           "angle": angle, "elements": C5_ELEMENTS, "density": density, "E": E, "tip_cells": tip}
+    return parent, child, fil
+
+
+def run_c5(precision: str, steps: int, sample: int = 1):
+    parent, child, fil = c5_filament()
+    tips, its = [], []
+    with GpuGrid.from_case(parent, precision) as gp, GpuGrid.from_case(child, precision) as gc:
+        gp.add_child(gc)
+        b = gc.ibm_add_filament(fil["start"], fil["length"], fil["height"], fil["angle"], fil["elements"], True,
+                                fil["density"], fil["E"])
+        gc.ibm_initialise()
+        t0 = time.perf_counter()
+        for s in range(1, steps + 1):
+            gp.LBM_multi_opt()
+            its.append(gc.ibm_subiterations()[0])
+            if s % sample == 0:
+                tips.append(gc.ibm_markers(b)["pos"][-1, 1])
+            if s % 2000 == 0:
+                print(f"[c5 {precision}] coarse step {s} tip y {tips[-1]:.6f} sub-iterations {its[-1]}", flush=True)
+        gp.sync()
+        wall = time.perf_counter() - t0
+    tips = np.array(tips)
+    dhf = child.info["dh"]
+    y0 = fil["start"][1]
+    tail = (tips[len(tips) // 2:] - y0) / dhf
+    cells = parent.N * parent.M + 2 * child.N * child.M
+    return {"config": "c5_filament", "precision": precision, "coarse_steps": steps, "fine_steps": 2 * steps,
+            "markers": int(np.floor(fil["length"] / dhf)) + 1, "elements": fil["elements"],
+            "initial_tip_cells": fil["tip_cells"], "tau_coarse": 1.0 / parent.omega, "tau_fine": 1.0 / child.omega,
+            "tip_amplitude_cells": float((tail.max() - tail.min()) / 2), "tip_mean_cells": float(tail.mean()),
+            "mean_subiterations": float(np.mean(its)), "wall_s": wall, "ms_per_coarse_step": 1e3 * wall / steps,
+            "mlups": cells * steps / wall / 1e6}, tips
+
+
 def rel_l2(a, b, mask):
     d = (a - b)[mask]
     return float(np.sqrt((d * d).sum() / (b[mask] * b[mask]).sum()))
@@ -217,7 +285,7 @@ def rel_l2(a, b, mask):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("config", choices=["c1", "c3", "c4"])
+    ap.add_argument("config", choices=["c1", "c3", "c4", "c5"])
     ap.add_argument("--precision", default="fp64,fp32s")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--sample", type=int, default=10)
@@ -225,6 +293,24 @@ def main(argv=None):
     a = ap.parse_args(argv)
     precs = a.precision.split(",")
     report = {"runs": []}
+    if a.config == "c5":
+        steps = a.steps or 100000
+        series = {}
+        for p in precs:
+            r, tips = run_c5(p, steps)
+            series[p] = tips
+            report["runs"].append(r)
+            print(json.dumps(r), flush=True)
+        if "fp64" in series:
+            dhf = c5_filament()[1].info["dh"]
+            for p in precs:
+                if p != "fp64":
+                    report[f"tip_max_diff_cells_{p}_vs_fp64"] = float(np.abs(series[p] - series["fp64"]).max() / dhf)
+        print(json.dumps({k: v for k, v in report.items() if k != "runs"}), flush=True)
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump(report, fh, indent=1)
+        return 0
     if a.config == "c4":
         report = run_c4(tuple(precs), steps=a.steps or 4000)
         print(json.dumps(report), flush=True)

@@ -137,7 +137,21 @@ struct Nbr {
     __device__ __forceinline__ size_t at(int dx, int dy, int dz, int nx, int ny) const {
         return cell_of(xs[dx + 1], ys[dy + 1], zs[dz + 1], nx, ny);
     }
+    __device__ __forceinline__ uint32_t at32(int dx, int dy, int dz, int nx, int ny) const {
+        return (uint32_t)xs[dx + 1] + (uint32_t)nx * ((uint32_t)ys[dy + 1] + (uint32_t)ny * (uint32_t)zs[dz + 1]);
+    }
 };
+
+// Element idx of a population plane.  With O32 the byte offset is a 32-bit value added to the
+// plane's (wave-uniform) base, which lets every load and store of a site share one offset
+// register (scalar base + vector offset addressing); valid while a plane is below 4 GiB.
+template <class S, bool O32>
+__device__ __forceinline__ S *elem_at(S *plane, size_t idx) {
+    if constexpr (O32)
+        return reinterpret_cast<S *>(reinterpret_cast<char *>(plane) + (uint32_t)((uint32_t)idx * (uint32_t)sizeof(S)));
+    else
+        return plane + idx;
+}
 
 // ----------------------------------------------------------------------------------------
 // exact (LUMA-order) per-site arithmetic
@@ -205,16 +219,57 @@ __device__ __forceinline__ T guo_exact(int v, const T *u, const double *F, T ome
     return fi;
 }
 
-// _LBM_collide_opt (BGK), :765-790; F == nullptr means no force term
+// _LBM_collide_opt (BGK), :765-790, one population at a time; F == nullptr means no force
+// term.  Kernels that store each population as soon as it is collided keep only the incoming
+// set live.
+template <int Q, class T>
+struct CollideExact {
+    T rho, u[3], omega, cs2, den2;
+    const double *F;
+    __device__ __forceinline__ T operator()(int v, T fin) const {
+        const T feq = feq_exact<Q, T>(v, rho, u, cs2, den2);
+        if (F) return fin + (omega * (feq - fin) + guo_exact<Q, T>(v, u, F, omega, cs2));
+        return fin + omega * (feq - fin);
+    }
+    // v and its opposite o = v + 1 (LUMA lists opposites in pairs), bit for bit the two
+    // single-population results: A(o) = -A(v) and B(o) = B(v) exactly (the same products,
+    // negated or not), and division is sign-symmetric, so the pair shares its two divisions;
+    // a zero of either sign added to 1.0 gives 1.0.
+    template <int V>
+    __device__ __forceinline__ void pair(T fin_v, T fin_o, T &out_v, T &out_o) const {
+        using L = Lattice<Q>;
+        static_assert(L::opp(V) == V + 1, "opposites must be adjacent");
+        constexpr int c0 = L::c(V, 0), c1 = L::c(V, 1), c2 = L::c(V, 2);
+        T A, B;
+        if constexpr (L::D == 3) {
+            A = (T(c0) * u[0]) + (T(c1) * u[1]) + (T(c2) * u[2]);
+            B = (T(c0 * c0) - cs2) * (u[0] * u[0]) + (T(c1 * c1) - cs2) * (u[1] * u[1]) +
+                (T(c2 * c2) - cs2) * (u[2] * u[2]) + T(2 * c0 * c1) * u[0] * u[1] +
+                T(2 * c0 * c2) * u[0] * u[2] + T(2 * c1 * c2) * u[1] * u[2];
+        } else {
+            A = (T(c0) * u[0]) + (T(c1) * u[1]);
+            B = (T(c0 * c0) - cs2) * (u[0] * u[0]) + (T(c1 * c1) - cs2) * (u[1] * u[1]) +
+                T(2 * c0 * c1) * u[0] * u[1];
+        }
+        const T a = A / cs2, b = B / den2;
+        const T feq_v = rho * T(L::w(V)) * (T(1.0) + a + b);
+        const T feq_o = rho * T(L::w(V + 1)) * (T(1.0) + (-a) + b);
+        if (F) {
+            out_v = fin_v + (omega * (feq_v - fin_v) + guo_exact<Q, T>(V, u, F, omega, cs2));
+            out_o = fin_o + (omega * (feq_o - fin_o) + guo_exact<Q, T>(V + 1, u, F, omega, cs2));
+        } else {
+            out_v = fin_v + omega * (feq_v - fin_v);
+            out_o = fin_o + omega * (feq_o - fin_o);
+        }
+    }
+};
+
 template <int Q, class T>
 __device__ __forceinline__ void collide_exact(const T *fin, T rho, const T *u, const double *F,
                                               T omega, T cs2, T den2, T *fout) {
+    const CollideExact<Q, T> c{rho, {u[0], u[1], u[2]}, omega, cs2, den2, F};
 #pragma unroll
-    for (int v = 0; v < Q; ++v) {
-        const T feq = feq_exact<Q, T>(v, rho, u, cs2, den2);
-        if (F) fout[v] = fin[v] + (omega * (feq - fin[v]) + guo_exact<Q, T>(v, u, F, omega, cs2));
-        else fout[v] = fin[v] + omega * (feq - fin[v]);
-    }
+    for (int v = 0; v < Q; ++v) fout[v] = c(v, fin[v]);
 }
 
 // _LBM_regularised_opt for a velocity site on a face (edgeCount == 1), :313-510.
@@ -370,36 +425,61 @@ __device__ __forceinline__ float mul_split(Split s, float x) { return fmaf(s.hi,
 // BGK on deviations, with rho = rho0 + drho:
 //   feq_v - w_v rho0 = w_v (drho + rho P_v),  P_v = 3 c.u + 4.5 (c.u)^2 - 1.5 u.u,
 // evaluated as w_v (drho + rho0 P_v + drho P_v) so rho itself is never rounded;
-// Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).
+// Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).  One population at a time.
 template <int Q>
-__device__ __forceinline__ void collide_shift(const float *fin, float drho, float rho0,
-                                              const float *u, const double *F, Split omega,
-                                              float *fout) {
-    using L = Lattice<Q>;
-    const float usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    float Fx = 0.f, Fy = 0.f, Fz = 0.f, uF = 0.f;
-    const float gpre = 1.0f - 0.5f * (omega.hi + omega.lo);
-    if (F) {
-        Fx = float(F[0]);
-        Fy = float(F[1]);
-        if constexpr (L::D == 3) Fz = float(F[2]);
-        uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+struct CollideShift {
+    float drho, rho0, u[3], usq15, Fx, Fy, Fz, uF, gpre;
+    Split omega;
+    bool forced;
+    __device__ __forceinline__ CollideShift(float drho_, float rho0_, const float *u_, const double *F,
+                                            Split om) {
+        using L = Lattice<Q>;
+        drho = drho_;
+        rho0 = rho0_;
+        u[0] = u_[0];
+        u[1] = u_[1];
+        u[2] = u_[2];
+        omega = om;
+        usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        gpre = 1.0f - 0.5f * (om.hi + om.lo);
+        forced = F != nullptr;
+        Fx = Fy = Fz = uF = 0.f;
+        if (forced) {
+            Fx = float(F[0]);
+            Fy = float(F[1]);
+            if constexpr (L::D == 3) Fz = float(F[2]);
+            uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+        }
     }
-#pragma unroll
-    for (int v = 0; v < Q; ++v) {
+    template <int V>
+    __device__ __forceinline__ void pair(float fin_v, float fin_o, float &out_v, float &out_o) const {
+        out_v = (*this)(V, fin_v);
+        out_o = (*this)(V + 1, fin_o);
+    }
+    __device__ __forceinline__ float operator()(int v, float fin) const {
+        using L = Lattice<Q>;
         const float cx = float(L::c(v, 0)), cy = float(L::c(v, 1)), cz = float(L::c(v, 2));
         const Split w = split_of(L::w(v));
         const float cu = cx * u[0] + cy * u[1] + cz * u[2];
         const float P = 3.0f * cu + 4.5f * cu * cu - usq15;
         const float feq = mul_split(w, drho + (rho0 * P + drho * P));
-        const float dlt = feq - fin[v];
-        float o = fmaf(omega.hi, dlt, fmaf(omega.lo, dlt, fin[v]));
-        if (F) {
+        const float dlt = feq - fin;
+        float o = fmaf(omega.hi, dlt, fmaf(omega.lo, dlt, fin));
+        if (forced) {
             const float cF = cx * Fx + cy * Fy + cz * Fz;
             o += gpre * mul_split(w, 3.0f * (cF - uF) + 9.0f * cu * cF);
         }
-        fout[v] = o;
+        return o;
     }
+};
+
+template <int Q>
+__device__ __forceinline__ void collide_shift(const float *fin, float drho, float rho0,
+                                              const float *u, const double *F, Split omega,
+                                              float *fout) {
+    const CollideShift<Q> c(drho, rho0, u, F, omega);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) fout[v] = c(v, fin[v]);
 }
 
 // equilibrium of (rho, u) in the storage convention of P (absolute or shifted)
@@ -413,21 +493,20 @@ __device__ __forceinline__ typename P::T feq_store(int v, double rho, const doub
 
 // ----------------------------------------------------------------------------------------
 // one-site update shared by all step kernels: macro, force, collide.
+// make_collider computes the site's macroscopic values (rho_o, u_o in FP64 for the macro
+// arrays) and returns the collision of one population at a time.
 template <int Q, class P>
-__device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *fout,
-                                            const double *F, const StepArgs &a, double &rho_o,
-                                            double *u_o, bool collide) {
+__device__ __forceinline__ auto make_collider(const typename P::T *fin, const double *F, const StepArgs &a,
+                                              double &rho_o, double *u_o) {
     using T = typename P::T;
     if constexpr (!P::kShift) {
         T rho, u[3];
         macro_exact<Q, T>(fin, F, a.fz_compat != 0, rho, u);
-        if (collide) collide_exact<Q, T>(fin, rho, u, F, T(a.omega), T(a.cs2), T(a.den2), fout);
-        else
-            for (int v = 0; v < Q; ++v) fout[v] = fin[v];
         rho_o = rho;
         u_o[0] = u[0];
         u_o[1] = u[1];
         u_o[2] = u[2];
+        return CollideExact<Q, T>{rho, {u[0], u[1], u[2]}, T(a.omega), T(a.cs2), T(a.den2), F};
     } else {
         float drho, j[3], u[3];
         macro_shift<Q>(fin, F, a.fz_compat != 0, drho, j);
@@ -439,14 +518,21 @@ __device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *f
         u[0] = (j[0] - j[0] * q) * r0inv;
         u[1] = (j[1] - j[1] * q) * r0inv;
         u[2] = (j[2] - j[2] * q) * r0inv;
-        if (collide) collide_shift<Q>(fin, drho, rho0, u, F, Split{a.om_hi, a.om_lo}, fout);
-        else
-            for (int v = 0; v < Q; ++v) fout[v] = fin[v];
         rho_o = double(float(a.rho0) + drho);
         u_o[0] = u[0];
         u_o[1] = u[1];
         u_o[2] = u[2];
+        return CollideShift<Q>(drho, rho0, u, F, Split{a.om_hi, a.om_lo});
     }
+}
+
+template <int Q, class P>
+__device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *fout,
+                                            const double *F, const StepArgs &a, double &rho_o,
+                                            double *u_o, bool collide) {
+    const auto col = make_collider<Q, P>(fin, F, a, rho_o, u_o);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) fout[v] = collide ? col(v, fin[v]) : fin[v];
 }
 
 template <int D>
@@ -459,8 +545,16 @@ __device__ __forceinline__ void store_macro(const StepArgs &a, size_t cell, doub
 
 // ----------------------------------------------------------------------------------------
 // periodic box, no labels: the metric kernel
-template <int Q, class P, bool ODD, bool STORE>
-__global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
+// Minimum waves per SIMD asked of the register allocator: the FP64 local (even) step fits five
+// without spilling; every other variant is left to the compiler.
+template <class P, bool ODD, bool STORE>
+constexpr int periodic_waves() {
+    return (!P::kShift && !ODD && !STORE) ? 5 : 1;
+}
+
+template <int Q, class P, bool ODD, bool STORE, bool O32 = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(periodic_waves<P, ODD, STORE>())))
+void k_aa_periodic(StepArgs a) {
     using L = Lattice<Q>;
     using S = typename P::S;
     using T = typename P::T;
@@ -470,30 +564,44 @@ __global__ __launch_bounds__(256) void k_aa_periodic(StepArgs a) {
     S *__restrict__ f = static_cast<S *>(a.f);
     const size_t ps = a.ps;
     const int nx = a.nx, ny = a.ny;
-    const size_t cell = cell_of(x, y, z, nx, ny);
+    const size_t cell = O32 ? (size_t)((uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z))
+                            : cell_of(x, y, z, nx, ny);
     Nbr nb;
     if constexpr (ODD) nb.init(x, y, z, nx, ny, a.nz);
+    auto nbr = [&](int dx, int dy, int dz) -> size_t {
+        if constexpr (O32) return nb.at32(dx, dy, dz, nx, ny);
+        else return nb.at(dx, dy, dz, nx, ny);
+    };
 
-    T fin[Q], fout[Q];
+    T fin[Q];
     static_for<Q>([&](auto vc) {
         constexpr int v = decltype(vc)::value;
         constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD)
-            fin[v] = ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + nb.at(-cx, -cy, -cz, nx, ny));
+            fin[v] = ld<P, cx == 0>(elem_at<S, O32>(f + (size_t)L::opp(v) * ps, nbr(-cx, -cy, -cz)));
         else
-            fin[v] = ld<P>(f + (size_t)v * ps + cell);
+            fin[v] = ld<P>(elem_at<S, O32>(f + (size_t)v * ps, cell));
     });
     double rho, u[3];
-    site_update<Q, P>(fin, fout, nullptr, a, rho, u, true);
+    const auto col = make_collider<Q, P>(fin, nullptr, a, rho, u);
     if constexpr (STORE) store_macro<L::D>(a, cell, rho, u);
-    static_for<Q>([&](auto vc) {
+    // collide and store an opposite pair at a time: only the incoming set stays live
+    auto put = [&](auto vc, T o) {
         constexpr int v = decltype(vc)::value;
         constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD)
-            st<P, cx == 0>(f + (size_t)v * ps + nb.at(cx, cy, cz, nx, ny), fout[v]);
+            st<P, cx == 0>(elem_at<S, O32>(f + (size_t)v * ps, nbr(cx, cy, cz)), o);
         else
-            st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
+            st<P>(elem_at<S, O32>(f + (size_t)L::opp(v) * ps, cell), o);
+    };
+    static_for<(Q - 1) / 2>([&](auto pc) {
+        constexpr int v = 2 * decltype(pc)::value;
+        T ov, oo;
+        col.template pair<v>(fin[v], fin[v + 1], ov, oo);
+        put(std::integral_constant<int, v>{}, ov);
+        put(std::integral_constant<int, v + 1>{}, oo);
     });
+    put(std::integral_constant<int, Q - 1>{}, col(Q - 1, fin[Q - 1]));
 }
 
 // Halo exchange of a z-slab (planes 1..n interior, 0 and n+1 halo; n = nz - 2) with itself as

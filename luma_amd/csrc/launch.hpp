@@ -113,6 +113,15 @@ hipError_t launch_step_q(const LaunchSpec &ls, const StepArgs &a, hipStream_t s)
             if (ls.store) k_aa_masked<Q, P, false, true><<<g, b, 0, s>>>(a);
             else k_aa_masked<Q, P, false, false><<<g, b, 0, s>>>(a);
         }
+    } else if (a.ncell * sizeof(typename P::S) < (size_t(1) << 32)) {
+        // population planes below 4 GiB: 32-bit offsets from the plane bases
+        if (ls.odd) {
+            if (ls.store) k_aa_periodic<Q, P, true, true, true><<<g, b, 0, s>>>(a);
+            else k_aa_periodic<Q, P, true, false, true><<<g, b, 0, s>>>(a);
+        } else {
+            if (ls.store) k_aa_periodic<Q, P, false, true, true><<<g, b, 0, s>>>(a);
+            else k_aa_periodic<Q, P, false, false, true><<<g, b, 0, s>>>(a);
+        }
     } else {
         if (ls.odd) {
             if (ls.store) k_aa_periodic<Q, P, true, true><<<g, b, 0, s>>>(a);
