@@ -193,3 +193,35 @@ def test_flexible_filament_fsi_fp64_bitwise():
         assert np.abs(a["pos"] - setups.filament_markers(start, length, 8.0, dh)).max() > 0.0
         compare_grids(c, gg, og, "filament")
         print("sub-iterations per step", its)
+
+
+def test_flexible_filament_on_refined_level_fp64_bitwise():
+    # the C5 arrangement at small size: clamped FEM filament on the fine level of a two-level
+    # channel, seeded initial angle, FSI sub-iterations every fine step
+    parent = setups.channel_2d(64, 32, 8.0, 0.05, 100.0)
+    child = setups.add_refined_region(parent, (10, 41, 8, 23), parent.info["dh"], parent.info["dt"])
+    setups.velocity_field_on(child, parent)
+    cdh, cdt, cdm = child.info["dh"], child.info["dt"], child.info["dm"]
+    length, height = 1.0, 0.05
+    density, E = setups.filament_material(length, height, cdh, mass_ratio=10.0, bending=1e-2)
+    start = (2.0, 2.0 + 0.013, 0.0)
+    print(f"[ibm] refined filament, child {child.N}x{child.M}", flush=True)
+    op, oc = oracle_grid(parent), oracle_grid(child)
+    op.add_child(oc)
+    ib = OracleIBM(oc, cdh, child.info["origin"])
+    ib.add_filament(start, length, height, cdh, 2.0, 8, True, density, E, cdt, cdm)
+    ib.initialise()
+    with gpu_grid(parent) as gp, gpu_grid(child) as gc:
+        gp.add_child(gc)
+        gc.ibm_add_filament(start, length, height, 2.0, 8, True, density, E)
+        gc.ibm_initialise()
+        for _ in range(12):
+            op.step()
+            gp.LBM_multi_opt()
+            assert gc.ibm_subiterations()[0] == ib.subiterations()[0]
+        assert ib.error() == 0
+        a, b = gc.ibm_markers(0), ib.markers(0)
+        for key in ("pos", "vel", "force", "eps", "ds"):
+            assert np.array_equal(a[key], b[key]), (key, np.abs(a[key] - b[key]).max())
+        compare_grids(parent, gp, op, "parent")
+        compare_grids(child, gc, oc, "child")
