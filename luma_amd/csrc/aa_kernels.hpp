@@ -526,13 +526,24 @@ __device__ __forceinline__ auto make_collider(const typename P::T *fin, const do
     }
 }
 
+// all populations through a collider, an opposite pair at a time (shared FP64 divisions)
+template <int Q, class C, class T>
+__device__ __forceinline__ void collide_pairs(const C &col, const T *fin, T *fout) {
+    static_for<(Q - 1) / 2>([&](auto pc) {
+        constexpr int v = 2 * decltype(pc)::value;
+        col.template pair<v>(fin[v], fin[v + 1], fout[v], fout[v + 1]);
+    });
+    fout[Q - 1] = col(Q - 1, fin[Q - 1]);
+}
+
 template <int Q, class P>
 __device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *fout,
                                             const double *F, const StepArgs &a, double &rho_o,
                                             double *u_o, bool collide) {
     const auto col = make_collider<Q, P>(fin, F, a, rho_o, u_o);
-#pragma unroll
-    for (int v = 0; v < Q; ++v) fout[v] = collide ? col(v, fin[v]) : fin[v];
+    if (collide) collide_pairs<Q>(col, fin, fout);
+    else
+        for (int v = 0; v < Q; ++v) fout[v] = fin[v];
 }
 
 template <int D>
@@ -778,17 +789,22 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
         T rr, uu[3];
         velocity_site<Q, P>(a, fl, y, fin, rr, uu);
         if constexpr (!P::kShift) {
-            collide_exact<Q, T>(fin, rr, uu, F, T(a.omega), T(a.cs2), T(a.den2), fout);
+            const CollideExact<Q, T> col{rr, {uu[0], uu[1], uu[2]}, T(a.omega), T(a.cs2), T(a.den2), F};
+            collide_pairs<Q>(col, fin, fout);
         } else {
-            collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
+            const CollideShift<Q> col(rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo});
+            collide_pairs<Q>(col, fin, fout);
         }
         rho = rr;
         u[0] = uu[0];
         u[1] = uu[1];
         u[2] = uu[2];
     } else {
+        const auto col = make_collider<Q, P>(fin, F, a, rho, u);
         // TL2C sites are streamed but not collided (:144)
-        site_update<Q, P>(fin, fout, F, a, rho, u, ty != kTL2C);
+        if (ty != kTL2C) collide_pairs<Q>(col, fin, fout);
+        else
+            for (int v = 0; v < Q; ++v) fout[v] = fin[v];
     }
     if constexpr (STORE) {
         if (ty == kFluid || ty == kTL2F || ty == kVelocity) store_macro<L::D>(a, cell, rho, u);
