@@ -113,8 +113,9 @@ hipError_t launch_step_q(const LaunchSpec &ls, const StepArgs &a, hipStream_t s)
             if (ls.store) k_aa_masked<Q, P, false, true><<<g, b, 0, s>>>(a);
             else k_aa_masked<Q, P, false, false><<<g, b, 0, s>>>(a);
         }
-    } else if (a.ncell * sizeof(typename P::S) < (size_t(1) << 32)) {
-        // population planes below 4 GiB: 32-bit offsets from the plane bases
+    } else if (!P::kShift && a.ncell * sizeof(typename P::S) < (size_t(1) << 32)) {
+        // FP64 planes below 4 GiB: 32-bit offsets from the plane bases (measured at 256^3:
+        // 920 us against 947 us; the shifted forms are faster with 64-bit addresses)
         if (ls.odd) {
             if (ls.store) k_aa_periodic<Q, P, true, true, true><<<g, b, 0, s>>>(a);
             else k_aa_periodic<Q, P, true, false, true><<<g, b, 0, s>>>(a);
