@@ -85,7 +85,7 @@ def strouhal(cl: np.ndarray, dstep: float, diameter: float, u0: float, tail: flo
     return diameter / (period * u0), period
 
 
-def run_c3(precision: str, steps: int, sample: int = 10, fields_at=(), geometry=None):
+def run_c3(precision: str, steps: int, sample: int = 20, fields_at=(), geometry=None):
     parent, child, pos = geometry if geometry is not None else c3_cylinder()
     u0 = C3_UMAX / 1.5
     d_fine = C3_RES * 2.0
@@ -97,12 +97,11 @@ def run_c3(precision: str, steps: int, sample: int = 10, fields_at=(), geometry=
         body = gc.ibm_add_body(pos)
         gc.ibm_initialise()
         t0 = time.perf_counter()
-        for s in range(1, steps + 1):
-            gp.LBM_multi_opt()
-            if s % sample == 0:
-                F = gc.ibm_body_force(body)
-                cd.append(2.0 * F[0] / (u0 * u0 * d_fine))
-                cl.append(2.0 * F[1] / (u0 * u0 * d_fine))
+        for s in range(sample, steps + 1, sample):
+            gp.run(sample)
+            F = gc.ibm_body_force(body)
+            cd.append(2.0 * F[0] / (u0 * u0 * d_fine))
+            cl.append(2.0 * F[1] / (u0 * u0 * d_fine))
             if s in fields_at:
                 snaps[s] = (gp.download_macro(), gc.download_macro())
             if s % 10000 == 0:
@@ -288,7 +287,7 @@ def main(argv=None):
     ap.add_argument("config", choices=["c1", "c3", "c4", "c5"])
     ap.add_argument("--precision", default="fp64,fp32s")
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--sample", type=int, default=10)
+    ap.add_argument("--sample", type=int, default=20)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     precs = a.precision.split(",")

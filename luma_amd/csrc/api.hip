@@ -126,20 +126,14 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
         for (int sc = 0; sc < 2; ++sc)
             if (int r = apply_step(c, sc, nullptr, nullptr)) return r;
         // the child reads this grid's populations, so order the two streams
-        hipEvent_t done;
-        HIP_TRY(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-        HIP_TRY(hipEventRecord(done, c->stream));
-        HIP_TRY(hipStreamWaitEvent(g->stream, done, 0));
-        HIP_TRY(hipEventDestroy(done));
+        HIP_TRY(hipEventRecord(c->ev_done, c->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, c->ev_done, 0));
         if (c->parity != 0) return fail(LUMAGPU_ESTATE, "child grid out of step with its parent");
     }
     if (g->parent) {
         // this grid reads the parent's populations: wait for the parent's previous work
-        hipEvent_t pe;
-        HIP_TRY(hipEventCreateWithFlags(&pe, hipEventDisableTiming));
-        HIP_TRY(hipEventRecord(pe, g->parent->stream));
-        HIP_TRY(hipStreamWaitEvent(g->stream, pe, 0));
-        HIP_TRY(hipEventDestroy(pe));
+        HIP_TRY(hipEventRecord(g->parent->ev_ready, g->parent->stream));
+        HIP_TRY(hipStreamWaitEvent(g->stream, g->parent->ev_ready, 0));
     }
     if (g->has_outlet) {
         hipError_t e = launch_extrap(g);
@@ -174,6 +168,75 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     }
     g->parity ^= 1;
     g->t += 1;
+    return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// no host work inside a step anywhere in the hierarchy (flexible bodies download forces and
+// run the FEM on the host every step)
+bool graph_ok(const lumagpu_grid *g) {
+    for (const lumagpu_grid *x = g; x; x = x->child) {
+        if (ibm_has_flexible(x)) return false;
+    }
+    return !g->parent;
+}
+
+void add_time(lumagpu_grid *g, int steps) {
+    for (lumagpu_grid *x = g; x; x = x->child, steps *= 2) x->t += steps;
+}
+
+// Capture two steps of g (and its descendants) into a graph and launch it reps times.
+// Returns 0 when the steps ran, 1 when the capture was not possible (nothing ran, host state
+// restored), or an error code.
+int run_graph(lumagpu_grid *g, int reps) {
+    std::vector<int> t0;
+    for (lumagpu_grid *x = g; x; x = x->child) t0.push_back(x->t);
+    auto restore = [&]() {
+        int i = 0;
+        for (lumagpu_grid *x = g; x; x = x->child) {
+            x->t = t0[i++];
+            x->parity = 0;
+        }
+    };
+    if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    int rc = 0;
+    for (int k = 0; k < 2 && !rc; ++k) rc = apply_step(g, 0, nullptr, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+    restore();
+    if (rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return rc ? rc : 1;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return 1;
+    }
+    // the graph runs on g's stream: order it after whatever the descendants' streams hold,
+    // and the descendants' later work after it
+    hipError_t e = hipSuccess;
+    for (lumagpu_grid *x = g->child; x && e == hipSuccess; x = x->child) {
+        e = hipEventRecord(x->ev_done, x->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, x->ev_done, 0);
+    }
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(exec, g->stream);
+    for (lumagpu_grid *x = g->child; x && e == hipSuccess; x = x->child) {
+        e = hipEventRecord(g->ev_ready, g->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(x->stream, g->ev_ready, 0);
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
+    add_time(g, 2 * reps);
     return 0;
 }
 
@@ -217,6 +280,9 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     };
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(LUMAGPU_EHIP, "hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming) != hipSuccess)
+        return cleanup(LUMAGPU_EHIP, "hipEventCreate failed");
     if (hipMalloc(&g->f, g->ps * g->Q * g->elem) != hipSuccess)
         return cleanup(LUMAGPU_ENOMEM, "population array allocation failed");
     if (hipMalloc(&g->rho, g->ncell * sizeof(double)) != hipSuccess ||
@@ -257,6 +323,8 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     ibm_destroy(g);
     if (g->parent && g->parent->child == g) g->parent->child = nullptr;
     if (g->child && g->child->parent == g) g->child->parent = nullptr;
+    if (g->ev_done) (void)hipEventDestroy(g->ev_done);
+    if (g->ev_ready) (void)hipEventDestroy(g->ev_ready);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return 0;
@@ -567,7 +635,23 @@ int lumagpu_step(lumagpu_grid *g, int subcycle) {
 int lumagpu_run(lumagpu_grid *g, int nsteps) {
     if (int r = check_grid(g)) return r;
     if (nsteps < 0) return fail(LUMAGPU_EINVAL, "negative step count");
-    for (int s = 0; s < nsteps; ++s)
+    int s = 0;
+    // Long runs of a hierarchy without host work per step replay a captured pair of steps: the
+    // step alternates between the two AA kernels, so two steps are the period (the child's
+    // four sub-steps included).  One eager step first settles the IB support flags.
+    if (nsteps >= 8 && graph_ok(g)) {
+        while (s < nsteps && (s == 0 || g->parity != 0)) {
+            if (int r = apply_step(g, 0, nullptr, nullptr)) return r;
+            ++s;
+        }
+        const int reps = (nsteps - s) / 2;
+        if (reps >= 2) {
+            int r = run_graph(g, reps);
+            if (r < 0) return r;
+            if (r == 0) s += 2 * reps;   // r > 0: not capturable here, steps continue eagerly
+        }
+    }
+    for (; s < nsteps; ++s)
         if (int r = apply_step(g, 0, nullptr, nullptr)) return r;
     return 0;
 }

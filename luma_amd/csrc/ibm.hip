@@ -329,6 +329,13 @@ int ensure_flags(lumagpu_grid *g) {
     return 0;
 }
 
+bool ibm_has_flexible(const lumagpu_grid *g) {
+    if (!g->ibm) return false;
+    for (const auto &b : g->ibm->bodies)
+        if (b.flexible) return true;
+    return false;
+}
+
 void ibm_destroy(lumagpu_grid *g) {
     delete g->ibm;
     g->ibm = nullptr;

@@ -28,6 +28,8 @@ struct lumagpu_grid {
     size_t stage_bytes = 0;
     double *moments_out = nullptr;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_done = nullptr;     // recorded on this grid's stream after its sub-steps
+    hipEvent_t ev_ready = nullptr;    // recorded before a child reads this grid's populations
     lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
     lumagpu_grid *child = nullptr;    // GridObj::subGrid[0] (non-owning; one region per level)
     double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
@@ -118,6 +120,8 @@ inline StepArgs make_args(const lumagpu_grid *g) {
 int ibm_apply(lumagpu_grid *g);
 void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
+// the grid carries a flexible (FEM) body, whose step runs host work
+bool ibm_has_flexible(const lumagpu_grid *g);
 // one LBM_multi_opt of the grid (children first), as lumagpu_step; defined in api.hip
 int step_grid(lumagpu_grid *g, int subcycle);
 // ExtrapolateRight side buffer from the grid's current populations; defined in api.hip

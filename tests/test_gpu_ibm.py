@@ -225,3 +225,33 @@ def test_flexible_filament_on_refined_level_fp64_bitwise():
             assert np.array_equal(a[key], b[key]), (key, np.abs(a[key] - b[key]).max())
         compare_grids(parent, gp, op, "parent")
         compare_grids(child, gc, oc, "child")
+
+
+def test_graph_replayed_runs_match_eager_steps():
+    # lumagpu_run replays a captured pair of steps (child sub-steps and IB included); replays
+    # interleaved with other stream work must equal the same number of eager steps bit for bit
+    parent = setups.channel_2d(64, 32, 8.0, 0.04, 80.0)
+    child = setups.add_refined_region(parent, (8, 31, 6, 25), parent.info["dh"], parent.info["dt"])
+    pos = setups.circle_markers((2.5 + 0.01, 2.0 + 0.13, 0.0), 0.5, child.info["dh"])
+    out = []
+    for mode in ("run", "eager"):
+        print(f"[graph] {mode}", flush=True)
+        with gpu_grid(parent) as gp, gpu_grid(child) as gc:
+            gp.add_child(gc)
+            gc.ibm_add_body(pos)
+            gc.ibm_initialise()
+            if mode == "run":
+                gp.run(37)
+                m1 = gp.moments()
+                gp.run(21)
+                assert gp.t == 58 and gc.t == 116
+            else:
+                for _ in range(37):
+                    gp.LBM_multi_opt()
+                m1 = gp.moments()
+                for _ in range(21):
+                    gp.LBM_multi_opt()
+            out.append((m1, gp.download_f(), gc.download_f(), gc.ibm_markers(0)["force"]))
+    (ma, pa, ca, fa), (mb, pb, cb, fb) = out
+    assert ma == mb
+    assert np.array_equal(pa, pb) and np.array_equal(ca, cb) and np.array_equal(fa, fb)
