@@ -159,6 +159,26 @@ int lumagpu_set_positions(lumagpu_grid *g, const double *xpos, const double *ypo
  * would replace with RCCL send/receive of the same contiguous planes. */
 int lumagpu_set_zslab(lumagpu_grid *g, int on);
 
+/* ---- LUMA's on-disk formats --------------------------------------------------------------
+ * Restart lines of this grid (GridObj::io_restart, GridObj_ops_io.cpp:406-590): level, region,
+ * X, Y, Z, u in dimensionless units (GridUnits::ulbm2ud), rho, and the time-scaled
+ * non-equilibrium populations ((f - feq) omega)/(feq dt); written at full precision (LUMA
+ * writes 6 significant digits).  append = 0 starts the file (level 0), 1 appends (sub-grids).
+ * Reading takes the lines of this grid's level and region, places them by position and sets
+ * f = feq (1 + dt fneq / omega) from the file's rho and u (LUMA evaluates that equilibrium on
+ * the level-0 grid's arrays for every level; here each grid uses its own).  Needs desc.dh/dt. */
+int lumagpu_write_restart(lumagpu_grid *g, const char *path, int append, int region);
+int lumagpu_read_restart(lumagpu_grid *g, const char *path, int region);
+/* GridObj::io_lite (GridObj_ops_io.cpp:706-810): one row per site, k-j-i order: rank, eType,
+ * X, Y, Z, rho, u (uz = 0.000000 in 2D), f[Q], fNew[Q], fixed format with the given number of
+ * decimals (L_OUTPUT_PRECISION).  This build keeps one population array, so the fNew block
+ * repeats the post-collision f. */
+int lumagpu_io_lite(lumagpu_grid *g, const char *path, int rank, int precision);
+/* ObjectManager::io_writeLiftDrag (ObjectManager_ops_io.cpp:64-122) for one body: appends
+ * "t<TAB>time" and, per marker, force * epsilon * volDepth * ds * dm dh / dt^2 (/dh in 2D) per
+ * dimension; the header line when t == 0. */
+int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, double time, int precision);
+
 /* ---- discrete adjoint (the differentiation form) ------------------------------------------
  * No LUMA counterpart.  Objective: J = (1/T) sum over the next T steps of dir . (the IB body
  * force, summed over all bodies of the grid: the per-marker terms LUMA writes to

@@ -80,6 +80,10 @@ _PROTOS = {
     "lumagpu_ibm_set_epsilon": [_vp, C.c_int, _dp],
     "lumagpu_ibm_get_markers": [_vp, C.c_int, _dp],
     "lumagpu_ibm_body_force": [_vp, C.c_int, _dp],
+    "lumagpu_write_restart": [_vp, C.c_char_p, C.c_int, C.c_int],
+    "lumagpu_read_restart": [_vp, C.c_char_p, C.c_int],
+    "lumagpu_io_lite": [_vp, C.c_char_p, C.c_int, C.c_int],
+    "lumagpu_ibm_write_ld": [_vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int],
     "lumagpu_adj_run": [_vp, C.c_int, C.c_int, _dp, C.c_int, C.POINTER(AdjResult), _dp, _dp],
     "lumagpu_last_error": [],
     "lumagpu_abi_version": [],

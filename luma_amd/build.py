@@ -35,6 +35,7 @@ UNITS = [
     ("api.hip", ["-ffp-contract=off"]),
     ("ibm.hip", ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"]),
     ("adjoint.hip", ["-ffp-contract=off"]),
+    ("io.hip", ["-ffp-contract=off"]),
 ]
 
 

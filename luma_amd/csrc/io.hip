@@ -1,0 +1,237 @@
+// io.hip -- LUMA's on-disk formats for a grid's state: the restart file (GridObj::io_restart,
+// LUMA/src/GridObj_ops_io.cpp:406-590), the io_lite row dump (GridObj::io_lite, :706-810) and
+// the per-body lift/drag file (ObjectManager::io_writeLiftDrag, ObjectManager_ops_io.cpp:64-122).
+// Host code: the state is downloaded in LUMA's layout and written line by line.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ibm_system.hpp"
+#include "internal.hpp"
+#include "lumagpu.h"
+
+namespace lumagpu {
+namespace {
+
+// _LBM_equilibrium_opt (GridObj_ops_lbm_optimised.cpp:674-705) on the host
+double feq_host(const lumagpu_grid *g, int v, double rho, const double *u) {
+    const double cs = 1.0 / std::sqrt(3.0);
+    const double cs2 = cs * cs, den2 = 2.0 * (cs * cs) * (cs * cs);
+    int c[3];
+    double w;
+    if (g->Q == 9) {
+        c[0] = Lattice<9>::c(v, 0);
+        c[1] = Lattice<9>::c(v, 1);
+        c[2] = 0;
+        w = Lattice<9>::w(v);
+    } else {
+        c[0] = Lattice<19>::c(v, 0);
+        c[1] = Lattice<19>::c(v, 1);
+        c[2] = Lattice<19>::c(v, 2);
+        w = Lattice<19>::w(v);
+    }
+    double A, B;
+    if (g->D == 3) {
+        A = (c[0] * u[0]) + (c[1] * u[1]) + (c[2] * u[2]);
+        B = (double(c[0] * c[0]) - cs2) * (u[0] * u[0]) + (double(c[1] * c[1]) - cs2) * (u[1] * u[1]) +
+            (double(c[2] * c[2]) - cs2) * (u[2] * u[2]) + double(2 * c[0] * c[1]) * u[0] * u[1] +
+            double(2 * c[0] * c[2]) * u[0] * u[2] + double(2 * c[1] * c[2]) * u[1] * u[2];
+    } else {
+        A = (c[0] * u[0]) + (c[1] * u[1]);
+        B = (double(c[0] * c[0]) - cs2) * (u[0] * u[0]) + (double(c[1] * c[1]) - cs2) * (u[1] * u[1]) +
+            double(2 * c[0] * c[1]) * u[0] * u[1];
+    }
+    return rho * w * (1.0 + (A / cs2) + (B / den2));
+}
+
+struct HostState {
+    std::vector<double> f, rho, u;
+};
+
+int download_state(lumagpu_grid *g, HostState &h) {
+    h.f.resize(g->ncell * g->Q);
+    h.rho.resize(g->ncell);
+    h.u.resize(g->ncell * g->D);
+    if (int r = lumagpu_download_f(g, h.f.data())) return r;
+    return lumagpu_download_macro(g, h.rho.data(), h.u.data());
+}
+
+int check_positions(const lumagpu_grid *g) {
+    if (g->xpos.size() != (size_t)g->d.N || g->ypos.size() != (size_t)g->d.M || g->zpos.size() != (size_t)g->d.K)
+        return fail(LUMAGPU_ESTATE, "grid positions are not set (lumagpu_set_labels / lumagpu_set_positions)");
+    return 0;
+}
+
+// index of a position on one axis (the site whose position matches to within 1e-6 dh): the
+// uniformly spaced guess first, a scan only if the positions are not uniform
+int locate(const std::vector<double> &pos, double x, double dh) {
+    const double tol = 1e-6 * dh;
+    if (pos.empty()) return -1;
+    const long guess = std::lround((x - pos[0]) / dh);
+    if (guess >= 0 && guess < (long)pos.size() && std::fabs(pos[guess] - x) <= tol) return (int)guess;
+    for (size_t i = 0; i < pos.size(); ++i)
+        if (std::fabs(pos[i] - x) <= tol) return (int)i;
+    return -1;
+}
+
+}  // namespace
+}  // namespace lumagpu
+
+using namespace lumagpu;
+
+extern "C" {
+
+int lumagpu_write_restart(lumagpu_grid *g, const char *path, int append, int region) {
+    if (!g || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    if (int r = check_positions(g)) return r;
+    HostState h;
+    if (int r = download_state(g, h)) return r;
+    std::ofstream out(path, append ? std::ios::out | std::ios::app : std::ios::out);
+    if (!out) return fail(LUMAGPU_EINVAL, std::string("cannot open ") + path);
+    const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q, D = g->D;
+    const double dh = g->d.dh, dt = g->d.dt, om = g->d.omega;
+    char buf[64];
+    auto put = [&](double v) {
+        std::snprintf(buf, sizeof buf, "%.17g\t", v);
+        out << buf;
+    };
+    for (int k = 0; k < K; ++k)
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < N; ++i) {
+                const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+                out << g->d.level << "\t" << region << "\t";
+                put(g->xpos[i]);
+                put(g->ypos[j]);
+                put(g->zpos[k]);
+                double u[3] = {0.0, 0.0, 0.0};
+                for (int v = 0; v < D; ++v) {
+                    u[v] = h.u[v + D * id];
+                    put((u[v] * dh) / dt);   // GridUnits::ulbm2ud
+                }
+                put(h.rho[id]);
+                for (int v = 0; v < Q; ++v) {
+                    const double fe = feq_host(g, v, h.rho[id], u);
+                    put(((h.f[v + Q * id] - fe) * om) / (fe * dt));
+                }
+                out << "\n";
+            }
+    if (!out) return fail(LUMAGPU_EINVAL, std::string("write failed: ") + path);
+    return 0;
+}
+
+int lumagpu_read_restart(lumagpu_grid *g, const char *path, int region) {
+    if (!g || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    if (int r = check_positions(g)) return r;
+    std::ifstream in(path);
+    if (!in) return fail(LUMAGPU_EINVAL, "Error opening LBM restart file. Exiting.");
+    const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q, D = g->D;
+    const double dh = g->d.dh, dt = g->d.dt, om = g->d.omega;
+    HostState h;
+    h.f.assign(g->ncell * Q, 0.0);
+    h.rho.assign(g->ncell, 0.0);
+    h.u.assign(g->ncell * D, 0.0);
+    std::vector<char> seen(g->ncell, 0);
+    size_t nseen = 0;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::istringstream iss(line);
+        int lev, reg;
+        double x, y, z;
+        if (!(iss >> lev >> reg >> x >> y >> z)) return fail(LUMAGPU_EINVAL, "malformed restart line");
+        if (lev != g->d.level || reg != region) continue;
+        const int i = locate(g->xpos, x, dh), j = locate(g->ypos, y, dh), k = locate(g->zpos, z, dh);
+        if (i < 0 || j < 0 || k < 0) continue;   // a site of another rank (GridUtils::isOnThisRank)
+        const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int v = 0; v < D; ++v) {
+            double ud;
+            if (!(iss >> ud)) return fail(LUMAGPU_EINVAL, "malformed restart line");
+            u[v] = (ud * dt) / dh;   // GridUnits::ud2ulbm
+            h.u[v + D * id] = u[v];
+        }
+        if (!(iss >> h.rho[id])) return fail(LUMAGPU_EINVAL, "malformed restart line");
+        for (int v = 0; v < Q; ++v) {
+            double fn;
+            if (!(iss >> fn)) return fail(LUMAGPU_EINVAL, "malformed restart line");
+            const double fe = feq_host(g, v, h.rho[id], u);
+            h.f[v + Q * id] = fe * (1 + (dt * fn) / om);
+        }
+        if (!seen[id]) {
+            seen[id] = 1;
+            ++nseen;
+        }
+    }
+    if (nseen != g->ncell)
+        return fail(LUMAGPU_EINVAL, "restart file covers " + std::to_string(nseen) + " of " +
+                                        std::to_string(g->ncell) + " sites of this grid");
+    if (int r = lumagpu_upload_f(g, h.f.data())) return r;
+    return lumagpu_upload_macro(g, h.rho.data(), h.u.data());
+}
+
+int lumagpu_io_lite(lumagpu_grid *g, const char *path, int rank, int precision) {
+    if (!g || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    if (int r = check_positions(g)) return r;
+    if (precision < 0 || precision > 30) return fail(LUMAGPU_EINVAL, "output precision out of range");
+    HostState h;
+    if (int r = download_state(g, h)) return r;
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return fail(LUMAGPU_EINVAL, std::string("cannot open ") + path);
+    const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q, D = g->D;
+    auto put = [&](double v) { std::fprintf(fp, "%.*f\t", precision, v); };
+    for (int k = 0; k < K; ++k)
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < N; ++i) {
+                const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+                const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
+                const int ty = g->hflags.size() == g->ncell ? (g->hflags[cell] & kTypeMask) : kFluid;
+                std::fprintf(fp, "%d\t%d\t", rank, ty);
+                put(g->xpos[i]);
+                put(g->ypos[j]);
+                put(g->zpos[k]);
+                put(h.rho[id]);
+                for (int v = 0; v < D; ++v) put(h.u[v + D * id]);
+                if (D != 3) std::fprintf(fp, "%s\t", std::to_string(0.0).c_str());
+                // f, then LUMA's fNew columns: this build keeps one population array, so the
+                // post-collision state is written in both blocks
+                for (int rep = 0; rep < 2; ++rep)
+                    for (int v = 0; v < Q; ++v) put(h.f[v + Q * id]);
+                std::fprintf(fp, "\n");
+            }
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    return ok ? 0 : fail(LUMAGPU_EINVAL, std::string("write failed: ") + path);
+}
+
+int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, double time, int precision) {
+    if (!g || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    if (!g->ibm || body < 0 || body >= (int)g->ibm->bodies.size()) return fail(LUMAGPU_EINVAL, "no such IB body");
+    if (precision < 1 || precision > 30) return fail(LUMAGPU_EINVAL, "output precision out of range");
+    const Body &b = g->ibm->bodies[body];
+    const int n = (int)b.m.size(), D = g->D;
+    std::vector<double> rec((size_t)n * 16);
+    if (int r = lumagpu_ibm_get_markers(g, body, rec.data())) return r;
+    FILE *fp = std::fopen(path, "a");
+    if (!fp) return fail(LUMAGPU_EINVAL, std::string("cannot open ") + path);
+    if (t == 0) std::fprintf(fp, "Timestep\tTime (s)\tDrag (N)\tLift (N)\n");
+    const double dm = g->d.dm, dh = g->d.dh, dt = g->d.dt;
+    const double scale = (D == 2) ? dm * dh / (dt * dt) * 1.0 / dh : dm * dh / (dt * dt);
+    std::fprintf(fp, "%d\t%.*g", t, precision, time);
+    for (int m = 0; m < n; ++m) {
+        const double *r = &rec[(size_t)m * 16];
+        const double eps = r[13], ds = r[14];
+        const double vd = (D == 3) ? ds : 1.0;
+        for (int d = 0; d < D; ++d) std::fprintf(fp, "\t%.*g", precision, r[6 + d] * eps * vd * ds * scale);
+    }
+    std::fprintf(fp, "\n");
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    return ok ? 0 : fail(LUMAGPU_EINVAL, std::string("write failed: ") + path);
+}
+
+}  // extern "C"

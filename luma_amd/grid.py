@@ -156,6 +156,19 @@ class GpuGrid:
                 out["lambda0"] = lam
         return out
 
+    # LUMA's on-disk formats ---------------------------------------------------------------
+    def write_restart(self, path, append=False, region=0):
+        call("lumagpu_write_restart", self.h, str(path).encode(), int(append), int(region))
+
+    def read_restart(self, path, region=0):
+        call("lumagpu_read_restart", self.h, str(path).encode(), int(region))
+
+    def io_lite(self, path, rank=0, precision=8):
+        call("lumagpu_io_lite", self.h, str(path).encode(), int(rank), int(precision))
+
+    def ibm_write_ld(self, path, body, t, time, precision=8):
+        call("lumagpu_ibm_write_ld", self.h, str(path).encode(), int(body), int(t), float(time), int(precision))
+
     def set_zslab(self, on=True):
         """z planes 0 and K-1 become halo planes exchanged around every odd step (one rank of a
         z-sharded periodic box, its own neighbour on both faces)."""

@@ -255,3 +255,26 @@ def test_graph_replayed_runs_match_eager_steps():
     (ma, pa, ca, fa), (mb, pb, cb, fb) = out
     assert ma == mb
     assert np.array_equal(pa, pb) and np.array_equal(ca, cb) and np.array_equal(fa, fb)
+
+
+def test_lift_drag_file(tmp_path):
+    # ObjectManager::io_writeLiftDrag: header at t = 0, then t, time and the scaled per-marker
+    # force terms, D per marker
+    c = setups.channel_2d(40, 20, 10.0, 0.04, 50.0)
+    pos = setups.circle_markers((1.5, 1.02, 0.0), 0.3, c.info["dh"])
+    path = tmp_path / "Body_0_LD_rank0.out"
+    with gpu_grid(c) as g:
+        g.ibm_add_body(pos)
+        g.ibm_initialise()
+        g.ibm_write_ld(path, 0, 0, 0.0, precision=12)
+        g.run(5)
+        g.ibm_write_ld(path, 0, 5, 5 * c.info["dt"], precision=12)
+        mk = g.ibm_markers(0)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "Timestep\tTime (s)\tDrag (N)\tLift (N)"
+    vals = lines[2].split("\t")
+    assert vals[0] == "5" and len(vals) == 2 + 2 * len(pos)
+    dh, dt, dm = c.info["dh"], c.info["dt"], c.info["dm"]
+    scale = dm * dh / (dt * dt) / dh
+    ref = mk["force"][:, :2] * (mk["eps"] * mk["ds"])[:, None] * scale
+    assert np.allclose(np.array(vals[2:], float).reshape(-1, 2), ref, rtol=1e-11, atol=0)

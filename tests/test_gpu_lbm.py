@@ -215,3 +215,46 @@ def test_full_size_256_mass_conservation(prec):
     tol = 1e-12 if prec == "fp64" else 2e-6
     assert abs(m1["mass"] - m0["mass"]) < tol * m0["mass"]
     assert m1["ke"] < m0["ke"]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32s"])
+def test_restart_file_round_trip_continues_the_run(prec, tmp_path):
+    # GridObj::io_restart format at full precision: write, read into a fresh grid, continue;
+    # the time-scaled non-equilibrium round trip is exact to rounding
+    case = setups.lid_driven_cavity(32, re=1280.0 * 32 / 256)
+    path = tmp_path / "restart_LBM_Rnk0.out"
+    with GpuGrid.from_case(case, prec, store_macro=True) as a:
+        a.run(57)
+        a.write_restart(path)
+        a.run(40)
+        fa = a.download_f()
+    lines = path.read_text().splitlines()
+    assert len(lines) == case.N * case.M * case.K
+    cols = lines[0].split("\t")
+    assert cols[0] == "0" and cols[1] == "0" and len([c for c in cols if c]) == 2 + 3 + case.dims + 1 + case.Q
+    with GpuGrid.from_case(case, prec, store_macro=True) as b:
+        b.read_restart(path)
+        b.run(40)
+        fb = b.download_f()
+    m = case.lattyp != setups.SOLID
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    assert np.abs(fa[m] - fb[m]).max() <= tol * np.abs(fa[m]).max()
+
+
+def test_io_lite_rows(tmp_path):
+    case = setups.lid_driven_cavity(16, re=1280.0 * 16 / 256)
+    path = tmp_path / "io_lite.Lev0.Reg0.Rnk0.0.dat"
+    with GpuGrid.from_case(case, "fp64", store_macro=True) as g:
+        g.run(3)
+        g.io_lite(path, rank=0, precision=8)
+        f = g.download_f()
+        rho, u = g.download_macro()
+    rows = [r.rstrip("\t").split("\t") for r in path.read_text().splitlines()]
+    assert len(rows) == case.N * case.M
+    # row order k, j, i; columns rank, type, X, Y, Z, rho, ux, uy, 0.000000, f[9], fNew[9]
+    i, j = 5, 3
+    r = rows[j * case.N + i]
+    assert int(r[0]) == 0 and int(r[1]) == int(case.lattyp[i, j, 0])
+    assert r[2] == f"{case.xpos[i]:.8f}" and r[5] == f"{rho[i, j, 0]:.8f}" and r[8] == "0.000000"
+    assert [float(x) for x in r[9:18]] == pytest.approx(list(f[i, j, 0]), abs=1e-8)
+    assert len(r) == 9 + 2 * case.Q
