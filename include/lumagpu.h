@@ -137,6 +137,16 @@ int lumagpu_ibm_add_filament(lumagpu_grid *g, const double *start, double length
 int lumagpu_ibm_subiterations(const lumagpu_grid *g, int *iterations, double *residual);
 /* ObjectManager::ibm_initialise (:190-246): support stencils, ds and epsilon of every body */
 int lumagpu_ibm_initialise(lumagpu_grid *g);
+/* ObjectManager::io_readInGeomConfig (ObjectManager_ops_io.cpp:322-800): builds the IBM bodies
+ * of a geometry.config file on the grids (matched by level, region 0): CIRCLE_SPHERE,
+ * SQUARE_CUBE and FILAMENT_ARRAY (FLEXIBLE filaments get their FEM beam, 2D).  shift[3] is the
+ * wall-thickness shift LUMA adds when the left/bottom/front walls are solid.  FROM_FILE, PLATE
+ * and non-IBM bodies return LUMAGPU_EUNSUPPORTED.  *nbodies receives the count built. */
+int lumagpu_ibm_read_geometry(lumagpu_grid *const *grids, int ngrids, const char *path, const double *shift,
+                              int *nbodies);
+/* number of IB bodies on the grid and, for body >= 0, the markers of that body (iBody.size(),
+ * iBody[b].markers.size()); either output may be NULL */
+int lumagpu_ibm_count(const lumagpu_grid *g, int *nbodies, int body, int *nmarkers);
 /* IBMarker::markerVel for every marker of a body (vel[3*m + a]) */
 int lumagpu_ibm_set_velocity(lumagpu_grid *g, int body, const double *vel);
 /* IBMarker::epsilon for every marker of a body (replaces the ibm_findEpsilon result) */

@@ -36,6 +36,7 @@ UNITS = [
     ("ibm.hip", ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"]),
     ("adjoint.hip", ["-ffp-contract=off"]),
     ("io.hip", ["-ffp-contract=off"]),
+    ("geometry.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -625,6 +625,17 @@ int lumagpu_ibm_get_markers(lumagpu_grid *g, int body, double *rec) {
     return 0;
 }
 
+int lumagpu_ibm_count(const lumagpu_grid *g, int *nbodies, int body, int *nmarkers) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    const int nb = g->ibm ? (int)g->ibm->bodies.size() : 0;
+    if (nbodies) *nbodies = nb;
+    if (nmarkers) {
+        if (body < 0 || body >= nb) return fail(LUMAGPU_EINVAL, "no such IB body");
+        *nmarkers = (int)g->ibm->bodies[body].m.size();
+    }
+    return 0;
+}
+
 int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]) {
     if (int r = check_body(g, body)) return r;
     if (!out) return fail(LUMAGPU_EINVAL, "null output");

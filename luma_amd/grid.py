@@ -42,7 +42,6 @@ class GpuGrid:
         for i in range(3):
             d.origin[i] = origin[i]
         self.desc = d
-        self.nmarkers = []
         self.dims, self.Q, self.shape = dims, Q, (N, M, K)
         h = C.c_void_p()
         call("lumagpu_grid_create", C.byref(d), C.byref(h))
@@ -142,7 +141,8 @@ class GpuGrid:
         (N, M, K, Q)) when requested."""
         res = _abi.AdjResult()
         d = np.ascontiguousarray(direction, dtype=np.float64)
-        nm = sum(self.nmarkers)
+        nb = self.ibm_count()[0]
+        nm = sum(self.ibm_count(b)[1] for b in range(nb))
         dpos = np.zeros((nm, 3)) if (backward and positions and nm) else None
         lam = np.zeros(self.shape + (self.Q,)) if (backward and initial) else None
         call("lumagpu_adj_run", self.h, int(nsteps), int(checkpoint_every), _dptr(d), int(bool(backward)),
@@ -213,7 +213,6 @@ class GpuGrid:
         p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         body = C.c_int()
         call("lumagpu_ibm_add_body", self.h, len(p), _dptr(p), int(flexible), C.byref(body))
-        self.nmarkers.append(len(p))
         return body.value
 
     def ibm_add_filament(self, start, length, height, angle_deg, n_elements, clamped, density, youngs, depth=0.0):
@@ -221,7 +220,6 @@ class GpuGrid:
         body = C.c_int()
         call("lumagpu_ibm_add_filament", self.h, _dptr(st), length, height, depth, angle_deg, int(n_elements),
              int(clamped), density, youngs, C.byref(body))
-        self.nmarkers.append(int(np.floor(length / self.desc.dh)) + 1)
         return body.value
 
     def ibm_subiterations(self):
@@ -240,8 +238,15 @@ class GpuGrid:
         e = _f64(eps)
         call("lumagpu_ibm_set_epsilon", self.h, int(body), _dptr(e))
 
+    def ibm_count(self, body=None):
+        """(number of bodies, markers of `body`) as the grid holds them."""
+        nb, nm = C.c_int(), C.c_int()
+        call("lumagpu_ibm_count", self.h, C.byref(nb), -1 if body is None else int(body),
+             C.byref(nm) if body is not None else None)
+        return nb.value, (nm.value if body is not None else None)
+
     def ibm_markers(self, body):
-        rec = np.empty((self.nmarkers[body], 16))
+        rec = np.empty((self.ibm_count(body)[1], 16))
         call("lumagpu_ibm_get_markers", self.h, int(body), _dptr(rec))
         return {"pos": rec[:, 0:3], "vel": rec[:, 3:6], "force": rec[:, 6:9], "mom": rec[:, 9:12],
                 "rho": rec[:, 12], "eps": rec[:, 13], "ds": rec[:, 14], "ns": rec[:, 15].astype(int)}
@@ -250,3 +255,13 @@ class GpuGrid:
         out = (C.c_double * 3)()
         call("lumagpu_ibm_body_force", self.h, int(body), out)
         return np.array(out[:])
+
+
+def read_geometry(grids, path, shift=(0.0, 0.0, 0.0)):
+    """ObjectManager::io_readInGeomConfig over GpuGrids: builds the file's IBM bodies on the grid
+    of their level; returns the number of bodies built."""
+    arr = (C.c_void_p * len(grids))(*[g.h for g in grids])
+    sh = np.ascontiguousarray(shift, dtype=np.float64)
+    n = C.c_int()
+    call("lumagpu_ibm_read_geometry", arr, len(grids), str(path).encode(), _dptr(sh), C.byref(n))
+    return n.value

@@ -278,3 +278,40 @@ def test_lift_drag_file(tmp_path):
     scale = dm * dh / (dt * dt) / dh
     ref = mk["force"][:, :2] * (mk["eps"] * mk["ds"])[:, None] * scale
     assert np.allclose(np.array(vals[2:], float).reshape(-1, 2), ref, rtol=1e-11, atol=0)
+
+
+def test_geometry_config_builds_luma_bodies(tmp_path):
+    # ObjectManager::io_readInGeomConfig: prefab IBM bodies from a geometry.config; the markers
+    # equal LUMA's generators (restated in setups) and the flexible filament gets its beam
+    from luma_amd.grid import read_geometry
+    c = setups.channel_2d(80, 40, 10.0, 0.04, 50.0)
+    dh = c.info["dh"]
+    cfg = tmp_path / "geometry.config"
+    cfg.write_text(
+        "# GEOMETRY.CONFIG\n#\n"
+        "CIRCLE_SPHERE\tIBM\t0\t0\t1.5\t2.0\t0.0\t0.5\tRIGID\n"
+        "# a comment between bodies\n"
+        "FILAMENT_ARRAY IBM 0 0 2 4.0 1.0 0.0 0.0 1.5 0.0 1.0 0.05 0 30 0 FLEXIBLE 8 CLAMPED 10000.0 1.4e6\n"
+        "SQUARE_CUBE IBM 0 0 6.0 2.0 0.0 0.6 0.4 0.0 15 0 RIGID\n")
+    shift = (dh, dh, 0.0)
+    with gpu_grid(c) as g:
+        n = read_geometry([g], cfg, shift)
+        assert n == 4 and g.ibm_count()[0] == 4
+        g.ibm_initialise()
+        recs = []
+        for b, expect in enumerate([
+                setups.circle_markers((1.5 + dh, 2.0 + dh, 0.0), 0.5, dh),
+                setups.filament_markers((4.0 + dh, 1.0 + dh, 0.0), 1.0, 30.0, dh),
+                setups.filament_markers((4.0 + dh, 2.5 + dh, 0.0), 1.0, 30.0, dh)]):
+            assert np.array_equal(g.ibm_markers(b)["pos"], expect), b
+        # square: perimeter markers of the rotated 0.6 x 0.4 rectangle
+        nl, nh = int(np.floor(0.6 / dh)) + 1, int(np.floor(0.4 / dh)) + 1
+        sq = g.ibm_markers(3)["pos"]
+        assert len(sq) == nl * nh - (nl - 2) * (nh - 2)
+        cen = np.array([6.0 + dh, 2.0 + dh])
+        a = np.radians(15.0)
+        rot = np.array([[np.cos(a), np.sin(a)], [-np.sin(a), np.cos(a)]])
+        local = (sq[:, :2] - cen) @ rot.T
+        assert np.allclose(np.max(np.abs(local), axis=0), [0.3, 0.2], atol=1e-12)
+        g.run(3)
+        assert g.ibm_subiterations()[0] >= 1
