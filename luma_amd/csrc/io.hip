@@ -129,7 +129,7 @@ int lumagpu_read_restart(lumagpu_grid *g, const char *path, int region) {
     if (int r = check_positions(g)) return r;
     std::ifstream in(path);
     if (!in) return fail(LUMAGPU_EINVAL, "Error opening LBM restart file. Exiting.");
-    const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q, D = g->D;
+    const int M = g->d.M, K = g->d.K, Q = g->Q, D = g->D;
     const double dh = g->d.dh, dt = g->d.dt, om = g->d.omega;
     HostState h;
     h.f.assign(g->ncell * Q, 0.0);
