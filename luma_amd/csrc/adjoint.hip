@@ -184,7 +184,6 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     if (int r = fin.alloc(fbytes)) return r;
     if (int r = accJ.alloc(4 * sizeof(double))) return r;
     HIP_TRY(hipMemsetAsync(accJ.p, 0, 4 * sizeof(double), g->stream));
-    const int t_start = g->t;
 
     // forward with checkpoints
     for (int t = 0; t < nsteps; ++t) {
@@ -309,7 +308,6 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     HIP_TRY(hipMemcpyAsync(g->f, fin.p, fbytes, hipMemcpyDeviceToDevice, g->stream));
     g->parity = fin_par;
     g->t = fin_t;
-    (void)t_start;
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
     std::vector<T> hl;

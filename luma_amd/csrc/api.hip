@@ -290,7 +290,8 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         return cleanup(LUMAGPU_ENOMEM, "macro array allocation failed");
     if (hipMalloc(&g->uin, 3 * (size_t)d.M * sizeof(double)) != hipSuccess)
         return cleanup(LUMAGPU_ENOMEM, "inlet array allocation failed");
-    (void)hipMemsetAsync(g->uin, 0, 3 * (size_t)d.M * sizeof(double), g->stream);
+    if (hipMemsetAsync(g->uin, 0, 3 * (size_t)d.M * sizeof(double), g->stream) != hipSuccess)
+        return cleanup(LUMAGPU_EHIP, "inlet array clear failed");
     const dim3 gr = cell_grid(d.N, d.M, d.K);
     const size_t nblocks = (size_t)gr.x * gr.y * gr.z;
     if (hipMalloc(&g->partials, nblocks * kMomentCount * sizeof(double)) != hipSuccess ||
@@ -299,7 +300,8 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     if (d.forced) {
         if (hipMalloc(&g->force, g->ncell * g->D * sizeof(double)) != hipSuccess)
             return cleanup(LUMAGPU_ENOMEM, "force array allocation failed");
-        (void)hipMemsetAsync(g->force, 0, g->ncell * g->D * sizeof(double), g->stream);
+        if (hipMemsetAsync(g->force, 0, g->ncell * g->D * sizeof(double), g->stream) != hipSuccess)
+            return cleanup(LUMAGPU_EHIP, "force array clear failed");
     }
     *out = g;
     return 0;
@@ -334,6 +336,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                        const double *zpos) {
     if (int r = check_grid(g)) return r;
     if (!lattyp) return fail(LUMAGPU_EINVAL, "null label array");
+    // the grid's labels change only once the whole array has been accepted
     if (xpos && ypos && zpos) {
         g->xpos.assign(xpos, xpos + g->d.N);
         g->ypos.assign(ypos, ypos + g->d.M);
@@ -341,9 +344,9 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     }
     const int N = g->d.N, M = g->d.M, K = g->d.K;
     // keep the force bits a previous lumagpu_set_force call placed
-    std::vector<uint8_t> prev = std::move(g->hflags);
+    std::vector<uint8_t> prev = g->hflags;
     if (prev.size() != g->ncell) prev.assign(g->ncell, 0);
-    g->hflags.assign(g->ncell, 0);
+    std::vector<uint8_t> nf(g->ncell, 0);
     int any = 0, outlet = 0;
     std::vector<int> ecell, enrm;   // edge/corner velocity sites (device cell, packed normal)
     for (int i = 0; i < N; ++i)
@@ -391,7 +394,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                 }
                 if (ty != kFluid) any = 1;
                 const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
-                g->hflags[cell] = (uint8_t)(fl | (prev[cell] & kForcedBit));
+                nf[cell] = (uint8_t)(fl | (prev[cell] & kForcedBit));
             }
     // Edge/corner sites extrapolate rho from x + n and x + 2n (_LBM_updateAndExtrapolate,
     // :1353-1412).  Those neighbours must be sites whose sweep 1 computes rho (fluid, TL2F);
@@ -408,7 +411,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
             if (ii < 0 || ii >= N || jj < 0 || jj >= M || kk < 0 || kk >= K)
                 return fail(LUMAGPU_EINVAL, "Extrapolation does not have enough available cells.");
             nb[s - 1] = ii + N * (jj + M * kk);
-            const int nty = g->hflags[nb[s - 1]] & kTypeMask;
+            const int nty = nf[nb[s - 1]] & kTypeMask;
             if (nty != kFluid && nty != kTL2F)
                 return fail(LUMAGPU_EUNSUPPORTED,
                             "edge/corner velocity site whose extrapolation neighbours are not fluid is not supported");
@@ -432,6 +435,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     }
     if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
+    g->hflags = std::move(nf);
     HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
     if (outlet && !g->xr &&
         hipMalloc(&g->xr, (size_t)g->Q * g->d.M * g->d.K * sizeof(double)) != hipSuccess)
