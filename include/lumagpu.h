@@ -189,6 +189,16 @@ int lumagpu_io_lite(lumagpu_grid *g, const char *path, int rank, int precision);
  * dimension; the header line when t == 0. */
 int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, double time, int precision);
 
+/* ---- device checkpoints --------------------------------------------------------------------
+ * A copy in HBM of a grid's populations, storage layout, macro arrays and step counter
+ * (the in-memory counterpart of io_restart, GridObj_ops_io.cpp:406-590, with no rounding):
+ * restoring it resumes the run bit for bit.  IB marker state is not included (rigid bodies
+ * do not change; flexible bodies are refused).  Free with lumagpu_checkpoint_free. */
+typedef struct lumagpu_checkpoint lumagpu_checkpoint;
+int lumagpu_checkpoint_save(lumagpu_grid *g, lumagpu_checkpoint **out);
+int lumagpu_checkpoint_restore(lumagpu_grid *g, const lumagpu_checkpoint *ck);
+int lumagpu_checkpoint_free(lumagpu_checkpoint *ck);
+
 /* ---- discrete adjoint (the differentiation form) ------------------------------------------
  * No LUMA counterpart.  Objective: J = (1/T) sum over the next T steps of dir . (the IB body
  * force, summed over all bodies of the grid: the per-marker terms LUMA writes to

@@ -86,6 +86,9 @@ _PROTOS = {
     "lumagpu_ibm_write_ld": [_vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int],
     "lumagpu_ibm_count": [_vp, _ip, C.c_int, _ip],
     "lumagpu_ibm_read_geometry": [C.POINTER(_vp), C.c_int, C.c_char_p, _dp, _ip],
+    "lumagpu_checkpoint_save": [_vp, C.POINTER(_vp)],
+    "lumagpu_checkpoint_restore": [_vp, _vp],
+    "lumagpu_checkpoint_free": [_vp],
     "lumagpu_adj_run": [_vp, C.c_int, C.c_int, _dp, C.c_int, C.POINTER(AdjResult), _dp, _dp],
     "lumagpu_last_error": [],
     "lumagpu_abi_version": [],

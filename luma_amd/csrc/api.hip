@@ -446,6 +446,68 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     return 0;
 }
 
+}  // extern "C"
+
+struct lumagpu_checkpoint {
+    const lumagpu_grid *owner = nullptr;
+    void *f = nullptr;
+    double *rho = nullptr, *u = nullptr;
+    size_t fbytes = 0;
+    int parity = 0, t = 0, has_state = 0;
+};
+
+extern "C" {
+
+int lumagpu_checkpoint_save(lumagpu_grid *g, lumagpu_checkpoint **out) {
+    if (int r = check_grid(g)) return r;
+    if (!out) return fail(LUMAGPU_EINVAL, "null output");
+    if (g->parent || g->child) return fail(LUMAGPU_EUNSUPPORTED, "checkpoints cover single-level grids");
+    if (ibm_has_flexible(g)) return fail(LUMAGPU_EUNSUPPORTED, "checkpoints do not hold flexible-body state");
+    auto *ck = new lumagpu_checkpoint();
+    ck->owner = g;
+    ck->fbytes = g->ps * g->Q * g->elem;
+    if (hipMalloc(&ck->f, ck->fbytes) != hipSuccess || hipMalloc(&ck->rho, g->ncell * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ck->u, g->ncell * g->D * sizeof(double)) != hipSuccess) {
+        lumagpu_checkpoint_free(ck);
+        return fail(LUMAGPU_ENOMEM, "checkpoint allocation failed");
+    }
+    if (hipMemcpyAsync(ck->f, g->f, ck->fbytes, hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
+        hipMemcpyAsync(ck->rho, g->rho, g->ncell * sizeof(double), hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
+        hipMemcpyAsync(ck->u, g->u, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess) {
+        lumagpu_checkpoint_free(ck);
+        return fail(LUMAGPU_EHIP, "checkpoint copy failed");
+    }
+    ck->parity = g->parity;
+    ck->t = g->t;
+    ck->has_state = g->has_state;
+    *out = ck;
+    return 0;
+}
+
+int lumagpu_checkpoint_restore(lumagpu_grid *g, const lumagpu_checkpoint *ck) {
+    if (int r = check_grid(g)) return r;
+    if (!ck) return fail(LUMAGPU_EINVAL, "null checkpoint");
+    if (ck->owner != g) return fail(LUMAGPU_EINVAL, "checkpoint belongs to another grid");
+    HIP_TRY(hipMemcpyAsync(g->f, ck->f, ck->fbytes, hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->rho, ck->rho, g->ncell * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->u, ck->u, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->parity = ck->parity;
+    g->t = ck->t;
+    g->has_state = ck->has_state;
+    return 0;
+}
+
+int lumagpu_checkpoint_free(lumagpu_checkpoint *ck) {
+    if (!ck) return 0;
+    (void)hipFree(ck->f);
+    (void)hipFree(ck->rho);
+    (void)hipFree(ck->u);
+    delete ck;
+    return 0;
+}
+
 int lumagpu_set_zslab(lumagpu_grid *g, int on) {
     if (int r = check_grid(g)) return r;
     if (!on) {

@@ -156,6 +156,21 @@ class GpuGrid:
                 out["lambda0"] = lam
         return out
 
+    # device checkpoints ---------------------------------------------------------------------
+    def checkpoint(self):
+        """Copy of the populations, macro arrays and step counter in HBM; pass it to restore()
+        (and release it with free_checkpoint())."""
+        ck = C.c_void_p()
+        call("lumagpu_checkpoint_save", self.h, C.byref(ck))
+        return ck
+
+    def restore(self, ck):
+        call("lumagpu_checkpoint_restore", self.h, ck)
+
+    @staticmethod
+    def free_checkpoint(ck):
+        call("lumagpu_checkpoint_free", ck)
+
     # LUMA's on-disk formats ---------------------------------------------------------------
     def write_restart(self, path, append=False, region=0):
         call("lumagpu_write_restart", self.h, str(path).encode(), int(append), int(region))

@@ -258,3 +258,19 @@ def test_io_lite_rows(tmp_path):
     assert r[2] == f"{case.xpos[i]:.8f}" and r[5] == f"{rho[i, j, 0]:.8f}" and r[8] == "0.000000"
     assert [float(x) for x in r[9:18]] == pytest.approx(list(f[i, j, 0]), abs=1e-8)
     assert len(r) == 9 + 2 * case.Q
+
+
+def test_device_checkpoint_resumes_bit_for_bit():
+    case = setups.lid_driven_cavity(32, re=1280.0 * 32 / 256)
+    with GpuGrid.from_case(case, "fp64", store_macro=True) as g:
+        g.run(23)
+        ck = g.checkpoint()
+        g.run(31)
+        f1 = g.download_f()
+        t1 = g.t
+        g.restore(ck)
+        assert g.t == 23
+        g.run(31)
+        assert g.t == t1
+        assert np.array_equal(g.download_f(), f1)
+        GpuGrid.free_checkpoint(ck)
