@@ -188,6 +188,16 @@ int lumagpu_io_lite(lumagpu_grid *g, const char *path, int rank, int precision);
  * "t<TAB>time" and, per marker, force * epsilon * volDepth * ds * dm dh / dt^2 (/dh in 2D) per
  * dimension; the header line when t == 0. */
 int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, double time, int precision);
+/* ObjectManager::io_restart (ObjectManager_ops_io.cpp:136-290) for the bodies of a hierarchy's
+ * grids, in grid order: the body count, then per body "\t/\t" n "\t/\t" and its marker
+ * positions (plus position0 for flexible bodies), at full precision (LUMA writes 6 digits).
+ * Reading checks the body and marker counts against the bodies already built and applies
+ * the positions only when the whole file is accepted; the bodies then need
+ * lumagpu_ibm_initialise (main_lbm.cpp:307-318).  LUMA's reader takes the marker values from
+ * the wrong '/' segment and sets nothing; this one reads each segment whole.  Flexible bodies
+ * return LUMAGPU_EUNSUPPORTED on read (their FEM state is not in the file). */
+int lumagpu_ibm_write_restart(lumagpu_grid *const *grids, int ngrids, const char *path);
+int lumagpu_ibm_read_restart(lumagpu_grid *const *grids, int ngrids, const char *path);
 
 /* ---- device checkpoints --------------------------------------------------------------------
  * A copy in HBM of a grid's populations, storage layout, macro arrays and step counter

@@ -169,6 +169,15 @@ class ObjectManager {
         int n = 0;
         check(lumagpu_ibm_read_geometry(hs.data(), (int)hs.size(), path.c_str(), shift.data(), &n));
     }
+    // io_restart (ObjectManager_ops_io.cpp:136-290) for the bodies of every grid of the
+    // hierarchy; path replaces LUMA's restart_IBBody_Rnk<r>.out.  After a read, call
+    // ibm_initialise, as main_lbm.cpp:307-318 does.
+    void io_restart(eIOFlag flag, const std::string &path) {
+        std::vector<lumagpu_grid *> hs;
+        for (GridObj *x = _Grids; x; x = x->subGrid.empty() ? nullptr : x->subGrid[0]) hs.push_back(x->handle());
+        if (flag == eWrite) check(lumagpu_ibm_write_restart(hs.data(), (int)hs.size(), path.c_str()));
+        else check(lumagpu_ibm_read_restart(hs.data(), (int)hs.size(), path.c_str()));
+    }
     // a rigid (or flexible) Body<IBMarker> from marker positions (3 per marker) on grid g
     int addBody(GridObj &g, const std::vector<double> &pos, bool flexible = false) {
         int b = -1;

@@ -86,6 +86,8 @@ _PROTOS = {
     "lumagpu_ibm_write_ld": [_vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int],
     "lumagpu_ibm_count": [_vp, _ip, C.c_int, _ip],
     "lumagpu_ibm_read_geometry": [C.POINTER(_vp), C.c_int, C.c_char_p, _dp, _ip],
+    "lumagpu_ibm_write_restart": [C.POINTER(_vp), C.c_int, C.c_char_p],
+    "lumagpu_ibm_read_restart": [C.POINTER(_vp), C.c_int, C.c_char_p],
     "lumagpu_checkpoint_save": [_vp, C.POINTER(_vp)],
     "lumagpu_checkpoint_restore": [_vp, _vp],
     "lumagpu_checkpoint_free": [_vp],

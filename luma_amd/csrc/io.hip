@@ -234,4 +234,85 @@ int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, dou
     return ok ? 0 : fail(LUMAGPU_EINVAL, std::string("write failed: ") + path);
 }
 
+// ObjectManager::io_restart (ObjectManager_ops_io.cpp:136-290) over the grids of a hierarchy,
+// bodies in grid order.  The layout is LUMA's: the body count on a line, then per body
+// "\t/\t" n "\t/\t" and the marker positions (and position0 for flexible bodies), written
+// at full precision.
+int lumagpu_ibm_write_restart(lumagpu_grid *const *grids, int ngrids, const char *path) {
+    if (!grids || ngrids <= 0 || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    size_t nb = 0;
+    for (int i = 0; i < ngrids; ++i) {
+        if (!grids[i]) return fail(LUMAGPU_EINVAL, "null grid handle");
+        if (grids[i]->ibm) nb += grids[i]->ibm->bodies.size();
+    }
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return fail(LUMAGPU_EINVAL, std::string("cannot open ") + path);
+    std::fprintf(fp, "%zu\n", nb);
+    for (int i = 0; i < ngrids; ++i) {
+        if (!grids[i]->ibm) continue;
+        for (const Body &b : grids[i]->ibm->bodies) {
+            std::fprintf(fp, "\t/\t%zu\t/\t", b.m.size());
+            for (const Marker &m : b.m) {
+                for (int a = 0; a < 3; ++a) std::fprintf(fp, "%.17g\t", m.pos[a]);
+                if (b.flexible)
+                    for (int a = 0; a < 3; ++a) std::fprintf(fp, "%.17g\t", m.pos0[a]);
+            }
+        }
+    }
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    return ok ? 0 : fail(LUMAGPU_EINVAL, std::string("write failed: ") + path);
+}
+
+// The read side of the same routine.  The file is split at the '/' separators: the body count,
+// then per body its marker count and its marker data (LUMA's own reader takes the marker data
+// from the count's segment and so reads nothing; here each segment is read whole).  Counts
+// must match the bodies already built; the positions are applied only when the whole file is
+// accepted, and the bodies then need lumagpu_ibm_initialise (as main_lbm.cpp:307-318 does).
+// Flexible bodies are refused: their FEM state is not part of the file.
+int lumagpu_ibm_read_restart(lumagpu_grid *const *grids, int ngrids, const char *path) {
+    if (!grids || ngrids <= 0 || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    std::vector<Body *> bodies;
+    for (int i = 0; i < ngrids; ++i) {
+        if (!grids[i]) return fail(LUMAGPU_EINVAL, "null grid handle");
+        if (grids[i]->ibm)
+            for (Body &b : grids[i]->ibm->bodies) bodies.push_back(&b);
+    }
+    std::ifstream in(path);
+    if (!in) return fail(LUMAGPU_EINVAL, "Error opening IBM restart file. Exiting.");
+    std::vector<std::string> seg;
+    std::string part;
+    while (std::getline(in, part, '/')) seg.push_back(part);
+    size_t nb = 0;
+    {
+        std::istringstream iss(seg.empty() ? std::string() : seg[0]);
+        if (!(iss >> nb)) return fail(LUMAGPU_EINVAL, "malformed IBM restart file");
+    }
+    if (nb != bodies.size())
+        return fail(LUMAGPU_EINVAL, "Number of IBM bodies does not match the number specified in the restart file. Exiting.");
+    if (seg.size() < 1 + 2 * nb) return fail(LUMAGPU_EINVAL, "IBM restart file ends early");
+    std::vector<std::vector<double>> data(nb);
+    for (size_t b = 0; b < nb; ++b) {
+        if (bodies[b]->flexible)
+            return fail(LUMAGPU_EUNSUPPORTED, "body " + std::to_string(b) +
+                                                  " is flexible: its FEM state is not part of the IBM restart file");
+        std::istringstream cnt(seg[1 + 2 * b]);
+        size_t nm = 0;
+        if (!(cnt >> nm) || nm != bodies[b]->m.size())
+            return fail(LUMAGPU_EINVAL, "Number of IBM markers does not match the number specified for body " +
+                                            std::to_string(b) + " in the restart file. Exiting.");
+        std::istringstream vals(seg[2 + 2 * b]);
+        data[b].resize(3 * nm);
+        for (double &v : data[b])
+            if (!(vals >> v)) return fail(LUMAGPU_EINVAL, "IBM restart file: too few marker values for body " +
+                                                              std::to_string(b));
+    }
+    for (size_t b = 0; b < nb; ++b)
+        for (size_t m = 0; m < bodies[b]->m.size(); ++m)
+            for (int a = 0; a < 3; ++a) bodies[b]->m[m].pos[a] = data[b][3 * m + a];
+    for (int i = 0; i < ngrids; ++i)
+        if (grids[i]->ibm && !grids[i]->ibm->bodies.empty()) grids[i]->ibm->initialised = 0;
+    return 0;
+}
+
 }  // extern "C"

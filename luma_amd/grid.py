@@ -280,3 +280,17 @@ def read_geometry(grids, path, shift=(0.0, 0.0, 0.0)):
     n = C.c_int()
     call("lumagpu_ibm_read_geometry", arr, len(grids), str(path).encode(), _dptr(sh), C.byref(n))
     return n.value
+
+
+def ibm_write_restart(grids, path):
+    """ObjectManager::io_restart(eWrite): the bodies of the grids, in grid order, in LUMA's IB
+    restart layout at full precision."""
+    arr = (C.c_void_p * len(grids))(*[g.h for g in grids])
+    call("lumagpu_ibm_write_restart", arr, len(grids), str(path).encode())
+
+
+def ibm_read_restart(grids, path):
+    """ObjectManager::io_restart(eRead): marker positions for the bodies already built on the
+    grids; call ibm_initialise on each grid with bodies afterwards."""
+    arr = (C.c_void_p * len(grids))(*[g.h for g in grids])
+    call("lumagpu_ibm_read_restart", arr, len(grids), str(path).encode())

@@ -315,3 +315,43 @@ def test_geometry_config_builds_luma_bodies(tmp_path):
         assert np.allclose(np.max(np.abs(local), axis=0), [0.3, 0.2], atol=1e-12)
         g.run(3)
         assert g.ibm_subiterations()[0] >= 1
+
+
+def test_ib_restart_round_trip(tmp_path):
+    # ObjectManager::io_restart: LUMA's layout (count, then "/ n /" and positions per body),
+    # read back into bodies built elsewhere; after ibm_initialise the two grids step alike
+    from luma_amd.grid import ibm_read_restart, ibm_write_restart
+    c = setups.channel_2d(48, 24, 10.0, 0.04, 50.0)
+    dh = c.info["dh"]
+    pa = setups.circle_markers((1.5, 1.2, 0.0), 0.3, dh)
+    pb = setups.circle_markers((2.0, 1.1, 0.0), 0.25, dh)
+    path = tmp_path / "restart_IBBody_Rnk0.out"
+    with gpu_grid(c) as g1, gpu_grid(c) as g2:
+        g1.ibm_add_body(pa)
+        g1.ibm_add_body(pb)
+        g1.ibm_initialise()
+        ibm_write_restart([g1], path)
+        segs = path.read_text().split("/")
+        assert segs[0].strip() == "2" and int(segs[1]) == len(pa) and int(segs[3]) == len(pb)
+        assert np.array_equal(np.array(segs[2].split(), float).reshape(-1, 3), pa)
+        # same marker counts, other positions: replaced by the file's
+        g2.ibm_add_body(pa + 0.1 * dh)
+        g2.ibm_add_body(pb - 0.2 * dh)
+        ibm_read_restart([g2], path)
+        g2.ibm_initialise()
+        for b, p in enumerate((pa, pb)):
+            assert np.array_equal(g2.ibm_markers(b)["pos"], p)
+        g1.run(6)
+        g2.run(6)
+        assert np.array_equal(g1.download_f(), g2.download_f())
+        for key in ("force", "eps", "ds"):
+            assert np.array_equal(g1.ibm_markers(1)[key], g2.ibm_markers(1)[key]), key
+    # a body count or marker count that does not match is refused, nothing applied
+    with gpu_grid(c) as g3:
+        g3.ibm_add_body(pa)
+        with pytest.raises(RuntimeError, match="Number of IBM bodies"):
+            ibm_read_restart([g3], path)
+        g3.ibm_add_body(pb[:-1])
+        with pytest.raises(RuntimeError, match="Number of IBM markers"):
+            ibm_read_restart([g3], path)
+        assert np.array_equal(g3.ibm_markers(0)["pos"], pa)
