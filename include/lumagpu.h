@@ -37,6 +37,8 @@ extern "C" {
 #define LUMAGPU_FP64 0   /* plain FP64 storage and arithmetic, LUMA's operation order */
 #define LUMAGPU_FP32S 1  /* FP32 storage of f - w*rho0, FP32 collision */
 #define LUMAGPU_FP16S 2  /* FP16 storage of f - w*rho0, FP32 collision */
+#define LUMAGPU_FP64T 3  /* tangent form: FP64 values with their derivative along one parameter
+                            (dual numbers, 16 B per population); values equal LUMAGPU_FP64's */
 
 /* moments returned by lumagpu_moments, all FP64 */
 #define LUMAGPU_MOMENT_MASS 0
@@ -95,6 +97,10 @@ int lumagpu_upload_f(lumagpu_grid *g, const double *f);
 int lumagpu_upload_macro(lumagpu_grid *g, const double *rho, const double *u);
 int lumagpu_download_f(lumagpu_grid *g, double *f);
 int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u);
+/* Tangent form only: derivatives of f (post-collision, LUMA layout), rho and u along the
+ * seeded direction; any output may be NULL.  No counterpart in LUMA (the tangent form is this
+ * build's forward-mode derivative of LBM_multi_opt). */
+int lumagpu_download_tangent(lumagpu_grid *g, double *df, double *drho, double *du);
 /* GridObj::force_xyz as left by the caller (body force); requires desc.forced */
 int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz);
 
@@ -157,6 +163,15 @@ int lumagpu_ibm_get_markers(lumagpu_grid *g, int body, double *rec);
 /* FP64 sum over the body's markers of force_xyz * epsilon * volDepth * ds (the per-marker
  * terms of ObjectManager::io_writeLiftDrag, ObjectManager_ops_io.cpp:98-121) */
 int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]);
+/* tangent form: the same sum and its derivative */
+int lumagpu_ibm_body_force_tangent(lumagpu_grid *g, int body, double out[3], double dout[3]);
+/* tangent form: derivatives of the 16 lumagpu_ibm_get_markers values of each marker (the
+ * support count's is 0) */
+int lumagpu_ibm_get_marker_tangents(lumagpu_grid *g, int body, double *rec);
+/* tangent form: seeds the derivative direction -- d(Young's modulus)/ds = dE for every element
+ * of a flexible filament (FEMElement::E, FEMElement.cpp:61-104), so every derivative the grid
+ * reports is d/ds along E -> E + s dE.  Nothing else is seeded. */
+int lumagpu_ibm_seed_youngs(lumagpu_grid *g, int body, double dE);
 /* XPos/YPos/ZPos of a grid that has no labels to set */
 int lumagpu_set_positions(lumagpu_grid *g, const double *xpos, const double *ypos, const double *zpos);
 

@@ -31,24 +31,38 @@
 #include <type_traits>
 #include <utility>
 
+#include "dual.hpp"
 #include "lattice.hpp"
 
 namespace lumagpu {
 
+// S: population storage type; T: per-site arithmetic type; M: type of the macro arrays,
+// force field and side buffers (FP64 in every form, a Dual in the tangent form).
 struct P64 {
     using S = double;
     using T = double;
+    using M = double;
     static constexpr bool kShift = false;
 };
 struct P32S {
     using S = float;
     using T = float;
+    using M = double;
     static constexpr bool kShift = true;
 };
 struct P16S {
     using S = _Float16;
     using T = float;
+    using M = double;
     static constexpr bool kShift = true;
+};
+// Tangent form: the FP64 exact arithmetic on Duals (value and derivative along one
+// parameter direction); values equal the FP64 form bit for bit.
+struct PDual {
+    using S = Dual;
+    using T = Dual;
+    using M = Dual;
+    static constexpr bool kShift = false;
 };
 
 struct StepArgs {
@@ -56,6 +70,8 @@ struct StepArgs {
     size_t ps;                    // population stride in elements
     int nx, ny, nz;               // x = LUMA i (N_lim), y = j (M_lim), z = k (K_lim)
     const uint8_t *flags;         // per-cell flag byte (masked kernels only)
+    // The double arrays below hold P::M elements: FP64, or (value, derivative) pairs in the
+    // tangent form; kernels address them through mv<M>.
     const double *force;          // force_xyz, SoA [d*ncell + cell], read where flagged
     const double *uin;            // inlet profile [a*ny + j]: ux_in, uy_in, uz_in
     double *rho_out;              // macro output [cell] (store kernels)
@@ -97,13 +113,25 @@ struct StepArgs {
 // unrolling, lets the compiler merge both branches into one load and drop the hint.
 template <class P, bool NT = true>
 __device__ __forceinline__ typename P::T ld(const typename P::S *p) {
-    if constexpr (NT) return static_cast<typename P::T>(__builtin_nontemporal_load(p));
+    if constexpr (std::is_same_v<typename P::S, Dual>) return *p;
+    else if constexpr (NT) return static_cast<typename P::T>(__builtin_nontemporal_load(p));
     else return static_cast<typename P::T>(*p);
 }
 template <class P, bool NT = true>
 __device__ __forceinline__ void st(typename P::S *p, typename P::T v) {
-    if constexpr (NT) __builtin_nontemporal_store(static_cast<typename P::S>(v), p);
+    if constexpr (std::is_same_v<typename P::S, Dual>) *p = v;
+    else if constexpr (NT) __builtin_nontemporal_store(static_cast<typename P::S>(v), p);
     else *p = static_cast<typename P::S>(v);
+}
+
+// an FP64 array of StepArgs viewed as elements of the form's macro type M
+template <class M>
+__device__ __forceinline__ M *mv(double *p) {
+    return reinterpret_cast<M *>(p);
+}
+template <class M>
+__device__ __forceinline__ const M *mv(const double *p) {
+    return reinterpret_cast<const M *>(p);
 }
 
 // Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1.
@@ -176,8 +204,8 @@ __device__ __forceinline__ T feq_exact(int v, T rho, const T *u, T cs2, T den2) 
 }
 
 // _LBM_macro_opt, :800-847
-template <int Q, class T>
-__device__ __forceinline__ void macro_exact(const T *fin, const double *F, bool fz_compat, T &rho,
+template <int Q, class T, class FT>
+__device__ __forceinline__ void macro_exact(const T *fin, const FT *F, bool fz_compat, T &rho,
                                             T *u) {
     using L = Lattice<Q>;
     T r = T(0), rx = T(0), ry = T(0), rz = T(0);
@@ -205,7 +233,7 @@ __device__ __forceinline__ void macro_exact(const T *fin, const double *F, bool 
 
 // _LBM_forceGrid_opt, :928-990 (one direction)
 template <int Q, class T>
-__device__ __forceinline__ T guo_exact(int v, const T *u, const double *F, T omega, T cs2) {
+__device__ __forceinline__ T guo_exact(int v, const T *u, const T *F, T omega, T cs2) {
     using L = Lattice<Q>;
     T beta = T(0);
     const T lambda = (T(1) - T(0.5) * omega) * (T(L::w(v)) / cs2);
@@ -225,7 +253,7 @@ __device__ __forceinline__ T guo_exact(int v, const T *u, const double *F, T ome
 template <int Q, class T>
 struct CollideExact {
     T rho, u[3], omega, cs2, den2;
-    const double *F;
+    const T *F;
     __device__ __forceinline__ T operator()(int v, T fin) const {
         const T feq = feq_exact<Q, T>(v, rho, u, cs2, den2);
         if (F) return fin + (omega * (feq - fin) + guo_exact<Q, T>(v, u, F, omega, cs2));
@@ -265,7 +293,7 @@ struct CollideExact {
 };
 
 template <int Q, class T>
-__device__ __forceinline__ void collide_exact(const T *fin, T rho, const T *u, const double *F,
+__device__ __forceinline__ void collide_exact(const T *fin, T rho, const T *u, const T *F,
                                               T omega, T cs2, T den2, T *fout) {
     const CollideExact<Q, T> c{rho, {u[0], u[1], u[2]}, omega, cs2, den2, F};
 #pragma unroll
@@ -496,8 +524,8 @@ __device__ __forceinline__ typename P::T feq_store(int v, double rho, const doub
 // make_collider computes the site's macroscopic values (rho_o, u_o in FP64 for the macro
 // arrays) and returns the collision of one population at a time.
 template <int Q, class P>
-__device__ __forceinline__ auto make_collider(const typename P::T *fin, const double *F, const StepArgs &a,
-                                              double &rho_o, double *u_o) {
+__device__ __forceinline__ auto make_collider(const typename P::T *fin, const typename P::M *F, const StepArgs &a,
+                                              typename P::M &rho_o, typename P::M *u_o) {
     using T = typename P::T;
     if constexpr (!P::kShift) {
         T rho, u[3];
@@ -538,20 +566,21 @@ __device__ __forceinline__ void collide_pairs(const C &col, const T *fin, T *fou
 
 template <int Q, class P>
 __device__ __forceinline__ void site_update(typename P::T *fin, typename P::T *fout,
-                                            const double *F, const StepArgs &a, double &rho_o,
-                                            double *u_o, bool collide) {
+                                            const typename P::M *F, const StepArgs &a, typename P::M &rho_o,
+                                            typename P::M *u_o, bool collide) {
     const auto col = make_collider<Q, P>(fin, F, a, rho_o, u_o);
     if (collide) collide_pairs<Q>(col, fin, fout);
     else
         for (int v = 0; v < Q; ++v) fout[v] = fin[v];
 }
 
-template <int D>
-__device__ __forceinline__ void store_macro(const StepArgs &a, size_t cell, double rho, const double *u) {
-    a.rho_out[cell] = rho;
-    a.u_out[cell] = u[0];
-    a.u_out[a.ncell + cell] = u[1];
-    if constexpr (D == 3) a.u_out[2 * a.ncell + cell] = u[2];
+template <int D, class M>
+__device__ __forceinline__ void store_macro(const StepArgs &a, size_t cell, M rho, const M *u) {
+    M *ro = mv<M>(a.rho_out), *uo = mv<M>(a.u_out);
+    ro[cell] = rho;
+    uo[cell] = u[0];
+    uo[a.ncell + cell] = u[1];
+    if constexpr (D == 3) uo[2 * a.ncell + cell] = u[2];
 }
 
 // ----------------------------------------------------------------------------------------
@@ -560,7 +589,7 @@ __device__ __forceinline__ void store_macro(const StepArgs &a, size_t cell, doub
 // without spilling; every other variant is left to the compiler.
 template <class P, bool ODD, bool STORE>
 constexpr int periodic_waves() {
-    return (!P::kShift && !ODD && !STORE) ? 5 : 1;
+    return (std::is_same_v<P, P64> && !ODD && !STORE) ? 5 : 1;
 }
 
 template <int Q, class P, bool ODD, bool STORE, bool O32 = false>
@@ -593,7 +622,7 @@ void k_aa_periodic(StepArgs a) {
         else
             fin[v] = ld<P>(elem_at<S, O32>(f + (size_t)v * ps, cell));
     });
-    double rho, u[3];
+    typename P::M rho, u[3];
     const auto col = make_collider<Q, P>(fin, nullptr, a, rho, u);
     if constexpr (STORE) store_macro<L::D>(a, cell, rho, u);
     // collide and store an opposite pair at a time: only the incoming set stays live
@@ -698,7 +727,7 @@ __device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z
             // HWBB: f*_{opp v}(x) of the previous step, in this cell's slot v in either layout
             fin[v] = ld<P>(f + (size_t)v * ps + cell);
         } else if (sty == kExtrapRight) {
-            fin[v] = T(a.xr[(size_t)v * ny * a.nz + nb.ys[1 - cy] + (size_t)ny * nb.zs[1 - cz]]);
+            fin[v] = T(mv<typename P::M>(a.xr)[(size_t)v * ny * a.nz + nb.ys[1 - cy] + (size_t)ny * nb.zs[1 - cz]]);
         } else if (!a.regularised && sty == kVelocity) {
             // forced-equilibrium velocity source, :254-270; its rho, u never change
             const int js = nb.ys[1 - cy];
@@ -774,16 +803,18 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     T fin[Q], fout[Q];
     gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
 
-    const double *F = nullptr;
-    double Fv[3];
+    using M = typename P::M;
+    const M *F = nullptr;
+    M Fv[3];
     if (fl & kForcedBit) {
-        Fv[0] = a.force[cell];
-        Fv[1] = a.force[a.ncell + cell];
-        Fv[2] = (L::D == 3) ? a.force[2 * a.ncell + cell] : 0.0;
+        const M *fx = mv<M>(a.force);
+        Fv[0] = fx[cell];
+        Fv[1] = fx[a.ncell + cell];
+        Fv[2] = (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
         F = Fv;
     }
 
-    double rho, u[3];
+    M rho, u[3];
     if (ty == kVelocity) {
         // regularised velocity face: the BC gives rho, u; no macro on this site (:101-107)
         T rr, uu[3];
@@ -815,8 +846,8 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
                 for (int jj = 0; jj < 2; ++jj)
                     for (int kk = 0; kk < (L::D == 3 ? 2 : 1); ++kk) {
                         const size_t cc = cell_of(fx + ii, fy + jj, fz + kk, a.cnx, a.cny);
-                        a.crho[cc] = rho;
-                        for (int d = 0; d < L::D; ++d) a.cu[d * a.cncell + cc] = u[d];
+                        mv<M>(a.crho)[cc] = rho;
+                        for (int d = 0; d < L::D; ++d) mv<M>(a.cu)[d * a.cncell + cc] = u[d];
                     }
         }
     }
@@ -856,7 +887,7 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
     nb.init(x, y, z, a.nx, a.ny, a.nz);
     T fin[Q];
     gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
-    double rho, u[3];
+    typename P::M rho, u[3];
     if (vel_type) {
         T rr, uu[3];
         velocity_site<Q, P>(a, fl, y, fin, rr, uu);
@@ -896,9 +927,9 @@ __global__ __launch_bounds__(256) void k_edge_rho(StepArgs a, EdgeArgs e) {
     nb.init(x, y, z, a.nx, a.ny, a.nz);
     T fin[Q], fout[Q];
     gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
-    double rho, u[3];
+    typename P::M rho, u[3];
     site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);
-    e.erho[i] = rho;
+    mv<typename P::M>(e.erho)[i] = rho;
 }
 
 // Edge/corner velocity sites: stream, regularise with the extrapolated density
@@ -925,16 +956,19 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
     T fin[Q], fout[Q];
     gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kVelocity, fin);
 
-    const double *F = nullptr;
-    double Fv[3];
+    using M = typename P::M;
+    const M *F = nullptr;
+    M Fv[3];
     if (fl & kForcedBit) {
-        Fv[0] = a.force[cell];
-        Fv[1] = a.force[a.ncell + cell];
-        Fv[2] = (L::D == 3) ? a.force[2 * a.ncell + cell] : 0.0;
+        const M *fx = mv<M>(a.force);
+        Fv[0] = fx[cell];
+        Fv[1] = fx[a.ncell + cell];
+        Fv[2] = (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
         F = Fv;
     }
     const T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
-    const T dens = T(2.0) * T(e.erho[2 * i]) - T(e.erho[2 * i + 1]);
+    const M *er = mv<M>(e.erho);
+    const T dens = T(2.0) * T(er[2 * i]) - T(er[2 * i + 1]);
     T fa[Q], rr, uu[3];
 #pragma unroll
     for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
@@ -947,8 +981,8 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
         collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
     }
     if (store) {
-        const double u[3] = {double(uu[0]), double(uu[1]), double(uu[2])};
-        store_macro<L::D>(a, cell, double(rr), u);
+        const M u[3] = {M(uu[0]), M(uu[1]), M(uu[2])};
+        store_macro<L::D>(a, cell, M(rr), u);
     }
     static_for<Q>([&](auto vc) {
         constexpr int v = decltype(vc)::value;
@@ -973,9 +1007,10 @@ __global__ void k_extrap_right(StepArgs a, int parity, double *xr) {
     const int z = blockIdx.y;
     if (y >= a.ny) return;
     const size_t plane = (size_t)a.ny * a.nz;
+    using M = typename P::M;
     for (int v = 0; v < Q; ++v)
-        xr[(size_t)v * plane + y + (size_t)a.ny * z] =
-            double(post_of<Q, P>(a.f, a.ps, a.nx, a.ny, a.nz, a.flags, parity, a.nx - 3, y, z, v));
+        mv<M>(xr)[(size_t)v * plane + y + (size_t)a.ny * z] =
+            M(post_of<Q, P>(a.f, a.ps, a.nx, a.ny, a.nz, a.flags, parity, a.nx - 3, y, z, v));
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1017,40 +1052,43 @@ __global__ void k_upload_aos(StepArgs a, const double *faos) {
     }
 }
 
-// gather the post-collision f*_v(x) from the current layout
-template <int Q, class P>
+// the post-collision f*_v(x) from the current layout: its value, or with TAN its derivative
+// (tangent form)
+template <int Q, class P, bool TAN = false>
 __device__ __forceinline__ double gather_post(const StepArgs &a, int x, int y, int z, int v, int parity,
                                               bool masked) {
     using L = Lattice<Q>;
     const typename P::S *f = static_cast<const typename P::S *>(a.f);
     const size_t cell = cell_of(x, y, z, a.nx, a.ny);
-    double fv;
+    typename P::T fe;
     if (parity == 0) {
-        fv = double(ld<P, false>(f + (size_t)L::opp(v) * a.ps + cell));
+        fe = ld<P, false>(f + (size_t)L::opp(v) * a.ps + cell);
     } else {
         Nbr nb;
         nb.init(x, y, z, a.nx, a.ny, a.nz);
         const size_t dst = nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), a.nx, a.ny);
         if (masked && (a.flags[dst] & kTypeMask) == kSolid)
-            fv = double(ld<P, false>(f + (size_t)L::opp(v) * a.ps + cell));
+            fe = ld<P, false>(f + (size_t)L::opp(v) * a.ps + cell);
         else
-            fv = double(ld<P, false>(f + (size_t)v * a.ps + dst));
+            fe = ld<P, false>(f + (size_t)v * a.ps + dst);
     }
+    if constexpr (TAN) return deriv_of(fe);
+    double fv = value_of(fe);
     if constexpr (P::kShift) fv += L::w(v) * a.rho0;
     return fv;
 }
 
-template <int Q, class P>
+template <int Q, class P, bool TAN = false>
 __global__ void k_download_aos(StepArgs a, double *faos, int parity, int masked) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
     if (x >= a.nx) return;
     const size_t id = luma_id(x, y, z, a.ny, a.nz);
-    for (int v = 0; v < Q; ++v) faos[v + Q * id] = gather_post<Q, P>(a, x, y, z, v, parity, masked != 0);
+    for (int v = 0; v < Q; ++v) faos[v + Q * id] = gather_post<Q, P, TAN>(a, x, y, z, v, parity, masked != 0);
 }
 
-// Macro arrays (device SoA) -> LUMA layout
-template <int D>
+// Macro arrays (device SoA of M) -> LUMA layout: values, or with TAN derivatives
+template <int D, class M = double, bool TAN = false>
 __global__ void k_macro_to_luma(const double *rho_s, const double *u_s, size_t ncell, int nx, int ny,
                                 int nz, double *rho_l, double *u_l) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1058,11 +1096,12 @@ __global__ void k_macro_to_luma(const double *rho_s, const double *u_s, size_t n
     if (x >= nx) return;
     const size_t cell = cell_of(x, y, z, nx, ny);
     const size_t id = luma_id(x, y, z, ny, nz);
-    rho_l[id] = rho_s[cell];
-    for (int d = 0; d < D; ++d) u_l[d + D * id] = u_s[d * ncell + cell];
+    const M *r = mv<M>(rho_s), *u = mv<M>(u_s);
+    rho_l[id] = TAN ? deriv_of(r[cell]) : value_of(r[cell]);
+    for (int d = 0; d < D; ++d) u_l[d + D * id] = TAN ? deriv_of(u[d * ncell + cell]) : value_of(u[d * ncell + cell]);
 }
 
-template <int D>
+template <int D, class M = double>
 __global__ void k_macro_from_luma(double *rho_s, double *u_s, size_t ncell, int nx, int ny, int nz,
                                   const double *rho_l, const double *u_l) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1070,8 +1109,9 @@ __global__ void k_macro_from_luma(double *rho_s, double *u_s, size_t ncell, int 
     if (x >= nx) return;
     const size_t cell = cell_of(x, y, z, nx, ny);
     const size_t id = luma_id(x, y, z, ny, nz);
-    rho_s[cell] = rho_l[id];
-    for (int d = 0; d < D; ++d) u_s[d * ncell + cell] = u_l[d + D * id];
+    M *r = mv<M>(rho_s), *u = mv<M>(u_s);
+    r[cell] = M(rho_l[id]);
+    for (int d = 0; d < D; ++d) u[d * ncell + cell] = M(u_l[d + D * id]);
 }
 
 // ----------------------------------------------------------------------------------------

@@ -80,29 +80,6 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     return 0;
 }
 
-// grad delta(x; s) = d/dx of prod_a phi((x_a - s_a)/dh)
-void delta_grad(const double *x, const double *s, double dh, double dil, int D, double out[3]) {
-    double ph[3], dph[3];
-    for (int a = 0; a < 3; ++a) {
-        const double r = (a < D) ? (x[a] - s[a]) / dh : 0.0;
-        ph[a] = (a < D) ? delta_kernel(r, dil) : 1.0;
-        dph[a] = (a < D) ? delta_kernel_deriv(r, dil) / dh : 0.0;
-    }
-    for (int a = 0; a < 3; ++a) {
-        out[a] = 0.0;
-        if (a >= D) continue;
-        double v = dph[a];
-        for (int b = 0; b < D; ++b)
-            if (b != a) v *= ph[b];
-        out[a] = v;
-    }
-}
-double delta_at(const double *x, const double *s, double dh, double dil, int D) {
-    double v = 1.0;
-    for (int a = 0; a < D; ++a) v *= delta_kernel((x[a] - s[a]) / dh, dil);
-    return v;
-}
-
 // epsilon and ds chains of one body: xg += dJ/dx through eps (A eps = 1) and ds
 int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsbar, const double *dsbar_dev,
                         double *xg) {

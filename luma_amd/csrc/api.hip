@@ -32,6 +32,7 @@ hipError_t launch_step(const lumagpu_grid *g, const LaunchSpec &ls, const StepAr
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_step_f64(ls, a, g->stream);
         case LUMAGPU_FP32S: return launch_step_f32s(ls, a, g->stream);
+        case LUMAGPU_FP64T: return launch_step_dual(ls, a, g->stream);
         default: return launch_step_f16s(ls, a, g->stream);
     }
 }
@@ -41,6 +42,7 @@ hipError_t launch_aux(const lumagpu_grid *g, int op, const double *p0, const dou
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_aux_f64(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
         case LUMAGPU_FP32S: return launch_aux_f32s(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
+        case LUMAGPU_FP64T: return launch_aux_dual(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
         default: return launch_aux_f16s(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
     }
 }
@@ -88,7 +90,8 @@ hipError_t launch_zslab(const lumagpu_grid *g, const StepArgs &a, int phase) {
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_zslab_f64(a, phase, g->stream);
         case LUMAGPU_FP32S: return launch_zslab_f32s(a, phase, g->stream);
-        default: return launch_zslab_f16s(a, phase, g->stream);
+        case LUMAGPU_FP16S: return launch_zslab_f16s(a, phase, g->stream);
+        default: return hipErrorInvalidValue;   // refused by lumagpu_set_zslab
     }
 }
 
@@ -105,8 +108,40 @@ hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_edges_f64(g->Q, a, e, odd, phase, store, g->stream);
         case LUMAGPU_FP32S: return launch_edges_f32s(g->Q, a, e, odd, phase, store, g->stream);
+        case LUMAGPU_FP64T: return launch_edges_dual(g->Q, a, e, odd, phase, store, g->stream);
         default: return launch_edges_f16s(g->Q, a, e, odd, phase, store, g->stream);
     }
+}
+
+// LUMA-layout rho, u <-> the grid's macro arrays (Duals with zero derivative in the tangent
+// form); macro_to_luma takes the values, or with tan the derivatives
+hipError_t macro_from_luma(lumagpu_grid *g, const double *rho, const double *u) {
+    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    const int N = g->d.N, M = g->d.M, K = g->d.K;
+    const bool t = g->d.precision == LUMAGPU_FP64T;
+    if (g->D == 2) {
+        if (t) k_macro_from_luma<2, Dual><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+        else k_macro_from_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+    } else {
+        if (t) k_macro_from_luma<3, Dual><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+        else k_macro_from_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+    }
+    return hipGetLastError();
+}
+hipError_t macro_to_luma(lumagpu_grid *g, double *rho, double *u, bool tan) {
+    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    const int N = g->d.N, M = g->d.M, K = g->d.K;
+    const bool t = g->d.precision == LUMAGPU_FP64T;
+    if (g->D == 2) {
+        if (t && tan) k_macro_to_luma<2, Dual, true><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+        else if (t) k_macro_to_luma<2, Dual><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+        else k_macro_to_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+    } else {
+        if (t && tan) k_macro_to_luma<3, Dual, true><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+        else if (t) k_macro_to_luma<3, Dual><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+        else k_macro_to_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_extrap(const lumagpu_grid *g) {
@@ -114,6 +149,7 @@ hipError_t launch_extrap(const lumagpu_grid *g) {
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_aux_f64(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
         case LUMAGPU_FP32S: return launch_aux_f32s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP64T: return launch_aux_dual(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
         default: return launch_aux_f16s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
     }
 }
@@ -260,7 +296,7 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         return fail(LUMAGPU_EUNSUPPORTED, "lattice must be D2Q9 or D3Q19");
     if (d.N < 3 || d.M < 3 || d.K < 1 || (d.dims == 2 && d.K != 1) || (d.dims == 3 && d.K < 3))
         return fail(LUMAGPU_EINVAL, "grid extents too small (need >= 3 per active dimension; K = 1 in 2D)");
-    if (d.precision < LUMAGPU_FP64 || d.precision > LUMAGPU_FP16S)
+    if (d.precision < LUMAGPU_FP64 || d.precision > LUMAGPU_FP64T)
         return fail(LUMAGPU_EINVAL, "unknown precision");
     // GridObj_init_grids.cpp:353-356
     if (!(d.omega > 0.0) || d.omega >= 2.0)
@@ -273,7 +309,8 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     g->D = d.dims;
     g->ncell = (size_t)d.N * d.M * d.K;
     g->ps = (g->ncell + 255) / 256 * 256;
-    g->elem = d.precision == LUMAGPU_FP64 ? 8 : d.precision == LUMAGPU_FP32S ? 4 : 2;
+    g->elem = d.precision == LUMAGPU_FP64T ? 16 : d.precision == LUMAGPU_FP64 ? 8 : d.precision == LUMAGPU_FP32S ? 4 : 2;
+    g->melem = d.precision == LUMAGPU_FP64T ? 16 : 8;
     auto cleanup = [&](int code, const std::string &m) {
         lumagpu_grid_destroy(g);
         return fail(code, m);
@@ -285,8 +322,8 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         return cleanup(LUMAGPU_EHIP, "hipEventCreate failed");
     if (hipMalloc(&g->f, g->ps * g->Q * g->elem) != hipSuccess)
         return cleanup(LUMAGPU_ENOMEM, "population array allocation failed");
-    if (hipMalloc(&g->rho, g->ncell * sizeof(double)) != hipSuccess ||
-        hipMalloc(&g->u, g->ncell * g->D * sizeof(double)) != hipSuccess)
+    if (hipMalloc(&g->rho, g->ncell * g->melem) != hipSuccess ||
+        hipMalloc(&g->u, g->ncell * g->D * g->melem) != hipSuccess)
         return cleanup(LUMAGPU_ENOMEM, "macro array allocation failed");
     if (hipMalloc(&g->uin, 3 * (size_t)d.M * sizeof(double)) != hipSuccess)
         return cleanup(LUMAGPU_ENOMEM, "inlet array allocation failed");
@@ -298,9 +335,9 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         hipMalloc(&g->moments_out, kMomentCount * sizeof(double)) != hipSuccess)
         return cleanup(LUMAGPU_ENOMEM, "reduction buffers allocation failed");
     if (d.forced) {
-        if (hipMalloc(&g->force, g->ncell * g->D * sizeof(double)) != hipSuccess)
+        if (hipMalloc(&g->force, g->ncell * g->D * g->melem) != hipSuccess)
             return cleanup(LUMAGPU_ENOMEM, "force array allocation failed");
-        if (hipMemsetAsync(g->force, 0, g->ncell * g->D * sizeof(double), g->stream) != hipSuccess)
+        if (hipMemsetAsync(g->force, 0, g->ncell * g->D * g->melem, g->stream) != hipSuccess)
             return cleanup(LUMAGPU_EHIP, "force array clear failed");
     }
     *out = g;
@@ -428,7 +465,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     g->nedge = 0;
     if (ne) {
         if (hipMalloc(&g->edge, etab.size() * sizeof(int)) != hipSuccess ||
-            hipMalloc(&g->erho, 2 * (size_t)ne * sizeof(double)) != hipSuccess)
+            hipMalloc(&g->erho, 2 * (size_t)ne * g->melem) != hipSuccess)
             return fail(LUMAGPU_ENOMEM, "edge site table allocation failed");
         HIP_TRY(hipMemcpyAsync(g->edge, etab.data(), etab.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
         g->nedge = ne;
@@ -438,7 +475,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     g->hflags = std::move(nf);
     HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
     if (outlet && !g->xr &&
-        hipMalloc(&g->xr, (size_t)g->Q * g->d.M * g->d.K * sizeof(double)) != hipSuccess)
+        hipMalloc(&g->xr, (size_t)g->Q * g->d.M * g->d.K * g->melem) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "outlet buffer allocation failed");
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->has_outlet = outlet;
@@ -466,14 +503,14 @@ int lumagpu_checkpoint_save(lumagpu_grid *g, lumagpu_checkpoint **out) {
     auto *ck = new lumagpu_checkpoint();
     ck->owner = g;
     ck->fbytes = g->ps * g->Q * g->elem;
-    if (hipMalloc(&ck->f, ck->fbytes) != hipSuccess || hipMalloc(&ck->rho, g->ncell * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ck->u, g->ncell * g->D * sizeof(double)) != hipSuccess) {
+    if (hipMalloc(&ck->f, ck->fbytes) != hipSuccess || hipMalloc(&ck->rho, g->ncell * g->melem) != hipSuccess ||
+        hipMalloc(&ck->u, g->ncell * g->D * g->melem) != hipSuccess) {
         lumagpu_checkpoint_free(ck);
         return fail(LUMAGPU_ENOMEM, "checkpoint allocation failed");
     }
     if (hipMemcpyAsync(ck->f, g->f, ck->fbytes, hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
-        hipMemcpyAsync(ck->rho, g->rho, g->ncell * sizeof(double), hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
-        hipMemcpyAsync(ck->u, g->u, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
+        hipMemcpyAsync(ck->rho, g->rho, g->ncell * g->melem, hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
+        hipMemcpyAsync(ck->u, g->u, g->ncell * g->D * g->melem, hipMemcpyDeviceToDevice, g->stream) != hipSuccess ||
         hipStreamSynchronize(g->stream) != hipSuccess) {
         lumagpu_checkpoint_free(ck);
         return fail(LUMAGPU_EHIP, "checkpoint copy failed");
@@ -490,8 +527,8 @@ int lumagpu_checkpoint_restore(lumagpu_grid *g, const lumagpu_checkpoint *ck) {
     if (!ck) return fail(LUMAGPU_EINVAL, "null checkpoint");
     if (ck->owner != g) return fail(LUMAGPU_EINVAL, "checkpoint belongs to another grid");
     HIP_TRY(hipMemcpyAsync(g->f, ck->f, ck->fbytes, hipMemcpyDeviceToDevice, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->rho, ck->rho, g->ncell * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
-    HIP_TRY(hipMemcpyAsync(g->u, ck->u, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->rho, ck->rho, g->ncell * g->melem, hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->u, ck->u, g->ncell * g->D * g->melem, hipMemcpyDeviceToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->parity = ck->parity;
     g->t = ck->t;
@@ -518,6 +555,7 @@ int lumagpu_set_zslab(lumagpu_grid *g, int on) {
         return fail(LUMAGPU_EINVAL, "a z-slab needs a D3Q19 grid with at least one interior and two halo planes");
     if (g->masked || g->d.forced || g->parent || g->child || g->ibm)
         return fail(LUMAGPU_EUNSUPPORTED, "z-slab halo exchange is implemented for unlabelled, unforced single-level grids");
+    if (g->d.precision == LUMAGPU_FP64T) return fail(LUMAGPU_EUNSUPPORTED, "z-slabs run in the FP64 and shifted forms");
     if (g->parity != 0) return fail(LUMAGPU_ESTATE, "switch to a z-slab between step pairs");
     g->zslab = 1;
     return 0;
@@ -590,10 +628,7 @@ int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u
     HIP_TRY(hipMemcpyAsync(drho, rho, g->ncell * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipMemcpyAsync(du, u, g->ncell * g->D * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(launch_aux(g, kAuxInitFeq, drho, du, nullptr));
-    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
-    if (g->D == 2) k_macro_from_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
-    else k_macro_from_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(macro_from_luma(g, drho, du));
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->parity = 0;
     g->has_state = 1;
@@ -621,10 +656,7 @@ int lumagpu_upload_macro(lumagpu_grid *g, const double *rho, const double *u) {
     double *drho = g->stage, *du = g->stage + g->ncell;
     HIP_TRY(hipMemcpyAsync(drho, rho, g->ncell * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipMemcpyAsync(du, u, g->ncell * g->D * sizeof(double), hipMemcpyHostToDevice, g->stream));
-    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
-    if (g->D == 2) k_macro_from_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
-    else k_macro_from_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(macro_from_luma(g, drho, du));
     HIP_TRY(hipStreamSynchronize(g->stream));
     return 0;
 }
@@ -647,13 +679,31 @@ int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u) {
     const size_t nb = g->ncell * (1 + g->D) * sizeof(double);
     if (int r = ensure_stage(g, nb)) return r;
     double *drho = g->stage, *du = g->stage + g->ncell;
-    const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
-    if (g->D == 2) k_macro_to_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
-    else k_macro_to_luma<3><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, g->d.N, g->d.M, g->d.K, drho, du);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(macro_to_luma(g, drho, du, false));
     HIP_TRY(hipMemcpyAsync(rho, drho, g->ncell * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipMemcpyAsync(u, du, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_download_tangent(lumagpu_grid *g, double *df, double *drho, double *du) {
+    if (int r = check_grid(g)) return r;
+    if (g->d.precision != LUMAGPU_FP64T) return fail(LUMAGPU_ESTATE, "derivatives exist on tangent-form grids only");
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "no state to download");
+    const size_t nf = g->ncell * g->Q * sizeof(double), nm = g->ncell * (1 + g->D) * sizeof(double);
+    if (int r = ensure_stage(g, nf > nm ? nf : nm)) return r;
+    if (df) {
+        HIP_TRY(launch_aux(g, kAuxDownloadTan, nullptr, nullptr, g->stage));
+        HIP_TRY(hipMemcpyAsync(df, g->stage, nf, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    if (drho || du) {
+        double *sr = g->stage, *su = g->stage + g->ncell;
+        HIP_TRY(macro_to_luma(g, sr, su, true));
+        if (drho) HIP_TRY(hipMemcpyAsync(drho, sr, g->ncell * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        if (du) HIP_TRY(hipMemcpyAsync(du, su, g->ncell * g->D * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
     return 0;
 }
 
@@ -663,7 +713,8 @@ int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz) {
     if (!force_xyz) return fail(LUMAGPU_EINVAL, "null force");
     const int N = g->d.N, M = g->d.M, K = g->d.K, D = g->D;
     if (g->hflags.empty()) g->hflags.assign(g->ncell, (uint8_t)kFluid);
-    std::vector<double> soa(g->ncell * D);
+    const size_t w = g->melem / sizeof(double);   // (value, derivative) pairs in the tangent form
+    std::vector<double> soa(g->ncell * D * w, 0.0);
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < M; ++j)
             for (int k = 0; k < K; ++k) {
@@ -672,7 +723,7 @@ int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz) {
                 bool nz = false;
                 for (int d = 0; d < D; ++d) {
                     const double v = force_xyz[d + D * id];
-                    soa[d * g->ncell + cell] = v;
+                    soa[w * (d * g->ncell + cell)] = v;
                     nz = nz || v != 0.0;
                 }
                 g->hflags[cell] = (uint8_t)((g->hflags[cell] & ~kForcedBit) | (nz ? kForcedBit : 0));

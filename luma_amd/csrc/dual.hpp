@@ -1,0 +1,87 @@
+// dual.hpp -- forward-mode derivative numbers for the tangent form of the step.
+//
+// A Dual carries a value and its derivative along one input direction (here: a parameter
+// such as a filament's Young's modulus).  Every operation computes the value exactly as the
+// plain double operation would (same operands, same order, no fused multiply-add in the
+// translation units that use it), so a kernel instantiated on Dual reproduces the FP64
+// kernel's values bit for bit and carries the exact derivative of that computation beside
+// them.  Derivative rules: (a b)' = a' b + a b', (a / b)' = (a' - (a/b) b') / b.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace lumagpu {
+
+struct Dual {
+    double v, d;
+    __host__ __device__ constexpr Dual() : v(0.0), d(0.0) {}
+    __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
+    __host__ __device__ constexpr Dual(double x, double dx) : v(x), d(dx) {}
+    __host__ __device__ explicit constexpr operator double() const { return v; }
+    __host__ __device__ Dual &operator+=(const Dual &b) {
+        v += b.v;
+        d += b.d;
+        return *this;
+    }
+    __host__ __device__ Dual &operator-=(const Dual &b) {
+        v -= b.v;
+        d -= b.d;
+        return *this;
+    }
+    __host__ __device__ Dual &operator*=(const Dual &b) {
+        const double nd = d * b.v + v * b.d;
+        v *= b.v;
+        d = nd;
+        return *this;
+    }
+    __host__ __device__ Dual &operator/=(const Dual &b) {
+        const double q = v / b.v;
+        d = (d - q * b.d) / b.v;
+        v = q;
+        return *this;
+    }
+};
+
+__host__ __device__ inline Dual operator+(const Dual &a, const Dual &b) { return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline Dual operator-(const Dual &a, const Dual &b) { return Dual(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline Dual operator-(const Dual &a) { return Dual(-a.v, -a.d); }
+__host__ __device__ inline Dual operator*(const Dual &a, const Dual &b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__host__ __device__ inline Dual operator/(const Dual &a, const Dual &b) {
+    const double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+// comparisons look at values only (branches follow the primal computation)
+__host__ __device__ inline bool operator<(const Dual &a, const Dual &b) { return a.v < b.v; }
+__host__ __device__ inline bool operator>(const Dual &a, const Dual &b) { return a.v > b.v; }
+
+// value part of either number type
+__host__ __device__ inline double value_of(double x) { return x; }
+__host__ __device__ inline double value_of(const Dual &x) { return x.v; }
+__host__ __device__ inline double deriv_of(double) { return 0.0; }
+__host__ __device__ inline double deriv_of(const Dual &x) { return x.d; }
+// both parts zero
+__host__ __device__ inline bool is_zero(double x) { return x == 0.0; }
+__host__ __device__ inline bool is_zero(const Dual &x) { return x.v == 0.0 && x.d == 0.0; }
+
+// host math on Duals (the FEM beam's corotational geometry); found by argument-dependent
+// lookup next to std:: for doubles
+inline Dual sqrt(const Dual &a) {
+    const double s = std::sqrt(a.v);
+    return Dual(s, a.d / (2.0 * s));
+}
+inline Dual cos(const Dual &a) { return Dual(std::cos(a.v), -std::sin(a.v) * a.d); }
+inline Dual sin(const Dual &a) { return Dual(std::sin(a.v), std::cos(a.v) * a.d); }
+inline Dual atan2(const Dual &y, const Dual &x) {
+    const double r2 = x.v * x.v + y.v * y.v;
+    return Dual(std::atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / r2);
+}
+inline Dual pow(const Dual &a, int n) {
+    // integer powers as the double code calls them (std::pow(x, 2))
+    const double p = std::pow(a.v, n);
+    return Dual(p, n * std::pow(a.v, n - 1) * a.d);
+}
+inline Dual fabs(const Dual &a) { return a.v < 0.0 ? -a : a; }
+
+}  // namespace lumagpu

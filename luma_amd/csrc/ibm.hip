@@ -42,6 +42,29 @@ double delta_kernel_deriv(double radius, double dilation) {
     return sgn * d / dilation;
 }
 
+double delta_at(const double *x, const double *s, double dh, double dil, int D) {
+    double v = 1.0;
+    for (int a = 0; a < D; ++a) v *= delta_kernel((x[a] - s[a]) / dh, dil);
+    return v;
+}
+
+void delta_grad(const double *x, const double *s, double dh, double dil, int D, double out[3]) {
+    double ph[3], dph[3];
+    for (int a = 0; a < 3; ++a) {
+        const double r = (a < D) ? (x[a] - s[a]) / dh : 0.0;
+        ph[a] = (a < D) ? delta_kernel(r, dil) : 1.0;
+        dph[a] = (a < D) ? delta_kernel_deriv(r, dil) / dh : 0.0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        out[a] = 0.0;
+        if (a >= D) continue;
+        double v = dph[a];
+        for (int b = 0; b < D; ++b)
+            if (b != a) v *= ph[b];
+        out[a] = v;
+    }
+}
+
 IbmArgs ibm_args(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
     IbmArgs b{};
@@ -136,62 +159,89 @@ int find_support(const lumagpu_grid *g, Body &b, int ib) {
     return 0;
 }
 
-// ibm_computeDs, :927-981
+// ibm_computeDs, :927-981; on a tangent-form grid also d ds along the marker tangents
+// (through the nearest marker: ds = |x_m - x_n| / dh)
 void compute_ds(const lumagpu_grid *g, Body &b) {
+    const double dh = g->d.dh;
     for (size_t mi = 0; mi < b.m.size(); ++mi) {
         double ds = 10.0;
+        size_t arg = mi;
         for (size_t n = 0; n < b.m.size(); ++n) {
             if (n == mi) continue;
             double acc = 0.0;
             for (int a = 0; a < 3; ++a) acc += std::pow(b.m[mi].pos[a] - b.m[n].pos[a], 2);
-            const double dist = std::sqrt(acc) / g->d.dh;
-            if (dist < ds) ds = dist;
+            const double dist = std::sqrt(acc) / dh;
+            if (dist < ds) {
+                ds = dist;
+                arg = n;
+            }
         }
-        b.m[mi].ds = ds;
+        Marker &m = b.m[mi];
+        m.ds = ds;
+        m.dds = 0.0;
+        if (tangent_form(g) && arg != mi && ds > 0.0) {
+            const Marker &q = b.m[arg];
+            double dd = 0.0;
+            for (int a = 0; a < 3; ++a) dd += (m.pos[a] - q.pos[a]) * (m.dpos[a] - q.dpos[a]);
+            m.dds = dd / (ds * dh * dh);
+        }
+    }
+}
+
+// derivatives of the support weights along the marker tangents (tangent-form grids)
+void support_tangents(const lumagpu_grid *g, Body &b) {
+    for (Marker &m : b.m) {
+        for (int k = 0; k < m.ns; ++k) {
+            double gr[3];
+            delta_grad(m.pos, m.sxyz[k], g->d.dh, m.dilation, g->D, gr);
+            m.ddelta[k] = gr[0] * m.dpos[0] + gr[1] * m.dpos[1] + gr[2] * m.dpos[2];
+        }
     }
 }
 
 }  // namespace
 
 // A x = rhs for row-major A: factor the column-major view (A^T) with row pivoting, then solve
-// through its transpose; the operation order of dgetf2 followed by dgetrs('T').
-int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
-    auto M = [&](int i, int j) -> double & { return a[(size_t)j * n + i]; };
+// through its transpose; the operation order of dgetf2 followed by dgetrs('T').  Pivots are
+// chosen on values; on Duals every operation also carries the derivative.
+template <class Sc>
+int lu_solve_t(int n, std::vector<Sc> &a, std::vector<Sc> &x) {
+    auto M = [&](int i, int j) -> Sc & { return a[(size_t)j * n + i]; };
     std::vector<int> piv(n), rows;
     rows.reserve(n);
     for (int j = 0; j < n; ++j) {
         int p = j;
-        double best = std::fabs(M(j, j));
+        double best = std::fabs(value_of(M(j, j)));
         for (int i = j + 1; i < n; ++i)
-            if (std::fabs(M(i, j)) > best) {
-                best = std::fabs(M(i, j));
+            if (std::fabs(value_of(M(i, j))) > best) {
+                best = std::fabs(value_of(M(i, j)));
                 p = i;
             }
         piv[j] = p;
-        if (M(p, j) == 0.0) return fail(LUMAGPU_EPHYS, "singular IB epsilon system");
+        if (value_of(M(p, j)) == 0.0) return fail(LUMAGPU_EPHYS, "singular IB epsilon system");
         if (p != j)
             for (int k = 0; k < n; ++k) std::swap(M(j, k), M(p, k));
-        const double r = 1.0 / M(j, j);
+        const Sc r = Sc(1.0) / M(j, j);
         rows.clear();
         for (int i = j + 1; i < n; ++i) {
             M(i, j) *= r;
-            if (M(i, j) != 0.0) rows.push_back(i);
+            if (!is_zero(M(i, j))) rows.push_back(i);
         }
         // rank-1 update; a zero multiplier or pivot-row entry changes nothing but the sign of a
         // zero, so only the nonzero pairs are visited (the IB systems are sparse and banded)
         for (int k = j + 1; k < n; ++k) {
-            const double t = M(j, k);
-            if (t != 0.0)
+            const Sc t = M(j, k);
+            if (!is_zero(t))
                 for (int i : rows) M(i, k) -= M(i, j) * t;
         }
     }
     for (int j = 0; j < n; ++j) {
-        double t = x[j];
+        Sc t = x[j];
         for (int i = 0; i < j; ++i) t -= M(i, j) * x[i];
         x[j] = t / M(j, j);
     }
     for (int j = n - 1; j >= 0; --j) {
-        double t = x[j];
+        Sc t = x[j];
         for (int i = j + 1; i < n; ++i) t -= M(i, j) * x[i];
         x[j] = t;
     }
@@ -200,13 +250,20 @@ int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) {
     return 0;
 }
 
+int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) { return lu_solve_t(n, a, x); }
+int lu_solve(int n, std::vector<Dual> &a, std::vector<Dual> &x) { return lu_solve_t(n, a, x); }
+
 namespace {
 
-// ibm_findEpsilon, :773-920: A_IJ = ds_J * sum_s delta_I(s) delta(x_J - x_s) A_I, A eps = 1
+// ibm_findEpsilon, :773-920: A_IJ = ds_J * sum_s delta_I(s) delta(x_J - x_s) A_I, A eps = 1.
+// On a tangent-form grid also d eps from A d(eps) = -dA eps, with dA from the marker, delta
+// and ds tangents.
 int find_epsilon(const lumagpu_grid *g, Body &b) {
     const int n = (int)b.m.size(), D = g->D;
     const double dh = g->d.dh;
-    std::vector<double> A((size_t)n * n, 0.0), rhs(n, 1.0);
+    const bool tan = tangent_form(g);
+    std::vector<double> A((size_t)n * n, 0.0), rhs(n, 1.0), dA;
+    if (tan) dA.assign((size_t)n * n, 0.0);
     for (int I = 0; I < n; ++I) {
         const Marker &mI = b.m[I];
         for (int J = 0; J < n; ++J) {
@@ -228,10 +285,33 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
                 acc += mI.delta[k] * dJ * mI.area;
             }
             A[(size_t)I * n + J] = acc * mJ.ds;
+            if (tan) {
+                double dacc = 0.0;
+                for (int k = 0; k < mI.ns; ++k) {
+                    double gJ[3];
+                    delta_grad(mJ.pos, mI.sxyz[k], dh, mJ.dilation, D, gJ);
+                    const double dJ = delta_at(mJ.pos, mI.sxyz[k], dh, mJ.dilation, D);
+                    const double ddJ = gJ[0] * mJ.dpos[0] + gJ[1] * mJ.dpos[1] + gJ[2] * mJ.dpos[2];
+                    dacc += (mI.ddelta[k] * dJ + mI.delta[k] * ddJ) * mI.area;
+                }
+                dA[(size_t)I * n + J] = acc * mJ.dds + dacc * mJ.ds;
+            }
         }
     }
+    std::vector<double> A0;
+    if (tan) A0 = A;
     if (int r = lu_solve(n, A, rhs)) return r;
     for (int i = 0; i < n; ++i) b.m[i].eps = rhs[i];
+    if (tan) {
+        std::vector<double> drhs(n, 0.0);
+        for (int I = 0; I < n; ++I) {
+            double t = 0.0;
+            for (int J = 0; J < n; ++J) t += dA[(size_t)I * n + J] * rhs[J];
+            drhs[I] = -t;
+        }
+        if (int r = lu_solve(n, A0, drhs)) return r;
+        for (int i = 0; i < n; ++i) b.m[i].deps = drhs[i];
+    }
     return 0;
 }
 
@@ -246,8 +326,14 @@ int upload(lumagpu_grid *g) {
         nm += (int)b.m.size();
     }
     s->nm = nm;
+    // tangent form: every FP64 marker quantity is a (value, derivative) pair
+    const size_t w = tangent_form(g) ? 2 : 1;
+    auto put = [w](std::vector<double> &v, size_t i, double x, double dx) {
+        v[w * i] = x;
+        if (w == 2) v[w * i + 1] = dx;
+    };
     std::vector<int> scell((size_t)nm * S, 0), ns(nm);
-    std::vector<double> sdelta((size_t)nm * S, 0.0), vel((size_t)nm * 3), eps(nm), ds(nm);
+    std::vector<double> sdelta((size_t)nm * S * w, 0.0), vel((size_t)nm * 3 * w), eps(nm * w), ds(nm * w);
     std::vector<int> site_of(g->ncell, -1), order;
     std::vector<std::vector<int>> entries;
     for (const auto &b : s->bodies)
@@ -255,13 +341,13 @@ int upload(lumagpu_grid *g) {
             const Marker &m = b.m[mi];
             const int gm = b.first + (int)mi;
             ns[gm] = m.ns;
-            for (int a = 0; a < 3; ++a) vel[3 * gm + a] = m.vel[a];
-            eps[gm] = m.eps;
-            ds[gm] = m.ds;
+            for (int a = 0; a < 3; ++a) put(vel, 3 * (size_t)gm + a, m.vel[a], m.dvel[a]);
+            put(eps, gm, m.eps, m.deps);
+            put(ds, gm, m.ds, m.dds);
             for (int k = 0; k < m.ns; ++k) {
                 const int cell = m.sijk[k][0] + N * (m.sijk[k][1] + M * m.sijk[k][2]);
                 scell[(size_t)gm * S + k] = cell;
-                sdelta[(size_t)gm * S + k] = m.delta[k];
+                put(sdelta, (size_t)gm * S + k, m.delta[k], m.ddelta[k]);
                 if (site_of[cell] < 0) {
                     const unsigned fl = g->hflags.empty() ? 0u : g->hflags[cell];
                     if ((fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)
@@ -286,10 +372,10 @@ int upload(lumagpu_grid *g) {
         (r = dev_upload(&s->d_sites, c + 2, order, g->stream)) || (r = dev_upload(&s->d_sptr, c + 3, sptr, g->stream)) ||
         (r = dev_upload(&s->d_sent, c + 4, sent, g->stream)) || (r = dev_upload(&s->d_sdelta, c + 5, sdelta, g->stream)) ||
         (r = dev_upload(&s->d_vel, c + 6, vel, g->stream)) || (r = dev_upload(&s->d_eps, c + 7, eps, g->stream)) ||
-        (r = dev_upload(&s->d_ds, c + 8, ds, g->stream)) || (r = dev_resize(&s->d_interp, c + 9, (size_t)nm * 4, g->stream)) ||
-        (r = dev_resize(&s->d_force, c + 10, (size_t)nm * 3, g->stream)))
+        (r = dev_upload(&s->d_ds, c + 8, ds, g->stream)) || (r = dev_resize(&s->d_interp, c + 9, (size_t)nm * 4 * w, g->stream)) ||
+        (r = dev_resize(&s->d_force, c + 10, (size_t)nm * 3 * w, g->stream)))
         return r;
-    if (!s->d_out && hipMalloc(&s->d_out, 3 * sizeof(double)) != hipSuccess)
+    if (!s->d_out && hipMalloc(&s->d_out, 6 * sizeof(double)) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
     HIP_TRY(hipStreamSynchronize(g->stream));
     if (order != s->sites) s->sites_dirty = 1;
@@ -302,6 +388,7 @@ hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int 
     switch (g->d.precision) {
         case LUMAGPU_FP64: return launch_sites_f64(g->Q, a, odd, n, cells, g->stream);
         case LUMAGPU_FP32S: return launch_sites_f32s(g->Q, a, odd, n, cells, g->stream);
+        case LUMAGPU_FP64T: return launch_sites_dual(g->Q, a, odd, n, cells, g->stream);
         default: return launch_sites_f16s(g->Q, a, odd, n, cells, g->stream);
     }
 }
@@ -358,7 +445,9 @@ int apply_core(lumagpu_grid *g) {
             size_t n_gone = 0;
             if (int r = dev_upload(&d_gone, &n_gone, gone, g->stream)) return r;
             IbmArgs b = ibm_args(g);
-            k_ibm_clear<<<((int)gone.size() + 127) / 128, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
+            const int nb = ((int)gone.size() + 127) / 128;
+            if (tangent_form(g)) k_ibm_clear<Dual><<<nb, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
+            else k_ibm_clear<double><<<nb, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipStreamSynchronize(g->stream));
             (void)hipFree(d_gone);
@@ -369,9 +458,15 @@ int apply_core(lumagpu_grid *g) {
     hipError_t e = launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB site macro launch: ") + hipGetErrorString(e));
     IbmArgs b = ibm_args(g);
-    k_ibm_interp_force<<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b);
-    HIP_TRY(hipGetLastError());
-    k_ibm_spread<<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
+    if (tangent_form(g)) {
+        k_ibm_interp_force<Dual><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b);
+        HIP_TRY(hipGetLastError());
+        k_ibm_spread<Dual><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
+    } else {
+        k_ibm_interp_force<double><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b);
+        HIP_TRY(hipGetLastError());
+        k_ibm_spread<double><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
+    }
     HIP_TRY(hipGetLastError());
     if (s->sites_dirty) {
         for (int c : s->sites) g->hflags[c] |= kForcedBit;
@@ -385,7 +480,58 @@ namespace {
 
 // ibm_moveBodies (:61-104): structural step of every flexible body from the marker forces,
 // new supports for flexible bodies, then ds and epsilon of all bodies on the grid.
+// The tangent form runs the same beam on Duals: marker forces, epsilon, positions and
+// velocities carry their derivatives through the Newton solve.
+int move_bodies_tangent(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    std::vector<Dual> force((size_t)s->nm * 3);
+    HIP_TRY(hipMemcpyAsync(force.data(), s->d_force, force.size() * sizeof(Dual), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    const FemBeamT<Dual>::Solver solver = [](int n, std::vector<Dual> &a, std::vector<Dual> &x) {
+        return lu_solve(n, a, x);
+    };
+    for (auto &b : s->bodies) {
+        if (!b.flexible || !b.femd) continue;
+        const size_t n = b.m.size();
+        std::vector<Dual> eps(n), pos(3 * n), pos0(3 * n), vel(3 * n), vkm1(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            const Marker &m = b.m[i];
+            eps[i] = Dual(m.eps, m.deps);
+            for (int a = 0; a < 3; ++a) {
+                pos[3 * i + a] = Dual(m.pos[a], m.dpos[a]);
+                pos0[3 * i + a] = Dual(m.pos0[a]);
+                vel[3 * i + a] = Dual(m.vel[a], m.dvel[a]);
+                vkm1[3 * i + a] = Dual(m.vel_km1[a], m.dvel_km1[a]);
+            }
+        }
+        if (int r = b.femd->dynamic(force.data() + 3 * (size_t)b.first, eps.data(), pos.data(), pos0.data(),
+                                    vel.data(), vkm1.data(), solver))
+            return r;
+        for (size_t i = 0; i < n; ++i) {
+            Marker &m = b.m[i];
+            for (int a = 0; a < 3; ++a) {
+                m.pos[a] = pos[3 * i + a].v;
+                m.dpos[a] = pos[3 * i + a].d;
+                m.vel[a] = vel[3 * i + a].v;
+                m.dvel[a] = vel[3 * i + a].d;
+                m.vel_km1[a] = vkm1[3 * i + a].v;
+                m.dvel_km1[a] = vkm1[3 * i + a].d;
+            }
+        }
+    }
+    for (size_t i = 0; i < s->bodies.size(); ++i)
+        if (s->bodies[i].flexible) {
+            if (int r = find_support(g, s->bodies[i], (int)i)) return r;
+            support_tangents(g, s->bodies[i]);
+        }
+    for (auto &b : s->bodies) compute_ds(g, b);
+    for (auto &b : s->bodies)
+        if (int r = find_epsilon(g, b)) return r;
+    return upload(g);
+}
+
 int move_bodies(lumagpu_grid *g) {
+    if (tangent_form(g)) return move_bodies_tangent(g);
     IbmSystem *s = g->ibm;
     std::vector<double> force((size_t)s->nm * 3);
     HIP_TRY(hipMemcpyAsync(force.data(), s->d_force, force.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
@@ -464,10 +610,69 @@ int ibm_apply(lumagpu_grid *g) {
     } while (res > 1e-4 && it < 10);
     s->sub_it = it;
     s->sub_res = res;
-    for (auto &b : s->bodies)
+    for (auto &b : s->bodies) {
         if (b.flexible && b.fem) b.fem->commit();
+        if (b.flexible && b.femd) b.femd->commit();
+    }
     return 0;
 }
+
+namespace {
+
+// 16 doubles per marker: position, markerVel, force_xyz, interpMom, interpRho, epsilon, ds and
+// the support count -- or, with tan, the derivatives of the same (the count's is 0)
+int marker_record(lumagpu_grid *g, int body, double *rec, bool tan) {
+    if (int r = check_body(g, body)) return r;
+    if (!rec) return fail(LUMAGPU_EINVAL, "null output");
+    IbmSystem *s = g->ibm;
+    const Body &b = s->bodies[body];
+    const size_t n = b.m.size(), w = tangent_form(g) ? 2 : 1, part = tan ? 1 : 0;
+    std::vector<double> interp(4 * n * w, 0.0), force(3 * n * w, 0.0);
+    if (s->initialised) {
+        HIP_TRY(hipMemcpyAsync(interp.data(), s->d_interp + 4 * w * b.first, interp.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipMemcpyAsync(force.data(), s->d_force + 3 * w * b.first, force.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+    for (size_t i = 0; i < n; ++i) {
+        double *o = rec + 16 * i;
+        const Marker &m = b.m[i];
+        for (int a = 0; a < 3; ++a) {
+            o[a] = tan ? m.dpos[a] : m.pos[a];
+            o[3 + a] = tan ? m.dvel[a] : m.vel[a];
+            o[6 + a] = force[w * (3 * i + a) + part];
+            o[9 + a] = interp[w * (4 * i + 1 + a) + part];
+        }
+        o[12] = interp[w * (4 * i) + part];
+        o[13] = tan ? m.deps : m.eps;
+        o[14] = tan ? m.dds : m.ds;
+        o[15] = tan ? 0.0 : m.ns;
+    }
+    return 0;
+}
+
+int body_force(lumagpu_grid *g, int body, double *out, double *dout) {
+    if (int r = check_body(g, body)) return r;
+    if (!out) return fail(LUMAGPU_EINVAL, "null output");
+    IbmSystem *s = g->ibm;
+    if (!s->initialised) return fail(LUMAGPU_ESTATE, "call lumagpu_ibm_initialise first");
+    const Body &b = s->bodies[body];
+    IbmArgs a = ibm_args(g);
+    if (tangent_form(g)) k_ibm_body_force<Dual><<<1, 64, 0, g->stream>>>(a, b.first, b.first + (int)b.m.size(), s->d_out);
+    else k_ibm_body_force<double><<<1, 64, 0, g->stream>>>(a, b.first, b.first + (int)b.m.size(), s->d_out);
+    HIP_TRY(hipGetLastError());
+    double h[6] = {0, 0, 0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(h, s->d_out, (tangent_form(g) ? 6 : 3) * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    for (int d = 0; d < 3; ++d) {
+        out[d] = h[d];
+        if (dout) dout[d] = h[3 + d];
+    }
+    return 0;
+}
+
+}  // namespace
 
 }  // namespace lumagpu
 
@@ -527,9 +732,15 @@ int lumagpu_ibm_add_filament(lumagpu_grid *g, const double *start, double length
     b.m.resize(n);
     for (int i = 0; i < n; ++i)
         for (int a = 0; a < 3; ++a) b.m[i].pos[a] = b.m[i].pos0[a] = pos[3 * i + a];
-    b.fem.reset(new FemBeam(start, length, height, angle_deg, n_elements, clamped != 0, density, youngs, dh,
-                            g->d.dt, g->d.dm));
-    if (b.fem->ib_nodes() != n) return fail(LUMAGPU_EINVAL, "filament IB-FEM node mapping mismatch");
+    if (tangent_form(g)) {
+        b.femd.reset(new FemBeamT<Dual>(start, length, height, angle_deg, n_elements, clamped != 0, Dual(density),
+                                        Dual(youngs), dh, g->d.dt, g->d.dm));
+        if (b.femd->ib_nodes() != n) return fail(LUMAGPU_EINVAL, "filament IB-FEM node mapping mismatch");
+    } else {
+        b.fem.reset(new FemBeam(start, length, height, angle_deg, n_elements, clamped != 0, density, youngs, dh,
+                                g->d.dt, g->d.dm));
+        if (b.fem->ib_nodes() != n) return fail(LUMAGPU_EINVAL, "filament IB-FEM node mapping mismatch");
+    }
     s->bodies.push_back(std::move(b));
     s->initialised = 0;
     if (body) *body = (int)s->bodies.size() - 1;
@@ -552,8 +763,10 @@ int lumagpu_ibm_initialise(lumagpu_grid *g) {
         return fail(LUMAGPU_ESTATE, "grid positions are not set (lumagpu_set_labels or lumagpu_set_positions)");
     if (int r = ensure_flags(g)) return r;
     IbmSystem *s = g->ibm;
-    for (size_t i = 0; i < s->bodies.size(); ++i)
+    for (size_t i = 0; i < s->bodies.size(); ++i) {
         if (int r = find_support(g, s->bodies[i], (int)i)) return r;
+        if (tangent_form(g)) support_tangents(g, s->bodies[i]);
+    }
     for (auto &b : s->bodies) compute_ds(g, b);
     for (auto &b : s->bodies)
         if (int r = find_epsilon(g, b)) return r;
@@ -570,12 +783,16 @@ int lumagpu_ibm_set_velocity(lumagpu_grid *g, int body, const double *vel) {
     if (!vel) return fail(LUMAGPU_EINVAL, "null velocity array");
     Body &b = g->ibm->bodies[body];
     for (size_t i = 0; i < b.m.size(); ++i)
-        for (int a = 0; a < 3; ++a) b.m[i].vel[a] = vel[3 * i + a];
+        for (int a = 0; a < 3; ++a) {
+            b.m[i].vel[a] = vel[3 * i + a];
+            b.m[i].dvel[a] = 0.0;
+        }
     if (g->ibm->initialised) {
-        std::vector<double> v(3 * b.m.size());
+        const size_t w = tangent_form(g) ? 2 : 1;
+        std::vector<double> v(3 * b.m.size() * w, 0.0);
         for (size_t i = 0; i < b.m.size(); ++i)
-            for (int a = 0; a < 3; ++a) v[3 * i + a] = b.m[i].vel[a];
-        HIP_TRY(hipMemcpyAsync(g->ibm->d_vel + 3 * b.first, v.data(), v.size() * sizeof(double),
+            for (int a = 0; a < 3; ++a) v[w * (3 * i + a)] = b.m[i].vel[a];
+        HIP_TRY(hipMemcpyAsync(g->ibm->d_vel + 3 * w * b.first, v.data(), v.size() * sizeof(double),
                                hipMemcpyHostToDevice, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
     }
@@ -587,41 +804,33 @@ int lumagpu_ibm_set_epsilon(lumagpu_grid *g, int body, const double *eps) {
     if (!eps) return fail(LUMAGPU_EINVAL, "null epsilon array");
     if (!g->ibm->initialised) return fail(LUMAGPU_ESTATE, "call lumagpu_ibm_initialise first");
     Body &b = g->ibm->bodies[body];
-    for (size_t i = 0; i < b.m.size(); ++i) b.m[i].eps = eps[i];
-    HIP_TRY(hipMemcpyAsync(g->ibm->d_eps + b.first, eps, b.m.size() * sizeof(double), hipMemcpyHostToDevice,
+    const size_t w = tangent_form(g) ? 2 : 1;
+    std::vector<double> h(b.m.size() * w, 0.0);
+    for (size_t i = 0; i < b.m.size(); ++i) {
+        b.m[i].eps = eps[i];
+        b.m[i].deps = 0.0;
+        h[w * i] = eps[i];
+    }
+    HIP_TRY(hipMemcpyAsync(g->ibm->d_eps + w * b.first, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                            g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     return 0;
 }
 
-int lumagpu_ibm_get_markers(lumagpu_grid *g, int body, double *rec) {
+int lumagpu_ibm_get_markers(lumagpu_grid *g, int body, double *rec) { return marker_record(g, body, rec, false); }
+
+int lumagpu_ibm_get_marker_tangents(lumagpu_grid *g, int body, double *rec) {
     if (int r = check_body(g, body)) return r;
-    if (!rec) return fail(LUMAGPU_EINVAL, "null output");
-    IbmSystem *s = g->ibm;
-    const Body &b = s->bodies[body];
-    const size_t n = b.m.size();
-    std::vector<double> interp(4 * n, 0.0), force(3 * n, 0.0);
-    if (s->initialised) {
-        HIP_TRY(hipMemcpyAsync(interp.data(), s->d_interp + 4 * b.first, 4 * n * sizeof(double),
-                               hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipMemcpyAsync(force.data(), s->d_force + 3 * b.first, 3 * n * sizeof(double),
-                               hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
-    }
-    for (size_t i = 0; i < n; ++i) {
-        double *o = rec + 16 * i;
-        const Marker &m = b.m[i];
-        for (int a = 0; a < 3; ++a) {
-            o[a] = m.pos[a];
-            o[3 + a] = m.vel[a];
-            o[6 + a] = force[3 * i + a];
-            o[9 + a] = interp[4 * i + 1 + a];
-        }
-        o[12] = interp[4 * i];
-        o[13] = m.eps;
-        o[14] = m.ds;
-        o[15] = m.ns;
-    }
+    if (!tangent_form(g)) return fail(LUMAGPU_ESTATE, "marker derivatives exist on tangent-form grids only");
+    return marker_record(g, body, rec, true);
+}
+
+int lumagpu_ibm_seed_youngs(lumagpu_grid *g, int body, double dE) {
+    if (int r = check_body(g, body)) return r;
+    if (!tangent_form(g)) return fail(LUMAGPU_ESTATE, "derivative seeds need a tangent-form grid (LUMAGPU_FP64T)");
+    Body &b = g->ibm->bodies[body];
+    if (!b.flexible || !b.femd) return fail(LUMAGPU_EINVAL, "body " + std::to_string(body) + " is not a flexible filament");
+    b.femd->seed_youngs(dE);
     return 0;
 }
 
@@ -636,18 +845,13 @@ int lumagpu_ibm_count(const lumagpu_grid *g, int *nbodies, int body, int *nmarke
     return 0;
 }
 
-int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]) {
+int lumagpu_ibm_body_force(lumagpu_grid *g, int body, double out[3]) { return body_force(g, body, out, nullptr); }
+
+int lumagpu_ibm_body_force_tangent(lumagpu_grid *g, int body, double out[3], double dout[3]) {
     if (int r = check_body(g, body)) return r;
-    if (!out) return fail(LUMAGPU_EINVAL, "null output");
-    IbmSystem *s = g->ibm;
-    if (!s->initialised) return fail(LUMAGPU_ESTATE, "call lumagpu_ibm_initialise first");
-    const Body &b = s->bodies[body];
-    IbmArgs a = ibm_args(g);
-    k_ibm_body_force<<<1, 64, 0, g->stream>>>(a, b.first, b.first + (int)b.m.size(), s->d_out);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, s->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    return 0;
+    if (!tangent_form(g)) return fail(LUMAGPU_ESTATE, "derivatives exist on tangent-form grids only");
+    if (!dout) return fail(LUMAGPU_EINVAL, "null output");
+    return body_force(g, body, out, dout);
 }
 
 }  // extern "C"

@@ -1,4 +1,5 @@
-// ibm_kernels.hpp -- immersed-boundary device kernels (FP64 in every storage form).
+// ibm_kernels.hpp -- immersed-boundary device kernels (FP64 in every storage form; Duals in the
+// tangent form).
 //
 // Markers of all bodies on one grid are stored back to back in body order, each with up to S
 // support slots (S = 9 in 2D, 27 in 3D).  The spread is a gather: every distinct support site
@@ -11,10 +12,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
+#include "dual.hpp"
 #include "lattice.hpp"
 
 namespace lumagpu {
 
+// The double arrays hold elements of the kernels' number type M: FP64, or (value, derivative)
+// pairs on a tangent-form grid (support indices and counts stay integers).
 struct IbmArgs {
     int nm, S, D;
     const int *scell;      // [nm*S] support cell index (device cell order)
@@ -36,68 +42,94 @@ struct IbmArgs {
     const int *sent;       // entries: m*S + slot
 };
 
+template <class M, class A>
+__device__ __forceinline__ M *as(A *p) {
+    return reinterpret_cast<M *>(p);
+}
+template <class M, class A>
+__device__ __forceinline__ const M *as(const A *p) {
+    return reinterpret_cast<const M *>(p);
+}
+
 // ibm_interpolate (:498-591) and ibm_computeForce (:598-614); one thread per marker, sums in
 // support order.  local_area = 1 (IBMarker.cpp:61).
-static __global__ void k_ibm_interp_force(IbmArgs b) {
+template <class M = double>
+__global__ void k_ibm_interp_force(IbmArgs b) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= b.nm) return;
-    const double area = 1.0;
-    double rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
+    const M *sdelta = as<M>(b.sdelta), *grho = as<M>(b.rho), *gu = as<M>(b.u), *vel = as<M>(b.vel);
+    const M area = 1.0;
+    M rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < b.ns[m]; ++k) {
         const size_t cell = (size_t)b.scell[m * b.S + k];
-        const double dl = b.sdelta[m * b.S + k];
-        const double r = b.rho[cell];
+        const M dl = sdelta[m * b.S + k];
+        const M r = grho[cell];
         rho += r * dl * area;
-        for (int d = 0; d < b.D; ++d) mom[d] += r * b.u[d * b.ncell + cell] * dl * area;
+        for (int d = 0; d < b.D; ++d) mom[d] += r * gu[d * b.ncell + cell] * dl * area;
     }
-    b.interp[4 * m] = rho;
-    for (int d = 0; d < 3; ++d) b.interp[4 * m + 1 + d] = mom[d];
-    for (int d = 0; d < b.D; ++d) b.force[3 * m + d] = 2.0 * (mom[d] - rho * b.vel[3 * m + d]) / 1.0;
+    M *interp = as<M>(b.interp), *force = as<M>(b.force);
+    interp[4 * m] = rho;
+    for (int d = 0; d < 3; ++d) interp[4 * m + 1 + d] = mom[d];
+    for (int d = 0; d < b.D; ++d) force[3 * m + d] = M(2.0) * (mom[d] - rho * vel[3 * m + d]) / M(1.0);
 }
 
 // ibm_spread (:621-689) as a per-site gather; the site's force starts from the reset value 0
 // (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
 // step kernel adds it in the macro (+F/2) and the Guo term.
-static __global__ void k_ibm_spread(IbmArgs b) {
+template <class M = double>
+__global__ void k_ibm_spread(IbmArgs b) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= b.nsites) return;
+    const M *sdelta = as<M>(b.sdelta), *force = as<M>(b.force), *eps = as<M>(b.eps), *ds = as<M>(b.ds);
     const size_t cell = (size_t)b.sites[s];
-    double acc[3] = {0.0, 0.0, 0.0};
+    M acc[3] = {0.0, 0.0, 0.0};
     for (int e = b.sptr[s]; e < b.sptr[s + 1]; ++e) {
         const int code = b.sent[e];
         const int m = code / b.S;
-        const double dl = b.sdelta[code];
-        const double volWidth = b.eps[m];
-        const double volDepth = (b.D == 3) ? b.ds[m] : 1.0;
-        for (int d = 0; d < b.D; ++d) acc[d] -= dl * b.force[3 * m + d] * volWidth * volDepth * b.ds[m];
+        const M dl = sdelta[code];
+        const M volWidth = eps[m];
+        const M volDepth = (b.D == 3) ? ds[m] : M(1.0);
+        for (int d = 0; d < b.D; ++d) acc[d] -= dl * force[3 * m + d] * volWidth * volDepth * ds[m];
     }
-    for (int d = 0; d < b.D; ++d) b.fxyz[d * b.ncell + cell] = acc[d];
+    M *fxyz = as<M>(b.fxyz);
+    for (int d = 0; d < b.D; ++d) fxyz[d * b.ncell + cell] = acc[d];
     b.flags[cell] |= kForcedBit;
 }
 
 // Clear the force and flag of sites that stop being supports.
-static __global__ void k_ibm_clear(IbmArgs b, int n, const int *cells) {
+template <class M = double>
+__global__ void k_ibm_clear(IbmArgs b, int n, const int *cells) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const size_t cell = (size_t)cells[s];
-    for (int d = 0; d < b.D; ++d) b.fxyz[d * b.ncell + cell] = 0.0;
+    M *fxyz = as<M>(b.fxyz);
+    for (int d = 0; d < b.D; ++d) fxyz[d * b.ncell + cell] = M(0.0);
     b.flags[cell] &= (uint8_t)~kForcedBit;
 }
 
 // FP64 sum over one body's markers of force * epsilon * volDepth * ds, the per-marker values
 // LUMA writes to Body_<id>_LD (ObjectManager_ops_io.cpp:98-121).  One 64-wide wave: strided
-// lane sums, then a fixed xor-shuffle tree, so the result does not depend on timing.
-static __global__ void k_ibm_body_force(IbmArgs b, int m0, int m1, double *out) {
+// lane sums, then a fixed xor-shuffle tree, so the result does not depend on timing.  out[0..2]
+// receives the sums; on a tangent-form grid out[3..5] their derivatives.
+template <class M = double>
+__global__ void k_ibm_body_force(IbmArgs b, int m0, int m1, double *out) {
     const int lane = threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
+    const M *force = as<M>(b.force), *eps = as<M>(b.eps), *ds = as<M>(b.ds);
+    M acc[3] = {0.0, 0.0, 0.0};
     for (int m = m0 + lane; m < m1; m += 64) {
-        const double w = b.eps[m] * ((b.D == 3) ? b.ds[m] : 1.0) * b.ds[m];
-        for (int d = 0; d < 3; ++d) acc[d] += b.force[3 * m + d] * w;
+        const M w = eps[m] * ((b.D == 3) ? ds[m] : M(1.0)) * ds[m];
+        for (int d = 0; d < 3; ++d) acc[d] += force[3 * m + d] * w;
     }
     for (int d = 0; d < 3; ++d) {
-        double v = acc[d];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0) out[d] = v;
+        double v = value_of(acc[d]), t = deriv_of(acc[d]);
+        for (int off = 32; off > 0; off >>= 1) {
+            v += __shfl_xor(v, off, 64);
+            t += __shfl_xor(t, off, 64);
+        }
+        if (lane == 0) {
+            out[d] = v;
+            if constexpr (std::is_same_v<M, Dual>) out[3 + d] = t;
+        }
     }
 }
 

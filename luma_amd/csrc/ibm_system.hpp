@@ -24,13 +24,18 @@ struct Marker {
     int sijk[kSmax][3];
     double sxyz[kSmax][3];
     double delta[kSmax];
+    // derivatives along the tangent direction (tangent-form grids only; zero otherwise)
+    double dpos[3] = {0, 0, 0}, dvel[3] = {0, 0, 0}, dvel_km1[3] = {0, 0, 0};
+    double deps = 0.0, dds = 0.0;
+    double ddelta[kSmax] = {};
 };
 
 struct Body {
     std::vector<Marker> m;
     int flexible = 0;
-    int first = 0;                  // index of the first marker in the grid-wide marker arrays
-    std::unique_ptr<FemBeam> fem;   // flexible filaments only
+    int first = 0;                            // index of the first marker in the grid-wide marker arrays
+    std::unique_ptr<FemBeam> fem;             // flexible filaments only
+    std::unique_ptr<FemBeamT<Dual>> femd;     // the same beam on a tangent-form grid
 };
 
 // Device buffer of n elements, reallocated only when n changes.  A fresh buffer is zeroed.
@@ -90,10 +95,16 @@ inline bool far_apart(const Marker &mI, const Marker &mJ, int D, double dh) {
 // ibm_deltaKernel (ObjectManager_ops_ibm.cpp:255-272) and its derivative in the radius
 double delta_kernel(double radius, double dilation);
 double delta_kernel_deriv(double radius, double dilation);
+// delta(x; s) = prod_a phi((x_a - s_a)/dh) and its gradient in x
+double delta_at(const double *x, const double *s, double dh, double dil, int D);
+void delta_grad(const double *x, const double *s, double dh, double dil, int D, double out[3]);
+// the grid stores (value, derivative) pairs: its IB device arrays hold Duals
+inline bool tangent_form(const lumagpu_grid *g) { return g->d.precision == LUMAGPU_FP64T; }
 IbmArgs ibm_args(lumagpu_grid *g);
 // interpolate, force and spread at the current supports from the grid's current populations
 int apply_core(lumagpu_grid *g);
 // A x = rhs, row-major A, LAPACK dgetf2 + dgetrs('T') operation order
 int lu_solve(int n, std::vector<double> &a, std::vector<double> &x);
+int lu_solve(int n, std::vector<Dual> &a, std::vector<Dual> &x);
 
 }  // namespace lumagpu

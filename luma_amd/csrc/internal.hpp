@@ -18,6 +18,7 @@ struct lumagpu_grid {
     lumagpu_grid_desc d;
     int Q = 0, D = 0;
     size_t ncell = 0, ps = 0, elem = 0;
+    size_t melem = 8;                 // bytes per element of the macro, force and side arrays
     void *f = nullptr;
     uint8_t *flags = nullptr;
     double *force = nullptr;

@@ -271,6 +271,31 @@ class GpuGrid:
         call("lumagpu_ibm_body_force", self.h, int(body), out)
         return np.array(out[:])
 
+    # --- tangent form (precision "fp64t"): derivatives along one seeded parameter direction --
+    def download_tangent(self):
+        """(df, drho, du) in the shapes of download_f / download_macro."""
+        N, M, K = self.shape
+        df = np.empty((N, M, K, self.Q))
+        dr = np.empty((N, M, K))
+        du = np.empty((N, M, K, self.dims))
+        call("lumagpu_download_tangent", self.h, _dptr(df), _dptr(dr), _dptr(du))
+        return df, dr, du
+
+    def ibm_seed_youngs(self, body, dE=1.0):
+        """Seed d(Young's modulus) = dE for a flexible filament: derivatives become d/dE."""
+        call("lumagpu_ibm_seed_youngs", self.h, int(body), float(dE))
+
+    def ibm_marker_tangents(self, body):
+        rec = np.empty((self.ibm_count(body)[1], 16))
+        call("lumagpu_ibm_get_marker_tangents", self.h, int(body), _dptr(rec))
+        return {"pos": rec[:, 0:3], "vel": rec[:, 3:6], "force": rec[:, 6:9], "mom": rec[:, 9:12],
+                "rho": rec[:, 12], "eps": rec[:, 13], "ds": rec[:, 14]}
+
+    def ibm_body_force_tangent(self, body):
+        out, dout = (C.c_double * 3)(), (C.c_double * 3)()
+        call("lumagpu_ibm_body_force_tangent", self.h, int(body), out, dout)
+        return np.array(out[:]), np.array(dout[:])
+
 
 def read_geometry(grids, path, shift=(0.0, 0.0, 0.0)):
     """ObjectManager::io_readInGeomConfig over GpuGrids: builds the file's IBM bodies on the grid

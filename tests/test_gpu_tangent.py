@@ -1,0 +1,189 @@
+"""Tangent form (precision "fp64t"): the step on dual numbers, carrying the derivative of every
+value along one seeded parameter direction.  Its values must equal the FP64 form bit for bit
+(the FP64 form equals the oracle, see the other GPU tests); with nothing seeded every
+derivative is exactly zero; with a filament's Young's modulus seeded, the derivatives of the
+marker trajectory and of the flow fields must match central finite differences of FP64 runs.
+Bound: 1e-4 relative (central differences with relative step 1e-4, or 1e-5 / 1e-6 where the
+FSI sub-iteration counts would differ across the larger stencil; the delta kernel's second
+derivative jumps at |r| = 1/2, so a wide stencil that crosses it is only first-order accurate)."""
+import numpy as np
+import pytest
+
+from luma_amd import setups
+from luma_amd.grid import GpuGrid
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+
+
+def grid(case, prec):
+    g = GpuGrid.from_case(case, prec, store_macro=True)
+    if case.lattyp is None and case.xpos is not None:
+        g.set_positions(case.xpos, case.ypos, case.zpos)
+    return g
+
+
+def assert_same_values(a, b, tag):
+    fa, fb = a.download_f(), b.download_f()
+    assert np.array_equal(fa, fb), f"{tag}: f differs by {np.abs(fa - fb).max()}"
+    (ra, ua), (rb, ub) = a.download_macro(), b.download_macro()
+    assert np.array_equal(ra, rb) and np.array_equal(ua, ub), tag
+
+
+def assert_zero_tangent(g, tag):
+    df, dr, du = g.download_tangent()
+    assert not df.any() and not dr.any() and not du.any(), tag
+
+
+def test_values_equal_fp64_and_derivatives_vanish_unseeded():
+    cases = [("cavity", setups.lid_driven_cavity(32, re=1280.0 * 32 / 256), 40),
+             ("channel", setups.channel_2d(48, 24, 8.0, 0.04, 50.0), 40),
+             ("tgv3d", setups.taylor_green((12, 10, 8), dims=3, tau=0.55, u0=0.05), 20)]
+    for name, case, steps in cases:
+        print(f"[tangent] {name}", flush=True)
+        with grid(case, "fp64") as a, grid(case, "fp64t") as b:
+            a.run(steps)
+            b.run(steps)
+            assert_same_values(a, b, name)
+            assert_zero_tangent(b, name)
+
+
+def test_refined_rigid_body_values_equal_fp64():
+    parent = setups.channel_2d(64, 32, 8.0, 0.04, 80.0)
+    child = setups.add_refined_region(parent, (8, 31, 6, 25), parent.info["dh"], parent.info["dt"])
+    pos = setups.circle_markers((2.5 + 0.01, 2.0 + 0.13, 0.0), 0.5, child.info["dh"])
+    out = {}
+    for prec in ("fp64", "fp64t"):
+        print(f"[tangent] refined rigid {prec}", flush=True)
+        with grid(parent, prec) as gp, grid(child, prec) as gc:
+            gp.add_child(gc)
+            gc.ibm_add_body(pos)
+            gc.ibm_initialise()
+            for _ in range(20):
+                gp.LBM_multi_opt()
+            out[prec] = (gp.download_f(), gc.download_f(), gc.ibm_markers(0), gc.ibm_body_force(0))
+            if prec == "fp64t":
+                assert_zero_tangent(gp, "parent")
+                assert_zero_tangent(gc, "child")
+                assert not gc.ibm_body_force_tangent(0)[1].any()
+    a, b = out["fp64"], out["fp64t"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for key in ("force", "rho", "mom", "eps", "ds"):
+        assert np.array_equal(a[2][key], b[2][key]), key
+    assert np.array_equal(a[3], b[3])
+
+
+def test_sphere_3d_values_equal_fp64():
+    c = setups.taylor_green((20, 16, 16), dims=3, tau=0.6, u0=0.0)
+    c.u[...] = 0.0
+    c.u[..., 0] = 0.03
+    c.rho[...] = 1.0
+    c.forced = True
+    c.xpos = 0.5 + np.arange(c.N, dtype=float)
+    c.ypos = 0.5 + np.arange(c.M, dtype=float)
+    c.zpos = 0.5 + np.arange(c.K, dtype=float)
+    c.edges = (float(c.N), float(c.M), float(c.K))
+    c.info.update({"dh": 1.0, "origin": (0.0, 0.0, 0.0)})
+    pos = setups.sphere_markers((8.3, 8.1, 7.9), 2.5, 1.0)
+    out = {}
+    for prec in ("fp64", "fp64t"):
+        with grid(c, prec) as g:
+            g.ibm_add_body(pos)
+            g.ibm_initialise()
+            g.run(12)
+            out[prec] = (g.download_f(), g.ibm_markers(0)["force"])
+    assert np.array_equal(out["fp64"][0], out["fp64t"][0])
+    assert np.array_equal(out["fp64"][1], out["fp64t"][1])
+
+
+# ---- flexible filament: d/dE against central differences ---------------------------------
+
+def filament_setup(refined):
+    if refined:
+        parent = setups.channel_2d(64, 32, 8.0, 0.05, 100.0)
+        child = setups.add_refined_region(parent, (10, 41, 8, 23), parent.info["dh"], parent.info["dt"])
+        setups.velocity_field_on(child, parent)
+        fine = child
+        start, angle = (2.0, 2.0 + 0.013, 0.0), 2.0
+    else:
+        parent, child = setups.channel_2d(64, 32, 8.0, 0.04, 100.0), None
+        fine = parent
+        start, angle = (1.5, 2.0 + 0.013, 0.0), 8.0
+    density, E = setups.filament_material(1.0, 0.05, fine.info["dh"], mass_ratio=10.0, bending=1e-2)
+    return parent, child, dict(start=start, length=1.0, height=0.05, angle=angle, elements=8,
+                               density=density, E=E)
+
+
+def run_filament(parent, child, fil, prec, E, steps, seed=False):
+    """Tip trajectory (y of the last marker, per coarse step), FSI sub-iteration counts, and
+    the final fine-grid state; derivatives too in the tangent form."""
+    grids = [grid(parent, prec)]
+    if child is not None:
+        grids.append(grid(child, prec))
+        grids[0].add_child(grids[1])
+    fg = grids[-1]
+    try:
+        b = fg.ibm_add_filament(fil["start"], fil["length"], fil["height"], fil["angle"], fil["elements"], True,
+                                fil["density"], E)
+        fg.ibm_initialise()
+        if seed:
+            fg.ibm_seed_youngs(b, 1.0)
+        tip, dtip, its = [], [], []
+        for _ in range(steps):
+            grids[0].LBM_multi_opt()
+            its.append(fg.ibm_subiterations()[0])
+            tip.append(fg.ibm_markers(b)["pos"][-1, 1])
+            if seed:
+                dtip.append(fg.ibm_marker_tangents(b)["pos"][-1, 1])
+        f = fg.download_f()
+        rho, u = fg.download_macro()
+        tan = fg.download_tangent() if seed else None
+        force = fg.ibm_body_force_tangent(b) if seed else (fg.ibm_body_force(b), None)
+        return dict(tip=np.array(tip), dtip=np.array(dtip), its=its, f=f, rho=rho, u=u, tan=tan, force=force)
+    finally:
+        for g in reversed(grids):
+            g.close()
+
+
+@pytest.mark.parametrize("refined,steps", [(False, 30), (True, 12)])
+def test_filament_youngs_derivative_matches_central_differences(refined, steps):
+    parent, child, fil = filament_setup(refined)
+    E = fil["E"]
+    print(f"[tangent] filament refined={refined} E={E:.5g} steps={steps}", flush=True)
+    ref = run_filament(parent, child, fil, "fp64", E, steps)
+    tan = run_filament(parent, child, fil, "fp64t", E, steps, seed=True)
+    # the values of the tangent form are the FP64 run's, seed or not
+    assert tan["its"] == ref["its"]
+    assert np.array_equal(tan["tip"], ref["tip"])
+    assert np.array_equal(tan["f"], ref["f"]) and np.array_equal(tan["u"], ref["u"])
+    # the FSI loop stops on a residual threshold: a central difference is a derivative only
+    # when E +- h run the same sub-iteration counts as E, so the step shrinks until they do
+    for rel in (1e-4, 1e-5, 1e-6):
+        h = rel * E
+        up = run_filament(parent, child, fil, "fp64", E + h, steps)
+        dn = run_filament(parent, child, fil, "fp64", E - h, steps)
+        print(f"h/E {rel:g}: sub-iterations equal {up['its'] == ref['its'] == dn['its']}", flush=True)
+        if up["its"] == ref["its"] == dn["its"]:
+            break
+    assert up["its"] == ref["its"] == dn["its"], "FSI sub-iteration counts differ across every stencil"
+    fd_tip = (up["tip"] - dn["tip"]) / (2 * h)
+    scale = np.abs(tan["dtip"]).max()
+    assert scale > 0.0
+    err = np.abs(tan["dtip"] - fd_tip).max() / scale
+    print(f"d tip/dE: max {scale:.4g}, max rel diff vs central differences {err:.3g}", flush=True)
+    assert err < RTOL
+    # mean tip displacement over the run (the C5 objective's form) and a projection of the fields
+    J_tan, J_fd = tan["dtip"].mean(), fd_tip.mean()
+    assert abs(J_tan - J_fd) <= RTOL * abs(J_tan)
+    rng = np.random.default_rng(5)
+    for key, d_tan in (("u", tan["tan"][2]), ("rho", tan["tan"][1]), ("f", tan["tan"][0])):
+        r = rng.standard_normal(d_tan.shape)
+        pt = float((d_tan * r).sum())
+        pf = float(((up[key] - dn[key]) / (2 * h) * r).sum())
+        print(f"d{key}/dE . r: tangent {pt:.8g} fd {pf:.8g}", flush=True)
+        assert abs(pt - pf) <= RTOL * abs(pt)
+    # body force derivative
+    (F, dF), (Fu, _), (Fd, _) = tan["force"], up["force"], dn["force"]
+    fdF = (Fu - Fd) / (2 * h)
+    assert np.abs(dF - fdF).max() <= RTOL * np.abs(dF).max()
