@@ -39,6 +39,8 @@ extern "C" {
 #define LUMAGPU_FP16S 2  /* FP16 storage of f - w*rho0, FP32 collision */
 #define LUMAGPU_FP64T 3  /* tangent form: FP64 values with their derivative along one parameter
                             (dual numbers, 16 B per population); values equal LUMAGPU_FP64's */
+#define LUMAGPU_FP32T 4  /* shifted tangent form: LUMAGPU_FP32S on float dual numbers (8 B per
+                            population); macro arrays, force field and IB in FP64 duals */
 
 /* moments returned by lumagpu_moments, all FP64 */
 #define LUMAGPU_MOMENT_MASS 0
@@ -56,7 +58,7 @@ typedef struct lumagpu_grid_desc {
     int dims;             /* L_DIMS: 2 or 3 */
     int Q;                /* L_NUM_VELS: 9 (D2Q9) or 19 (D3Q19) */
     int N, M, K;          /* N_lim, M_lim, K_lim (K = 1 in 2D) */
-    int precision;        /* LUMAGPU_FP64 / FP32S / FP16S */
+    int precision;        /* LUMAGPU_FP64 / FP32S / FP16S / FP64T / FP32T */
     int regularised;      /* L_REGULARISED_BOUNDARIES */
     int forced;           /* L_IBM_ON || L_GRAVITY_ON: force_xyz enters macro and collision */
     int fz_compat;        /* 3D: add F_z/2 to x-momentum as GridObj_ops_lbm_optimised.cpp:833 */
@@ -97,7 +99,7 @@ int lumagpu_upload_f(lumagpu_grid *g, const double *f);
 int lumagpu_upload_macro(lumagpu_grid *g, const double *rho, const double *u);
 int lumagpu_download_f(lumagpu_grid *g, double *f);
 int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u);
-/* Tangent form only: derivatives of f (post-collision, LUMA layout), rho and u along the
+/* Tangent forms only: derivatives of f (post-collision, LUMA layout), rho and u along the
  * seeded direction; any output may be NULL.  No counterpart in LUMA (the tangent form is this
  * build's forward-mode derivative of LBM_multi_opt). */
 int lumagpu_download_tangent(lumagpu_grid *g, double *df, double *drho, double *du);
@@ -141,6 +143,13 @@ int lumagpu_ibm_add_filament(lumagpu_grid *g, const double *start, double length
                              int *body);
 /* sub-iterations and residual of the last step's FSI loop (ObjectManager::timeav_subResidual) */
 int lumagpu_ibm_subiterations(const lumagpu_grid *g, int *iterations, double *residual);
+/* FSI sub-iteration counts of every step since lumagpu_ibm_initialise (the per-step values of
+ * ObjectManager::timeav_subIterations): *n receives the number of steps, up to max are copied.
+ * And a prescribed schedule: the next n steps run exactly counts[k] sub-iterations (1..10)
+ * instead of stopping on the residual (n = 0 clears).  No counterpart in LUMA: pinning lets
+ * finite differences in a parameter follow the schedule a derivative run took. */
+int lumagpu_ibm_subiteration_log(const lumagpu_grid *g, int *counts, int max, int *n);
+int lumagpu_ibm_pin_subiterations(lumagpu_grid *g, const int *counts, int n);
 /* ObjectManager::ibm_initialise (:190-246): support stencils, ds and epsilon of every body */
 int lumagpu_ibm_initialise(lumagpu_grid *g);
 /* ObjectManager::io_readInGeomConfig (ObjectManager_ops_io.cpp:322-800): builds the IBM bodies

@@ -16,8 +16,8 @@ EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ESTATE, EPHYS = -1, -2, -3, -4, -5, -6
 ERROR_NAMES = {EINVAL: "EINVAL", EHIP: "EHIP", ENOMEM: "ENOMEM", EUNSUPPORTED: "EUNSUPPORTED",
                ESTATE: "ESTATE", EPHYS: "EPHYS"}
 
-FP64, FP32S, FP16S, FP64T = 0, 1, 2, 3
-PRECISIONS = {"fp64": FP64, "fp32s": FP32S, "fp16s": FP16S, "fp64t": FP64T}
+FP64, FP32S, FP16S, FP64T, FP32T = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": FP64, "fp32s": FP32S, "fp16s": FP16S, "fp64t": FP64T, "fp32t": FP32T}
 NUM_MOMENTS = 5
 
 
@@ -91,6 +91,8 @@ _PROTOS = {
     "lumagpu_ibm_body_force_tangent": [_vp, C.c_int, _dp, _dp],
     "lumagpu_ibm_get_marker_tangents": [_vp, C.c_int, _dp],
     "lumagpu_ibm_seed_youngs": [_vp, C.c_int, C.c_double],
+    "lumagpu_ibm_subiteration_log": [_vp, _ip, C.c_int, _ip],
+    "lumagpu_ibm_pin_subiterations": [_vp, _ip, C.c_int],
     "lumagpu_ibm_read_restart": [C.POINTER(_vp), C.c_int, C.c_char_p],
     "lumagpu_checkpoint_save": [_vp, C.POINTER(_vp)],
     "lumagpu_checkpoint_restore": [_vp, _vp],

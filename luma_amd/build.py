@@ -33,6 +33,7 @@ UNITS = [
     ("lbm_f64.hip", ["-ffp-contract=off"]),
     ("lbm_f32.hip", ["-ffp-contract=fast"]),
     ("lbm_dual.hip", ["-ffp-contract=off"]),
+    ("lbm_dual32.hip", ["-ffp-contract=fast"]),
     ("api.hip", ["-ffp-contract=off"]),
     ("ibm.hip", ["-ffp-contract=off", "-fno-builtin-sin", "-fno-builtin-cos"]),
     ("adjoint.hip", ["-ffp-contract=off"]),

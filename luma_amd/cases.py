@@ -277,6 +277,97 @@ def run_c5(precision: str, steps: int, sample: int = 1):
             "mlups": cells * steps / wall / 1e6}, tips
 
 
+def c5_tip_series(precision: str, steps: int, youngs: float, seed: bool = False, pin=None):
+    """C5 tip trajectory: the last marker's y per coarse step, its derivative along Young's
+    modulus (tangent form, seeded), and the FSI sub-iteration count of every fine step (pin:
+    a schedule of counts to follow instead of the residual test)."""
+    parent, child, fil = c5_filament()
+    tips, dtips, its = np.empty(steps), np.empty(steps), []
+    with GpuGrid.from_case(parent, precision) as gp, GpuGrid.from_case(child, precision) as gc:
+        gp.add_child(gc)
+        b = gc.ibm_add_filament(fil["start"], fil["length"], fil["height"], fil["angle"], fil["elements"], True,
+                                fil["density"], youngs)
+        gc.ibm_initialise()
+        if seed:
+            gc.ibm_seed_youngs(b, 1.0)
+        if pin is not None:
+            gc.ibm_pin_subiterations(pin)
+        t0 = time.perf_counter()
+        for s in range(steps):
+            gp.LBM_multi_opt()
+            tips[s] = gc.ibm_markers(b)["pos"][-1, 1]
+            if seed:
+                dtips[s] = gc.ibm_marker_tangents(b)["pos"][-1, 1]
+            if (s + 1) % 2000 == 0:
+                print(f"[c5 grad {precision}] coarse step {s + 1} tip y {tips[s]:.6f}", flush=True)
+        wall = time.perf_counter() - t0
+        its = gc.ibm_subiteration_log()
+    return tips, (dtips if seed else None), its, wall
+
+
+def c5_objectives(tips, dtips, y0, dhf):
+    """Mean tip amplitude J = mean_t |y_tip - y0| / dh_f over the second half of the run (the
+    window run_c5 reports), the mean signed deflection, and their derivatives when dtips is
+    given (per unit of the seeded parameter)."""
+    w = slice(len(tips) // 2, None)
+    dev = (tips[w] - y0) / dhf
+    out = {"J_mean_amplitude_cells": float(np.abs(dev).mean()), "J_mean_deflection_cells": float(dev.mean())}
+    if dtips is not None:
+        ddev = dtips[w] / dhf
+        out["dJ_mean_amplitude"] = float((np.sign(dev) * ddev).mean())
+        out["dJ_mean_deflection"] = float(ddev.mean())
+    return out
+
+
+def run_c5_grad(steps: int, check_steps: int, h_rel: float = 1e-3, form: str = "fp64t"):
+    """d(mean tip amplitude)/d(bending stiffness) for C5 in the tangent form, with the tip
+    trajectory's derivative checked against central differences of FP64 runs at E(1 +- h_rel)
+    over the first check_steps coarse steps.  The FSI loop stops on a residual threshold, so
+    the difference runs follow the derivative run's sub-iteration schedule (pinned): both
+    then differentiate the same discrete program."""
+    parent, child, fil = c5_filament()
+    E, KB = fil["E"], 1e-3
+    y0, dhf = fil["start"][1], child.info["dh"]
+    tips, dtips, its, wall = c5_tip_series(form, steps, E, seed=True)
+    obj = c5_objectives(tips, dtips, y0, dhf)
+    # E = KB * rho_f u^2 L^3 / I (setups.filament_material): dJ/dKB = dJ/dE * E / KB
+    rep = {"config": "c5_filament_gradient", "form": form,
+           "coarse_steps": steps, "fine_steps": 2 * steps, "bending_stiffness": KB, "youngs": E,
+           "wall_s": wall, "ms_per_coarse_step": 1e3 * wall / steps,
+           "mean_subiterations_per_fine_step": float(np.mean(its)), **obj,
+           "dJ_mean_amplitude_dKB": obj["dJ_mean_amplitude"] * E / KB,
+           "dJ_mean_deflection_dKB": obj["dJ_mean_deflection"] * E / KB}
+    rep["_series"] = (tips, dtips)
+    if check_steps and form == "fp64t":
+        n = min(check_steps, steps)
+        h = h_rel * E
+        sched = its[:2 * n]
+        up, _, its_u, _ = c5_tip_series("fp64", n, E + h, pin=sched)
+        dn, _, its_d, _ = c5_tip_series("fp64", n, E - h, pin=sched)
+        assert np.array_equal(its_u, sched) and np.array_equal(its_d, sched)
+        # free (unpinned) stencil: the first fine step whose count differs
+        fu = c5_tip_series("fp64", n, E + h)[2]
+        fdn = c5_tip_series("fp64", n, E - h)[2]
+        diff = [i for i in range(2 * n) if not (fu[i] == sched[i] == fdn[i])]
+        rep["check_free_stencil_first_differing_fine_step"] = diff[0] if diff else None
+        k = n
+        fd = (up[:k] - dn[:k]) / (2 * h)
+        scale = float(np.abs(dtips[:k]).max()) if k else 0.0
+        tail = slice(k // 2, k)
+        rep.update({"check_h_rel": h_rel, "check_steps": n,
+                    "check_max_rel_diff_dtip": float(np.abs(dtips[:k] - fd).max() / scale) if scale else None,
+                    "check_max_rel_diff_dtip_second_half": float(np.abs(dtips[tail] - fd[tail]).max() /
+                                                                 np.abs(dtips[tail]).max()),
+                    "check_dtip_scale": scale})
+        if k:
+            fo = c5_objectives(up[:k], None, y0, dhf)["J_mean_deflection_cells"]
+            fm = c5_objectives(dn[:k], None, y0, dhf)["J_mean_deflection_cells"]
+            to = c5_objectives(tips[:k], dtips[:k], y0, dhf)
+            rep["check_dJ_mean_deflection_tangent"] = to["dJ_mean_deflection"]
+            rep["check_dJ_mean_deflection_fd"] = (fo - fm) / (2 * h)
+    return rep
+
+
 def rel_l2(a, b, mask):
     d = (a - b)[mask]
     return float(np.sqrt((d * d).sum() / (b[mask] * b[mask]).sum()))
@@ -289,9 +380,34 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--sample", type=int, default=20)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--grad", action="store_true",
+                    help="c5: gradient of the mean tip amplitude with respect to bending stiffness")
+    ap.add_argument("--check-steps", type=int, default=2000,
+                    help="c5 --grad: coarse steps of the central-difference check (0: none)")
+    ap.add_argument("--check-h", type=float, default=1e-3,
+                    help="c5 --grad: relative step of the central differences in Young's modulus")
     a = ap.parse_args(argv)
     precs = a.precision.split(",")
     report = {"runs": []}
+    if a.config == "c5" and a.grad:
+        forms = [p for p in precs if p in ("fp64t", "fp32t")] or ["fp64t"]
+        runs = {f: run_c5_grad(a.steps or 100000, a.check_steps, a.check_h, form=f) for f in forms}
+        series = {f: r.pop("_series") for f, r in runs.items()}
+        report = {"runs": list(runs.values())}
+        if "fp64t" in series and "fp32t" in series:
+            (t64, d64), (t32, d32) = series["fp64t"], series["fp32t"]
+            dhf = c5_filament()[1].info["dh"]
+            report["fp32t_vs_fp64t"] = {
+                "tip_max_diff_cells": float(np.abs(t32 - t64).max() / dhf),
+                "dtip_max_rel_diff": float(np.abs(d32 - d64).max() / np.abs(d64).max()),
+                "dJ_mean_amplitude_dKB_rel_diff": abs(runs["fp32t"]["dJ_mean_amplitude_dKB"] -
+                                                      runs["fp64t"]["dJ_mean_amplitude_dKB"]) /
+                abs(runs["fp64t"]["dJ_mean_amplitude_dKB"])}
+        print(json.dumps(report), flush=True)
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump(report, fh, indent=1)
+        return 0
     if a.config == "c5":
         steps = a.steps or 100000
         series = {}

@@ -64,6 +64,14 @@ struct PDual {
     using M = Dual;
     static constexpr bool kShift = false;
 };
+// Shifted tangent form: the FP32-shifted storage and collision on float Duals, FP64 Duals for
+// the macro arrays and force field.
+struct P32T {
+    using S = DualF;
+    using T = DualF;
+    using M = Dual;
+    static constexpr bool kShift = true;
+};
 
 struct StepArgs {
     void *f;                      // population array
@@ -113,13 +121,13 @@ struct StepArgs {
 // unrolling, lets the compiler merge both branches into one load and drop the hint.
 template <class P, bool NT = true>
 __device__ __forceinline__ typename P::T ld(const typename P::S *p) {
-    if constexpr (std::is_same_v<typename P::S, Dual>) return *p;
+    if constexpr (std::is_same_v<typename P::S, Dual> || std::is_same_v<typename P::S, DualF>) return *p;
     else if constexpr (NT) return static_cast<typename P::T>(__builtin_nontemporal_load(p));
     else return static_cast<typename P::T>(*p);
 }
 template <class P, bool NT = true>
 __device__ __forceinline__ void st(typename P::S *p, typename P::T v) {
-    if constexpr (std::is_same_v<typename P::S, Dual>) *p = v;
+    if constexpr (std::is_same_v<typename P::S, Dual> || std::is_same_v<typename P::S, DualF>) *p = v;
     else if constexpr (NT) __builtin_nontemporal_store(static_cast<typename P::S>(v), p);
     else *p = static_cast<typename P::S>(v);
 }
@@ -405,13 +413,14 @@ __device__ __forceinline__ void regularised_edge(T *fa, const int *n, const T *v
 }
 
 // ----------------------------------------------------------------------------------------
-// shifted (deviation) per-site arithmetic, FP32 compute
+// shifted (deviation) per-site arithmetic, FP32 compute (TF = float, or DualF in the shifted
+// tangent form)
 
-template <int Q>
-__device__ __forceinline__ void macro_shift(const float *fin, const double *F, bool fz_compat,
-                                            float &drho, float *j) {
+template <int Q, class TF, class FT>
+__device__ __forceinline__ void macro_shift(const TF *fin, const FT *F, bool fz_compat,
+                                            TF &drho, TF *j) {
     using L = Lattice<Q>;
-    float r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+    TF r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
     for (int v = 0; v < Q; ++v) {
         r += fin[v];
@@ -423,11 +432,11 @@ __device__ __forceinline__ void macro_shift(const float *fin, const double *F, b
         if (L::c(v, 2) == -1) jz -= fin[v];
     }
     if (F) {
-        jx += 0.5f * float(F[0]);
-        jy += 0.5f * float(F[1]);
+        jx += 0.5f * TF(F[0]);
+        jy += 0.5f * TF(F[1]);
         if constexpr (L::D == 3) {
-            if (fz_compat) jx += 0.5f * float(F[2]);
-            else jz += 0.5f * float(F[2]);
+            if (fz_compat) jx += 0.5f * TF(F[2]);
+            else jz += 0.5f * TF(F[2]);
         }
     }
     drho = r;
@@ -449,18 +458,22 @@ __host__ __device__ constexpr Split split_of(double x) {
     return Split{float(x), float(x - double(float(x)))};
 }
 __device__ __forceinline__ float mul_split(Split s, float x) { return fmaf(s.hi, x, s.lo * x); }
+__device__ __forceinline__ DualF mul_split(Split s, const DualF &x) {
+    return DualF(fmaf(s.hi, x.v, s.lo * x.v), fmaf(s.hi, x.d, s.lo * x.d));
+}
 
 // BGK on deviations, with rho = rho0 + drho:
 //   feq_v - w_v rho0 = w_v (drho + rho P_v),  P_v = 3 c.u + 4.5 (c.u)^2 - 1.5 u.u,
 // evaluated as w_v (drho + rho0 P_v + drho P_v) so rho itself is never rounded;
 // Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).  One population at a time.
-template <int Q>
+template <int Q, class TF = float>
 struct CollideShift {
-    float drho, rho0, u[3], usq15, Fx, Fy, Fz, uF, gpre;
+    TF drho, u[3], usq15, Fx, Fy, Fz, uF;
+    float rho0, gpre;
     Split omega;
     bool forced;
-    __device__ __forceinline__ CollideShift(float drho_, float rho0_, const float *u_, const double *F,
-                                            Split om) {
+    template <class FT>
+    __device__ __forceinline__ CollideShift(TF drho_, float rho0_, const TF *u_, const FT *F, Split om) {
         using L = Lattice<Q>;
         drho = drho_;
         rho0 = rho0_;
@@ -473,39 +486,39 @@ struct CollideShift {
         forced = F != nullptr;
         Fx = Fy = Fz = uF = 0.f;
         if (forced) {
-            Fx = float(F[0]);
-            Fy = float(F[1]);
-            if constexpr (L::D == 3) Fz = float(F[2]);
+            Fx = TF(F[0]);
+            Fy = TF(F[1]);
+            if constexpr (L::D == 3) Fz = TF(F[2]);
             uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
         }
     }
     template <int V>
-    __device__ __forceinline__ void pair(float fin_v, float fin_o, float &out_v, float &out_o) const {
+    __device__ __forceinline__ void pair(TF fin_v, TF fin_o, TF &out_v, TF &out_o) const {
         out_v = (*this)(V, fin_v);
         out_o = (*this)(V + 1, fin_o);
     }
-    __device__ __forceinline__ float operator()(int v, float fin) const {
+    __device__ __forceinline__ TF operator()(int v, TF fin) const {
         using L = Lattice<Q>;
         const float cx = float(L::c(v, 0)), cy = float(L::c(v, 1)), cz = float(L::c(v, 2));
         const Split w = split_of(L::w(v));
-        const float cu = cx * u[0] + cy * u[1] + cz * u[2];
-        const float P = 3.0f * cu + 4.5f * cu * cu - usq15;
-        const float feq = mul_split(w, drho + (rho0 * P + drho * P));
-        const float dlt = feq - fin;
-        float o = fmaf(omega.hi, dlt, fmaf(omega.lo, dlt, fin));
+        const TF cu = cx * u[0] + cy * u[1] + cz * u[2];
+        const TF P = 3.0f * cu + 4.5f * cu * cu - usq15;
+        const TF feq = mul_split(w, drho + (rho0 * P + drho * P));
+        const TF dlt = feq - fin;
+        TF o = fmac(omega.hi, dlt, fmac(omega.lo, dlt, fin));
         if (forced) {
-            const float cF = cx * Fx + cy * Fy + cz * Fz;
+            const TF cF = cx * Fx + cy * Fy + cz * Fz;
             o += gpre * mul_split(w, 3.0f * (cF - uF) + 9.0f * cu * cF);
         }
         return o;
     }
 };
 
-template <int Q>
-__device__ __forceinline__ void collide_shift(const float *fin, float drho, float rho0,
-                                              const float *u, const double *F, Split omega,
-                                              float *fout) {
-    const CollideShift<Q> c(drho, rho0, u, F, omega);
+template <int Q, class TF, class FT>
+__device__ __forceinline__ void collide_shift(const TF *fin, TF drho, float rho0,
+                                              const TF *u, const FT *F, Split omega,
+                                              TF *fout) {
+    const CollideShift<Q, TF> c(drho, rho0, u, F, omega);
 #pragma unroll
     for (int v = 0; v < Q; ++v) fout[v] = c(v, fin[v]);
 }
@@ -536,21 +549,22 @@ __device__ __forceinline__ auto make_collider(const typename P::T *fin, const ty
         u_o[2] = u[2];
         return CollideExact<Q, T>{rho, {u[0], u[1], u[2]}, T(a.omega), T(a.cs2), T(a.den2), F};
     } else {
-        float drho, j[3], u[3];
+        using M = typename P::M;
+        T drho, j[3], u[3];
         macro_shift<Q>(fin, F, a.fz_compat != 0, drho, j);
         // u = j / (rho0 + drho) = (j - j*q)/rho0 with q = drho/(rho0 + drho): the rounding
         // of rho0 + drho only reaches the small correction term
         const float rho0 = float(a.rho0);
-        const float q = drho / (rho0 + drho);
+        const T q = drho / (rho0 + drho);
         const float r0inv = float(1.0 / a.rho0);
         u[0] = (j[0] - j[0] * q) * r0inv;
         u[1] = (j[1] - j[1] * q) * r0inv;
         u[2] = (j[2] - j[2] * q) * r0inv;
-        rho_o = double(float(a.rho0) + drho);
-        u_o[0] = u[0];
-        u_o[1] = u[1];
-        u_o[2] = u[2];
-        return CollideShift<Q>(drho, rho0, u, F, Split{a.om_hi, a.om_lo});
+        rho_o = M(float(a.rho0) + drho);
+        u_o[0] = M(u[0]);
+        u_o[1] = M(u[1]);
+        u_o[2] = M(u[2]);
+        return CollideShift<Q, T>(drho, rho0, u, F, Split{a.om_hi, a.om_lo});
     }
 }
 
@@ -823,13 +837,13 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
             const CollideExact<Q, T> col{rr, {uu[0], uu[1], uu[2]}, T(a.omega), T(a.cs2), T(a.den2), F};
             collide_pairs<Q>(col, fin, fout);
         } else {
-            const CollideShift<Q> col(rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo});
+            const CollideShift<Q, T> col(rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo});
             collide_pairs<Q>(col, fin, fout);
         }
-        rho = rr;
-        u[0] = uu[0];
-        u[1] = uu[1];
-        u[2] = uu[2];
+        rho = M(rr);
+        u[0] = M(uu[0]);
+        u[1] = M(uu[1]);
+        u[2] = M(uu[2]);
     } else {
         const auto col = make_collider<Q, P>(fin, F, a, rho, u);
         // TL2C sites are streamed but not collided (:144)
@@ -891,10 +905,10 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
     if (vel_type) {
         T rr, uu[3];
         velocity_site<Q, P>(a, fl, y, fin, rr, uu);
-        rho = rr;
-        u[0] = uu[0];
-        u[1] = uu[1];
-        u[2] = uu[2];
+        rho = typename P::M(rr);
+        u[0] = typename P::M(uu[0]);
+        u[1] = typename P::M(uu[1]);
+        u[2] = typename P::M(uu[2]);
     } else {
         T fout[Q];
         site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);

@@ -33,6 +33,7 @@ hipError_t launch_step(const lumagpu_grid *g, const LaunchSpec &ls, const StepAr
         case LUMAGPU_FP64: return launch_step_f64(ls, a, g->stream);
         case LUMAGPU_FP32S: return launch_step_f32s(ls, a, g->stream);
         case LUMAGPU_FP64T: return launch_step_dual(ls, a, g->stream);
+        case LUMAGPU_FP32T: return launch_step_dual32(ls, a, g->stream);
         default: return launch_step_f16s(ls, a, g->stream);
     }
 }
@@ -43,6 +44,7 @@ hipError_t launch_aux(const lumagpu_grid *g, int op, const double *p0, const dou
         case LUMAGPU_FP64: return launch_aux_f64(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
         case LUMAGPU_FP32S: return launch_aux_f32s(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
         case LUMAGPU_FP64T: return launch_aux_dual(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
+        case LUMAGPU_FP32T: return launch_aux_dual32(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
         default: return launch_aux_f16s(op, g->Q, a, p0, p1, out, g->parity, g->masked, g->stream);
     }
 }
@@ -109,6 +111,7 @@ hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
         case LUMAGPU_FP64: return launch_edges_f64(g->Q, a, e, odd, phase, store, g->stream);
         case LUMAGPU_FP32S: return launch_edges_f32s(g->Q, a, e, odd, phase, store, g->stream);
         case LUMAGPU_FP64T: return launch_edges_dual(g->Q, a, e, odd, phase, store, g->stream);
+        case LUMAGPU_FP32T: return launch_edges_dual32(g->Q, a, e, odd, phase, store, g->stream);
         default: return launch_edges_f16s(g->Q, a, e, odd, phase, store, g->stream);
     }
 }
@@ -118,7 +121,7 @@ hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
 hipError_t macro_from_luma(lumagpu_grid *g, const double *rho, const double *u) {
     const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
     const int N = g->d.N, M = g->d.M, K = g->d.K;
-    const bool t = g->d.precision == LUMAGPU_FP64T;
+    const bool t = g->melem != sizeof(double);
     if (g->D == 2) {
         if (t) k_macro_from_luma<2, Dual><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
         else k_macro_from_luma<2><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
@@ -131,7 +134,7 @@ hipError_t macro_from_luma(lumagpu_grid *g, const double *rho, const double *u) 
 hipError_t macro_to_luma(lumagpu_grid *g, double *rho, double *u, bool tan) {
     const dim3 b = cell_block(g->d.N), gr = cell_grid(g->d.N, g->d.M, g->d.K);
     const int N = g->d.N, M = g->d.M, K = g->d.K;
-    const bool t = g->d.precision == LUMAGPU_FP64T;
+    const bool t = g->melem != sizeof(double);
     if (g->D == 2) {
         if (t && tan) k_macro_to_luma<2, Dual, true><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
         else if (t) k_macro_to_luma<2, Dual><<<gr, b, 0, g->stream>>>(g->rho, g->u, g->ncell, N, M, K, rho, u);
@@ -150,6 +153,7 @@ hipError_t launch_extrap(const lumagpu_grid *g) {
         case LUMAGPU_FP64: return launch_aux_f64(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
         case LUMAGPU_FP32S: return launch_aux_f32s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
         case LUMAGPU_FP64T: return launch_aux_dual(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP32T: return launch_aux_dual32(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
         default: return launch_aux_f16s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
     }
 }
@@ -296,7 +300,7 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         return fail(LUMAGPU_EUNSUPPORTED, "lattice must be D2Q9 or D3Q19");
     if (d.N < 3 || d.M < 3 || d.K < 1 || (d.dims == 2 && d.K != 1) || (d.dims == 3 && d.K < 3))
         return fail(LUMAGPU_EINVAL, "grid extents too small (need >= 3 per active dimension; K = 1 in 2D)");
-    if (d.precision < LUMAGPU_FP64 || d.precision > LUMAGPU_FP64T)
+    if (d.precision < LUMAGPU_FP64 || d.precision > LUMAGPU_FP32T)
         return fail(LUMAGPU_EINVAL, "unknown precision");
     // GridObj_init_grids.cpp:353-356
     if (!(d.omega > 0.0) || d.omega >= 2.0)
@@ -309,8 +313,12 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     g->D = d.dims;
     g->ncell = (size_t)d.N * d.M * d.K;
     g->ps = (g->ncell + 255) / 256 * 256;
-    g->elem = d.precision == LUMAGPU_FP64T ? 16 : d.precision == LUMAGPU_FP64 ? 8 : d.precision == LUMAGPU_FP32S ? 4 : 2;
-    g->melem = d.precision == LUMAGPU_FP64T ? 16 : 8;
+    const bool tan = d.precision == LUMAGPU_FP64T || d.precision == LUMAGPU_FP32T;
+    g->elem = d.precision == LUMAGPU_FP64T                                  ? 16
+              : (d.precision == LUMAGPU_FP64 || d.precision == LUMAGPU_FP32T) ? 8
+              : d.precision == LUMAGPU_FP32S                                 ? 4
+                                                                             : 2;
+    g->melem = tan ? 16 : 8;
     auto cleanup = [&](int code, const std::string &m) {
         lumagpu_grid_destroy(g);
         return fail(code, m);
@@ -555,7 +563,7 @@ int lumagpu_set_zslab(lumagpu_grid *g, int on) {
         return fail(LUMAGPU_EINVAL, "a z-slab needs a D3Q19 grid with at least one interior and two halo planes");
     if (g->masked || g->d.forced || g->parent || g->child || g->ibm)
         return fail(LUMAGPU_EUNSUPPORTED, "z-slab halo exchange is implemented for unlabelled, unforced single-level grids");
-    if (g->d.precision == LUMAGPU_FP64T) return fail(LUMAGPU_EUNSUPPORTED, "z-slabs run in the FP64 and shifted forms");
+    if (g->melem != sizeof(double)) return fail(LUMAGPU_EUNSUPPORTED, "z-slabs run in the FP64 and shifted forms");
     if (g->parity != 0) return fail(LUMAGPU_ESTATE, "switch to a z-slab between step pairs");
     g->zslab = 1;
     return 0;
@@ -688,7 +696,7 @@ int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u) {
 
 int lumagpu_download_tangent(lumagpu_grid *g, double *df, double *drho, double *du) {
     if (int r = check_grid(g)) return r;
-    if (g->d.precision != LUMAGPU_FP64T) return fail(LUMAGPU_ESTATE, "derivatives exist on tangent-form grids only");
+    if (g->melem == sizeof(double)) return fail(LUMAGPU_ESTATE, "derivatives exist on tangent-form grids only");
     if (!g->has_state) return fail(LUMAGPU_ESTATE, "no state to download");
     const size_t nf = g->ncell * g->Q * sizeof(double), nm = g->ncell * (1 + g->D) * sizeof(double);
     if (int r = ensure_stage(g, nf > nm ? nf : nm)) return r;

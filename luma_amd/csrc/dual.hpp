@@ -14,11 +14,14 @@
 
 namespace lumagpu {
 
+struct DualF;
+
 struct Dual {
     double v, d;
     __host__ __device__ constexpr Dual() : v(0.0), d(0.0) {}
     __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
     __host__ __device__ constexpr Dual(double x, double dx) : v(x), d(dx) {}
+    __host__ __device__ explicit constexpr Dual(const DualF &x);
     __host__ __device__ explicit constexpr operator double() const { return v; }
     __host__ __device__ Dual &operator+=(const Dual &b) {
         v += b.v;
@@ -56,11 +59,66 @@ __host__ __device__ inline Dual operator/(const Dual &a, const Dual &b) {
 __host__ __device__ inline bool operator<(const Dual &a, const Dual &b) { return a.v < b.v; }
 __host__ __device__ inline bool operator>(const Dual &a, const Dual &b) { return a.v > b.v; }
 
-// value part of either number type
+// The FP32 counterpart for the shifted tangent form (FP32 storage and collision of values and
+// derivatives).  fmac and mul_split mirror the fused operations of the FP32 collision.
+struct DualF {
+    float v, d;
+    __host__ __device__ constexpr DualF() : v(0.f), d(0.f) {}
+    __host__ __device__ constexpr DualF(float x) : v(x), d(0.f) {}
+    __host__ __device__ constexpr DualF(int x) : v(float(x)), d(0.f) {}
+    __host__ __device__ constexpr DualF(float x, float dx) : v(x), d(dx) {}
+    __host__ __device__ explicit constexpr DualF(double x) : v(float(x)), d(0.f) {}
+    __host__ __device__ explicit constexpr DualF(const Dual &x) : v(float(x.v)), d(float(x.d)) {}
+    __host__ __device__ explicit constexpr operator double() const { return v; }
+    __host__ __device__ DualF &operator+=(const DualF &b) {
+        v += b.v;
+        d += b.d;
+        return *this;
+    }
+    __host__ __device__ DualF &operator-=(const DualF &b) {
+        v -= b.v;
+        d -= b.d;
+        return *this;
+    }
+    __host__ __device__ DualF &operator*=(const DualF &b) {
+        const float nd = d * b.v + v * b.d;
+        v *= b.v;
+        d = nd;
+        return *this;
+    }
+    __host__ __device__ DualF &operator/=(const DualF &b) {
+        const float q = v / b.v;
+        d = (d - q * b.d) / b.v;
+        v = q;
+        return *this;
+    }
+};
+__host__ __device__ constexpr Dual::Dual(const DualF &x) : v(x.v), d(x.d) {}
+
+__host__ __device__ inline DualF operator+(const DualF &a, const DualF &b) { return DualF(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline DualF operator-(const DualF &a, const DualF &b) { return DualF(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline DualF operator-(const DualF &a) { return DualF(-a.v, -a.d); }
+__host__ __device__ inline DualF operator*(const DualF &a, const DualF &b) {
+    return DualF(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+__host__ __device__ inline DualF operator/(const DualF &a, const DualF &b) {
+    const float q = a.v / b.v;
+    return DualF(q, (a.d - q * b.d) / b.v);
+}
+
+// a x + y with one rounding of the value (fmaf); the derivative a x' + y'
+__device__ inline float fmac(float a, float x, float y) { return fmaf(a, x, y); }
+__device__ inline DualF fmac(float a, const DualF &x, const DualF &y) {
+    return DualF(fmaf(a, x.v, y.v), fmaf(a, x.d, y.d));
+}
+
+// value part of every number type
 __host__ __device__ inline double value_of(double x) { return x; }
 __host__ __device__ inline double value_of(const Dual &x) { return x.v; }
+__host__ __device__ inline double value_of(const DualF &x) { return x.v; }
 __host__ __device__ inline double deriv_of(double) { return 0.0; }
 __host__ __device__ inline double deriv_of(const Dual &x) { return x.d; }
+__host__ __device__ inline double deriv_of(const DualF &x) { return x.d; }
 // both parts zero
 __host__ __device__ inline bool is_zero(double x) { return x == 0.0; }
 __host__ __device__ inline bool is_zero(const Dual &x) { return x.v == 0.0 && x.d == 0.0; }

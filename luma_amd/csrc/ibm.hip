@@ -389,6 +389,7 @@ hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int 
         case LUMAGPU_FP64: return launch_sites_f64(g->Q, a, odd, n, cells, g->stream);
         case LUMAGPU_FP32S: return launch_sites_f32s(g->Q, a, odd, n, cells, g->stream);
         case LUMAGPU_FP64T: return launch_sites_dual(g->Q, a, odd, n, cells, g->stream);
+        case LUMAGPU_FP32T: return launch_sites_dual32(g->Q, a, odd, n, cells, g->stream);
         default: return launch_sites_f16s(g->Q, a, odd, n, cells, g->stream);
     }
 }
@@ -600,6 +601,8 @@ int ibm_apply(lumagpu_grid *g) {
     for (const auto &b : s->bodies) flexible = flexible || b.flexible;
     if (!flexible) return 0;
     if (int r = move_bodies(g)) return r;
+    // a pinned count replaces the residual test for this step (lumagpu_ibm_pin_subiterations)
+    const int pin = s->pinned_pos < s->pinned.size() ? s->pinned[s->pinned_pos++] : 0;
     int it = 0;
     double res = 0.0;
     do {
@@ -607,9 +610,10 @@ int ibm_apply(lumagpu_grid *g) {
         if (int r = move_bodies(g)) return r;
         res = vel_diff(g);
         it++;
-    } while (res > 1e-4 && it < 10);
+    } while (pin > 0 ? it < pin : (res > 1e-4 && it < 10));
     s->sub_it = it;
     s->sub_res = res;
+    s->sub_log.push_back(it);
     for (auto &b : s->bodies) {
         if (b.flexible && b.fem) b.fem->commit();
         if (b.flexible && b.femd) b.femd->commit();
@@ -755,6 +759,28 @@ int lumagpu_ibm_subiterations(const lumagpu_grid *g, int *iterations, double *re
     return 0;
 }
 
+int lumagpu_ibm_subiteration_log(const lumagpu_grid *g, int *counts, int max, int *n) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->ibm) return fail(LUMAGPU_ESTATE, "no IB bodies on this grid");
+    const std::vector<int> &l = g->ibm->sub_log;
+    if (n) *n = (int)l.size();
+    if (counts)
+        for (int i = 0; i < max && i < (int)l.size(); ++i) counts[i] = l[i];
+    return 0;
+}
+
+int lumagpu_ibm_pin_subiterations(lumagpu_grid *g, const int *counts, int n) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->ibm) return fail(LUMAGPU_ESTATE, "no IB bodies on this grid");
+    if (n < 0 || (n > 0 && !counts)) return fail(LUMAGPU_EINVAL, "bad sub-iteration schedule");
+    std::vector<int> v(counts, counts + n);
+    for (int c : v)
+        if (c < 1 || c > 10) return fail(LUMAGPU_EINVAL, "pinned sub-iteration counts must lie in 1..10");
+    g->ibm->pinned = std::move(v);
+    g->ibm->pinned_pos = 0;
+    return 0;
+}
+
 int lumagpu_ibm_initialise(lumagpu_grid *g) {
     if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
     if (!g->ibm || g->ibm->bodies.empty()) return fail(LUMAGPU_ESTATE, "no IB bodies on this grid");
@@ -775,6 +801,7 @@ int lumagpu_ibm_initialise(lumagpu_grid *g) {
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->masked = 1;
     s->initialised = 1;
+    s->sub_log.clear();
     return 0;
 }
 
@@ -827,7 +854,7 @@ int lumagpu_ibm_get_marker_tangents(lumagpu_grid *g, int body, double *rec) {
 
 int lumagpu_ibm_seed_youngs(lumagpu_grid *g, int body, double dE) {
     if (int r = check_body(g, body)) return r;
-    if (!tangent_form(g)) return fail(LUMAGPU_ESTATE, "derivative seeds need a tangent-form grid (LUMAGPU_FP64T)");
+    if (!tangent_form(g)) return fail(LUMAGPU_ESTATE, "derivative seeds need a tangent-form grid (LUMAGPU_FP64T or LUMAGPU_FP32T)");
     Body &b = g->ibm->bodies[body];
     if (!b.flexible || !b.femd) return fail(LUMAGPU_EINVAL, "body " + std::to_string(body) + " is not a flexible filament");
     b.femd->seed_youngs(dE);

@@ -75,6 +75,9 @@ struct IbmSystem {
     int sites_dirty = 0;        // the support sites changed since the last application
     int sub_it = 0;             // sub-iterations of the last step
     double sub_res = 0.0;
+    std::vector<int> sub_log;   // sub-iterations of every FSI step since the bodies were initialised
+    std::vector<int> pinned;    // prescribed sub-iteration counts of the next steps
+    size_t pinned_pos = 0;
 
     ~IbmSystem() {
         for (void *p : {(void *)d_scell, (void *)d_ns, (void *)d_sites, (void *)d_sptr, (void *)d_sent,
@@ -99,7 +102,9 @@ double delta_kernel_deriv(double radius, double dilation);
 double delta_at(const double *x, const double *s, double dh, double dil, int D);
 void delta_grad(const double *x, const double *s, double dh, double dil, int D, double out[3]);
 // the grid stores (value, derivative) pairs: its IB device arrays hold Duals
-inline bool tangent_form(const lumagpu_grid *g) { return g->d.precision == LUMAGPU_FP64T; }
+inline bool tangent_form(const lumagpu_grid *g) {
+    return g->d.precision == LUMAGPU_FP64T || g->d.precision == LUMAGPU_FP32T;
+}
 IbmArgs ibm_args(lumagpu_grid *g);
 // interpolate, force and spread at the current supports from the grid's current populations
 int apply_core(lumagpu_grid *g);

@@ -22,6 +22,7 @@ hipError_t launch_step_f64(const LaunchSpec &ls, const StepArgs &a, hipStream_t 
 hipError_t launch_step_f32s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
 hipError_t launch_step_f16s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
 hipError_t launch_step_dual(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
+hipError_t launch_step_dual32(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
 
 hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                           double *out, int parity, int masked, hipStream_t s);
@@ -31,12 +32,15 @@ hipError_t launch_aux_f16s(int op, int Q, const StepArgs &a, const double *p0, c
                            double *out, int parity, int masked, hipStream_t s);
 hipError_t launch_aux_dual(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                            double *out, int parity, int masked, hipStream_t s);
+hipError_t launch_aux_dual32(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
+                             double *out, int parity, int masked, hipStream_t s);
 
 // k_site_macro over a list of cells (IB support sites), in the layout the next step reads
 hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
 hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
 hipError_t launch_sites_f16s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
 hipError_t launch_sites_dual(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
+hipError_t launch_sites_dual32(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
 
 template <class P>
 hipError_t launch_sites_p(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
@@ -58,6 +62,7 @@ hipError_t launch_edges_f64(int Q, const StepArgs &a, const EdgeArgs &e, bool od
 hipError_t launch_edges_f32s(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
 hipError_t launch_edges_f16s(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
 hipError_t launch_edges_dual(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
+hipError_t launch_edges_dual32(int Q, const StepArgs &a, const EdgeArgs &e, bool odd, int phase, int store, hipStream_t s);
 
 template <int Q, class P, bool ODD>
 void launch_edges_qo(const StepArgs &a, const EdgeArgs &e, int phase, int store, hipStream_t s) {
@@ -157,7 +162,7 @@ hipError_t launch_aux_q(int op, const StepArgs &a, const double *p0, const doubl
         case kAuxUpload: k_upload_aos<Q, P><<<g, b, 0, s>>>(a, p0); break;
         case kAuxDownload: k_download_aos<Q, P><<<g, b, 0, s>>>(a, out, parity, masked); break;
         case kAuxDownloadTan:
-            if constexpr (!std::is_same_v<typename P::S, Dual>) return hipErrorInvalidValue;
+            if constexpr (!std::is_same_v<typename P::M, Dual>) return hipErrorInvalidValue;
             else k_download_aos<Q, P, true><<<g, b, 0, s>>>(a, out, parity, masked);
             break;
         case kAuxMoments: k_moments<Q, P><<<g, b, 0, s>>>(a, parity, masked, out); break;

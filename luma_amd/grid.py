@@ -271,6 +271,19 @@ class GpuGrid:
         call("lumagpu_ibm_body_force", self.h, int(body), out)
         return np.array(out[:])
 
+    def ibm_subiteration_log(self):
+        """FSI sub-iterations of every step since ibm_initialise."""
+        n = C.c_int()
+        call("lumagpu_ibm_subiteration_log", self.h, None, 0, C.byref(n))
+        out = np.zeros(n.value, dtype=np.int32)
+        call("lumagpu_ibm_subiteration_log", self.h, out.ctypes.data_as(_ip), n.value, C.byref(n))
+        return out
+
+    def ibm_pin_subiterations(self, counts):
+        """The next len(counts) steps run exactly these FSI sub-iteration counts."""
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        call("lumagpu_ibm_pin_subiterations", self.h, c.ctypes.data_as(_ip), len(c))
+
     # --- tangent form (precision "fp64t"): derivatives along one seeded parameter direction --
     def download_tangent(self):
         """(df, drho, du) in the shapes of download_f / download_macro."""
