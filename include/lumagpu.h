@@ -103,6 +103,11 @@ int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u);
  * seeded direction; any output may be NULL.  No counterpart in LUMA (the tangent form is this
  * build's forward-mode derivative of LBM_multi_opt). */
 int lumagpu_download_tangent(lumagpu_grid *g, double *df, double *drho, double *du);
+/* FP64 tangent form: seeds d(omega) = d_omega for this grid's collision (GridObj::omega,
+ * GridObj_init_grids.cpp:339-356).  Grids of a hierarchy are seeded one by one; a refined
+ * level's omega follows its parent's as omega_f = 1/(2(1/omega_c - 1/2) + 1/2) (:527), so
+ * its seed is d_omega_c * domega_f/domega_c. */
+int lumagpu_tangent_seed_omega(lumagpu_grid *g, double d_omega);
 /* GridObj::force_xyz as left by the caller (body force); requires desc.forced */
 int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz);
 

@@ -86,6 +86,7 @@ struct StepArgs {
     double *u_out;                // macro output [d*ncell + cell]
     size_t ncell;
     double omega, rho0, rho_in;
+    double d_omega;               // derivative of omega along the seeded direction (FP64 tangent form)
     double cs2, den2;             // cs*cs and 2*cs^2*cs^2, evaluated as LUMA does
     float om_hi, om_lo;           // omega as a float pair (shifted forms)
     int regularised;              // L_REGULARISED_BOUNDARIES
@@ -130,6 +131,13 @@ __device__ __forceinline__ void st(typename P::S *p, typename P::T v) {
     if constexpr (std::is_same_v<typename P::S, Dual> || std::is_same_v<typename P::S, DualF>) *p = v;
     else if constexpr (NT) __builtin_nontemporal_store(static_cast<typename P::S>(v), p);
     else *p = static_cast<typename P::S>(v);
+}
+
+// omega in the exact arithmetic type: a Dual carries the seeded derivative d_omega
+template <class T>
+__device__ __forceinline__ T omega_of(const StepArgs &a) {
+    if constexpr (std::is_same_v<T, Dual>) return Dual(a.omega, a.d_omega);
+    else return T(a.omega);
 }
 
 // an FP64 array of StepArgs viewed as elements of the form's macro type M
@@ -547,7 +555,7 @@ __device__ __forceinline__ auto make_collider(const typename P::T *fin, const ty
         u_o[0] = u[0];
         u_o[1] = u[1];
         u_o[2] = u[2];
-        return CollideExact<Q, T>{rho, {u[0], u[1], u[2]}, T(a.omega), T(a.cs2), T(a.den2), F};
+        return CollideExact<Q, T>{rho, {u[0], u[1], u[2]}, omega_of<T>(a), T(a.cs2), T(a.den2), F};
     } else {
         using M = typename P::M;
         T drho, j[3], u[3];
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
         T rr, uu[3];
         velocity_site<Q, P>(a, fl, y, fin, rr, uu);
         if constexpr (!P::kShift) {
-            const CollideExact<Q, T> col{rr, {uu[0], uu[1], uu[2]}, T(a.omega), T(a.cs2), T(a.den2), F};
+            const CollideExact<Q, T> col{rr, {uu[0], uu[1], uu[2]}, omega_of<T>(a), T(a.cs2), T(a.den2), F};
             collide_pairs<Q>(col, fin, fout);
         } else {
             const CollideShift<Q, T> col(rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo});
@@ -990,7 +998,7 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
 #pragma unroll
     for (int v = 0; v < Q; ++v) fin[v] = P::kShift ? fa[v] - T(L::w(v) * a.rho0) : fa[v];
     if constexpr (!P::kShift) {
-        collide_exact<Q, T>(fin, rr, uu, F, T(a.omega), T(a.cs2), T(a.den2), fout);
+        collide_exact<Q, T>(fin, rr, uu, F, omega_of<T>(a), T(a.cs2), T(a.den2), fout);
     } else {
         collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
     }

@@ -694,6 +694,14 @@ int lumagpu_download_macro(lumagpu_grid *g, double *rho, double *u) {
     return 0;
 }
 
+int lumagpu_tangent_seed_omega(lumagpu_grid *g, double d_omega) {
+    if (int r = check_grid(g)) return r;
+    if (g->d.precision != LUMAGPU_FP64T)
+        return fail(LUMAGPU_ESTATE, "an omega seed needs an FP64 tangent-form grid (LUMAGPU_FP64T)");
+    g->d_omega = d_omega;
+    return 0;
+}
+
 int lumagpu_download_tangent(lumagpu_grid *g, double *df, double *drho, double *du) {
     if (int r = check_grid(g)) return r;
     if (g->melem == sizeof(double)) return fail(LUMAGPU_ESTATE, "derivatives exist on tangent-form grids only");

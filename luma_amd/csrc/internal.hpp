@@ -38,6 +38,7 @@ struct lumagpu_grid {
     int nedge = 0;                    // regularised velocity sites on edges and corners
     int *edge = nullptr;              // [4*nedge]: cells, packed normals, neighbour x+n, x+2n
     double *erho = nullptr;           // [2*nedge] neighbour densities of the current step
+    double d_omega = 0.0;             // tangent form: seeded derivative of omega
     int masked = 0;
     int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
@@ -75,6 +76,7 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.u_out = g->u;
     a.ncell = g->ncell;
     a.omega = g->d.omega;
+    a.d_omega = g->d_omega;
     a.rho0 = g->d.rho0;
     a.rho_in = g->d.rho_in;
     const double cs = 1.0 / std::sqrt(3.0);    // LUMA/src/stdafx.cpp:152

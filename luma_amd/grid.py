@@ -294,6 +294,10 @@ class GpuGrid:
         call("lumagpu_download_tangent", self.h, _dptr(df), _dptr(dr), _dptr(du))
         return df, dr, du
 
+    def seed_omega(self, d_omega=1.0):
+        """Seed d(omega) = d_omega (FP64 tangent form): derivatives become d/domega."""
+        call("lumagpu_tangent_seed_omega", self.h, float(d_omega))
+
     def ibm_seed_youngs(self, body, dE=1.0):
         """Seed d(Young's modulus) = dE for a flexible filament: derivatives become d/dE."""
         call("lumagpu_ibm_seed_youngs", self.h, int(body), float(dE))

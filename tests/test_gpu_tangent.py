@@ -228,3 +228,37 @@ def test_fp32_shifted_tangent_tracks_fp64_tangent():
         rel = np.abs(d32 - d64).max() / np.abs(d64).max()
         print(f"fp32t vs fp64t tangent field {k}: max rel diff {rel:.3g}", flush=True)
         assert rel < 1e-2
+
+
+def test_tangent_and_reverse_adjoint_agree_on_domega():
+    # two independent derivative codes: the FP64 tangent form seeded in omega and the reverse
+    # discrete adjoint (lumagpu_adj_run) give the same d(time-averaged drag)/d(omega)
+    from test_gpu_adjoint import channel_case, make_grid
+    case, pos = channel_case()
+    T = 60
+    with make_grid(case, pos) as g:
+        adj = g.adj_run(T, 16)
+    with make_grid(case, pos, prec="fp64t") as g:
+        g.seed_omega(1.0)
+        J, dJ = 0.0, 0.0
+        for _ in range(T):
+            g.LBM_multi_opt()
+            F, dF = g.ibm_body_force_tangent(0)
+            J += F[0] / T
+            dJ += dF[0] / T
+    print(f"J {J!r} adjoint {adj['J']!r}; dJ/domega tangent {dJ!r} adjoint {adj['dJ_domega']!r}", flush=True)
+    assert abs(J - adj["J"]) <= 1e-12 * abs(adj["J"])   # same values, another summation order
+    assert abs(dJ - adj["dJ_domega"]) <= 1e-9 * abs(adj["dJ_domega"])
+
+
+def test_tangent_entry_points_refuse_other_forms():
+    from luma_amd._abi import LumaGpuError
+    case = setups.channel_2d(48, 24, 8.0, 0.04, 50.0)
+    with grid(case, "fp64") as g:
+        with pytest.raises(LumaGpuError, match="tangent-form"):
+            g.download_tangent()
+        with pytest.raises(LumaGpuError, match="omega seed"):
+            g.seed_omega(1.0)
+    with grid(case, "fp32t") as g:
+        with pytest.raises(LumaGpuError, match="omega seed"):
+            g.seed_omega(1.0)
