@@ -158,10 +158,12 @@ int lumagpu_ibm_pin_subiterations(lumagpu_grid *g, const int *counts, int n);
 /* ObjectManager::ibm_initialise (:190-246): support stencils, ds and epsilon of every body */
 int lumagpu_ibm_initialise(lumagpu_grid *g);
 /* ObjectManager::io_readInGeomConfig (ObjectManager_ops_io.cpp:322-800): builds the IBM bodies
- * of a geometry.config file on the grids (matched by level, region 0): CIRCLE_SPHERE,
- * SQUARE_CUBE and FILAMENT_ARRAY (FLEXIBLE filaments get their FEM beam, 2D).  shift[3] is the
- * wall-thickness shift LUMA adds when the left/bottom/front walls are solid.  FROM_FILE, PLATE
- * and non-IBM bodies return LUMAGPU_EUNSUPPORTED.  *nbodies receives the count built. */
+ * of a geometry.config file on the grids (matched by level, region 0): FROM_FILE point clouds
+ * (io_readInCloud :812-1049, one marker per voxel; the cloud file is read from the geometry
+ * file's directory, as LUMA reads both from ./input/; rigid only), CIRCLE_SPHERE, SQUARE_CUBE,
+ * PLATE (3D) and FILAMENT_ARRAY (FLEXIBLE filaments get their FEM beam, 2D).  shift[3] is the
+ * wall-thickness shift LUMA adds to the prefabs when the left/bottom/front walls are solid.
+ * Bounce-back and BFL bodies return LUMAGPU_EUNSUPPORTED.  *nbodies receives the count built. */
 int lumagpu_ibm_read_geometry(lumagpu_grid *const *grids, int ngrids, const char *path, const double *shift,
                               int *nbodies);
 /* number of IB bodies on the grid and, for body >= 0, the markers of that body (iBody.size(),

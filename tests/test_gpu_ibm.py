@@ -355,3 +355,80 @@ def test_ib_restart_round_trip(tmp_path):
         with pytest.raises(RuntimeError, match="Number of IBM markers"):
             ibm_read_restart([g3], path)
         assert np.array_equal(g3.ibm_markers(0)["pos"], pa)
+
+
+def test_geometry_config_point_cloud_and_plate(tmp_path):
+    # FROM_FILE IBM: LUMA's test-suite point cloud (tests/golden/ibb_input.in, a unit circle of
+    # 2000 points) scaled to a length of 1.0 along x and centred; one marker per occupied voxel
+    # at the mean of its points.  PLATE IBM (3D): floor(L/dh)+1 by floor(W/dh)+1 markers in the
+    # plane of the rotated x-z frame.
+    import shutil
+    from pathlib import Path
+    from luma_amd.grid import read_geometry
+    src = Path(__file__).resolve().parent / "golden" / "ibb_input.in"
+    shutil.copy(src, tmp_path / "ibb_input.in")
+    c = setups.channel_2d(80, 40, 10.0, 0.04, 50.0)
+    dh = c.info["dh"]
+    cfg = tmp_path / "geometry.config"
+    cfg.write_text("FROM_FILE\tIBM\tibb_input.in\t0\t0\tCENTRE\t2.0\tCENTRE\t2.0\tSTART\t0.0\t1.0\tX\tRIGID\tNONE\n")
+    with gpu_grid(c) as g:
+        assert read_geometry([g], cfg) == 1
+        mk = g.ibm_markers(0)["pos"]
+        g.ibm_initialise()
+        g.run(2)
+    # the cloud as LUMA reads it: a line holding no number repeats the previous point, so the
+    # file's final newline adds one more copy of its last point (1, 0, 0)
+    pts = np.loadtxt(src)
+    pts = np.vstack([pts, pts[-1]])[:, :2]
+    # scaling (ObjectManager_ops_io.cpp:909-930): the x extent becomes round(1.0/dh)*dh - 2e-8*dh;
+    # placement of a CENTRE reference (:933-955): the extent, rounded to whole voxels, centred on
+    # the centre of the voxel the rounded reference starts
+    blen = round(1.0 / dh) * dh
+    scale = (blen - 2e-8 * dh) / (pts[:, 0].max() - pts[:, 0].min())
+    shift = np.empty(2)
+    for a in range(2):
+        r = round(2.0 / dh) * dh + dh / 2.0
+        dist = round(scale * (pts[:, a].max() - pts[:, a].min()) / dh) * dh
+        shift[a] = (r - dist / 2.0) - scale * pts[:, a].min()
+    qq = pts * scale + shift
+    assert abs((qq[:, 0].max() - qq[:, 0].min()) - (blen - 2e-8 * dh)) < 1e-12
+    origin = np.array(c.info["origin"][:2])
+    vox = np.floor((qq - origin) / dh).astype(int)
+    # one marker per occupied voxel, at the mean of that voxel's points
+    assert len(mk) == len({tuple(v) for v in vox})
+    assert np.allclose(mk[:, 2], 0.0)
+    for m in mk:
+        v = np.floor((m[:2] - origin) / dh).astype(int)
+        sel = np.all(vox == v, axis=1)
+        assert sel.any()
+        assert np.abs(qq[sel].mean(axis=0) - m[:2]).max() <= 1e-12
+    # markers in first-appearance order: each marker's voxel first occurs after the previous one's
+    first = [int(np.argmax(np.all(vox == np.floor((m[:2] - origin) / dh).astype(int), axis=1))) for m in mk]
+    assert first == sorted(first)
+
+    # PLATE, 3D
+    box = periodic_box(24, 24, 24, 0.0, 0.6)
+    cfg3 = tmp_path / "plate.config"
+    cfg3.write_text("PLATE IBM 0 0 12.2 11.9 12.1 8.0 5.0 30.0 0.0 45.0 RIGID\n")
+    with gpu_grid(box) as g:
+        assert read_geometry([g], cfg3) == 1
+        mk = g.ibm_markers(0)["pos"]
+    assert len(mk) == 9 * 6
+    tx, tz = np.radians(30.0), np.radians(45.0)
+    Tx = np.array([[1, 0, 0], [0, np.cos(tx), -np.sin(tx)], [0, np.sin(tx), np.cos(tx)]])
+    Tz = np.array([[np.cos(tz), -np.sin(tz), 0], [np.sin(tz), np.cos(tz), 0], [0, 0, 1]])
+    R = Tx @ Tz
+    local = (mk - np.array([12.2, 11.9, 12.1])) @ R      # back to the plate's frame
+    assert np.abs(local[:, 1]).max() < 1e-12              # a plane
+    assert np.allclose([local[:, 0].min(), local[:, 0].max()], [-4.0, 4.0], atol=1e-12)
+    assert np.allclose([local[:, 2].min(), local[:, 2].max()], [-2.5, 2.5], atol=1e-12)
+    # refused forms
+    bad = tmp_path / "bad.config"
+    bad.write_text("PLATE IBM 0 0 12 12 12 8 5 0 0 0 FLEXIBLE\n")
+    with gpu_grid(box) as g:
+        with pytest.raises(RuntimeError, match="Plate cannot be flexible"):
+            read_geometry([g], bad)
+    bad.write_text("FROM_FILE BFL ibb_input.in 0 0 CENTRE 2 CENTRE 2 START 0 1 X RIGID NONE\n")
+    with gpu_grid(c) as g:
+        with pytest.raises(RuntimeError, match="not on the path"):
+            read_geometry([g], bad)
