@@ -92,6 +92,11 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                        const double *zpos);
 /* ux_in/uy_in/uz_in, one value per j (GridObj_init_grids.cpp:1322-1360) */
 int lumagpu_set_inlet(lumagpu_grid *g, const double *ux_in, const double *uy_in, const double *uz_in);
+/* GridObj::_LBM_initGetInletProfileFromFile (GridObj_init_grids.cpp:1233-1312, L_USE_INLET_PROFILE):
+ * ux_in/uy_in/uz_in from a file of x y z ux uy uz rows, interpolated along y at the grid's YPos
+ * with the reference's end extrapolation; path replaces ./input/inlet_profile.in.  Values are
+ * taken as the file gives them, as the reference does. */
+int lumagpu_read_inlet_profile(lumagpu_grid *g, const char *path);
 /* rho and u, then f = feq(rho, u) (GridObj_init_grids.cpp:315-333) */
 int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u);
 /* GridObj::f after a step (post-collision populations) and the macro arrays rho, u */

@@ -91,6 +91,11 @@ class GridObj {
         check(lumagpu_init_equilibrium(h_, rho.data(), u.data()));
         check(lumagpu_set_store_macro(h_, 1));
     }
+    // _LBM_initGetInletProfileFromFile (GridObj_init_grids.cpp:1233-1312): ux_in/uy_in/uz_in from
+    // a profile file (L_USE_INLET_PROFILE); path replaces ./input/inlet_profile.in
+    void LBM_initGetInletProfileFromFile(const std::string &path) {
+        check(lumagpu_read_inlet_profile(h_, path.c_str()));
+    }
     // start from LUMA-layout post-collision populations already in f (a restart)
     void LBM_initFromF() {
         check(lumagpu_upload_f(h_, f.data()));

@@ -315,4 +315,64 @@ int lumagpu_ibm_read_restart(lumagpu_grid *const *grids, int ngrids, const char 
     return 0;
 }
 
+// GridObj::_LBM_initGetInletProfileFromFile (GridObj_init_grids.cpp:1233-1312) with
+// GridUtils::readVelocityFromFile (GridUtils.cpp:1467-1524): columns x y z ux uy uz, the
+// profile taken along y and interpolated (or extrapolated from the end pair) at this grid's
+// YPos.  The same stream handling as the reference: one istringstream re-pointed at each line,
+// so a line holding no number (the file's final newline) appends the last value read to every
+// column.  The values go to ux_in/uy_in/uz_in as the file gives them, without the ud2ulbm
+// conversion the parabolic and uniform profiles get (:1336-1355).
+int lumagpu_read_inlet_profile(lumagpu_grid *g, const char *path) {
+    if (!g || !path) return fail(LUMAGPU_EINVAL, "null argument");
+    if (g->ypos.size() != (size_t)g->d.M)
+        return fail(LUMAGPU_ESTATE, "grid positions are not set (lumagpu_set_labels / lumagpu_set_positions)");
+    std::ifstream datafile(path, std::ios::in);
+    if (!datafile.is_open())
+        return fail(LUMAGPU_EINVAL, std::string("Cannot open velocity data file named ") + path + ". Exiting.");
+    std::vector<double> xb, yb, zb, uxb, uyb, uzb;
+    std::string line_in;
+    std::istringstream iss;
+    double tmp = 0.0;
+    while (!datafile.eof()) {
+        std::getline(datafile, line_in, '\n');
+        iss.str(line_in);
+        iss.seekg(0);
+        for (std::vector<double> *col : {&xb, &yb, &zb, &uxb, &uyb, &uzb}) {
+            iss >> tmp;
+            col->push_back(tmp);
+        }
+    }
+    const size_t n = yb.size();
+    if (n < 2) return fail(LUMAGPU_EINVAL, "Failed to read in inlet profile data. Exiting.");
+    const int M = g->d.M;
+    std::vector<double> ux(M, 0.0), uy(M, 0.0), uz(M, 0.0);
+    for (int j = 0; j < M; j++) {
+        const double y = g->ypos[j];
+        for (size_t i = 0; i < n; i++) {
+            if (i == 0 && yb[i] > y) {
+                ux[j] = -(uxb[i + 1] - uxb[i]) + uxb[i];
+                uy[j] = -(uyb[i + 1] - uyb[i]) + uyb[i];
+                uz[j] = -(uzb[i + 1] - uzb[i]) + uzb[i];
+                break;
+            } else if (i == n - 1 && yb[i] < y) {
+                ux[j] = (uxb[i] - uxb[i - 1]) + uxb[i];
+                uy[j] = (uyb[i] - uyb[i - 1]) + uyb[i];
+                uz[j] = (uzb[i] - uzb[i - 1]) + uzb[i];
+                break;
+            } else if (i + 1 < n && yb[i] < y && yb[i + 1] > y) {
+                ux[j] = ((uxb[i + 1] - uxb[i]) / (yb[i + 1] - yb[i])) * (y - yb[i]) + uxb[i];
+                uy[j] = ((uyb[i + 1] - uyb[i]) / (yb[i + 1] - yb[i])) * (y - yb[i]) + uyb[i];
+                uz[j] = ((uzb[i + 1] - uzb[i]) / (yb[i + 1] - yb[i])) * (y - yb[i]) + uzb[i];
+                break;
+            } else if (yb[i] == y) {
+                ux[j] = uxb[i];
+                uy[j] = uyb[i];
+                uz[j] = uzb[i];
+                break;
+            }
+        }
+    }
+    return lumagpu_set_inlet(g, ux.data(), uy.data(), uz.data());
+}
+
 }  // extern "C"

@@ -294,6 +294,10 @@ class GpuGrid:
         call("lumagpu_download_tangent", self.h, _dptr(df), _dptr(dr), _dptr(du))
         return df, dr, du
 
+    def read_inlet_profile(self, path):
+        """GridObj::_LBM_initGetInletProfileFromFile: the inlet profile from an x y z ux uy uz file."""
+        call("lumagpu_read_inlet_profile", self.h, str(path).encode())
+
     def seed_omega(self, d_omega=1.0):
         """Seed d(omega) = d_omega (FP64 tangent form): derivatives become d/domega."""
         call("lumagpu_tangent_seed_omega", self.h, float(d_omega))

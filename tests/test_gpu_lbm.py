@@ -274,3 +274,30 @@ def test_device_checkpoint_resumes_bit_for_bit():
         assert g.t == t1
         assert np.array_equal(g.download_f(), f1)
         GpuGrid.free_checkpoint(ck)
+
+
+def test_inlet_profile_file(tmp_path):
+    # GridObj::_LBM_initGetInletProfileFromFile on LUMA's own input/inlet_profile.in
+    # (tests/golden/inlet_profile.in: rows x y z ux uy uz, y from 0.288 to 1.543): linear
+    # interpolation in y inside the data; below it the first pair's step backwards (2 u0 - u1);
+    # above it the reference's reader has appended a zero row for the file's final newline, so
+    # the top extrapolation gives (0 - u_last) + 0.  The regularised inlet sites carry u_in.
+    from pathlib import Path
+    src = Path(__file__).resolve().parent / "golden" / "inlet_profile.in"
+    data = np.loadtxt(src)
+    yb, uxb = data[:, 1], data[:, 3]
+    c = setups.channel_2d(16, 40, 20.0, 0.04, 50.0)
+    with GpuGrid.from_case(c, "fp64", store_macro=True) as g:
+        g.read_inlet_profile(src)
+        g.run(1)
+        rho, u = g.download_macro()
+    ys = np.asarray(c.ypos)
+    inlet = c.lattyp[0, :, 0] == setups.VELOCITY
+    got = u[0, :, 0, 0][inlet]
+    y = ys[inlet]
+    exp = np.interp(y, yb, uxb)
+    exp[y < yb[0]] = 2 * uxb[0] - uxb[1]
+    exp[y > yb[-1]] = -uxb[-1]
+    assert (y < yb[0]).any() and (y > yb[-1]).any() and ((y > yb[0]) & (y < yb[-1])).any()
+    assert np.allclose(got, exp, rtol=0, atol=1e-14), np.abs(got - exp).max()
+    assert np.all(u[0, :, 0, 1][inlet] == 0.0)
