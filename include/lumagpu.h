@@ -127,6 +127,16 @@ int lumagpu_run(lumagpu_grid *g, int nsteps);
 int lumagpu_run_timed(lumagpu_grid *g, int nsteps, double *ms_total, double *ms_kernel_mean);
 /* write rho and u from every step (LUMA keeps them in GridObj::rho/u); off by default */
 int lumagpu_set_store_macro(lumagpu_grid *g, int on);
+/* L_GRAVITY_ON with L_GRAVITY_DIRECTION (definitions.h:114-117): gravity = GridObj::gravity in
+ * lattice units (fd2flbm(L_GRAVITY_FORCE), GridObj_init_grids.cpp:185).  Each step of a grid
+ * that holds IB bodies then starts with _LBM_resetForces (GridObj_ops_lbm_optimised.cpp:53-57,
+ * :1327-1337): the gravity component of force_xyz at every site becomes rho * gravity *
+ * refinement_ratio (refinement_ratio = 1/2^level) from the previous step's rho, and, as in the
+ * reference, the other components are not reset, so IB forces in them add up over the steps.
+ * LUMA resets nothing on levels without IB bodies: there the force stays at its initial
+ * rho * gravity * refinement_ratio (:291-297), which lumagpu_set_force gives.  Turns the macro
+ * store on.  Flexible bodies are refused at the step. */
+int lumagpu_set_gravity(lumagpu_grid *g, double gravity, int direction);
 /* completed steps on this grid (GridObj::t) */
 int lumagpu_get_time(const lumagpu_grid *g, int *t);
 int lumagpu_sync(lumagpu_grid *g);

@@ -91,6 +91,7 @@ struct StepArgs {
     float om_hi, om_lo;           // omega as a float pair (shifted forms)
     int regularised;              // L_REGULARISED_BOUNDARIES
     int fz_compat;                // 3D macro quirk (:833): F_z/2 added to x-momentum
+    int all_forced;               // every site carries force_xyz (gravity reset on this level)
     // --- refinement coupling (GridObj::parentGrid / subGrid) -----------------------------
     const void *pf;               // parent population array (explode source)
     size_t pps;
@@ -828,7 +829,7 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     using M = typename P::M;
     const M *F = nullptr;
     M Fv[3];
-    if (fl & kForcedBit) {
+    if (a.all_forced || (fl & kForcedBit)) {
         const M *fx = mv<M>(a.force);
         Fv[0] = fx[cell];
         Fv[1] = fx[a.ncell + cell];
@@ -918,8 +919,20 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
         u[1] = typename P::M(uu[1]);
         u[2] = typename P::M(uu[2]);
     } else {
+        // sweep 1's macro adds F/2 of the force the step started from: zero after an IB reset,
+        // the gravity field when the reset writes gravity
+        using M = typename P::M;
+        M Fv[3];
+        const M *F = nullptr;
+        if (a.all_forced) {
+            const M *fx = mv<M>(a.force);
+            Fv[0] = fx[cell];
+            Fv[1] = fx[a.ncell + cell];
+            Fv[2] = (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
+            F = Fv;
+        }
         T fout[Q];
-        site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);
+        site_update<Q, P>(fin, fout, F, a, rho, u, false);
     }
     store_macro<L::D>(a, cell, rho, u);
 }
@@ -981,7 +994,7 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
     using M = typename P::M;
     const M *F = nullptr;
     M Fv[3];
-    if (fl & kForcedBit) {
+    if (a.all_forced || (fl & kForcedBit)) {
         const M *fx = mv<M>(a.force);
         Fv[0] = fx[cell];
         Fv[1] = fx[a.ncell + cell];
@@ -1019,6 +1032,16 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
             st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
         }
     });
+}
+
+// _LBM_resetForces with L_GRAVITY_ON (:1327-1337): the gravity component of force_xyz at every
+// site becomes rho * gravity * refinement_ratio from the rho of the previous step; the other
+// components are left as they are, as the reference leaves them.
+template <class M>
+__global__ void k_gravity_reset(StepArgs a, double *force, double gravity, double rr, int dir) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.ncell) return;
+    mv<M>(force)[(size_t)dir * a.ncell + cell] = mv<M>(a.rho_out)[cell] * M(gravity) * M(rr);
 }
 
 // Side buffer for ExtrapolateRight sources (:246-251): xr[v][y, z] = f*_v(N-3, y, z), taken

@@ -179,6 +179,20 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
         hipError_t e = launch_extrap(g);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
     }
+    // _LBM_resetForces at the start of the step on a level holding IB bodies (:53-57); with
+    // gravity it writes the gravity component from the previous step's rho, and every site
+    // then carries a force
+    g->all_forced = (g->gravity_on && ibm_marker_count(g) > 0) ? 1 : 0;
+    if (g->all_forced) {
+        const StepArgs ga = make_args(g);
+        const double rr = 1.0 / std::pow(2.0, (double)g->d.level);   // GridObj.cpp:54
+        const unsigned nb = (unsigned)((g->ncell + 255) / 256);
+        if (g->melem == sizeof(double))
+            k_gravity_reset<double><<<nb, 256, 0, g->stream>>>(ga, g->force, g->gravity, rr, g->gravity_dir);
+        else
+            k_gravity_reset<Dual><<<nb, 256, 0, g->stream>>>(ga, g->force, g->gravity, rr, g->gravity_dir);
+        HIP_TRY(hipGetLastError());
+    }
     // ObjectManager::ibm_apply between sweep 1 and sweep 2 (:120-121)
     if (g->ibm)
         if (int r = ibm_apply(g)) return r;
@@ -750,6 +764,17 @@ int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz) {
     HIP_TRY(hipMemcpyAsync(g->force, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->masked = 1;
+    return 0;
+}
+
+int lumagpu_set_gravity(lumagpu_grid *g, double gravity, int direction) {
+    if (int r = check_grid(g)) return r;
+    if (!g->d.forced) return fail(LUMAGPU_ESTATE, "gravity needs a grid created with desc.forced = 1");
+    if (direction < 0 || direction >= g->D) return fail(LUMAGPU_EINVAL, "gravity direction out of range");
+    g->gravity_on = 1;
+    g->gravity = gravity;
+    g->gravity_dir = direction;
+    g->store_macro = 1;   // the reset reads every site's rho of the previous step
     return 0;
 }
 

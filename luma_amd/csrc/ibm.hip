@@ -88,6 +88,7 @@ IbmArgs ibm_args(lumagpu_grid *g) {
     b.sites = s->d_sites;
     b.sptr = s->d_sptr;
     b.sent = s->d_sent;
+    b.accumulate = g->all_forced;
     return b;
 }
 
@@ -417,6 +418,13 @@ int ensure_flags(lumagpu_grid *g) {
     return 0;
 }
 
+int ibm_marker_count(const lumagpu_grid *g) {
+    if (!g->ibm) return 0;
+    int n = 0;
+    for (const auto &b : g->ibm->bodies) n += (int)b.m.size();
+    return n;
+}
+
 bool ibm_has_flexible(const lumagpu_grid *g) {
     if (!g->ibm) return false;
     for (const auto &b : g->ibm->bodies)
@@ -435,7 +443,8 @@ void ibm_destroy(lumagpu_grid *g) {
 // current supports are overwritten by the spread, which starts from zero.
 int apply_core(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
-    if (s->sites_dirty && !s->flagged.empty()) {
+    // a gravity reset has rewritten every site's force: nothing to clear
+    if (s->sites_dirty && !s->flagged.empty() && !g->all_forced) {
         std::vector<char> now(g->ncell, 0);
         for (int c : s->sites) now[c] = 1;
         std::vector<int> gone;
@@ -596,6 +605,9 @@ int ibm_apply(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
     if (!s || s->nm == 0) return 0;
     if (!s->initialised) return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
+    if (g->all_forced && ibm_has_flexible(g))
+        return fail(LUMAGPU_EUNSUPPORTED, "gravity with flexible bodies is not built (each FSI sub-iteration "
+                                          "re-resets gravity from the updated density)");
     if (int r = apply_core(g)) return r;
     bool flexible = false;
     for (const auto &b : s->bodies) flexible = flexible || b.flexible;

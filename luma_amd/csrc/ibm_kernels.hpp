@@ -40,6 +40,7 @@ struct IbmArgs {
     const int *sites;      // [nsites] distinct support cells
     const int *sptr;       // [nsites+1] entry ranges
     const int *sent;       // entries: m*S + slot
+    int accumulate;        // spread onto the force field's current values (gravity reset) instead of 0
 };
 
 template <class M, class A>
@@ -82,7 +83,10 @@ __global__ void k_ibm_spread(IbmArgs b) {
     if (s >= b.nsites) return;
     const M *sdelta = as<M>(b.sdelta), *force = as<M>(b.force), *eps = as<M>(b.eps), *ds = as<M>(b.ds);
     const size_t cell = (size_t)b.sites[s];
+    M *fxyz = as<M>(b.fxyz);
     M acc[3] = {0.0, 0.0, 0.0};
+    if (b.accumulate)
+        for (int d = 0; d < b.D; ++d) acc[d] = fxyz[d * b.ncell + cell];
     for (int e = b.sptr[s]; e < b.sptr[s + 1]; ++e) {
         const int code = b.sent[e];
         const int m = code / b.S;
@@ -91,7 +95,6 @@ __global__ void k_ibm_spread(IbmArgs b) {
         const M volDepth = (b.D == 3) ? ds[m] : M(1.0);
         for (int d = 0; d < b.D; ++d) acc[d] -= dl * force[3 * m + d] * volWidth * volDepth * ds[m];
     }
-    M *fxyz = as<M>(b.fxyz);
     for (int d = 0; d < b.D; ++d) fxyz[d * b.ncell + cell] = acc[d];
     b.flags[cell] |= kForcedBit;
 }

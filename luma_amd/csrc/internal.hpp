@@ -39,6 +39,9 @@ struct lumagpu_grid {
     int *edge = nullptr;              // [4*nedge]: cells, packed normals, neighbour x+n, x+2n
     double *erho = nullptr;           // [2*nedge] neighbour densities of the current step
     double d_omega = 0.0;             // tangent form: seeded derivative of omega
+    int gravity_on = 0, gravity_dir = 0;   // L_GRAVITY_ON, L_GRAVITY_DIRECTION
+    double gravity = 0.0;             // GridObj::gravity (lattice units)
+    int all_forced = 0;               // this step reset gravity: every site carries force_xyz
     int masked = 0;
     int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
@@ -87,6 +90,7 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.zoff = g->zslab ? 1 : 0;
     a.regularised = g->d.regularised;
     a.fz_compat = g->d.fz_compat;
+    a.all_forced = g->all_forced;
     if (g->parent) {
         const lumagpu_grid *p = g->parent;
         a.pf = p->f;
@@ -125,6 +129,8 @@ void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
 // the grid carries a flexible (FEM) body, whose step runs host work
 bool ibm_has_flexible(const lumagpu_grid *g);
+// markers of all IB bodies on the grid
+int ibm_marker_count(const lumagpu_grid *g);
 // one LBM_multi_opt of the grid (children first), as lumagpu_step; defined in api.hip
 int step_grid(lumagpu_grid *g, int subcycle);
 // ExtrapolateRight side buffer from the grid's current populations; defined in api.hip

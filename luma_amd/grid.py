@@ -294,6 +294,10 @@ class GpuGrid:
         call("lumagpu_download_tangent", self.h, _dptr(df), _dptr(dr), _dptr(du))
         return df, dr, du
 
+    def set_gravity(self, gravity, direction):
+        """L_GRAVITY_ON: the IB force reset writes rho * gravity * refinement_ratio each step."""
+        call("lumagpu_set_gravity", self.h, float(gravity), int(direction))
+
     def read_inlet_profile(self, path):
         """GridObj::_LBM_initGetInletProfileFromFile: the inlet profile from an x y z ux uy uz file."""
         call("lumagpu_read_inlet_profile", self.h, str(path).encode())

@@ -55,6 +55,8 @@ struct ora_grid {
     ora_ibm_hook hook;
     void *hook_ctx;
     int copy_u_n;
+    int gravity_on, gravity_dir;   /* L_GRAVITY_ON, L_GRAVITY_DIRECTION */
+    double gravity;                /* GridObj::gravity (lattice units) */
 };
 
 double ora_cs(void) { return 1.0 / sqrt(3.0); }   /* stdafx.cpp:152 */
@@ -435,8 +437,24 @@ static void collide_site(ora_grid *g, int id) {
     }
 }
 
-/* ---- _LBM_resetForces (gravity off): :1327-1337 -------------------------------------- */
-void ora_reset_forces(ora_grid *g) { memset(g->force_xyz, 0, g->ncell * g->D * sizeof(double)); }
+/* ---- _LBM_resetForces: :1327-1337 -------------------------------------------------------
+ * Gravity off: every component zeroed.  Gravity on: only the gravity component is written,
+ * rho[id] * gravity * refinement_ratio with refinement_ratio = 1 / 2^level (GridObj.cpp:54);
+ * the other components keep their values. */
+void ora_reset_forces(ora_grid *g) {
+    if (!g->gravity_on) {
+        memset(g->force_xyz, 0, g->ncell * g->D * sizeof(double));
+        return;
+    }
+    const double rr = 1.0 / pow(2.0, (double)g->d.level);
+    for (size_t id = 0; id < g->ncell; ++id)
+        g->force_xyz[g->gravity_dir + id * g->D] = g->rho[id] * g->gravity * rr;
+}
+void ora_set_gravity(ora_grid *g, double gravity, int dir) {
+    g->gravity_on = 1;
+    g->gravity = gravity;
+    g->gravity_dir = dir;
+}
 
 /* ---- sweep 1 of LBM_multi_opt: :68-107 ------------------------------------------------ */
 static void sweep1_plane(ora_grid *g, int i, int subcycle) {

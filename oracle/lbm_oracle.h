@@ -61,7 +61,8 @@ void ora_multi_opt(ora_grid *g, int subcycle);
 /* The two halves, for callers that put IBM work in between (no recursion into children). */
 void ora_sweep1(ora_grid *g, int subcycle);        /* stream + BC + macro */
 void ora_sweep2(ora_grid *g);                      /* force + collide + swap + ++t */
-void ora_reset_forces(ora_grid *g);                /* _LBM_resetForces (gravity off) */
+void ora_reset_forces(ora_grid *g);                /* _LBM_resetForces */
+void ora_set_gravity(ora_grid *g, double gravity, int dir);   /* L_GRAVITY_ON in the reset */
 void ora_macro_site(ora_grid *g, int i, int j, int k);  /* _LBM_macro_opt on one site */
 /* Run n steps of LBM_multi_opt on g with an optional OpenMP sweep (nthreads > 1). */
 void ora_run(ora_grid *g, int nsteps, int nthreads);

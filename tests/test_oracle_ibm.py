@@ -103,3 +103,37 @@ def test_support_stencil_3d_has_27_sites_led_by_the_enclosing_voxel():
         ijk, dl = ib.support(0, m)
         assert (ijk[0] == np.floor(pos[m]).astype(int)).all()
         assert abs(dl.sum() - 1.0) < 1e-14
+
+
+def test_gravity_reset_momentum_balance():
+    # L_GRAVITY_ON with an IB body on the level (GridObj_ops_lbm_optimised.cpp:53-57, :1327-1337).
+    # A forced BGK step adds exactly the final force field to the total momentum, and streaming
+    # in a periodic box keeps it, so per step:
+    #   along gravity: dP = sum_sites rho_prev * g + (this step's spread),
+    #   across gravity: dP = sum of every spread so far -- the reset leaves that component alone,
+    # where a spread gives the grid -sum_m F_m eps_m ds_m (the balance pinned above).
+    from oracle.oracle import lattice
+    grav = 3e-6
+    g = periodic(40, 36, (0.0, 0.0))
+    g.set_gravity(grav, 0)
+    pos = setups.circle_markers((20.3, 18.2, 0.0), 6.0, 1.0)
+    ib = OracleIBM(g, 1.0)
+    ib.add_body(pos)
+    ib.initialise()
+    c, _, _ = lattice(9)
+    p_prev = (g.get_f() @ c[:, :2]).sum((0, 1, 2))
+    rho_prev, _ = g.get_macro()
+    spread_y = 0.0
+    for step in range(6):
+        g.step()
+        mk = ib.markers(0)
+        sp = -(mk["force"][:, :2] * (mk["eps"] * mk["ds"])[:, None]).sum(0)
+        spread_y += sp[1]
+        p = (g.get_f() @ c[:, :2]).sum((0, 1, 2))
+        dp = p - p_prev
+        # momentum sums over 1440 sites x 9 populations round at ~1e-13
+        assert abs(dp[0] - (rho_prev.sum() * grav + sp[0])) <= 1e-10 * abs(dp[0]) + 1e-13, step
+        assert abs(dp[1] - spread_y) <= 1e-10 * abs(dp[1]) + 1e-13, step
+        p_prev = p
+        rho_prev, _ = g.get_macro()
+    assert abs(spread_y) > 0.0
