@@ -92,6 +92,7 @@ struct StepArgs {
     int regularised;              // L_REGULARISED_BOUNDARIES
     int fz_compat;                // 3D macro quirk (:833): F_z/2 added to x-momentum
     int all_forced;               // every site carries force_xyz (gravity reset on this level)
+    int sweep1_zero;              // the IB reset zeroes force_xyz at the start of the step
     // --- refinement coupling (GridObj::parentGrid / subGrid) -----------------------------
     const void *pf;               // parent population array (explode source)
     size_t pps;
@@ -317,31 +318,12 @@ __device__ __forceinline__ void collide_exact(const T *fin, T rho, const T *u, c
     for (int v = 0; v < Q; ++v) fout[v] = c(v, fin[v]);
 }
 
-// _LBM_regularised_opt for a velocity site on a face (edgeCount == 1), :313-510.
-// fa holds absolute populations after streaming; on return it holds the regularised set and
-// rho/u the site's macroscopic values.
+// The reconstruction shared by the face conditions (:432-508): unknown populations (those
+// entering through the face) by non-equilibrium bounce-back, then every population rebuilt from
+// feq(rho, u) and the non-equilibrium stress of the result.
 template <int Q, class T>
-__device__ __forceinline__ void regularised_face(T *fa, int ncode, const T *vel_in, T rho_in,
-                                                 T cs2, T den2, T &rho, T *u) {
+__device__ __forceinline__ void regularise_face_set(T *fa, int ndir, int nsgn, T rho, const T *u, T cs2, T den2) {
     using L = Lattice<Q>;
-    const int ndir = (ncode - 1) >> 1;              // 0:x 1:y 2:z
-    const int nsgn = ((ncode - 1) & 1) ? -1 : 1;    // inward normal component
-    T f_plus = T(0), f_zero = T(0);
-#pragma unroll
-    for (int v = 0; v < Q; ++v) {
-        const int cn = L::c(v, ndir);
-        if (cn == -nsgn) f_plus += fa[v];
-        else if (cn == 0) f_zero += fa[v];
-    }
-    T vel[3] = {vel_in[0], vel_in[1], vel_in[2]};
-    T nv = vel[ndir];
-    if (nsgn == -1) nv *= T(-1.0);
-    (void)rho_in;
-    const T dens = (T(1.0) / (T(1.0) - nv)) * (T(2.0) * f_plus + f_zero);
-    rho = dens;
-    u[0] = vel[0];
-    u[1] = vel[1];
-    u[2] = (L::D == 3) ? vel[2] : T(0);
     T Sxx = 0, Syy = 0, Sxy = 0, Szz = 0, Sxz = 0, Syz = 0;
     // Unknown populations first (non-equilibrium bounce-back), in v order as LUMA does:
     // each v's update reads fa[opp v], which for an unknown v is a known population.
@@ -370,6 +352,61 @@ __device__ __forceinline__ void regularised_face(T *fa, int ncode, const T *vel_
                      ((T(c2 * c2) - cs2) * Szz) + (T(2.0) * T(c0) * T(c1) * Sxy) +
                      (T(2.0) * T(c0) * T(c2) * Sxz) + (T(2.0) * T(c1) * T(c2) * Syz));
     }
+}
+
+// _LBM_regularised_opt for a velocity site on a face (edgeCount == 1), :313-510.
+// fa holds absolute populations after streaming; on return it holds the regularised set and
+// rho/u the site's macroscopic values.
+template <int Q, class T>
+__device__ __forceinline__ void regularised_face(T *fa, int ncode, const T *vel_in, T rho_in,
+                                                 T cs2, T den2, T &rho, T *u) {
+    using L = Lattice<Q>;
+    const int ndir = (ncode - 1) >> 1;              // 0:x 1:y 2:z
+    const int nsgn = ((ncode - 1) & 1) ? -1 : 1;    // inward normal component
+    T f_plus = T(0), f_zero = T(0);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int cn = L::c(v, ndir);
+        if (cn == -nsgn) f_plus += fa[v];
+        else if (cn == 0) f_zero += fa[v];
+    }
+    T vel[3] = {vel_in[0], vel_in[1], vel_in[2]};
+    T nv = vel[ndir];
+    if (nsgn == -1) nv *= T(-1.0);
+    (void)rho_in;
+    const T dens = (T(1.0) / (T(1.0) - nv)) * (T(2.0) * f_plus + f_zero);
+    rho = dens;
+    u[0] = vel[0];
+    u[1] = vel[1];
+    u[2] = (L::D == 3) ? vel[2] : T(0);
+    regularise_face_set<Q, T>(fa, ndir, nsgn, rho, u, cs2, den2);
+}
+
+// _LBM_regularised_opt for a pressure site on a face (:389-402): rho is the prescribed density
+// (L_RHOIN), the tangential velocities are extrapolated from the two inward neighbours by the
+// caller (vel_t), the wall-normal one follows from the known populations.
+template <int Q, class T>
+__device__ __forceinline__ void regularised_pressure(T *fa, int ncode, const T *vel_t, T rho_in, T cs2, T den2,
+                                                     T &rho, T *u) {
+    using L = Lattice<Q>;
+    const int ndir = (ncode - 1) >> 1;
+    const int nsgn = ((ncode - 1) & 1) ? -1 : 1;
+    T f_plus = T(0), f_zero = T(0);
+#pragma unroll
+    for (int v = 0; v < Q; ++v) {
+        const int cn = L::c(v, ndir);
+        if (cn == -nsgn) f_plus += fa[v];
+        else if (cn == 0) f_zero += fa[v];
+    }
+    const T dens = rho_in;
+    T vel[3] = {vel_t[0], vel_t[1], vel_t[2]};
+    vel[ndir] = T(1.0) - ((T(1.0) / dens) * (T(2.0) * f_plus + f_zero));
+    if (nsgn == -1) vel[ndir] *= T(-1.0);
+    rho = dens;
+    u[0] = vel[0];
+    u[1] = vel[1];
+    u[2] = (L::D == 3) ? vel[2] : T(0);
+    regularise_face_set<Q, T>(fa, ndir, nsgn, rho, u, cs2, den2);
 }
 
 // _LBM_regularised_opt for a velocity site on an edge or corner (edgeCount > 1), :346-364 and
@@ -816,8 +853,8 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     const int ty = fl & kTypeMask;
     // sites LUMA's sweep 1 skips (:91-95); outlet sites are not read by any other site
     if (ty == kSolid || ty == kRefined || ty == kExtrapRight || (!a.regularised && ty == kVelocity)) return;
-    // edge and corner velocity sites run in k_edge_sites after this kernel
-    if (ty == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge) return;
+    // edge and corner velocity sites and pressure sites run in k_edge_sites after this kernel
+    if (ty == kPressure || (ty == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)) return;
     S *__restrict__ f = static_cast<S *>(a.f);
     const size_t ps = a.ps;
     Nbr nb;
@@ -937,19 +974,20 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
     store_macro<L::D>(a, cell, rho, u);
 }
 
-// Regularised velocity sites on edges and corners.  Each entry e holds the site's cell, its
-// packed inward normal (n_a + 1 in bits 2a..2a+1) and the cells of the two inward neighbours
-// x + n and x + 2n, which are fluid or TL2F sites.
+// Regularised sites that extrapolate from their inward neighbours: velocity sites on edges and
+// corners, and pressure sites (faces).  Each entry e holds the site's cell, its packed inward
+// normal (n_a + 1 in bits 2a..2a+1) and the cells of the two inward neighbours x + n and x + 2n,
+// which are fluid or TL2F sites.
 struct EdgeArgs {
     int n;
     const int *cell, *nrm, *n1, *n2;
-    double *erho;   // [2n]: sweep-1 rho of the two neighbours
+    double *erho;   // [2n][4] (M elements): sweep-1 rho, u of the two neighbours
 };
 
-// Sweep-1 density of the extrapolation neighbours, read from the layout the step kernel is
-// about to consume.  It equals LUMA's rho[id1], rho[id2] whether the neighbour was visited
-// before the edge site or is streamed and macro-updated on the fly (:1376-1404): both give
-// the neighbour's own stream + macro of this step.
+// Sweep-1 density and velocity of the extrapolation neighbours, read from the layout the step
+// kernel is about to consume.  They equal LUMA's rho[id1], u[id1], ... whether the neighbour
+// was visited before the site or is streamed and macro-updated on the fly (:1376-1404): both
+// give the neighbour's own stream + macro of this step, with the force the step started from.
 template <int Q, class P, bool ODD>
 __global__ __launch_bounds__(256) void k_edge_rho(StepArgs a, EdgeArgs e) {
     using T = typename P::T;
@@ -962,9 +1000,23 @@ __global__ __launch_bounds__(256) void k_edge_rho(StepArgs a, EdgeArgs e) {
     nb.init(x, y, z, a.nx, a.ny, a.nz);
     T fin[Q], fout[Q];
     gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
-    typename P::M rho, u[3];
-    site_update<Q, P>(fin, fout, nullptr, a, rho, u, false);
-    mv<typename P::M>(e.erho)[i] = rho;
+    using M = typename P::M;
+    M Fv[3];
+    const M *F = nullptr;
+    if (!a.sweep1_zero && (a.all_forced || (a.flags[cell] & kForcedBit))) {
+        const M *fx = mv<M>(a.force);
+        Fv[0] = fx[cell];
+        Fv[1] = fx[a.ncell + cell];
+        Fv[2] = (Lattice<Q>::D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
+        F = Fv;
+    }
+    M rho, u[3];
+    site_update<Q, P>(fin, fout, F, a, rho, u, false);
+    M *er = mv<M>(e.erho) + 4 * (size_t)i;
+    er[0] = rho;
+    er[1] = u[0];
+    er[2] = u[1];
+    er[3] = u[2];
 }
 
 // Edge/corner velocity sites: stream, regularise with the extrapolated density
@@ -1002,12 +1054,19 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
         F = Fv;
     }
     const T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
-    const M *er = mv<M>(e.erho);
-    const T dens = T(2.0) * T(er[2 * i]) - T(er[2 * i + 1]);
+    const M *er1 = mv<M>(e.erho) + 8 * (size_t)i, *er2 = er1 + 4;
     T fa[Q], rr, uu[3];
 #pragma unroll
     for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
-    regularised_edge<Q, T>(fa, n, vin, dens, T(a.cs2), T(a.den2), rr, uu);
+    if ((fl & kTypeMask) == kPressure) {
+        // tangential velocities: 2 u(x+n) - u(x+2n) (GridUtils::extrapolate, order 1)
+        const T vt[3] = {T(2.0) * T(er1[1]) - T(er2[1]), T(2.0) * T(er1[2]) - T(er2[2]),
+                         (L::D == 3) ? T(2.0) * T(er1[3]) - T(er2[3]) : T(0)};
+        regularised_pressure<Q, T>(fa, (fl >> kNormalShift) & 7, vt, T(a.rho_in), T(a.cs2), T(a.den2), rr, uu);
+    } else {
+        const T dens = T(2.0) * T(er1[0]) - T(er2[0]);
+        regularised_edge<Q, T>(fa, n, vin, dens, T(a.cs2), T(a.den2), rr, uu);
+    }
 #pragma unroll
     for (int v = 0; v < Q; ++v) fin[v] = P::kShift ? fa[v] - T(L::w(v) * a.rho0) : fa[v];
     if constexpr (!P::kShift) {

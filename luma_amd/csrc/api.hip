@@ -182,7 +182,9 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     // _LBM_resetForces at the start of the step on a level holding IB bodies (:53-57); with
     // gravity it writes the gravity component from the previous step's rho, and every site
     // then carries a force
-    g->all_forced = (g->gravity_on && ibm_marker_count(g) > 0) ? 1 : 0;
+    const int has_bodies = ibm_marker_count(g) > 0;
+    g->all_forced = (g->gravity_on && has_bodies) ? 1 : 0;
+    g->sweep1_zero = (has_bodies && !g->gravity_on) ? 1 : 0;
     if (g->all_forced) {
         const StepArgs ga = make_args(g);
         const double rr = 1.0 / std::pow(2.0, (double)g->d.level);   // GridObj.cpp:54
@@ -436,6 +438,23 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                         }
                         break;
                     }
+                    case kPressure: {
+                        // regularised pressure BC on a face (:389-402); LUMA has no other form
+                        if (!g->d.regularised)
+                            return fail(LUMAGPU_EUNSUPPORTED, "pressure sites need regularised boundaries");
+                        if (!xpos || !ypos || !zpos)
+                            return fail(LUMAGPU_EINVAL, "positions are required for regularised pressure sites");
+                        int code = 0, n[3];
+                        const int cnt = wall_normal(g->d, xpos[i], ypos[j], zpos[k], &code, n);
+                        if (cnt == 0)
+                            return fail(LUMAGPU_EINVAL,
+                                        "Velocity site not outside domain walls is currently not supported.");
+                        if (cnt > 1) return fail(LUMAGPU_EINVAL, "Pressure BC not supported at corner/edge site. Exiting.");
+                        fl = (uint8_t)(ty | (code << kNormalShift));
+                        ecell.push_back(i + N * (j + M * k));
+                        enrm.push_back((n[0] + 1) | ((n[1] + 1) << 2) | ((n[2] + 1) << 4));
+                        break;
+                    }
                     case kRefined:
                     case kTL2C:
                     case kTL2F: fl = (uint8_t)ty; break;
@@ -455,9 +474,10 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
                 const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
                 nf[cell] = (uint8_t)(fl | (prev[cell] & kForcedBit));
             }
-    // Edge/corner sites extrapolate rho from x + n and x + 2n (_LBM_updateAndExtrapolate,
-    // :1353-1412).  Those neighbours must be sites whose sweep 1 computes rho (fluid, TL2F);
-    // for any other type LUMA reads a value left from an earlier step or its initial state.
+    // Edge/corner velocity sites extrapolate rho, pressure sites their tangential u, from x + n
+    // and x + 2n (_LBM_updateAndExtrapolate, :1353-1412).  Those neighbours must be sites whose
+    // sweep 1 computes rho and u (fluid, TL2F); for any other type LUMA reads a value left from
+    // an earlier step or its initial state.
     const int ne = (int)ecell.size();
     std::vector<int> etab(4 * (size_t)ne);
     for (int e = 0; e < ne; ++e) {
@@ -473,7 +493,8 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
             const int nty = nf[nb[s - 1]] & kTypeMask;
             if (nty != kFluid && nty != kTL2F)
                 return fail(LUMAGPU_EUNSUPPORTED,
-                            "edge/corner velocity site whose extrapolation neighbours are not fluid is not supported");
+                            "edge/corner velocity or pressure site whose extrapolation neighbours are not fluid "
+                            "is not supported");
         }
         etab[e] = c;
         etab[ne + e] = enrm[e];
@@ -487,7 +508,7 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     g->nedge = 0;
     if (ne) {
         if (hipMalloc(&g->edge, etab.size() * sizeof(int)) != hipSuccess ||
-            hipMalloc(&g->erho, 2 * (size_t)ne * g->melem) != hipSuccess)
+            hipMalloc(&g->erho, 8 * (size_t)ne * g->melem) != hipSuccess)
             return fail(LUMAGPU_ENOMEM, "edge site table allocation failed");
         HIP_TRY(hipMemcpyAsync(g->edge, etab.data(), etab.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
         g->nedge = ne;

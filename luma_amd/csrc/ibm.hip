@@ -351,8 +351,10 @@ int upload(lumagpu_grid *g) {
                 put(sdelta, (size_t)gm * S + k, m.delta[k], m.ddelta[k]);
                 if (site_of[cell] < 0) {
                     const unsigned fl = g->hflags.empty() ? 0u : g->hflags[cell];
-                    if ((fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)
-                        return fail(LUMAGPU_EUNSUPPORTED, "IB support on an edge/corner velocity site is not supported");
+                    if (((fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge) ||
+                        (fl & kTypeMask) == kPressure)
+                        return fail(LUMAGPU_EUNSUPPORTED,
+                                    "IB support on an edge/corner velocity site or a pressure site is not supported");
                     site_of[cell] = (int)order.size();
                     order.push_back(cell);
                     entries.emplace_back();

@@ -37,11 +37,12 @@ struct lumagpu_grid {
     int has_outlet = 0;
     int nedge = 0;                    // regularised velocity sites on edges and corners
     int *edge = nullptr;              // [4*nedge]: cells, packed normals, neighbour x+n, x+2n
-    double *erho = nullptr;           // [2*nedge] neighbour densities of the current step
+    double *erho = nullptr;           // [8*nedge] neighbour rho, u of the current step (edge/pressure)
     double d_omega = 0.0;             // tangent form: seeded derivative of omega
     int gravity_on = 0, gravity_dir = 0;   // L_GRAVITY_ON, L_GRAVITY_DIRECTION
     double gravity = 0.0;             // GridObj::gravity (lattice units)
     int all_forced = 0;               // this step reset gravity: every site carries force_xyz
+    int sweep1_zero = 0;              // this step's IB reset zeroes force_xyz (sweep 1 sees no force)
     int masked = 0;
     int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
@@ -91,6 +92,7 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.regularised = g->d.regularised;
     a.fz_compat = g->d.fz_compat;
     a.all_forced = g->all_forced;
+    a.sweep1_zero = g->sweep1_zero;
     if (g->parent) {
         const lumagpu_grid *p = g->parent;
         a.pf = p->f;

@@ -238,6 +238,28 @@ def uniform_inflow_3d(N: int, M: int, K: int, U: float, tau: float, dh: float = 
                 info={"dh": dh, "origin": (0.0, 0.0, 0.0), "U": U})
 
 
+def pressure_outlet_duct(N: int, M: int, K: int = 1, dims: int = 2, U: float = 0.03, tau: float = 0.8,
+                         rho_out: float = 1.0, dh: float = 1.0, forced: bool = False) -> Case:
+    """Duct periodic in y (and z): a regularised eVelocity inlet at x = 0 with u = (U, 0, 0) and a
+    regularised ePressure outlet at x = N-1 holding rho = rho_out (L_RHOIN, GridObj_ops_lbm_optimised.cpp
+    :333-343, :389-402); one-cell wall thicknesses in x put both planes on faces (isWithinDomainWall).
+    Uniform initial flow U, rho = 1."""
+    Q = 9 if dims == 2 else 19
+    lt = np.full((N, M, K), FLUID, dtype=np.int32)
+    lt[0] = VELOCITY
+    lt[N - 1] = PRESSURE
+    u = np.zeros((N, M, K, dims))
+    u[..., 0] = U
+    rho = np.ones((N, M, K))
+    edges = (N * dh, M * dh, K * dh if dims == 3 else 0.0)
+    return Case(name=f"pressure{dims}d_{N}x{M}x{K}", dims=dims, Q=Q, N=N, M=M, K=K, omega=1.0 / tau, rho=rho, u=u,
+                lattyp=lt, xpos=(np.arange(N) + 0.5) * dh, ypos=(np.arange(M) + 0.5) * dh,
+                zpos=(np.arange(K) + 0.5) * dh if dims == 3 else np.array([0.0]), ux_in=np.full(M, U),
+                uy_in=np.zeros(M), uz_in=np.zeros(M), regularised=True, forced=forced, rho_in=rho_out,
+                edges=edges, wall_thk=(dh, dh, 0.0, 0.0, 0.0, 0.0),
+                info={"dh": dh, "origin": (0.0, 0.0, 0.0), "U": U})
+
+
 def body_force_channel(N: int, M: int, K: int = 1, dims: int = 2, tau: float = 0.8,
                        g: float = 1e-6) -> Case:
     """Channel with solid walls at j = 0 and j = M-1, periodic in x (and z), driven by a

@@ -333,8 +333,23 @@ static double update_and_extrapolate_rho(ora_grid *g, int subcycle, const int di
     if (id2 > id) update_interior_site(g, i2, j2, k2, subcycle);
     return 2.0 * g->rho[id1] - g->rho[id2];
 }
+/* the same for velocity component d (the pressure face's tangential velocities, :393-399) */
+static double update_and_extrapolate_u(ora_grid *g, int subcycle, const int dir[3], int i, int j, int k, int d,
+                                       int *err) {
+    int id = k + j * g->d.K + i * g->d.K * g->d.M;
+    int i1 = i + dir[0], i2 = i1 + dir[0];
+    int j1 = j + dir[1], j2 = j1 + dir[1];
+    int k1 = k + dir[2], k2 = k1 + dir[2];
+    if (off_grid(g, i1, j1, k1) || off_grid(g, i2, j2, k2)) { *err = 1; return 0.0; }
+    int id1 = k1 + j1 * g->d.K + i1 * g->d.K * g->d.M;
+    int id2 = k2 + j2 * g->d.K + i2 * g->d.K * g->d.M;
+    if (id1 > id) update_interior_site(g, i1, j1, k1, subcycle);
+    if (id2 > id) update_interior_site(g, i2, j2, k2, subcycle);
+    return 2.0 * g->u[d + (size_t)id1 * g->D] - g->u[d + (size_t)id2 * g->D];
+}
 
-/* ---- _LBM_regularised_opt: GridObj_ops_lbm_optimised.cpp:313-510 (velocity sites) ---- */
+/* ---- _LBM_regularised_opt: GridObj_ops_lbm_optimised.cpp:313-510 (velocity sites; pressure
+ *      sites on faces, :389-402, with rho = L_RHOIN as desc.rho_in carries it) ------------ */
 static void regularised_site(ora_grid *g, int i, int j, int k, int id, int type, int subcycle) {
     const int Q = g->Q, D = g->D;
     const double cs = g->cs;
@@ -351,7 +366,7 @@ static void regularised_site(ora_grid *g, int i, int j, int k, int id, int type,
     vel[1] = g->uy_in[j] * ramp;
     vel[2] = g->uz_in[j] * ramp;
     if (cnt > 1) {
-        if (type != O_VELOCITY) abort();
+        if (type != O_VELOCITY) abort();   /* L_ERROR: pressure BC not supported at corner/edge */
         int err = 0;
         dens = update_and_extrapolate_rho(g, subcycle, nrm, i, j, k, &err);
         if (err) abort();
@@ -360,10 +375,20 @@ static void regularised_site(ora_grid *g, int i, int j, int k, int id, int type,
             if (g->c[v][ndir] == -nrm[ndir]) f_plus += fn[v];
             else if (g->c[v][ndir] == 0) f_zero += fn[v];
         }
-        if (type != O_VELOCITY) abort();   /* pressure BC is not on the path */
-        double nv = vel[ndir];
-        if (nrm[ndir] == -1) nv *= -1.0;
-        dens = (1.0 / (1.0 - nv)) * (2.0 * f_plus + f_zero);
+        if (type == O_PRESSURE) {
+            for (int d = 0; d < D; d++)
+                if (d != ndir) {
+                    int err = 0;
+                    vel[d] = update_and_extrapolate_u(g, subcycle, nrm, i, j, k, d, &err);
+                    if (err) abort();
+                }
+            vel[ndir] = 1.0 - ((1.0 / dens) * (2.0 * f_plus + f_zero));
+            if (nrm[ndir] == -1) vel[ndir] *= -1.0;
+        } else {
+            double nv = vel[ndir];
+            if (nrm[ndir] == -1) nv *= -1.0;
+            dens = (1.0 / (1.0 - nv)) * (2.0 * f_plus + f_zero);
+        }
     }
     g->rho[id] = dens;
     g->u[0 + (size_t)id * D] = vel[0];

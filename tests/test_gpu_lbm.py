@@ -301,3 +301,40 @@ def test_inlet_profile_file(tmp_path):
     assert (y < yb[0]).any() and (y > yb[-1]).any() and ((y > yb[0]) & (y < yb[-1])).any()
     assert np.allclose(got, exp, rtol=0, atol=1e-14), np.abs(got - exp).max()
     assert np.all(u[0, :, 0, 1][inlet] == 0.0)
+
+
+def _pressure_case(dims, forced):
+    K = 6 if dims == 3 else 1
+    c = setups.pressure_outlet_duct(16, 10, K, dims=dims, U=0.04, tau=0.7, rho_out=1.001, forced=forced)
+    # a sheared inflow, so the outlet's tangential extrapolation carries information
+    c.uy_in[:] = 0.01 * np.sin(2 * np.pi * (np.arange(c.M) + 0.5) / c.M)
+    if dims == 3:
+        c.uz_in[:] = 0.005 * np.cos(2 * np.pi * (np.arange(c.M) + 0.5) / c.M)
+    if forced:
+        # a constant body force field (set_force): sweep 1's macro of the outlet's neighbours
+        # includes F/2, and so does their extrapolated tangential velocity
+        F = np.zeros((c.N, c.M, c.K, dims))
+        F[..., 1] = 2e-6 * np.cos(2 * np.pi * (np.arange(c.N) + 0.5) / c.N)[:, None, None]
+        F[..., 0] = 1e-6
+        c.force = F
+    return c
+
+
+@pytest.mark.parametrize("dims,forced", [(2, False), (3, False), (2, True)])
+def test_pressure_outlet_fp64_bitwise(dims, forced):
+    # _LBM_regularised_opt on ePressure faces (:389-402) in the deferred site kernel
+    assert_bitwise(_pressure_case(dims, forced), 120 if dims == 2 else 40)
+
+
+def test_pressure_outlet_shifted_forms_track_fp64():
+    c = _pressure_case(2, False)
+    out = {}
+    for prec in ("fp64", "fp32s"):
+        with GpuGrid.from_case(c, prec, store_macro=True) as g:
+            g.run(300)
+            out[prec] = g.download_macro()
+    (r64, u64), (r32, u32) = out["fp64"], out["fp32s"]
+    m = c.lattyp != setups.SOLID
+    assert np.abs(r32[m] - r64[m]).max() < 1e-6
+    assert np.abs(u32[m] - u64[m]).max() < 1e-6
+    assert np.abs(r64[-1] - 1.001).max() < 1e-15
