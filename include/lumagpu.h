@@ -97,6 +97,12 @@ int lumagpu_set_inlet(lumagpu_grid *g, const double *ux_in, const double *uy_in,
  * with the reference's end extrapolation; path replaces ./input/inlet_profile.in.  Values are
  * taken as the file gives them, as the reference does. */
 int lumagpu_read_inlet_profile(lumagpu_grid *g, const char *path);
+/* GridObj::LBM_initVelocity with L_INIT_VELOCITY_FROM_FILE (GridObj_init_grids.cpp:43-78): fills
+ * u[d + D*id] (LUMA layout, for lumagpu_init_equilibrium) from the first N*M*K rows of an
+ * x y z ux uy uz file in dimensionless units (ud2ulbm: u dt/dh), each row at the voxel that
+ * encloses its position; path replaces ./input/initial_velocity.in.  Fewer rows than sites is
+ * an error, as in the reference. */
+int lumagpu_read_initial_velocity(const lumagpu_grid *g, const char *path, double *u);
 /* rho and u, then f = feq(rho, u) (GridObj_init_grids.cpp:315-333) */
 int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u);
 /* GridObj::f after a step (post-collision populations) and the macro arrays rho, u */

@@ -96,6 +96,9 @@ class GridObj {
     void LBM_initGetInletProfileFromFile(const std::string &path) {
         check(lumagpu_read_inlet_profile(h_, path.c_str()));
     }
+    // LBM_initVelocity with L_INIT_VELOCITY_FROM_FILE (GridObj_init_grids.cpp:43-78): u from a
+    // velocity file; path replaces ./input/initial_velocity.in
+    void LBM_initVelocityFromFile(const std::string &path) { check(lumagpu_read_initial_velocity(h_, path.c_str(), u.data())); }
     // start from LUMA-layout post-collision populations already in f (a restart)
     void LBM_initFromF() {
         check(lumagpu_upload_f(h_, f.data()));

@@ -315,6 +315,52 @@ int lumagpu_ibm_read_restart(lumagpu_grid *const *grids, int ngrids, const char 
     return 0;
 }
 
+// GridObj::LBM_initVelocity with L_INIT_VELOCITY_FROM_FILE (GridObj_init_grids.cpp:43-78): the
+// first N*M*K rows of an x y z ux uy uz file (fewer is an error), each placed at the voxel that
+// encloses its position (rows off the grid skipped, a later row wins a voxel) as
+// u = ud2ulbm(u_file) = u_file dt / dh, into u[d + D*id] (LUMA layout).  Sites no row reaches
+// keep what u held.  Read with the same stream handling as the inlet profile.
+int lumagpu_read_initial_velocity(const lumagpu_grid *g, const char *path, double *u) {
+    if (!g || !path || !u) return fail(LUMAGPU_EINVAL, "null argument");
+    if (!(g->d.dh > 0.0) || !(g->d.dt > 0.0)) return fail(LUMAGPU_EINVAL, "desc.dh and desc.dt must be positive");
+    std::ifstream datafile(path, std::ios::in);
+    if (!datafile.is_open())
+        return fail(LUMAGPU_EINVAL, std::string("Cannot open velocity data file named ") + path + ". Exiting.");
+    std::vector<double> col[6];
+    std::string line_in;
+    std::istringstream iss;
+    double tmp = 0.0;
+    while (!datafile.eof()) {
+        std::getline(datafile, line_in, '\n');
+        iss.str(line_in);
+        iss.seekg(0);
+        for (auto &c : col) {
+            iss >> tmp;
+            c.push_back(tmp);
+        }
+    }
+    const int N = g->d.N, M = g->d.M, K = g->d.K, D = g->D;
+    const size_t n = (size_t)N * M * K;
+    if (col[0].size() < n)
+        return fail(LUMAGPU_EINVAL, "The initial velocity file has less points than the LBM grid -- change "
+                                    "L_BX/L_BY/L_BZ/L_RESOLUTION or change the file initial_velocity.in. Exiting.");
+    const double dh = g->d.dh, dt = g->d.dt;
+    for (size_t r = 0; r < n; ++r) {
+        const double x = col[0][r], y = col[1][r], z = col[2][r];
+        // GridUtils::isOnThisRank, serial: inside the level-0 edges, then the enclosing voxel
+        if (!(0.0 <= x && x < g->d.xmax && 0.0 <= y && y < g->d.ymax)) continue;
+        if (D == 3 && !(0.0 <= z && z < g->d.zmax)) continue;
+        const int i = (int)std::floor((x - g->d.origin[0]) / dh), j = (int)std::floor((y - g->d.origin[1]) / dh);
+        const int k = (D == 3) ? (int)std::floor((z - g->d.origin[2]) / dh) : 0;
+        if (i < 0 || i >= N || j < 0 || j >= M || k < 0 || k >= K) continue;
+        const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
+        u[0 + D * id] = (col[3][r] * dt) / dh;
+        u[1 + D * id] = (col[4][r] * dt) / dh;
+        if (D == 3) u[2 + D * id] = (col[5][r] * dt) / dh;
+    }
+    return 0;
+}
+
 // GridObj::_LBM_initGetInletProfileFromFile (GridObj_init_grids.cpp:1233-1312) with
 // GridUtils::readVelocityFromFile (GridUtils.cpp:1467-1524): columns x y z ux uy uz, the
 // profile taken along y and interpolated (or extrapolated from the end pair) at this grid's

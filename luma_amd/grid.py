@@ -298,6 +298,14 @@ class GpuGrid:
         """L_GRAVITY_ON: the IB force reset writes rho * gravity * refinement_ratio each step."""
         call("lumagpu_set_gravity", self.h, float(gravity), int(direction))
 
+    def read_initial_velocity(self, path, u=None):
+        """GridObj::LBM_initVelocity from a velocity file: u (N, M, K, D) in lattice units,
+        starting from `u` (zeros by default) where no row lands."""
+        N, M, K = self.shape
+        out = np.zeros((N, M, K, self.dims)) if u is None else np.array(u, dtype=np.float64, order="C")
+        call("lumagpu_read_initial_velocity", self.h, str(path).encode(), _dptr(out))
+        return out
+
     def read_inlet_profile(self, path):
         """GridObj::_LBM_initGetInletProfileFromFile: the inlet profile from an x y z ux uy uz file."""
         call("lumagpu_read_inlet_profile", self.h, str(path).encode())

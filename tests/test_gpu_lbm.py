@@ -338,3 +338,39 @@ def test_pressure_outlet_shifted_forms_track_fp64():
     assert np.abs(r32[m] - r64[m]).max() < 1e-6
     assert np.abs(u32[m] - u64[m]).max() < 1e-6
     assert np.abs(r64[-1] - 1.001).max() < 1e-15
+
+
+def test_initial_velocity_file(tmp_path):
+    # L_INIT_VELOCITY_FROM_FILE: a Taylor-Green field written at the cell centres in
+    # dimensionless units, rows in an arbitrary order plus rows off the grid, read back in
+    # lattice units (u dt / dh) at the voxels that enclose the rows' positions
+    c = setups.taylor_green((12, 10, 8), dims=3, tau=0.55, u0=0.05)
+    dh, dt = 0.1, 0.004
+    xs = (np.arange(c.N) + 0.5) * dh
+    ys = (np.arange(c.M) + 0.5) * dh
+    zs = (np.arange(c.K) + 0.5) * dh
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    U = c.u * dh / dt                                      # dimensionless
+    rows = np.column_stack([X.ravel(), Y.ravel(), Z.ravel(), U[..., 0].ravel(), U[..., 1].ravel(), U[..., 2].ravel()])
+    rows = rows[np.random.default_rng(4).permutation(len(rows))]
+    extra = np.array([[-0.05, 0.1, 0.1, 9.0, 9.0, 9.0], [5.0, 0.1, 0.1, 9.0, 9.0, 9.0]])
+    path = tmp_path / "initial_velocity.in"
+    np.savetxt(path, np.vstack([extra, rows]), fmt="%.17g", delimiter="\t")
+    with GpuGrid(3, 19, c.N, c.M, c.K, omega=c.omega, edges=(c.N * dh, c.M * dh, c.K * dh), dh=dh, dt=dt) as g:
+        # only the first N*M*K rows are read: the last two of the field fall outside
+        u = g.read_initial_velocity(path)
+    expect = (U * dt) / dh
+    n = c.N * c.M * c.K
+    read = np.vstack([extra, rows])[:n]
+    got_rows = [(int(r[0] / dh), int(r[1] / dh), int(r[2] / dh)) for r in read if 0 <= r[0] < c.N * dh]
+    mask = np.zeros((c.N, c.M, c.K), bool)
+    for i, j, k in got_rows:
+        mask[i, j, k] = True
+    assert mask.sum() == n - 2
+    assert np.array_equal(u[mask], expect[mask])
+    assert not u[~mask].any()
+    short = tmp_path / "short.in"
+    np.savetxt(short, rows[:10], fmt="%.17g", delimiter="\t")
+    with GpuGrid(3, 19, c.N, c.M, c.K, omega=c.omega, edges=(c.N * dh, c.M * dh, c.K * dh), dh=dh, dt=dt) as g:
+        with pytest.raises(RuntimeError, match="less points than the LBM grid"):
+            g.read_initial_velocity(short)
