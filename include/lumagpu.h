@@ -143,6 +143,13 @@ int lumagpu_set_store_macro(lumagpu_grid *g, int on);
  * rho * gravity * refinement_ratio (:291-297), which lumagpu_set_force gives.  Turns the macro
  * store on.  Flexible bodies are refused at the step. */
 int lumagpu_set_gravity(lumagpu_grid *g, double gravity, int direction);
+/* L_VELOCITY_RAMP (definitions.h:244): the inlet profile is scaled by
+ * GridUtils::getVelocityRampCoefficient (GridUtils.cpp:1808-1816), (1 - cos(pi t / T)) / 2 for
+ * t <= T and 1 after, with t = (GridObj::t + 1) dt at regularised velocity sites (:326) and
+ * t = GridObj::t dt at forced-equilibrium sources (:258-266; these take the profile of the
+ * pulling site's row, as the reference writes it).  ramp_time = T in dimensionless time; 0 turns
+ * it off.  Needs desc.dt.  lumagpu_run steps eagerly while a ramp is set. */
+int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time);
 /* completed steps on this grid (GridObj::t) */
 int lumagpu_get_time(const lumagpu_grid *g, int *t);
 int lumagpu_sync(lumagpu_grid *g);

@@ -93,6 +93,10 @@ struct StepArgs {
     int fz_compat;                // 3D macro quirk (:833): F_z/2 added to x-momentum
     int all_forced;               // every site carries force_xyz (gravity reset on this level)
     int sweep1_zero;              // the IB reset zeroes force_xyz at the start of the step
+    // L_VELOCITY_RAMP: inlet factors of this step, (1 - cos(pi t / T)) / 2 at t = (step+1) dt for
+    // the regularised sites and t = step dt for forced-equilibrium sources (1 when off)
+    int ramp_on;
+    double ramp_reg, ramp_feq;
     // --- refinement coupling (GridObj::parentGrid / subGrid) -----------------------------
     const void *pf;               // parent population array (explode source)
     size_t pps;
@@ -789,9 +793,12 @@ __device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z
         } else if (sty == kExtrapRight) {
             fin[v] = T(mv<typename P::M>(a.xr)[(size_t)v * ny * a.nz + nb.ys[1 - cy] + (size_t)ny * nb.zs[1 - cz]]);
         } else if (!a.regularised && sty == kVelocity) {
-            // forced-equilibrium velocity source, :254-270; its rho, u never change
-            const int js = nb.ys[1 - cy];
-            const double us[3] = {a.uin[js], a.uin[ny + js], (L::D == 3) ? a.uin[2 * ny + js] : 0.0};
+            // forced-equilibrium velocity source, :254-270: its rho never changes; its u is the
+            // source row's profile, or with the ramp the ramped profile of this site's row y
+            // (the reference writes u[src] = ux_in[j] * ramp with j the pulling site's row)
+            const int js = a.ramp_on ? y : nb.ys[1 - cy];
+            const double r = a.ramp_feq;
+            const double us[3] = {a.uin[js] * r, a.uin[ny + js] * r, (L::D == 3) ? a.uin[2 * ny + js] * r : 0.0};
             fin[v] = feq_store<Q, P>(v, a.rho_in, us, a);
         } else if (a.pf && sty == kTL2C && a.subcycle == 0) {
             // explode, :646-663: the parent's post-collision value at the coarse cell of the source
@@ -828,7 +835,8 @@ __device__ __forceinline__ void velocity_site(const StepArgs &a, unsigned fl, in
     using T = typename P::T;
     const int ny = a.ny;
     const int ncode = (fl >> kNormalShift) & 7;
-    T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
+    const double r = a.ramp_reg;
+    T vin[3] = {T(a.uin[y] * r), T(a.uin[ny + y] * r), (L::D == 3) ? T(a.uin[2 * ny + y] * r) : T(0)};
     T fa[Q];
 #pragma unroll
     for (int v = 0; v < Q; ++v) fa[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
@@ -1053,7 +1061,8 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
         Fv[2] = (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
         F = Fv;
     }
-    const T vin[3] = {T(a.uin[y]), T(a.uin[ny + y]), (L::D == 3) ? T(a.uin[2 * ny + y]) : T(0)};
+    const double r = a.ramp_reg;
+    const T vin[3] = {T(a.uin[y] * r), T(a.uin[ny + y] * r), (L::D == 3) ? T(a.uin[2 * ny + y] * r) : T(0)};
     const M *er1 = mv<M>(e.erho) + 8 * (size_t)i, *er2 = er1 + 4;
     T fa[Q], rr, uu[3];
 #pragma unroll

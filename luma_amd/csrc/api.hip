@@ -236,6 +236,7 @@ namespace {
 bool graph_ok(const lumagpu_grid *g) {
     for (const lumagpu_grid *x = g; x; x = x->child) {
         if (ibm_has_flexible(x)) return false;
+        if (x->ramp_time > 0.0) return false;   // the ramp factors change every step
     }
     return !g->parent;
 }
@@ -785,6 +786,14 @@ int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz) {
     HIP_TRY(hipMemcpyAsync(g->force, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->masked = 1;
+    return 0;
+}
+
+int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time) {
+    if (int r = check_grid(g)) return r;
+    if (!(ramp_time >= 0.0)) return fail(LUMAGPU_EINVAL, "ramp time must be >= 0");
+    if (ramp_time > 0.0 && !(g->d.dt > 0.0)) return fail(LUMAGPU_EINVAL, "a velocity ramp needs desc.dt > 0");
+    g->ramp_time = ramp_time;
     return 0;
 }
 

@@ -43,6 +43,7 @@ struct lumagpu_grid {
     double gravity = 0.0;             // GridObj::gravity (lattice units)
     int all_forced = 0;               // this step reset gravity: every site carries force_xyz
     int sweep1_zero = 0;              // this step's IB reset zeroes force_xyz (sweep 1 sees no force)
+    double ramp_time = 0.0;           // L_VELOCITY_RAMP in dimensionless time (0: off)
     int masked = 0;
     int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
@@ -92,6 +93,16 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.regularised = g->d.regularised;
     a.fz_compat = g->d.fz_compat;
     a.all_forced = g->all_forced;
+    // GridUtils::getVelocityRampCoefficient (GridUtils.cpp:1808-1816) at (t+1) dt (regularised
+    // sites, GridObj_ops_lbm_optimised.cpp:326) and t dt (forced-equilibrium sources, :258-266)
+    auto ramp = [g](double t) {
+        if (g->ramp_time > 0.0 && t <= g->ramp_time)
+            return (1.0 - std::cos(3.14159265358979323846 * t / g->ramp_time)) / 2.0;
+        return 1.0;
+    };
+    a.ramp_on = g->ramp_time > 0.0;
+    a.ramp_reg = ramp((g->t + 1) * g->d.dt);
+    a.ramp_feq = ramp(g->t * g->d.dt);
     a.sweep1_zero = g->sweep1_zero;
     if (g->parent) {
         const lumagpu_grid *p = g->parent;

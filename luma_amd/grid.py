@@ -294,6 +294,10 @@ class GpuGrid:
         call("lumagpu_download_tangent", self.h, _dptr(df), _dptr(dr), _dptr(du))
         return df, dr, du
 
+    def set_velocity_ramp(self, ramp_time):
+        """L_VELOCITY_RAMP: ramp the inlet over ramp_time (dimensionless time); 0 turns it off."""
+        call("lumagpu_set_velocity_ramp", self.h, float(ramp_time))
+
     def set_gravity(self, gravity, direction):
         """L_GRAVITY_ON: the IB force reset writes rho * gravity * refinement_ratio each step."""
         call("lumagpu_set_gravity", self.h, float(gravity), int(direction))

@@ -57,7 +57,18 @@ struct ora_grid {
     int copy_u_n;
     int gravity_on, gravity_dir;   /* L_GRAVITY_ON, L_GRAVITY_DIRECTION */
     double gravity;                /* GridObj::gravity (lattice units) */
+    double ramp_time, dt;          /* L_VELOCITY_RAMP (0: undefined), GridObj::dt */
 };
+
+/* GridUtils::getVelocityRampCoefficient (GridUtils.cpp:1808-1816) */
+static double ramp_coefficient(const ora_grid *g, double t) {
+    if (g->ramp_time > 0.0 && t <= g->ramp_time) return (1.0 - cos(3.14159265358979323846 * t / g->ramp_time)) / 2.0;
+    return 1.0;
+}
+void ora_set_ramp(ora_grid *g, double ramp_time, double dt) {
+    g->ramp_time = ramp_time;
+    g->dt = dt;
+}
 
 double ora_cs(void) { return 1.0 / sqrt(3.0); }   /* stdafx.cpp:152 */
 
@@ -233,6 +244,13 @@ static void stream_site(ora_grid *g, int i, int j, int k, int id, int type_local
         } else if (st == O_EXTRAP_RIGHT) {
             g->fNew[v + id * Q] = g->f[v + (sid - 2 * (K * M)) * Q];
         } else if (!g->d.regularised && st == O_VELOCITY) {
+            if (g->ramp_time > 0.0) {
+                /* :258-266: the source's u set from the ramped profile of this site's row j */
+                const double r = ramp_coefficient(g, g->t * g->dt);
+                g->u[0 + (size_t)sid * g->D] = g->ux_in[j] * r;
+                g->u[1 + (size_t)sid * g->D] = g->uy_in[j] * r;
+                if (g->D == 3) g->u[2 + (size_t)sid * g->D] = g->uz_in[j] * r;
+            }
             g->fNew[v + id * Q] = ora_equilibrium(g, sid, v);
         } else if (g->parent && st == O_TL2C && subcycle == 0) {
             explode(g, id, v, sx, sy, sz);
@@ -361,7 +379,7 @@ static void regularised_site(ora_grid *g, int i, int j, int k, int id, int type,
     double *fn = &g->fNew[(size_t)id * Q];
     int cnt = within_wall(g, g->XPos[i], g->YPos[j], g->ZPos[k], nrm, &ndir);
     if (cnt == 0) abort();   /* L_ERROR: velocity site not on a domain wall */
-    double ramp = 1.0;        /* GridUtils::getVelocityRampCoefficient without L_VELOCITY_RAMP */
+    const double ramp = ramp_coefficient(g, (g->t + 1) * g->dt);   /* :326 */
     vel[0] = g->ux_in[j] * ramp;
     vel[1] = g->uy_in[j] * ramp;
     vel[2] = g->uz_in[j] * ramp;

@@ -63,6 +63,7 @@ void ora_sweep1(ora_grid *g, int subcycle);        /* stream + BC + macro */
 void ora_sweep2(ora_grid *g);                      /* force + collide + swap + ++t */
 void ora_reset_forces(ora_grid *g);                /* _LBM_resetForces */
 void ora_set_gravity(ora_grid *g, double gravity, int dir);   /* L_GRAVITY_ON in the reset */
+void ora_set_ramp(ora_grid *g, double ramp_time, double dt);  /* L_VELOCITY_RAMP (0: off), GridObj::dt */
 void ora_macro_site(ora_grid *g, int i, int j, int k);  /* _LBM_macro_opt on one site */
 /* Run n steps of LBM_multi_opt on g with an optional OpenMP sweep (nthreads > 1). */
 void ora_run(ora_grid *g, int nsteps, int nthreads);

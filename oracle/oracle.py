@@ -50,6 +50,7 @@ def _load():
             "ora_sweep1": ([_vp, C.c_int], None), "ora_sweep2": ([_vp], None),
             "ora_reset_forces": ([_vp], None), "ora_run": ([_vp, C.c_int, C.c_int], None),
             "ora_set_gravity": ([_vp, C.c_double, C.c_int], None),
+            "ora_set_ramp": ([_vp, C.c_double, C.c_double], None),
             "ora_equilibrium": ([_vp, C.c_int, C.c_int], C.c_double),
             "ora_lattice": ([C.c_int, _ip, _dp, _ip], None), "ora_cs": ([], C.c_double),
             "ora_ibm_create": ([_vp, C.c_double, _dp], _vp), "ora_ibm_destroy": ([_vp], None),
@@ -153,6 +154,9 @@ class OracleGrid:
 
     def set_gravity(self, gravity, direction):
         self.lib.ora_set_gravity(self.h, float(gravity), int(direction))
+
+    def set_ramp(self, ramp_time, dt):
+        self.lib.ora_set_ramp(self.h, float(ramp_time), float(dt))
 
     def set_f(self, f):
         a, p = _d(f)

@@ -262,3 +262,29 @@ def test_pressure_outlet_sets_density_and_extrapolates_tangential_velocity():
     assert np.allclose(u1[-1, :, 0, 1], ext, rtol=0, atol=1e-16)
     # mass flux balance is not exact, but the outlet normal velocity follows the known populations
     assert np.abs(u1[-1, :, 0, 0] - 0.03).max() < 5e-3
+
+
+def test_velocity_ramp_scales_the_regularised_inlet():
+    # L_VELOCITY_RAMP (GridUtils.cpp:1808-1816): the regularised inlet carries u_in * r((t+1) dt)
+    # with r rising from 0 to 1 over T as half a cosine period, symmetric about T/2, and 1 after
+    case = setups.channel_2d(24, 12, 8.0, 0.04, 50.0)
+    dt = case.info["dt"]
+    T = 8 * dt
+    g = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega, regularised=True,
+                   forced=case.forced, rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk)
+    g.set_labels(case.lattyp)
+    g.set_positions(case.xpos, case.ypos, case.zpos)
+    g.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+    g.init_feq(case.rho, case.u)
+    g.set_ramp(T, dt)
+    j = case.M // 2
+    ratios = []
+    for _ in range(10):
+        g.step()
+        _, u = g.get_macro()
+        ratios.append(u[0, j, 0, 0] / case.ux_in[j])
+    r = np.array(ratios)                      # r[n] at t = (n + 1) dt
+    assert np.all(np.diff(r[:8]) > 0) and 0 < r[0] < 1
+    assert abs(r[3] - 0.5) < 1e-15            # t = T/2
+    assert abs(r[1] + r[5] - 1.0) < 1e-15     # T/2 -+ 2 dt
+    assert r[7] == 1.0 and r[8] == 1.0 and r[9] == 1.0
