@@ -150,6 +150,23 @@ int lumagpu_set_gravity(lumagpu_grid *g, double gravity, int direction);
  * pulling site's row, as the reference writes it).  ramp_time = T in dimensionless time; 0 turns
  * it off.  Needs desc.dt.  lumagpu_run steps eagerly while a ramp is set. */
 int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time);
+/* L_REYNOLDS_RAMP (definitions.h:204): at the start of every step of this grid and of each grid
+ * below it (LBM_multi_opt, GridObj_ops_lbm_optimised.cpp:39-42, before the children's
+ * sub-steps), omega is reset by _LBM_updateReynolds (:1313-1321) to
+ * 1 / (nu / cs^2 + 0.5) with nu = (dt / Re) / dh^2 (GridUnits::nud2nulbm, GridUnits.h:128-131)
+ * and Re = re * getReynoldsRampCoefficient((t + 1) dt) (GridUtils.cpp:1825-1833), the
+ * coefficient being 1 - cos(pi t / T) for t <= T, which rises to 2 at T, and 1 after, as the
+ * reference computes it.  re = L_RE, ramp_time = T in dimensionless time (0 turns it off),
+ * with each level's own desc.dt and desc.dh.  The omega seed of the tangent form stays
+ * constant; the adjoint refuses a grid with a Reynolds ramp. */
+int lumagpu_set_reynolds_ramp(lumagpu_grid *g, double re, double ramp_time);
+/* GridObj::omega as the last step used it (the descriptor's until a Reynolds ramp changes it) */
+int lumagpu_get_omega(const lumagpu_grid *g, double *omega);
+/* L_PRESSURE_DELTA (definitions.h:245): regularised pressure sites hold
+ * rho = L_RHOIN + GridUnits::pd2dlbm(p) = L_RHOIN + (p dh dt^2 / dm) / cs^2
+ * (GridObj_ops_lbm_optimised.cpp:317, :343-345; GridUnits.h:177-180), p dimensionless.  The
+ * initial density and the forced-equilibrium sources keep L_RHOIN.  Needs desc.dh, dt, dm. */
+int lumagpu_set_pressure_delta(lumagpu_grid *g, double p);
 /* completed steps on this grid (GridObj::t) */
 int lumagpu_get_time(const lumagpu_grid *g, int *t);
 int lumagpu_sync(lumagpu_grid *g);

@@ -92,6 +92,9 @@ _PROTOS = {
     "lumagpu_read_initial_velocity": [_vp, C.c_char_p, _dp],
     "lumagpu_set_gravity": [_vp, C.c_double, C.c_int],
     "lumagpu_set_velocity_ramp": [_vp, C.c_double],
+    "lumagpu_set_pressure_delta": [_vp, C.c_double],
+    "lumagpu_set_reynolds_ramp": [_vp, C.c_double, C.c_double],
+    "lumagpu_get_omega": [_vp, _dp],
     "lumagpu_tangent_seed_omega": [_vp, C.c_double],
     "lumagpu_ibm_body_force_tangent": [_vp, C.c_int, _dp, _dp],
     "lumagpu_ibm_get_marker_tangents": [_vp, C.c_int, _dp],

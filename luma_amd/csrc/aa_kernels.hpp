@@ -86,6 +86,7 @@ struct StepArgs {
     double *u_out;                // macro output [d*ncell + cell]
     size_t ncell;
     double omega, rho0, rho_in;
+    double rho_p;                 // pressure sites: L_RHOIN + pd2dlbm(L_PRESSURE_DELTA)
     double d_omega;               // derivative of omega along the seeded direction (FP64 tangent form)
     double cs2, den2;             // cs*cs and 2*cs^2*cs^2, evaluated as LUMA does
     float om_hi, om_lo;           // omega as a float pair (shifted forms)
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
         // tangential velocities: 2 u(x+n) - u(x+2n) (GridUtils::extrapolate, order 1)
         const T vt[3] = {T(2.0) * T(er1[1]) - T(er2[1]), T(2.0) * T(er1[2]) - T(er2[2]),
                          (L::D == 3) ? T(2.0) * T(er1[3]) - T(er2[3]) : T(0)};
-        regularised_pressure<Q, T>(fa, (fl >> kNormalShift) & 7, vt, T(a.rho_in), T(a.cs2), T(a.den2), rr, uu);
+        regularised_pressure<Q, T>(fa, (fl >> kNormalShift) & 7, vt, T(a.rho_p), T(a.cs2), T(a.den2), rr, uu);
     } else {
         const T dens = T(2.0) * T(er1[0]) - T(er2[0]);
         regularised_edge<Q, T>(fa, n, vin, dens, T(a.cs2), T(a.den2), rr, uu);

@@ -158,8 +158,23 @@ hipError_t launch_extrap(const lumagpu_grid *g) {
     }
 }
 
+// _LBM_updateReynolds (GridObj_ops_lbm_optimised.cpp:1313-1321) at the start of
+// LBM_multi_opt (:39-42): Re = L_RE * getReynoldsRampCoefficient((t + 1) dt)
+// (GridUtils.cpp:1825-1833: 1 - cos(pi t / T) up to T, 1 after), nu = nud2nulbm(1 / Re)
+// (GridUnits.h:128-131), omega = 1 / (nu / cs^2 + 0.5)
+void update_reynolds(lumagpu_grid *g) {
+    const double t = (g->t + 1) * g->d.dt;
+    double coef = 1.0;
+    if (t <= g->re_ramp) coef = 1.0 - std::cos(3.14159265358979323846 * t / g->re_ramp);
+    const double re = g->re * coef;
+    const double nu = ((1.0 / re) * g->d.dt) / (g->d.dh * g->d.dh);
+    const double cs = 1.0 / std::sqrt(3.0);
+    g->d.omega = 1.0 / ((nu / (cs * cs)) + 0.5);
+}
+
 int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
+    if (g->re_ramp > 0.0) update_reynolds(g);
     // GridObj_ops_lbm_optimised.cpp:45-49: two sub-steps of the child first
     if (g->child) {
         lumagpu_grid *c = g->child;
@@ -237,6 +252,7 @@ bool graph_ok(const lumagpu_grid *g) {
     for (const lumagpu_grid *x = g; x; x = x->child) {
         if (ibm_has_flexible(x)) return false;
         if (x->ramp_time > 0.0) return false;   // the ramp factors change every step
+        if (x->re_ramp > 0.0) return false;     // so does omega under a Reynolds ramp
     }
     return !g->parent;
 }
@@ -794,6 +810,37 @@ int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time) {
     if (!(ramp_time >= 0.0)) return fail(LUMAGPU_EINVAL, "ramp time must be >= 0");
     if (ramp_time > 0.0 && !(g->d.dt > 0.0)) return fail(LUMAGPU_EINVAL, "a velocity ramp needs desc.dt > 0");
     g->ramp_time = ramp_time;
+    return 0;
+}
+
+int lumagpu_set_reynolds_ramp(lumagpu_grid *g, double re, double ramp_time) {
+    if (int r = check_grid(g)) return r;
+    if (!(ramp_time >= 0.0)) return fail(LUMAGPU_EINVAL, "Reynolds ramp time must be >= 0");
+    if (ramp_time > 0.0 && !(re > 0.0)) return fail(LUMAGPU_EINVAL, "a Reynolds ramp needs L_RE > 0");
+    for (lumagpu_grid *x = g; x; x = x->child)
+        if (ramp_time > 0.0 && !(x->d.dt > 0.0 && x->d.dh > 0.0))
+            return fail(LUMAGPU_EINVAL, "a Reynolds ramp needs desc.dt and desc.dh > 0 on every level");
+    for (lumagpu_grid *x = g; x; x = x->child) {
+        x->re = re;
+        x->re_ramp = ramp_time;
+    }
+    return 0;
+}
+
+int lumagpu_get_omega(const lumagpu_grid *g, double *omega) {
+    if (int r = check_grid(g)) return r;
+    if (omega) *omega = g->d.omega;
+    return 0;
+}
+
+int lumagpu_set_pressure_delta(lumagpu_grid *g, double p) {
+    if (int r = check_grid(g)) return r;
+    if (!std::isfinite(p)) return fail(LUMAGPU_EINVAL, "pressure delta must be finite");
+    if (p != 0.0 && !(g->d.dm > 0.0 && g->d.dt > 0.0 && g->d.dh > 0.0))
+        return fail(LUMAGPU_EINVAL, "a pressure delta needs desc.dh, dt and dm > 0");
+    // GridUnits::pd2dlbm (GridUnits.h:177-180)
+    const double cs = 1.0 / std::sqrt(3.0);
+    g->pressure_drho = p == 0.0 ? 0.0 : (p * g->d.dh * (g->d.dt * g->d.dt) / g->d.dm) / (cs * cs);
     return 0;
 }
 

@@ -44,6 +44,8 @@ struct lumagpu_grid {
     int all_forced = 0;               // this step reset gravity: every site carries force_xyz
     int sweep1_zero = 0;              // this step's IB reset zeroes force_xyz (sweep 1 sees no force)
     double ramp_time = 0.0;           // L_VELOCITY_RAMP in dimensionless time (0: off)
+    double pressure_drho = 0.0;       // GridUnits::pd2dlbm(L_PRESSURE_DELTA)
+    double re = 0.0, re_ramp = 0.0;   // L_RE and L_REYNOLDS_RAMP in dimensionless time (0: off)
     int masked = 0;
     int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
@@ -84,6 +86,9 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.d_omega = g->d_omega;
     a.rho0 = g->d.rho0;
     a.rho_in = g->d.rho_in;
+    // tmpDensity = L_RHOIN; tmpDensity += pd2dlbm(L_PRESSURE_DELTA) (GridObj_ops_lbm_optimised.cpp:317, :343-345)
+    a.rho_p = g->d.rho_in;
+    a.rho_p += g->pressure_drho;
     const double cs = 1.0 / std::sqrt(3.0);    // LUMA/src/stdafx.cpp:152
     a.cs2 = cs * cs;
     a.den2 = 2.0 * (cs * cs) * (cs * cs);

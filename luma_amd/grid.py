@@ -298,6 +298,20 @@ class GpuGrid:
         """L_VELOCITY_RAMP: ramp the inlet over ramp_time (dimensionless time); 0 turns it off."""
         call("lumagpu_set_velocity_ramp", self.h, float(ramp_time))
 
+    def set_reynolds_ramp(self, re, ramp_time):
+        """L_REYNOLDS_RAMP over L_RE = re on this grid and the grids below it (0 turns it off)."""
+        call("lumagpu_set_reynolds_ramp", self.h, float(re), float(ramp_time))
+
+    @property
+    def omega(self):
+        o = C.c_double()
+        call("lumagpu_get_omega", self.h, C.byref(o))
+        return o.value
+
+    def set_pressure_delta(self, p):
+        """L_PRESSURE_DELTA: pressure sites hold L_RHOIN + pd2dlbm(p) (p dimensionless)."""
+        call("lumagpu_set_pressure_delta", self.h, float(p))
+
     def set_gravity(self, gravity, direction):
         """L_GRAVITY_ON: the IB force reset writes rho * gravity * refinement_ratio each step."""
         call("lumagpu_set_gravity", self.h, float(gravity), int(direction))

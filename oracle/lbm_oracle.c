@@ -58,6 +58,8 @@ struct ora_grid {
     int gravity_on, gravity_dir;   /* L_GRAVITY_ON, L_GRAVITY_DIRECTION */
     double gravity;                /* GridObj::gravity (lattice units) */
     double ramp_time, dt;          /* L_VELOCITY_RAMP (0: undefined), GridObj::dt */
+    double pressure_drho;          /* GridUnits::pd2dlbm(L_PRESSURE_DELTA) */
+    double re, re_ramp, re_dt, re_dh;   /* L_RE, L_REYNOLDS_RAMP (0: undefined), GridObj::dt, dh */
 };
 
 /* GridUtils::getVelocityRampCoefficient (GridUtils.cpp:1808-1816) */
@@ -68,6 +70,35 @@ static double ramp_coefficient(const ora_grid *g, double t) {
 void ora_set_ramp(ora_grid *g, double ramp_time, double dt) {
     g->ramp_time = ramp_time;
     g->dt = dt;
+}
+/* L_REYNOLDS_RAMP: GridUtils::getReynoldsRampCoefficient (GridUtils.cpp:1825-1833), which
+ * is 1 - cos(pi t / T) up to t = T (rising to 2) and 1 after */
+void ora_set_reynolds_ramp(ora_grid *g, double re, double ramp_time, double dt, double dh) {
+    g->re = re;
+    g->re_ramp = ramp_time;
+    g->re_dt = dt;
+    g->re_dh = dh;
+}
+/* _LBM_updateReynolds (GridObj_ops_lbm_optimised.cpp:1313-1321) with Re = L_RE * coefficient
+ * at (t + 1) dt, at the start of LBM_multi_opt (:39-42): nu = nud2nulbm(1 / Re)
+ * (GridUnits.h:128-131), omega = 1 / (nu / cs^2 + 0.5) */
+static void update_reynolds(ora_grid *g) {
+    const double t = (g->t + 1) * g->re_dt;
+    double coef = 1.0;
+    if (t <= g->re_ramp) coef = 1.0 - cos(3.14159265358979323846 * t / g->re_ramp);
+    const double re = g->re * coef;
+    const double nu = ((1.0 / re) * g->re_dt) / (g->re_dh * g->re_dh);
+    const double cs = 1.0 / sqrt(3.0);
+    g->d.omega = 1.0 / ((nu / (cs * cs)) + 0.5);
+}
+
+double ora_get_omega(const ora_grid *g) { return g->d.omega; }
+
+/* L_PRESSURE_DELTA added to the pressure sites' density (GridObj_ops_lbm_optimised.cpp:343-345)
+ * through GridUnits::pd2dlbm (GridUnits.h:177-180): p dh dt^2 / dm / cs^2 */
+void ora_set_pressure_delta(ora_grid *g, double p, double dh, double dt, double dm) {
+    const double cs = 1.0 / sqrt(3.0);
+    g->pressure_drho = (p * dh * (dt * dt) / dm) / (cs * cs);
 }
 
 double ora_cs(void) { return 1.0 / sqrt(3.0); }   /* stdafx.cpp:152 */
@@ -367,12 +398,13 @@ static double update_and_extrapolate_u(ora_grid *g, int subcycle, const int dir[
 }
 
 /* ---- _LBM_regularised_opt: GridObj_ops_lbm_optimised.cpp:313-510 (velocity sites; pressure
- *      sites on faces, :389-402, with rho = L_RHOIN as desc.rho_in carries it) ------------ */
+ *      sites on faces, :389-402, with rho = L_RHOIN + pd2dlbm(L_PRESSURE_DELTA), :317, :343) --- */
 static void regularised_site(ora_grid *g, int i, int j, int k, int id, int type, int subcycle) {
     const int Q = g->Q, D = g->D;
     const double cs = g->cs;
     double vel[3] = {0, 0, 0};
     double dens = g->d.rho_in;
+    dens += g->pressure_drho;
     int nrm[3], ndir;
     double f_plus = 0.0, f_zero = 0.0;
     double Sxx = 0, Syy = 0, Sxy = 0, Szz = 0, Sxz = 0, Syz = 0;
@@ -541,6 +573,7 @@ void ora_sweep2(ora_grid *g) {
 
 /* ---- LBM_multi_opt: GridObj_ops_lbm_optimised.cpp:36-193 ---------------------------- */
 void ora_multi_opt(ora_grid *g, int subcycle) {
+    if (g->re_ramp > 0.0) update_reynolds(g);
     if (g->child) {
         ora_multi_opt(g->child, 0);
         ora_multi_opt(g->child, 1);
@@ -556,7 +589,7 @@ void ora_multi_opt(ora_grid *g, int subcycle) {
  * the x index like LUMA's own experimental pragma (:69-71).  Valid for grids without
  * regularised corner sites (whose serial order the parallel sweep cannot keep). */
 void ora_run(ora_grid *g, int nsteps, int nthreads) {
-    if (nthreads <= 1 || g->child || g->hook) {
+    if (nthreads <= 1 || g->child || g->hook || g->re_ramp > 0.0) {
         for (int s = 0; s < nsteps; ++s) ora_multi_opt(g, 0);
         return;
     }

@@ -64,6 +64,12 @@ void ora_sweep2(ora_grid *g);                      /* force + collide + swap + +
 void ora_reset_forces(ora_grid *g);                /* _LBM_resetForces */
 void ora_set_gravity(ora_grid *g, double gravity, int dir);   /* L_GRAVITY_ON in the reset */
 void ora_set_ramp(ora_grid *g, double ramp_time, double dt);  /* L_VELOCITY_RAMP (0: off), GridObj::dt */
+/* L_REYNOLDS_RAMP T (0: off) over L_RE with GridObj::dt, dh of the grid; omega is then set
+ * at the start of every step */
+void ora_set_reynolds_ramp(ora_grid *g, double re, double ramp_time, double dt, double dh);
+double ora_get_omega(const ora_grid *g);
+/* L_PRESSURE_DELTA (dimensionless) with GridObj::dh, dt, dm of the grid */
+void ora_set_pressure_delta(ora_grid *g, double p, double dh, double dt, double dm);
 void ora_macro_site(ora_grid *g, int i, int j, int k);  /* _LBM_macro_opt on one site */
 /* Run n steps of LBM_multi_opt on g with an optional OpenMP sweep (nthreads > 1). */
 void ora_run(ora_grid *g, int nsteps, int nthreads);

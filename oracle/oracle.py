@@ -51,6 +51,9 @@ def _load():
             "ora_reset_forces": ([_vp], None), "ora_run": ([_vp, C.c_int, C.c_int], None),
             "ora_set_gravity": ([_vp, C.c_double, C.c_int], None),
             "ora_set_ramp": ([_vp, C.c_double, C.c_double], None),
+            "ora_set_pressure_delta": ([_vp, C.c_double, C.c_double, C.c_double, C.c_double], None),
+            "ora_set_reynolds_ramp": ([_vp, C.c_double, C.c_double, C.c_double, C.c_double], None),
+            "ora_get_omega": ([_vp], C.c_double),
             "ora_equilibrium": ([_vp, C.c_int, C.c_int], C.c_double),
             "ora_lattice": ([C.c_int, _ip, _dp, _ip], None), "ora_cs": ([], C.c_double),
             "ora_ibm_create": ([_vp, C.c_double, _dp], _vp), "ora_ibm_destroy": ([_vp], None),
@@ -157,6 +160,16 @@ class OracleGrid:
 
     def set_ramp(self, ramp_time, dt):
         self.lib.ora_set_ramp(self.h, float(ramp_time), float(dt))
+
+    def set_reynolds_ramp(self, re, ramp_time, dt, dh):
+        self.lib.ora_set_reynolds_ramp(self.h, float(re), float(ramp_time), float(dt), float(dh))
+
+    @property
+    def omega(self):
+        return self.lib.ora_get_omega(self.h)
+
+    def set_pressure_delta(self, p, dh, dt, dm):
+        self.lib.ora_set_pressure_delta(self.h, float(p), float(dh), float(dt), float(dm))
 
     def set_f(self, f):
         a, p = _d(f)

@@ -326,6 +326,53 @@ def test_pressure_outlet_fp64_bitwise(dims, forced):
     assert_bitwise(_pressure_case(dims, forced), 120 if dims == 2 else 40)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32s"])
+def test_reynolds_ramp(prec):
+    # L_REYNOLDS_RAMP: omega follows the ramped Reynolds number every step, the same double on
+    # both sides; FP64 populations bit for bit, the shifted form within its storage rounding
+    c = setups.channel_2d(48, 24, 8.0, 0.04, 50.0)
+    dt, dh = c.info["dt"], c.info["dh"]
+    T = 30 * dt
+    print(f"[parity] Reynolds ramp {prec}", flush=True)
+    ref = oracle_for(c)
+    ref.set_reynolds_ramp(50.0, T, dt, dh)
+    m = active(c) & (c.lattyp != setups.EXTRAP_RIGHT)
+    with GpuGrid.from_case(c, prec, store_macro=True) as g:
+        g.set_reynolds_ramp(50.0, T)
+        for n in (13, 47):
+            ref.run(n)
+            g.run(n)
+            assert g.omega == ref.omega
+            rg, ug = g.download_macro()
+            rr, ur = ref.get_macro()
+            if prec == "fp64":
+                assert np.array_equal(g.download_f()[m], ref.get_f()[m])
+                assert np.array_equal(rg[m], rr[m]) and np.array_equal(ug[m], ur[m])
+            else:
+                assert np.abs(ug[m] - ur[m]).max() < 1e-6
+    assert ref.omega == c.omega
+
+
+def test_pressure_delta_fp64_bitwise():
+    # L_PRESSURE_DELTA: the outlet holds L_RHOIN + pd2dlbm(p) with the grid's dh, dt, dm
+    c = _pressure_case(2, False)
+    c.info.update(dh=0.05, dt=0.002, dm=0.25)
+    p = 1.25                       # 3 p dh dt^2 / dm = 3.0e-6
+    print("[parity] pressure delta", flush=True)
+    ref = oracle_for(c)
+    ref.set_pressure_delta(p, 0.05, 0.002, 0.25)
+    with GpuGrid.from_case(c, "fp64", store_macro=True) as g:
+        g.set_pressure_delta(p)
+        ref.run(90)
+        g.run(90)
+        m = active(c)
+        assert np.array_equal(g.download_f()[m], ref.get_f()[m])
+        rg, ug = g.download_macro()
+        rr, ur = ref.get_macro()
+        assert np.array_equal(rg[m], rr[m]) and np.array_equal(ug[m], ur[m])
+    assert np.abs(rg[-1] - (1.001 + 3.0e-6)).max() < 1e-14
+
+
 def test_pressure_outlet_shifted_forms_track_fp64():
     c = _pressure_case(2, False)
     out = {}

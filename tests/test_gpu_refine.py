@@ -25,13 +25,18 @@ def oracle_grid(case):
     return g
 
 
-def run_pair(parent, child, nsteps):
+def run_pair(parent, child, nsteps, re_ramp=None):
     print(f"[refine] {parent.name} + {child.name} steps={nsteps}", flush=True)
     op, oc = oracle_grid(parent), oracle_grid(child)
     op.add_child(oc)
     with GpuGrid.from_case(parent, "fp64", store_macro=True) as gp, \
             GpuGrid.from_case(child, "fp64", store_macro=True) as gc:
         gp.add_child(gc)
+        if re_ramp is not None:
+            re, T = re_ramp
+            for o, case in ((op, parent), (oc, child)):
+                o.set_reynolds_ramp(re, T, case.info["dt"], case.info["dh"])
+            gp.set_reynolds_ramp(re, T)
         for _ in range(nsteps):
             op.step()
             gp.LBM_multi_opt()
@@ -66,6 +71,14 @@ def test_two_level_cavity_interior_patch_fp64_bitwise():
     parent = setups.lid_driven_cavity(32, re=1280.0 * 32 / 256)
     child = setups.add_refined_region(parent, (8, 23, 12, 27), parent.info["dh"], parent.info["dt"])
     run_pair(parent, child, 60)
+
+
+def test_two_level_cavity_reynolds_ramp_fp64_bitwise():
+    # L_REYNOLDS_RAMP on both levels: each resets its omega from its own dt and dh at the start
+    # of each of its steps (the parent before its child's two sub-steps)
+    parent = setups.lid_driven_cavity(32, re=1280.0 * 32 / 256)
+    child = setups.add_refined_region(parent, (8, 23, 12, 27), parent.info["dh"], parent.info["dt"])
+    run_pair(parent, child, 40, re_ramp=(parent.info["re"], 25 * parent.info["dt"]))
 
 
 def test_refined_patch_touching_the_lid_is_refused():

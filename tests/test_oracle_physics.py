@@ -264,6 +264,64 @@ def test_pressure_outlet_sets_density_and_extrapolates_tangential_velocity():
     assert np.abs(u1[-1, :, 0, 0] - 0.03).max() < 5e-3
 
 
+def test_reynolds_ramp_sets_omega_from_the_ramped_viscosity():
+    # L_REYNOLDS_RAMP (GridObj_ops_lbm_optimised.cpp:39-42, :1313-1321; GridUtils.cpp:1825-1833):
+    # the step that starts at t uses Re(t) = Re (1 - cos(pi (t+1) dt / T)) up to T and Re after;
+    # in lattice units nu = dt / (Re dh^2) and, from the BGK Chapman-Enskog relation
+    # nu = cs^2 (tau - 1/2) with cs^2 = 1/3, tau = 3 nu + 1/2.  Checked against that relation in
+    # closed form at the quarter points of the ramp, where the coefficient is 1 - cos(pi/4) etc.
+    case = setups.channel_2d(24, 12, 8.0, 0.04, 50.0)
+    dt, dh = case.info["dt"], case.info["dh"]
+    re = 50.0
+    T = 8 * dt
+    g = OracleGrid(
+        case.dims, case.Q, case.N, case.M, case.K, omega=case.omega, regularised=True,
+        forced=case.forced, rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk)
+    g.set_labels(case.lattyp)
+    g.set_positions(case.xpos, case.ypos, case.zpos)
+    g.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+    g.init_feq(case.rho, case.u)
+    g.set_reynolds_ramp(re, T, dt, dh)
+    coef = {2: 1 - np.sqrt(0.5), 4: 1.0, 6: 1 + np.sqrt(0.5), 8: 2.0, 9: 1.0, 12: 1.0}
+    for n in range(1, 13):
+        g.step()
+        if n in coef:
+            tau = 3.0 * dt / (re * coef[n] * dh * dh) + 0.5
+            assert abs(1.0 / g.omega - tau) < 1e-13 * tau, (n, 1.0 / g.omega, tau)
+    # after the ramp the omega of L_RE itself, the value the grid started with
+    assert abs(g.omega - case.omega) < 1e-15
+    # the ramp changes the flow: early steps run at a far higher viscosity
+    g0 = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega, regularised=True,
+                    forced=case.forced, rho_in=case.rho_in, edges=case.edges, wall_thk=case.wall_thk)
+    g0.set_labels(case.lattyp)
+    g0.set_positions(case.xpos, case.ypos, case.zpos)
+    g0.set_inlet(case.ux_in, case.uy_in, case.uz_in)
+    g0.init_feq(case.rho, case.u)
+    g0.run(12)
+    assert np.abs(g0.get_f() - g.get_f()).max() > 1e-8
+
+
+def test_pressure_delta_shifts_the_outlet_density():
+    # L_PRESSURE_DELTA (GridObj_ops_lbm_optimised.cpp:343-345) through pd2dlbm (GridUnits.h:177-180):
+    # a dimensionless pressure p is p dh dt^2 / dm in lattice pressure units, and the lattice
+    # equation of state p = rho cs^2 with cs^2 = 1/3 turns it into a density step of
+    # 3 p dh dt^2 / dm on top of L_RHOIN at the outlet (only there: the inlet keeps its velocity)
+    case = setups.pressure_outlet_duct(12, 6, dims=2, U=0.03, tau=0.8, rho_out=1.0)
+    dh, dt, dm, p = 0.05, 0.002, 0.25, 1.25     # 3 p dh dt^2 / dm = 3.0e-6
+    g = _pressure_oracle(case)
+    g.set_pressure_delta(p, dh, dt, dm)
+    for _ in range(20):
+        g.step()
+    rho, u = g.get_macro()
+    assert np.abs(rho[-1] - 1.000003).max() < 1e-15
+    assert np.all(u[0, ..., 0] == 0.03)
+    # the delta is what moves the outlet: without it the outlet holds L_RHOIN
+    g0 = _pressure_oracle(case)
+    for _ in range(20):
+        g0.step()
+    assert np.abs(g0.get_macro()[0][-1] - 1.0).max() < 1e-15
+
+
 def test_velocity_ramp_scales_the_regularised_inlet():
     # L_VELOCITY_RAMP (GridUtils.cpp:1808-1816): the regularised inlet carries u_in * r((t+1) dt)
     # with r rising from 0 to 1 over T as half a cosine period, symmetric about T/2, and 1 after
