@@ -150,6 +150,15 @@ int lumagpu_set_gravity(lumagpu_grid *g, double gravity, int direction);
  * pulling site's row, as the reference writes it).  ramp_time = T in dimensionless time; 0 turns
  * it off.  Needs desc.dt.  lumagpu_run steps eagerly while a ramp is set. */
 int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time);
+/* Temporal blocking of LBM_multi_opt (GridObj_ops_lbm_optimised.cpp:36-193) for a periodic D3Q19
+ * box without labels, force, refinement or bodies: lumagpu_run and lumagpu_run_timed then
+ * advance such a grid two steps per kernel launch (even and odd AA step fused through LDS, a
+ * TX x TY tile of columns per block marching `zchunk` z planes; shape 0 = 64 x 8, 1 = 32 x 16,
+ * 2 = 128 x 4, N and M whole numbers of tiles), bit for bit the same populations and macro
+ * arrays as two single steps.  Allocates a second population array (the kernel reads one and
+ * writes the other); shape < 0 frees it and turns the two-step kernel off.  Grids that stop
+ * qualifying (labels or a force set later) step one step at a time. */
+int lumagpu_set_temporal_blocking(lumagpu_grid *g, int shape, int zchunk);
 /* L_REYNOLDS_RAMP (definitions.h:204): at the start of every step of this grid and of each grid
  * below it (LBM_multi_opt, GridObj_ops_lbm_optimised.cpp:39-42, before the children's
  * sub-steps), omega is reset by _LBM_updateReynolds (:1313-1321) to

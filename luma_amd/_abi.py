@@ -94,6 +94,7 @@ _PROTOS = {
     "lumagpu_set_velocity_ramp": [_vp, C.c_double],
     "lumagpu_set_pressure_delta": [_vp, C.c_double],
     "lumagpu_set_reynolds_ramp": [_vp, C.c_double, C.c_double],
+    "lumagpu_set_temporal_blocking": [_vp, C.c_int, C.c_int],
     "lumagpu_get_omega": [_vp, _dp],
     "lumagpu_tangent_seed_omega": [_vp, C.c_double],
     "lumagpu_ibm_body_force_tangent": [_vp, C.c_int, _dp, _dp],

@@ -246,6 +246,34 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
 
 namespace {
 
+// The two-step kernel applies to a grid that the unlabelled periodic kernel steps on its own,
+// with the same omega for both steps.  It fuses the local (even) step with the following
+// neighbour (odd) step, so it starts at parity 1 (the local step's input layout) and leaves the
+// grid there.
+bool fused_ready(const lumagpu_grid *g) {
+    return g->fuse_shape >= 0 && g->f2 && !g->masked && !g->zslab && !g->parent && !g->child &&
+           ibm_marker_count(g) == 0 && !(g->re_ramp > 0.0) && g->parity == 1 && g->has_state;
+}
+
+// two steps (local, then neighbour) in one launch from g->f into g->f2, then the arrays trade places
+int fused_pair(lumagpu_grid *g, hipEvent_t e0, hipEvent_t e1) {
+    const StepArgs a = make_args(g);
+    const bool store = g->store_macro != 0;
+    if (e0) HIP_TRY(hipEventRecord(e0, g->stream));
+    hipError_t e;
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: e = launch_fused2_f64(g->fuse_shape, a, g->f2, g->fuse_zc, store, g->stream); break;
+        case LUMAGPU_FP32S: e = launch_fused2_f32s(g->fuse_shape, a, g->f2, g->fuse_zc, store, g->stream); break;
+        case LUMAGPU_FP16S: e = launch_fused2_f16s(g->fuse_shape, a, g->f2, g->fuse_zc, store, g->stream); break;
+        default: return fail(LUMAGPU_EUNSUPPORTED, "no two-step kernel for this precision");
+    }
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("two-step launch: ") + hipGetErrorString(e));
+    if (e1) HIP_TRY(hipEventRecord(e1, g->stream));
+    std::swap(g->f, g->f2);
+    g->t += 2;
+    return 0;
+}
+
 // no host work inside a step anywhere in the hierarchy (flexible bodies download forces and
 // run the FEM on the host every step)
 bool graph_ok(const lumagpu_grid *g) {
@@ -253,6 +281,7 @@ bool graph_ok(const lumagpu_grid *g) {
         if (ibm_has_flexible(x)) return false;
         if (x->ramp_time > 0.0) return false;   // the ramp factors change every step
         if (x->re_ramp > 0.0) return false;     // so does omega under a Reynolds ramp
+        if (x->fuse_shape >= 0) return false;   // the two-step kernel alternates two arrays
     }
     return !g->parent;
 }
@@ -389,6 +418,7 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     if (!g) return 0;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     (void)hipFree(g->f);
+    (void)hipFree(g->f2);
     (void)hipFree(g->flags);
     (void)hipFree(g->force);
     (void)hipFree(g->uin);
@@ -813,6 +843,35 @@ int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time) {
     return 0;
 }
 
+int lumagpu_set_temporal_blocking(lumagpu_grid *g, int shape, int zchunk) {
+    if (int r = check_grid(g)) return r;
+    if (shape < 0) {
+        if (g->f2) {
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            HIP_TRY(hipFree(g->f2));
+        }
+        g->f2 = nullptr;
+        g->fuse_shape = -1;
+        return 0;
+    }
+    int tx, ty;
+    if (!fused_tile(shape, tx, ty)) return fail(LUMAGPU_EINVAL, "unknown tile shape");
+    if (g->Q != 19) return fail(LUMAGPU_EUNSUPPORTED, "the two-step kernel runs on D3Q19 grids");
+    const int p = g->d.precision;
+    if (p != LUMAGPU_FP64 && p != LUMAGPU_FP32S && p != LUMAGPU_FP16S)
+        return fail(LUMAGPU_EUNSUPPORTED, "the two-step kernel runs in the FP64, FP32- and FP16-shifted forms");
+    if (!fused_shape_fits(shape, g->d.N, g->d.M))
+        return fail(LUMAGPU_EINVAL, "N and M must be whole numbers of the tile's x and y extents");
+    if (zchunk < 1 || g->d.K < 3) return fail(LUMAGPU_EINVAL, "z chunk must be >= 1 and K >= 3");
+    if (!g->f2 && hipMalloc(&g->f2, g->ps * g->Q * g->elem) != hipSuccess) {
+        g->f2 = nullptr;
+        return fail(LUMAGPU_ENOMEM, "second population array");
+    }
+    g->fuse_shape = shape;
+    g->fuse_zc = zchunk;
+    return 0;
+}
+
 int lumagpu_set_reynolds_ramp(lumagpu_grid *g, double re, double ramp_time) {
     if (int r = check_grid(g)) return r;
     if (!(ramp_time >= 0.0)) return fail(LUMAGPU_EINVAL, "Reynolds ramp time must be >= 0");
@@ -871,6 +930,16 @@ int lumagpu_run(lumagpu_grid *g, int nsteps) {
     if (int r = check_grid(g)) return r;
     if (nsteps < 0) return fail(LUMAGPU_EINVAL, "negative step count");
     int s = 0;
+    if (g->fuse_shape >= 0) {
+        if (g->parity == 0 && nsteps >= 3) {
+            if (int r = apply_step(g, 0, nullptr, nullptr)) return r;
+            ++s;
+        }
+        while (nsteps - s >= 2 && fused_ready(g)) {
+            if (int r = fused_pair(g, nullptr, nullptr)) return r;
+            s += 2;
+        }
+    }
     // Long runs of a hierarchy without host work per step replay a captured pair of steps: the
     // step alternates between the two AA kernels, so two steps are the period (the child's
     // four sub-steps included).  One eager step first settles the IB support flags.
@@ -897,18 +966,30 @@ int lumagpu_run_timed(lumagpu_grid *g, int nsteps, double *ms_total, double *ms_
     std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     int rc = 0;
-    for (int s = 0; s < nsteps && !rc; ++s) rc = apply_step(g, 0, ev[2 * s], ev[2 * s + 1]);
+    // launches: event pairs in use, each covering one step or (two-step kernel) two
+    int nl = 0;
+    for (int s = 0; s < nsteps && !rc;) {
+        if (nsteps - s >= 2 && fused_ready(g)) {
+            rc = fused_pair(g, ev[2 * nl], ev[2 * nl + 1]);
+            s += 2;
+        } else {
+            rc = apply_step(g, 0, ev[2 * nl], ev[2 * nl + 1]);
+            s += 1;
+        }
+        ++nl;
+    }
     if (!rc) {
-        HIP_TRY(hipEventSynchronize(ev.back()));
+        HIP_TRY(hipEventSynchronize(ev[2 * nl - 1]));
         float tot = 0.f;
-        HIP_TRY(hipEventElapsedTime(&tot, ev.front(), ev.back()));
+        HIP_TRY(hipEventElapsedTime(&tot, ev.front(), ev[2 * nl - 1]));
         double sum = 0.0;
-        for (int s = 0; s < nsteps; ++s) {
+        for (int l = 0; l < nl; ++l) {
             float k = 0.f;
-            HIP_TRY(hipEventElapsedTime(&k, ev[2 * s], ev[2 * s + 1]));
+            HIP_TRY(hipEventElapsedTime(&k, ev[2 * l], ev[2 * l + 1]));
             sum += k;
         }
         if (ms_total) *ms_total = tot;
+        // per step: a two-step launch counts half its duration for each of its steps
         if (ms_kernel_mean) *ms_kernel_mean = sum / nsteps;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);

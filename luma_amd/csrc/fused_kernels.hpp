@@ -1,0 +1,202 @@
+// fused_kernels.hpp -- two AA steps per launch on a periodic D3Q19 box (temporal blocking).
+//
+// The AA pattern's even step is local (a cell reads and writes its own slots) and its odd step
+// reads and writes the neighbours' slots.  Fusing an even step with the following odd step lets
+// the even step's post-collision populations stay on chip: a block owns a TX x TY tile of
+// (x, y) columns plus a one-cell ring around it, marches up a chunk of z planes and, per plane,
+//
+//   1. runs the even step of plane p+1 on every cell of tile + ring (loads from `src`),
+//   2. publishes to LDS what the odd step of plane p reads: the c_z = -1 populations of plane
+//      p+1, the c_z = 0 populations of plane p and the c_z = +1 populations of plane p-1 (each
+//      thread keeps its own column's later-needed populations in registers until then),
+//   3. runs the odd step of plane p on the tile cells, gathering population v from the LDS copy
+//      of cell x - c_v, and pushes its post-collision populations to `dst` at x + c_v.
+//
+// The ring costs a second even step on (TX+2)(TY+2)/(TX TY) - 1 of the cells and their reads.
+// In return each cell is read once and written once per two steps instead of twice each, so
+// the HBM traffic per two steps falls from 4 Q s to about (1 + ring) Q s + Q s bytes per cell.
+// The odd step's pushes cross tile borders, so `dst` is a second population array: two
+// launches in a row go src -> dst -> src.  Both arrays hold the state in the layout the even
+// step reads (every cell's pre-collision populations in its own slots), which is the layout of
+// a grid after a neighbour step (odd parity); a run from even parity takes one single
+// neighbour step first.
+//
+// Every cell update is the same code as k_aa_periodic's (make_collider and the pair-wise
+// collision), and the even step's populations go through the storage type S on the way through
+// LDS, as they would through HBM, so the result is bit for bit that of an even launch followed
+// by an odd launch.
+#pragma once
+
+#include "aa_kernels.hpp"
+
+namespace lumagpu {
+
+template <int TX, int TY>
+struct FuseTile {
+    static constexpr int WX = TX + 2, WY = TY + 2, C = WX * WY;
+};
+
+// Linear block id -> tile id such that the blocks one XCD receives (ids congruent mod 8 under the
+// round-robin dispatch) take consecutive tiles: neighbouring tiles, whose ring rows overlap,
+// then run on the same XCD and share its L2.
+__device__ __forceinline__ int fused_tile_of(int b, int nblocks) {
+    if (nblocks % 8) return b;
+    return (b % 8) * (nblocks / 8) + b / 8;
+}
+
+template <class P, int TX, int TY, bool STORE>
+__global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, void *dst_v, int zc) {
+    using L = Lattice<19>;
+    using S = typename P::S;
+    using T = typename P::T;
+    constexpr int Q = 19;
+    constexpr int WX = FuseTile<TX, TY>::WX, C = FuseTile<TX, TY>::C;
+    __shared__ S sh[Q * C];
+
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int ntx = nx / TX, nty = ny / TY, ntz = (nz + zc - 1) / zc;
+    const int tile = fused_tile_of(blockIdx.x, ntx * nty * ntz);
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int t = threadIdx.x;
+    const int lx = t % WX, ly = t / WX;
+    int gx = tx * TX - 1 + lx, gy = ty * TY - 1 + ly;
+    gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+    gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+    const bool inner = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY;
+    const int z0 = tz * zc, z1 = min(z0 + zc, nz);
+
+    const S *__restrict__ src = static_cast<const S *>(a.f);
+    S *__restrict__ dst = static_cast<S *>(dst_v);
+    const size_t ps = a.ps;
+    const size_t col = (size_t)gx + (size_t)nx * (size_t)gy;
+    const size_t plane = (size_t)nx * (size_t)ny;
+
+    // even step of plane z (own cell): post-collision populations in storage precision
+    auto even = [&](int z, S *e) {
+        const size_t cell = col + plane * (size_t)z;
+        T fin[Q];
+        static_for<Q>([&](auto vc) {
+            constexpr int v = decltype(vc)::value;
+            fin[v] = ld<P, false>(src + (size_t)v * ps + cell);
+        });
+        typename P::M rho, u[3];
+        const auto c = make_collider<Q, P>(fin, nullptr, a, rho, u);
+        static_for<(Q - 1) / 2>([&](auto pc) {
+            constexpr int v = 2 * decltype(pc)::value;
+            T ov, oo;
+            c.template pair<v>(fin[v], fin[v + 1], ov, oo);
+            e[v] = static_cast<S>(ov);
+            e[v + 1] = static_cast<S>(oo);
+        });
+        e[Q - 1] = static_cast<S>(c(Q - 1, fin[Q - 1]));
+    };
+    auto wrapz = [&](int z) { return z < 0 ? z + nz : (z >= nz ? z - nz : z); };
+
+    // own column's populations awaiting publication: c_z = 0 of plane p, c_z = +1 of planes p
+    // and p - 1 (indexed by population; only the entries of the matching group are used)
+    S rmid[Q], rup[Q], rup_prev[Q];
+    {
+        S e[Q];
+        even(wrapz(z0 - 1), e);
+        static_for<Q>([&](auto vc) {
+            constexpr int v = decltype(vc)::value;
+            if constexpr (L::c(v, 2) == 1) rup_prev[v] = e[v];
+        });
+        even(z0, e);
+        static_for<Q>([&](auto vc) {
+            constexpr int v = decltype(vc)::value;
+            if constexpr (L::c(v, 2) == 1) rup[v] = e[v];
+            if constexpr (L::c(v, 2) == 0) rmid[v] = e[v];
+        });
+    }
+
+    for (int p = z0; p < z1; ++p) {
+        S e[Q];
+        even(wrapz(p + 1), e);
+        __syncthreads();   // the previous plane's odd step has read the LDS copy
+        static_for<Q>([&](auto vc) {
+            constexpr int v = decltype(vc)::value;
+            constexpr int cz = L::c(v, 2);
+            if constexpr (cz == -1) sh[v * C + t] = e[v];
+            if constexpr (cz == 0) {
+                sh[v * C + t] = rmid[v];
+                rmid[v] = e[v];
+            }
+            if constexpr (cz == 1) {
+                sh[v * C + t] = rup_prev[v];
+                rup_prev[v] = rup[v];
+                rup[v] = e[v];
+            }
+        });
+        __syncthreads();
+        if (inner) {
+            T fin[Q];
+            static_for<Q>([&](auto vc) {
+                constexpr int v = decltype(vc)::value;
+                constexpr int cx = L::c(v, 0), cy = L::c(v, 1);
+                fin[v] = static_cast<T>(sh[v * C + t - cy * WX - cx]);
+            });
+            typename P::M rho, u[3];
+            const auto c = make_collider<Q, P>(fin, nullptr, a, rho, u);
+            if constexpr (STORE) store_macro<3>(a, col + plane * (size_t)p, rho, u);
+            const int xm = gx == 0 ? nx - 1 : gx - 1, xp = gx == nx - 1 ? 0 : gx + 1;
+            const int ym = gy == 0 ? ny - 1 : gy - 1, yp = gy == ny - 1 ? 0 : gy + 1;
+            const int zm = p == 0 ? nz - 1 : p - 1, zp = p == nz - 1 ? 0 : p + 1;
+            auto put = [&](auto vc, T o) {
+                constexpr int v = decltype(vc)::value;
+                constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+                const int xx = cx == 0 ? gx : (cx > 0 ? xp : xm);
+                const int yy = cy == 0 ? gy : (cy > 0 ? yp : ym);
+                const int zz = cz == 0 ? p : (cz > 0 ? zp : zm);
+                st<P, cx == 0>(dst + (size_t)v * ps + cell_of(xx, yy, zz, nx, ny), o);
+            };
+            static_for<(Q - 1) / 2>([&](auto pc) {
+                constexpr int v = 2 * decltype(pc)::value;
+                T ov, oo;
+                c.template pair<v>(fin[v], fin[v + 1], ov, oo);
+                put(std::integral_constant<int, v>{}, ov);
+                put(std::integral_constant<int, v + 1>{}, oo);
+            });
+            put(std::integral_constant<int, Q - 1>{}, c(Q - 1, fin[Q - 1]));
+        }
+    }
+}
+
+// Tile shapes the launcher offers; the grid must be a whole number of tiles in x and y.
+enum FusedShape : int { kFuse64x8 = 0, kFuse32x16 = 1, kFuse128x4 = 2 };
+
+inline bool fused_tile(int shape, int &tx, int &ty) {
+    static const int t[3][2] = {{64, 8}, {32, 16}, {128, 4}};
+    if (shape < 0 || shape > 2) return false;
+    tx = t[shape][0];
+    ty = t[shape][1];
+    return true;
+}
+
+inline bool fused_shape_fits(int shape, int nx, int ny) {
+    int tx, ty;
+    if (!fused_tile(shape, tx, ty)) return false;
+    return nx % tx == 0 && ny % ty == 0 && nx >= tx && ny >= ty;
+}
+
+template <class P, int TX, int TY>
+hipError_t launch_fused2_t(const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
+    const int nblocks = (a.nx / TX) * (a.ny / TY) * ((a.nz + zc - 1) / zc);
+    const dim3 g(nblocks), b(FuseTile<TX, TY>::C);
+    if (store) k_aa_fused2<P, TX, TY, true><<<g, b, 0, s>>>(a, dst, zc);
+    else k_aa_fused2<P, TX, TY, false><<<g, b, 0, s>>>(a, dst, zc);
+    return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_fused2_p(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
+    if (!fused_shape_fits(shape, a.nx, a.ny) || zc < 1 || a.nz < 3) return hipErrorInvalidValue;
+    switch (shape) {
+        case kFuse64x8: return launch_fused2_t<P, 64, 8>(a, dst, zc, store, s);
+        case kFuse32x16: return launch_fused2_t<P, 32, 16>(a, dst, zc, store, s);
+        case kFuse128x4: return launch_fused2_t<P, 128, 4>(a, dst, zc, store, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lumagpu

@@ -20,6 +20,8 @@ struct lumagpu_grid {
     size_t ncell = 0, ps = 0, elem = 0;
     size_t melem = 8;                 // bytes per element of the macro, force and side arrays
     void *f = nullptr;
+    void *f2 = nullptr;               // second population array of the two-step (temporal-blocking) kernel
+    int fuse_shape = -1, fuse_zc = 0; // its tile shape (FusedShape, -1: off) and z planes per block
     uint8_t *flags = nullptr;
     double *force = nullptr;
     double *uin = nullptr;

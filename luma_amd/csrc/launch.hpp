@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "aa_kernels.hpp"
+#include "fused_kernels.hpp"
 
 namespace lumagpu {
 
@@ -23,6 +24,11 @@ hipError_t launch_step_f32s(const LaunchSpec &ls, const StepArgs &a, hipStream_t
 hipError_t launch_step_f16s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
 hipError_t launch_step_dual(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
 hipError_t launch_step_dual32(const LaunchSpec &ls, const StepArgs &a, hipStream_t s);
+
+// two AA steps per launch on a periodic D3Q19 box, src = a.f -> dst (fused_kernels.hpp)
+hipError_t launch_fused2_f64(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s);
+hipError_t launch_fused2_f32s(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s);
+hipError_t launch_fused2_f16s(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s);
 
 hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                           double *out, int parity, int masked, hipStream_t s);

@@ -9,6 +9,12 @@ hipError_t launch_step_f32s(const LaunchSpec &ls, const StepArgs &a, hipStream_t
 hipError_t launch_step_f16s(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
     return launch_step_p<P16S>(ls, a, s);
 }
+hipError_t launch_fused2_f32s(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
+    return launch_fused2_p<P32S>(shape, a, dst, zc, store, s);
+}
+hipError_t launch_fused2_f16s(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
+    return launch_fused2_p<P16S>(shape, a, dst, zc, store, s);
+}
 hipError_t launch_aux_f32s(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                            double *out, int parity, int masked, hipStream_t s) {
     return launch_aux_p<P32S>(op, Q, a, p0, p1, out, parity, masked, s);

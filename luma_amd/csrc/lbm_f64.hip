@@ -7,6 +7,9 @@ namespace lumagpu {
 hipError_t launch_step_f64(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
     return launch_step_p<P64>(ls, a, s);
 }
+hipError_t launch_fused2_f64(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
+    return launch_fused2_p<P64>(shape, a, dst, zc, store, s);
+}
 
 hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                           double *out, int parity, int masked, hipStream_t s) {

@@ -298,6 +298,10 @@ class GpuGrid:
         """L_VELOCITY_RAMP: ramp the inlet over ramp_time (dimensionless time); 0 turns it off."""
         call("lumagpu_set_velocity_ramp", self.h, float(ramp_time))
 
+    def set_temporal_blocking(self, shape=0, zchunk=16):
+        """Two steps per launch on a periodic D3Q19 box (shape < 0 turns it off)."""
+        call("lumagpu_set_temporal_blocking", self.h, int(shape), int(zchunk))
+
     def set_reynolds_ramp(self, re, ramp_time):
         """L_REYNOLDS_RAMP over L_RE = re on this grid and the grids below it (0 turns it off)."""
         call("lumagpu_set_reynolds_ramp", self.h, float(re), float(ramp_time))
