@@ -170,7 +170,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--slab-steps", type=int, default=200, help="0 skips the per-GPU slab measurement")
     ap.add_argument("--forms", default="fp64,fp32s,fp16s")
-    ap.add_argument("--two-step", default="0,128",
+    ap.add_argument("--two-step", default="10,128",
                     help="FP32-shifted two-steps-per-launch kernel: tile shape code, z planes per block "
                          "('' skips it)")
     args = ap.parse_args()

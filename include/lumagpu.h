@@ -154,8 +154,9 @@ int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time);
  * box without labels, force, refinement or bodies: lumagpu_run and lumagpu_run_timed then
  * advance such a grid two steps per kernel launch (even and odd AA step fused through LDS, a
  * TX x TY tile of columns per block marching `zchunk` z planes; shape 0 = 64 x 8, 1 = 32 x 16,
- * 2 = 128 x 4, N and M whole numbers of tiles), bit for bit the same populations and macro
- * arrays as two single steps.  Allocates a second population array (the kernel reads one and
+ * 2 = 128 x 4, plus 8 for two alternating LDS copies (shapes 8 and 10, FP32- and FP16-shifted
+ * only); N and M whole numbers of tiles), bit for bit the same populations and macro arrays as
+ * two single steps in FP64 and FP32-shifted storage, within one half ulp in FP16-shifted.  Allocates a second population array (the kernel reads one and
  * writes the other); shape < 0 frees it and turns the two-step kernel off.  Grids that stop
  * qualifying (labels or a force set later) step one step at a time. */
 int lumagpu_set_temporal_blocking(lumagpu_grid *g, int shape, int zchunk);

@@ -860,6 +860,8 @@ int lumagpu_set_temporal_blocking(lumagpu_grid *g, int shape, int zchunk) {
     const int p = g->d.precision;
     if (p != LUMAGPU_FP64 && p != LUMAGPU_FP32S && p != LUMAGPU_FP16S)
         return fail(LUMAGPU_EUNSUPPORTED, "the two-step kernel runs in the FP64, FP32- and FP16-shifted forms");
+    if ((shape & 8) && g->elem > 4)
+        return fail(LUMAGPU_EINVAL, "double-buffered tiles need storage of at most 4 bytes (FP32- or FP16-shifted)");
     if (!fused_shape_fits(shape, g->d.N, g->d.M))
         return fail(LUMAGPU_EINVAL, "N and M must be whole numbers of the tile's x and y extents");
     if (zchunk < 1 || g->d.K < 3) return fail(LUMAGPU_EINVAL, "z chunk must be >= 1 and K >= 3");

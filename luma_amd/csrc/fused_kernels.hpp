@@ -44,14 +44,16 @@ __device__ __forceinline__ int fused_tile_of(int b, int nblocks) {
     return (b % 8) * (nblocks / 8) + b / 8;
 }
 
-template <class P, int TX, int TY, bool STORE>
+template <class P, int TX, int TY, bool STORE, bool DB>
 __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, void *dst_v, int zc) {
     using L = Lattice<19>;
     using S = typename P::S;
     using T = typename P::T;
     constexpr int Q = 19;
     constexpr int WX = FuseTile<TX, TY>::WX, C = FuseTile<TX, TY>::C;
-    __shared__ S sh[Q * C];
+    // DB: two LDS copies used alternately, so a plane needs one barrier (between publishing and
+    // reading) instead of two
+    __shared__ S shm[(DB ? 2 : 1) * Q * C];
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int ntx = nx / TX, nty = ny / TY, ntz = (nz + zc - 1) / zc;
@@ -113,7 +115,8 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
     for (int p = z0; p < z1; ++p) {
         S e[Q];
         even(wrapz(p + 1), e);
-        __syncthreads();   // the previous plane's odd step has read the LDS copy
+        S *sh = DB ? shm + (p & 1) * (Q * C) : shm;
+        if constexpr (!DB) __syncthreads();   // the previous plane's odd step has read the LDS copy
         static_for<Q>([&](auto vc) {
             constexpr int v = decltype(vc)::value;
             constexpr int cz = L::c(v, 2);
@@ -167,9 +170,10 @@ enum FusedShape : int { kFuse64x8 = 0, kFuse32x16 = 1, kFuse128x4 = 2 };
 
 inline bool fused_tile(int shape, int &tx, int &ty) {
     static const int t[3][2] = {{64, 8}, {32, 16}, {128, 4}};
-    if (shape < 0 || shape > 2) return false;
-    tx = t[shape][0];
-    ty = t[shape][1];
+    const int k = shape & 7;
+    if (shape < 0 || shape > 15 || k > 2 || shape == 8 + kFuse32x16) return false;
+    tx = t[k][0];
+    ty = t[k][1];
     return true;
 }
 
@@ -179,12 +183,12 @@ inline bool fused_shape_fits(int shape, int nx, int ny) {
     return nx % tx == 0 && ny % ty == 0 && nx >= tx && ny >= ty;
 }
 
-template <class P, int TX, int TY>
+template <class P, int TX, int TY, bool DB>
 hipError_t launch_fused2_t(const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
     const int nblocks = (a.nx / TX) * (a.ny / TY) * ((a.nz + zc - 1) / zc);
     const dim3 g(nblocks), b(FuseTile<TX, TY>::C);
-    if (store) k_aa_fused2<P, TX, TY, true><<<g, b, 0, s>>>(a, dst, zc);
-    else k_aa_fused2<P, TX, TY, false><<<g, b, 0, s>>>(a, dst, zc);
+    if (store) k_aa_fused2<P, TX, TY, true, DB><<<g, b, 0, s>>>(a, dst, zc);
+    else k_aa_fused2<P, TX, TY, false, DB><<<g, b, 0, s>>>(a, dst, zc);
     return hipGetLastError();
 }
 
@@ -192,11 +196,17 @@ template <class P>
 hipError_t launch_fused2_p(int shape, const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
     if (!fused_shape_fits(shape, a.nx, a.ny) || zc < 1 || a.nz < 3) return hipErrorInvalidValue;
     switch (shape) {
-        case kFuse64x8: return launch_fused2_t<P, 64, 8>(a, dst, zc, store, s);
-        case kFuse32x16: return launch_fused2_t<P, 32, 16>(a, dst, zc, store, s);
-        case kFuse128x4: return launch_fused2_t<P, 128, 4>(a, dst, zc, store, s);
-        default: return hipErrorInvalidValue;
+        case kFuse64x8: return launch_fused2_t<P, 64, 8, false>(a, dst, zc, store, s);
+        case kFuse32x16: return launch_fused2_t<P, 32, 16, false>(a, dst, zc, store, s);
+        case kFuse128x4: return launch_fused2_t<P, 128, 4, false>(a, dst, zc, store, s);
+        default: break;
     }
+    // two LDS copies fit the 160 KiB of a CU for storage types up to 4 bytes
+    if constexpr (sizeof(typename P::S) <= 4) {
+        if (shape == 8 + kFuse64x8) return launch_fused2_t<P, 64, 8, true>(a, dst, zc, store, s);
+        if (shape == 8 + kFuse128x4) return launch_fused2_t<P, 128, 4, true>(a, dst, zc, store, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace lumagpu

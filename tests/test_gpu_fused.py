@@ -27,11 +27,15 @@ def _run(case, prec, nsteps, fuse=None, parity_first=False):
         return g.download_f(), g.download_macro()
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32s", "fp16s"])
-@pytest.mark.parametrize("shape,zc", [(0, 7), (1, 5), (2, 64), (0, 128)])
+CASES = [(p, s, z) for p in ("fp64", "fp32s", "fp16s") for s, z in ((0, 7), (1, 5), (2, 64), (0, 128))]
+CASES += [(p, s, z) for p in ("fp32s", "fp16s") for s, z in ((8, 6), (10, 128), (10, 3))]
+
+
+@pytest.mark.parametrize("prec,shape,zc", CASES)
 def test_two_step_kernel_matches_single_steps(prec, shape, zc):
     # 128 x 32 x 20: two or four tiles in x, two to eight in y, z chunks that do not divide the
-    # depth (zc = 7, 5) and one chunk over the whole depth (zc = 64, 128)
+    # depth (zc = 7, 5, 6, 3) and one chunk over the whole depth (zc = 64, 128); shapes 8 and 10
+    # alternate two LDS copies
     case = _tgv((128, 32, 20))
     print(f"[fused] {prec} shape={shape} zc={zc}", flush=True)
     f1, (r1, u1) = _run(case, prec, 6)
@@ -53,7 +57,7 @@ def test_two_step_kernel_odd_counts_and_parity():
     print("[fused] odd counts", flush=True)
     for parity_first in (False, True):
         f1, (r1, u1) = _run(case, "fp32s", 7, parity_first=parity_first)
-        f2, (r2, u2) = _run(case, "fp32s", 7, fuse=(0, 4), parity_first=parity_first)
+        f2, (r2, u2) = _run(case, "fp32s", 7, fuse=(8, 4), parity_first=parity_first)
         assert np.array_equal(f1, f2) and np.array_equal(u1, u2)
 
 
@@ -76,7 +80,7 @@ def test_two_step_kernel_run_timed_and_switch_off():
     print("[fused] timed, off", flush=True)
     f1, _ = _run(case, "fp32s", 8)
     with GpuGrid.from_case(case, "fp32s") as g:
-        g.set_temporal_blocking(0, 6)
+        g.set_temporal_blocking(8, 6)
         tot, ker = g.run_timed(4)
         assert tot > 0 and ker > 0
         g.set_temporal_blocking(-1, 0)
@@ -88,6 +92,9 @@ def test_two_step_kernel_refuses_unsuitable_grids():
     with GpuGrid.from_case(_tgv((64, 12, 8)), "fp32s") as g:
         with pytest.raises(LumaGpuError, match="whole numbers"):
             g.set_temporal_blocking(0, 4)     # M = 12 is not a whole number of 8-row tiles
+    with GpuGrid.from_case(_tgv((64, 16, 8)), "fp64") as g:
+        with pytest.raises(LumaGpuError, match="at most 4 bytes"):
+            g.set_temporal_blocking(8, 4)     # two FP64 LDS copies exceed a CU's LDS
     c2 = setups.taylor_green((32, 32, 1), u0=0.05, tau=0.56, dims=2)
     with GpuGrid.from_case(c2, "fp32s") as g:
         with pytest.raises(LumaGpuError, match="D3Q19"):
