@@ -2,7 +2,7 @@
 
 Each builder returns the LUMA-layout set-up (labels, inlet, initial state, refined region,
 markers) as LUMA's initialisation would produce it; each runner steps it with
-``LBM_multi_opt`` and reports what the configuration is judged on (fields, drag, lift,
+``LBM_multi_opt`` and reports the quantities each configuration is measured by (fields, drag, lift,
 Strouhal number, throughput).  Command line::
 
     python -m luma_amd.cases c1 --precision fp64 --steps 20000
