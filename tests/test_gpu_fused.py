@@ -43,7 +43,8 @@ def test_two_step_kernel_matches_single_steps(prec, shape, zc):
     if prec == "fp16s":
         # the compiler may round a collision's last FMA straight to half in one kernel and
         # through float in the other: equal to within the half storage rounding (the shifted
-        # values are below 2^-4, where one half ulp is at most 2^-15 = 3.05e-5)
+        # values are below 2^-4, where one unit in the last place of a half is at most
+        # 2^-15 = 3.05e-5)
         assert np.abs(f1 - f2).max() <= 3.05e-5
         assert np.abs(u1 - u2).max() < 1e-5
         return
