@@ -170,7 +170,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--slab-steps", type=int, default=200, help="0 skips the per-GPU slab measurement")
     ap.add_argument("--forms", default="fp64,fp32s,fp16s")
-    ap.add_argument("--two-step", default="10,128",
+    ap.add_argument("--two-step", default="8,128",
                     help="FP32-shifted two-steps-per-launch kernel: tile shape code, z planes per block "
                          "('' skips it)")
     args = ap.parse_args()
@@ -230,6 +230,8 @@ def main():
     # times the steps one launch advances) over its mean duration
     steps_per_launch = 2 if "two_step" in r else 1
     launch_ms = r["kernel_ms_mean"] * steps_per_launch
+    # HBM bytes actually moved (rocprofv3 counters) per launch over the live launch duration
+    traffic_frac = None if traffic is None else traffic / (launch_ms * 1e-3) / (PEAK_GBS * 1e9)
     line = {
         "metric": METRIC,
         "value": round(r["mlups"], 1),
@@ -251,6 +253,7 @@ def main():
                    "bytes_per_cell_step": r["bytes_per_cell_step"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["roofline_frac"], 4), "traffic": traffic,
+                     "traffic_frac": None if traffic_frac is None else round(traffic_frac, 4),
                      "steps_per_launch": steps_per_launch, "launch_ms": round(launch_ms, 5)},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}

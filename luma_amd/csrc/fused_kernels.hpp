@@ -60,11 +60,28 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
     const int tile = fused_tile_of(blockIdx.x, ntx * nty * ntz);
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int t = threadIdx.x;
-    const int lx = t % WX, ly = t / WX;
+    // threads 0 .. TX*TY-1 take the tile row by row (whole waves, rows aligned to the tile's
+    // first x), the rest the ring: the row below, the row above, then the two side columns
+    int lx, ly;
+    if (t < TX * TY) {
+        lx = 1 + t % TX;
+        ly = 1 + t / TX;
+    } else if (t < TX * TY + WX) {
+        lx = t - TX * TY;
+        ly = 0;
+    } else if (t < TX * TY + 2 * WX) {
+        lx = t - TX * TY - WX;
+        ly = TY + 1;
+    } else {
+        const int k = t - TX * TY - 2 * WX;
+        lx = (k & 1) ? WX - 1 : 0;
+        ly = 1 + (k >> 1);
+    }
+    const int lc = ly * WX + lx;   // the cell's slot in the LDS copy
     int gx = tx * TX - 1 + lx, gy = ty * TY - 1 + ly;
     gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
     gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
-    const bool inner = lx >= 1 && lx <= TX && ly >= 1 && ly <= TY;
+    const bool inner = t < TX * TY;
     const int z0 = tz * zc, z1 = min(z0 + zc, nz);
 
     const S *__restrict__ src = static_cast<const S *>(a.f);
@@ -120,13 +137,13 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
         static_for<Q>([&](auto vc) {
             constexpr int v = decltype(vc)::value;
             constexpr int cz = L::c(v, 2);
-            if constexpr (cz == -1) sh[v * C + t] = e[v];
+            if constexpr (cz == -1) sh[v * C + lc] = e[v];
             if constexpr (cz == 0) {
-                sh[v * C + t] = rmid[v];
+                sh[v * C + lc] = rmid[v];
                 rmid[v] = e[v];
             }
             if constexpr (cz == 1) {
-                sh[v * C + t] = rup_prev[v];
+                sh[v * C + lc] = rup_prev[v];
                 rup_prev[v] = rup[v];
                 rup[v] = e[v];
             }
@@ -137,7 +154,7 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
             static_for<Q>([&](auto vc) {
                 constexpr int v = decltype(vc)::value;
                 constexpr int cx = L::c(v, 0), cy = L::c(v, 1);
-                fin[v] = static_cast<T>(sh[v * C + t - cy * WX - cx]);
+                fin[v] = static_cast<T>(sh[v * C + lc - cy * WX - cx]);
             });
             typename P::M rho, u[3];
             const auto c = make_collider<Q, P>(fin, nullptr, a, rho, u);
