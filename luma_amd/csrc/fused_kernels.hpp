@@ -188,7 +188,7 @@ enum FusedShape : int { kFuse64x8 = 0, kFuse32x16 = 1, kFuse128x4 = 2 };
 inline bool fused_tile(int shape, int &tx, int &ty) {
     static const int t[3][2] = {{64, 8}, {32, 16}, {128, 4}};
     const int k = shape & 7;
-    if (shape < 0 || shape > 15 || k > 2 || shape == 8 + kFuse32x16) return false;
+    if (shape < 0 || shape > 15 || k > 2 || (shape > 2 && shape < 8)) return false;
     tx = t[k][0];
     ty = t[k][1];
     return true;
@@ -221,6 +221,7 @@ hipError_t launch_fused2_p(int shape, const StepArgs &a, void *dst, int zc, bool
     // two LDS copies fit the 160 KiB of a CU for storage types up to 4 bytes
     if constexpr (sizeof(typename P::S) <= 4) {
         if (shape == 8 + kFuse64x8) return launch_fused2_t<P, 64, 8, true>(a, dst, zc, store, s);
+        if (shape == 8 + kFuse32x16) return launch_fused2_t<P, 32, 16, true>(a, dst, zc, store, s);
         if (shape == 8 + kFuse128x4) return launch_fused2_t<P, 128, 4, true>(a, dst, zc, store, s);
     }
     return hipErrorInvalidValue;

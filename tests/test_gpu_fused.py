@@ -28,7 +28,7 @@ def _run(case, prec, nsteps, fuse=None, parity_first=False):
 
 
 CASES = [(p, s, z) for p in ("fp64", "fp32s", "fp16s") for s, z in ((0, 7), (1, 5), (2, 64), (0, 128))]
-CASES += [(p, s, z) for p in ("fp32s", "fp16s") for s, z in ((8, 6), (10, 128), (10, 3))]
+CASES += [(p, s, z) for p in ("fp32s", "fp16s") for s, z in ((8, 6), (9, 7), (10, 128), (10, 3))]
 
 
 @pytest.mark.parametrize("prec,shape,zc", CASES)
