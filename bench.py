@@ -155,7 +155,15 @@ def cpu_baseline(case, target_s=15.0):
     g.run(n, threads)
     dt = time.perf_counter() - t0
     g.close()
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": case.ncell * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model,
+                     "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")},
             "sample": f"D3Q19 BGK TGV {case.N}^3 FP64, {n} steps after 1 warm-up: oracle restating "
                       f"LUMA's LBM_multi_opt (AoS f/fNew pull, two sweeps), OpenMP over x"}
 
