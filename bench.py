@@ -106,8 +106,10 @@ def slab_share(n, nz, steps, warmup, form="fp32s"):
     return out
 
 
-def error_run(case, nsteps):
-    """Run every form nsteps from the same state; u error and KE of shifted forms vs FP64."""
+def error_run(case, nsteps, two_step=None):
+    """Run every form nsteps from the same state; u error and KE of shifted forms vs FP64.
+    two_step = (shape, z chunk): the FP32-shifted run uses the two-steps-per-launch kernel, the
+    path whose throughput is reported."""
     import numpy as np
 
     from luma_amd.grid import GpuGrid
@@ -124,6 +126,8 @@ def error_run(case, nsteps):
     norm = np.linalg.norm(u64)
     for form in ("fp32s", "fp16s"):
         with GpuGrid.from_case(case, form) as g:
+            if form == "fp32s" and two_step is not None:
+                g.set_temporal_blocking(*two_step)
             g.run(nsteps - 1)
             g.set_store_macro(True)
             g.run(1)
@@ -212,7 +216,13 @@ def main():
     err = None
     if args.error_steps > 0 and "fp64" in forms:
         log(f"[bench] error run: {args.error_steps} steps per form")
-        err = error_run(case, args.error_steps)
+        two = None
+        if "fp32s_2step" in res:
+            ts = res["fp32s_2step"]["two_step"]
+            two = (ts["shape"], ts["z_chunk"])
+        err = error_run(case, args.error_steps, two_step=two)
+        if two is not None:
+            err["fp32s"]["kernel"] = "two steps per launch"
         log(f"[bench] errors: {json.dumps(err)}")
     slab = None
     if args.slab_steps > 0:
