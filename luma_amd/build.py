@@ -5,6 +5,7 @@ The library is written to luma_amd/lib/libluma_amd.so; object files to luma_amd/
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -53,14 +54,22 @@ def build_library(verbose: bool = False) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
     LIB.parent.mkdir(parents=True, exist_ok=True)
     cc = hipcc()
-    objs = []
+    objs, cmds = [], []
     for src, extra in UNITS:
         obj = OBJ / (Path(src).stem + ".o")
-        cmd = [cc, "-c", str(CSRC / src), "-o", str(obj), *COMMON, *extra]
+        cmds.append([cc, "-c", str(CSRC / src), "-o", str(obj), *COMMON, *extra])
+        objs.append(str(obj))
+    # the translation units are independent: compile them side by side (MAX_JOBS caps it)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1)))
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        for fut in [pool.submit(compile_one, c) for c in cmds]:
+            fut.result()
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs]
     if verbose:
