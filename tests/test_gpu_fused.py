@@ -100,3 +100,25 @@ def test_two_step_kernel_refuses_unsuitable_grids():
     with GpuGrid.from_case(c2, "fp32s") as g:
         with pytest.raises(LumaGpuError, match="D3Q19"):
             g.set_temporal_blocking(1, 4)
+
+
+def test_two_step_kernel_full_size_256():
+    # the metric box, 256^3 FP32-shifted: the two-step kernel's state after six steps equals six
+    # single steps exactly (FP64 moments of the populations and the last step's macro arrays),
+    # and mass is conserved to the shifted storage's rounding
+    case = _tgv(256)
+    print("[fused] 256^3 fp32s", flush=True)
+    out = []
+    for fuse in (None, (8, 128)):
+        with GpuGrid.from_case(case, "fp32s", store_macro=True) as g:
+            m0 = g.moments()
+            if fuse is not None:
+                g.set_temporal_blocking(*fuse)
+            g.run(6)
+            m = g.moments()
+            r, u = g.download_macro()
+        out.append((m, r, u))
+        assert abs(m["mass"] - m0["mass"]) < 1e-6 * m0["mass"]
+    (ma, ra, ua), (mb, rb, ub) = out
+    assert ma == mb
+    assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
