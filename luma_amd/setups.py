@@ -404,9 +404,10 @@ def filament_markers(start, length, angle_deg, dh):
 
 
 def channel_2d(N: int, M: int, resolution: float, umax_lbm: float, re: float, regularised: bool = True,
-               outlet: bool = True, no_flow: bool = False) -> Case:
+               outlet=True, no_flow: bool = False) -> Case:
     """D2Q9 channel as LUMA sets it up: parabolic inlet (eVelocity, L_PARABOLIC_INLET) on the left,
-    eExtrapolateRight (or periodic) on the right, eSolid top and bottom, one-cell walls.
+    eExtrapolateRight on the right (outlet=True; False: periodic in x; "pressure": an ePressure
+    outlet at L_RHOIN, LUMA's default L_WALL_RIGHT), eSolid top and bottom, one-cell walls.
 
     resolution = L_RESOLUTION (cells per unit length); umax_lbm = 1.5 * L_UX0 in lattice units;
     re = L_RE based on L_UX0 (the mean velocity) and unit length."""
@@ -419,7 +420,9 @@ def channel_2d(N: int, M: int, resolution: float, umax_lbm: float, re: float, re
     ypos = linspace(dh / 2.0, edges[1] - dh / 2.0, M)
     zpos = np.array([0.0])
     lt = label_walls(N, M, 1, 2, xpos, ypos, zpos, edges, thk,
-                     dict(left=VELOCITY, right=EXTRAP_RIGHT if outlet else FLUID, bottom=SOLID, top=SOLID))
+                     dict(left=VELOCITY,
+                          right=PRESSURE if outlet == "pressure" else (EXTRAP_RIGHT if outlet else FLUID),
+                          bottom=SOLID, top=SOLID))
 
     def inlet_fn(y, dt_, dh_):
         # _LBM_initSetInletProfile, L_PARABOLIC_INLET (GridObj_init_grids.cpp:1334-1347)

@@ -85,6 +85,33 @@ def test_ib_cylinder_channel_fp64_bitwise():
         assert np.allclose(gg.ibm_body_force(0), ref, rtol=1e-13, atol=1e-16)
 
 
+def test_luma_default_arrangement_fp64_bitwise():
+    # LUMA's default definitions.h arrangement in 2D: regularised parabolic velocity inlet ramped
+    # in (L_VELOCITY_RAMP), regularised pressure outlet (L_WALL_RIGHT ePressure), solid top and
+    # bottom, a rigid IB cylinder (input/geometry.config's CIRCLE_SPHERE IBM ... RIGID)
+    c = setups.channel_2d(90, 30, 10.0, 0.04, 100.0, outlet="pressure")
+    assert (c.lattyp == setups.PRESSURE).sum() == c.M - 2
+    dt = c.info["dt"]
+    pos = setups.circle_markers((2.05, 1.53, 0.0), 0.5, c.info["dh"])
+    print(f"[ibm] default arrangement {c.N}x{c.M}, {len(pos)} markers", flush=True)
+    og = oracle_grid(c)
+    og.set_ramp(40 * dt, dt)
+    ib = OracleIBM(og, c.info["dh"])
+    ib.add_body(pos)
+    ib.initialise()
+    with gpu_grid(c) as gg:
+        gg.set_velocity_ramp(40 * dt)
+        gg.ibm_add_body(pos)
+        gg.ibm_initialise()
+        for _ in range(120):
+            og.step()
+            gg.LBM_multi_opt()
+        compare_markers(gg, ib)
+        compare_grids(c, gg, og, "default arrangement")
+        r, _ = gg.download_macro()
+        assert np.abs(r[-1, 1:-1] - 1.0).max() < 1e-15
+
+
 def test_ib_sphere_3d_fp64_bitwise():
     c = periodic_box(20, 16, 16, 0.03, 0.6)
     pos = setups.sphere_markers((8.3, 8.1, 7.9), 3.0, 1.0)
