@@ -153,19 +153,27 @@ def test_ib_moving_markers_fp64_bitwise():
         compare_grids(c, gg, og, "rotating")
 
 
-def test_ib_cylinder_on_refined_level_fp64_bitwise():
+@pytest.mark.parametrize("re_ramp", [False, True])
+def test_ib_cylinder_on_refined_level_fp64_bitwise(re_ramp):
+    # re_ramp: L_REYNOLDS_RAMP over 25 coarse steps on both levels as well
     parent = setups.channel_2d(64, 32, 8.0, 0.04, 80.0)
     child = setups.add_refined_region(parent, (8, 31, 6, 25), parent.info["dh"], parent.info["dt"])
     cdh = child.info["dh"]
     pos = setups.circle_markers((2.5 + 0.01, 2.0 + 0.13, 0.0), 0.5, cdh)
-    print(f"[ibm] refined cylinder, child {child.N}x{child.M}, {len(pos)} markers", flush=True)
+    print(f"[ibm] refined cylinder, child {child.N}x{child.M}, {len(pos)} markers, re_ramp={re_ramp}", flush=True)
     op, oc = oracle_grid(parent), oracle_grid(child)
     op.add_child(oc)
     ib = OracleIBM(oc, cdh, child.info["origin"])
     ib.add_body(pos)
     ib.initialise()
+    if re_ramp:
+        T = 25 * parent.info["dt"]
+        for o, case in ((op, parent), (oc, child)):
+            o.set_reynolds_ramp(80.0, T, case.info["dt"], case.info["dh"])
     with gpu_grid(parent) as gp, gpu_grid(child) as gc:
         gp.add_child(gc)
+        if re_ramp:
+            gp.set_reynolds_ramp(80.0, T)
         gc.ibm_add_body(pos)
         gc.ibm_initialise()
         for _ in range(40):
