@@ -23,8 +23,10 @@
 //
 // Every cell update is the same code as k_aa_periodic's (make_collider and the pair-wise
 // collision), and the even step's populations go through the storage type S on the way through
-// LDS, as they would through HBM, so the result is bit for bit that of an even launch followed
-// by an odd launch.
+// LDS, as they would through HBM, so in FP64 and FP32-shifted storage the result is bit for bit
+// that of an even launch followed by an odd launch.  (In FP16 storage the compiler may round a
+// collision's last FMA straight to half in one kernel and through float in the other: the two
+// then differ by one unit in the last place of the stored half.)
 #pragma once
 
 #include "aa_kernels.hpp"
