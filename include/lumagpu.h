@@ -192,6 +192,12 @@ int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]);
  * :33-54) runs inside lumagpu_step between the stream/macro and force/collide halves.  The
  * grid must be created with desc.forced = 1 (L_IBM_ON).  Positions use the units of the
  * grid's XPos/YPos/ZPos (lumagpu_set_labels / lumagpu_set_positions) and desc.dh. */
+/* L_UNIVERSAL_EPSILON_CALC (definitions.h:220): ibm_findEpsilon solves one RKPM system over
+ * every marker of the grid's bodies, in body order (ibm_universalEpsilonGather /
+ * ibm_universalEpsilonScatter, ObjectManager_ops_ibm.cpp:775-786, :905-909, :1067-1141), for
+ * bodies whose supports overlap.  Set before lumagpu_ibm_initialise; 0 = per body (default).
+ * The adjoint refuses a grid in this mode. */
+int lumagpu_ibm_set_universal_epsilon(lumagpu_grid *g, int on);
 /* Body<IBMarker>::addMarker (Body.h:144-255) + IBMarker ctor (IBMarker.cpp:56-86):
  * a body of n markers at pos[3*m + a]; epsilon 0, ds 1, markerVel 0 until initialised. */
 int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int flexible, int *body);

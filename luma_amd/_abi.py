@@ -93,6 +93,7 @@ _PROTOS = {
     "lumagpu_set_gravity": [_vp, C.c_double, C.c_int],
     "lumagpu_set_velocity_ramp": [_vp, C.c_double],
     "lumagpu_set_pressure_delta": [_vp, C.c_double],
+    "lumagpu_ibm_set_universal_epsilon": [_vp, C.c_int],
     "lumagpu_set_reynolds_ramp": [_vp, C.c_double, C.c_double],
     "lumagpu_set_temporal_blocking": [_vp, C.c_int, C.c_int],
     "lumagpu_get_omega": [_vp, _dp],

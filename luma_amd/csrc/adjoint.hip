@@ -70,6 +70,8 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     if (g->gravity_on) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover the gravity reset");
     if (g->ramp_time > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a velocity ramp");
     if (g->re_ramp > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a Reynolds ramp");
+    if (g->ibm && g->ibm->universal)
+        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint chains epsilon body by body, not the universal system");
     for (size_t c = 0; c < g->hflags.size(); ++c) {
         const int ty = g->hflags[c] & kTypeMask;
         if (ty == kFluid || ty == kSolid || ty == kExtrapRight) continue;

@@ -316,6 +316,31 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
     return 0;
 }
 
+// ibm_findEpsilon over the grid's bodies: per body, or with L_UNIVERSAL_EPSILON_CALC one system
+// over every marker of the level in body order (ibm_universalEpsilonGather, :1067-1107), its
+// epsilon (and on a tangent-form grid its derivative) scattered back in the same order
+// (ibm_universalEpsilonScatter, :1113-1141)
+int find_epsilon_all(const lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    if (!s->universal) {
+        for (auto &b : s->bodies)
+            if (int r = find_epsilon(g, b)) return r;
+        return 0;
+    }
+    Body all;
+    for (const auto &b : s->bodies) all.m.insert(all.m.end(), b.m.begin(), b.m.end());
+    if (all.m.empty()) return 0;
+    if (int r = find_epsilon(g, all)) return r;
+    size_t k = 0;
+    for (auto &b : s->bodies)
+        for (auto &m : b.m) {
+            m.eps = all.m[k].eps;
+            m.deps = all.m[k].deps;
+            ++k;
+        }
+    return 0;
+}
+
 // Pack markers and supports for the device; build the per-site gather lists in LUMA's spread
 // order; clear sites that are no longer supports.
 int upload(lumagpu_grid *g) {
@@ -537,8 +562,7 @@ int move_bodies_tangent(lumagpu_grid *g) {
             support_tangents(g, s->bodies[i]);
         }
     for (auto &b : s->bodies) compute_ds(g, b);
-    for (auto &b : s->bodies)
-        if (int r = find_epsilon(g, b)) return r;
+    if (int r = find_epsilon_all(g)) return r;
     return upload(g);
 }
 
@@ -576,8 +600,7 @@ int move_bodies(lumagpu_grid *g) {
         if (s->bodies[i].flexible)
             if (int r = find_support(g, s->bodies[i], (int)i)) return r;
     for (auto &b : s->bodies) compute_ds(g, b);
-    for (auto &b : s->bodies)
-        if (int r = find_epsilon(g, b)) return r;
+    if (int r = find_epsilon_all(g)) return r;
     return upload(g);
 }
 
@@ -707,6 +730,14 @@ int lumagpu_set_positions(lumagpu_grid *g, const double *x, const double *y, con
     return 0;
 }
 
+int lumagpu_ibm_set_universal_epsilon(lumagpu_grid *g, int on) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    IbmSystem *s = system_of(g);
+    if (s->initialised) return fail(LUMAGPU_ESTATE, "set the epsilon mode before lumagpu_ibm_initialise");
+    s->universal = on ? 1 : 0;
+    return 0;
+}
+
 int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int flexible, int *body) {
     if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
     if (!g->d.forced) return fail(LUMAGPU_ESTATE, "IB bodies need a grid created with desc.forced = 1");
@@ -808,8 +839,7 @@ int lumagpu_ibm_initialise(lumagpu_grid *g) {
         if (tangent_form(g)) support_tangents(g, s->bodies[i]);
     }
     for (auto &b : s->bodies) compute_ds(g, b);
-    for (auto &b : s->bodies)
-        if (int r = find_epsilon(g, b)) return r;
+    if (int r = find_epsilon_all(g)) return r;
     if (int r = upload(g)) return r;
     HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));

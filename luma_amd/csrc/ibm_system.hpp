@@ -64,6 +64,7 @@ struct IbmSystem {
     int nm = 0;
     int S = 9;
     int initialised = 0;
+    int universal = 0;          // L_UNIVERSAL_EPSILON_CALC: one epsilon system over all bodies
     // device copies
     int *d_scell = nullptr, *d_ns = nullptr, *d_sites = nullptr, *d_sptr = nullptr, *d_sent = nullptr;
     double *d_sdelta = nullptr, *d_vel = nullptr, *d_eps = nullptr, *d_ds = nullptr;

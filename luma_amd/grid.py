@@ -312,6 +312,10 @@ class GpuGrid:
         call("lumagpu_get_omega", self.h, C.byref(o))
         return o.value
 
+    def ibm_set_universal_epsilon(self, on=True):
+        """L_UNIVERSAL_EPSILON_CALC: one epsilon system over all bodies (before ibm_initialise)."""
+        call("lumagpu_ibm_set_universal_epsilon", self.h, int(on))
+
     def set_pressure_delta(self, p):
         """L_PRESSURE_DELTA: pressure sites hold L_RHOIN + pd2dlbm(p) (p dimensionless)."""
         call("lumagpu_set_pressure_delta", self.h, float(p))

@@ -38,6 +38,7 @@ typedef struct ora_ibm {
     int err;
     int sub_it;       /* sub-iterations of the last step */
     double sub_res;
+    int universal;    /* L_UNIVERSAL_EPSILON_CALC */
 } ora_ibm;
 
 static int D_of(const ora_ibm *s) { return ora_get_desc(s->g)->dims; }
@@ -272,13 +273,38 @@ static int find_epsilon(ora_ibm *s, obody *b) {
     return r;
 }
 
+/* ibm_findEpsilon over the level (:773-920): per body, or with L_UNIVERSAL_EPSILON_CALC one
+ * system over every marker of the level in body order (ibm_universalEpsilonGather, :1067-1107),
+ * its epsilon scattered back in the same order (ibm_universalEpsilonScatter, :1113-1141) */
+static int find_epsilon_all(ora_ibm *s) {
+    if (!s->universal) {
+        for (int i = 0; i < s->nb; ++i)
+            if (find_epsilon(s, &s->b[i])) return -1;
+        return 0;
+    }
+    int n = 0;
+    for (int i = 0; i < s->nb; ++i) n += s->b[i].nm;
+    if (n == 0) return 0;
+    obody all = {0};
+    all.nm = n;
+    all.m = (omarker *)malloc(sizeof(omarker) * (size_t)n);
+    for (int i = 0, k = 0; i < s->nb; ++i)
+        for (int m = 0; m < s->b[i].nm; ++m) all.m[k++] = s->b[i].m[m];
+    int r = find_epsilon(s, &all);
+    for (int i = 0, k = 0; i < s->nb && r == 0; ++i)
+        for (int m = 0; m < s->b[i].nm; ++m) s->b[i].m[m].eps = all.m[k++].eps;
+    free(all.m);
+    return r;
+}
+
+void ora_ibm_set_universal_epsilon(ora_ibm *s, int on) { s->universal = on ? 1 : 0; }
+
 /* ObjectManager::ibm_initialise, :190-246 */
 int ora_ibm_initialise(ora_ibm *s) {
     for (int i = 0; i < s->nb; ++i)
         if (find_support(s, &s->b[i])) return -1;
     for (int i = 0; i < s->nb; ++i) compute_ds(s, &s->b[i]);
-    for (int i = 0; i < s->nb; ++i)
-        if (find_epsilon(s, &s->b[i])) return -2;
+    if (find_epsilon_all(s)) return -2;
     return 0;
 }
 
@@ -324,8 +350,7 @@ static void move_bodies(ora_ibm *s) {
     for (int ib = 0; ib < s->nb; ++ib)
         if (s->b[ib].flexible && find_support(s, &s->b[ib])) s->err = 2;
     for (int ib = 0; ib < s->nb; ++ib) compute_ds(s, &s->b[ib]);
-    for (int ib = 0; ib < s->nb; ++ib)
-        if (find_epsilon(s, &s->b[ib])) s->err = 3;
+    if (find_epsilon_all(s)) s->err = 3;
 }
 
 /* ibm_checkVelDiff, :988-1032 */

@@ -61,6 +61,7 @@ def _load():
             "ora_ibm_initialise": ([_vp], C.c_int), "ora_ibm_attach": ([_vp], None),
             "ora_ibm_set_velocity": ([_vp, C.c_int, _dp], None),
             "ora_ibm_set_epsilon": ([_vp, C.c_int, _dp], None),
+            "ora_ibm_set_universal_epsilon": ([_vp, C.c_int], None),
             "ora_ibm_get_markers": ([_vp, C.c_int, _dp], None),
             "ora_ibm_get_support": ([_vp, C.c_int, C.c_int, _ip, _dp], C.c_int),
             "ora_ibm_delta": ([C.c_double, C.c_double], C.c_double),
@@ -259,6 +260,10 @@ class OracleIBM:
     def set_velocity(self, ib, vel):
         v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
         self.lib.ora_ibm_set_velocity(self.h, ib, v.ctypes.data_as(_dp))
+
+    def set_universal_epsilon(self, on=True):
+        """L_UNIVERSAL_EPSILON_CALC: one epsilon system over every body (call before initialise)."""
+        self.lib.ora_ibm_set_universal_epsilon(self.h, int(on))
 
     def set_epsilon(self, ib, eps):
         e = np.ascontiguousarray(eps, dtype=np.float64)

@@ -112,6 +112,34 @@ def test_luma_default_arrangement_fp64_bitwise():
         assert np.abs(r[-1, 1:-1] - 1.0).max() < 1e-15
 
 
+def test_universal_epsilon_fp64_bitwise():
+    # L_UNIVERSAL_EPSILON_CALC with two cylinders whose supports overlap: one epsilon system over
+    # both bodies' markers, then the coupled IB step, bit for bit against the oracle
+    c = periodic_box(44, 26, 1, 0.02, 0.6, dims=2)
+    a = setups.circle_markers((10.3, 12.2, 0.0), 4.0, 1.0)
+    b = setups.circle_markers((16.6, 12.9, 0.0), 3.0, 1.0)
+    print(f"[ibm] universal epsilon, {len(a)} + {len(b)} markers", flush=True)
+    og = oracle_grid(c)
+    ib = OracleIBM(og, 1.0)
+    ib.set_universal_epsilon(True)
+    ib.add_body(a)
+    ib.add_body(b)
+    ib.initialise()
+    with gpu_grid(c) as gg:
+        gg.ibm_set_universal_epsilon(True)
+        gg.ibm_add_body(a)
+        gg.ibm_add_body(b)
+        gg.ibm_initialise()
+        for body in (0, 1):
+            assert np.array_equal(gg.ibm_markers(body)["eps"], ib.markers(body)["eps"]), body
+        for _ in range(60):
+            og.step()
+            gg.LBM_multi_opt()
+        compare_markers(gg, ib, 0)
+        compare_markers(gg, ib, 1)
+        compare_grids(c, gg, og, "universal epsilon")
+
+
 def test_ib_sphere_3d_fp64_bitwise():
     c = periodic_box(20, 16, 16, 0.03, 0.6)
     pos = setups.sphere_markers((8.3, 8.1, 7.9), 3.0, 1.0)

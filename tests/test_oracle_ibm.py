@@ -87,6 +87,45 @@ def test_epsilon_against_independent_numpy_assembly_and_spread_balance():
     assert np.abs(mk["force"]).max() > 1e-6
 
 
+def _joint_epsilon(ib, bodies, pos_all):
+    """A eps = 1 over the markers of several bodies together, assembled from the support lists."""
+    n = len(pos_all)
+    ds = np.concatenate([ib.markers(b)["ds"] for b in bodies])
+    sup = [ib.support(b, m) for b in bodies for m in range(ib.nm[b])]
+    A = np.zeros((n, n))
+    for I, (ijk, dI) in enumerate(sup):
+        xs, ys = 0.5 + ijk[:, 0], 0.5 + ijk[:, 1]
+        for J in range(n):
+            dJ = np.array([delta_kernel(pos_all[J, 0] - x) * delta_kernel(pos_all[J, 1] - y) for x, y in zip(xs, ys)])
+            A[I, J] = (dI * dJ).sum() * ds[J]
+    return np.linalg.solve(A, np.ones(n))
+
+
+def test_universal_epsilon_couples_overlapping_bodies_only():
+    # L_UNIVERSAL_EPSILON_CALC (ObjectManager_ops_ibm.cpp:775-786, :905-909, :1067-1141): one
+    # system over every marker in body order.  Bodies far apart give a block-diagonal system, so
+    # the joint solve returns each body's own epsilon exactly; bodies whose supports overlap are
+    # coupled, and the joint epsilon solves the joint numpy assembly
+    a = setups.circle_markers((10.3, 12.2, 0.0), 4.0, 1.0)
+    far = setups.circle_markers((30.1, 12.4, 0.0), 3.0, 1.0)
+    near = setups.circle_markers((16.6, 12.9, 0.0), 3.0, 1.0)       # supports overlap a's
+    eps = {}
+    for name, b2, uni in (("sep", far, False), ("sep_u", far, True), ("ov", near, False), ("ov_u", near, True)):
+        g = periodic(44, 26, (0.02, 0.0))
+        ib = OracleIBM(g, 1.0)
+        if uni:
+            ib.set_universal_epsilon(True)
+        ib.add_body(a)
+        ib.add_body(b2)
+        ib.initialise()
+        eps[name] = np.concatenate([ib.markers(0)["eps"], ib.markers(1)["eps"]])
+        if name == "ov_u":
+            ref = _joint_epsilon(ib, (0, 1), np.concatenate([a, near]))
+            assert np.allclose(eps[name], ref, rtol=1e-11)
+    assert np.array_equal(eps["sep"], eps["sep_u"])
+    assert np.abs(eps["ov"] - eps["ov_u"]).max() > 1e-3
+
+
 def test_support_stencil_3d_has_27_sites_led_by_the_enclosing_voxel():
     N = 12
     g = OracleGrid(3, 19, N, N, N, omega=1.2, forced=True, edges=(float(N),) * 3)
