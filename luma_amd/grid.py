@@ -184,6 +184,9 @@ class GpuGrid:
     def ibm_write_ld(self, path, body, t, time, precision=8):
         call("lumagpu_ibm_write_ld", self.h, str(path).encode(), int(body), int(t), float(time), int(precision))
 
+    def ibm_write_tip(self, path, body, t, precision=10):
+        call("lumagpu_ibm_write_tip", self.h, str(path).encode(), int(body), int(t), int(precision))
+
     def set_zslab(self, on=True):
         """z planes 0 and K-1 become halo planes exchanged around every odd step (one rank of a
         z-sharded periodic box, its own neighbour on both faces)."""

@@ -343,6 +343,35 @@ def test_lift_drag_file(tmp_path):
     assert np.allclose(np.array(vals[2:], float).reshape(-1, 2), ref, rtol=1e-11, atol=0)
 
 
+def test_tip_positions_file(tmp_path):
+    # ObjectManager::io_writeTipPositions: header at t = 0, then t, t*dt and the last marker's
+    # position of a flexible filament; rigid bodies are refused
+    from luma_amd._abi import LumaGpuError
+    c = setups.channel_2d(64, 32, 8.0, 0.04, 100.0)
+    dh, dt = c.info["dh"], c.info["dt"]
+    density, E = setups.filament_material(1.0, 0.05, dh, mass_ratio=10.0, bending=1e-2)
+    path = tmp_path / "Body_0_TipPositions.out"
+    with gpu_grid(c) as g:
+        g.ibm_add_filament((1.5, 2.013, 0.0), 1.0, 0.05, 8.0, 8, True, density, E)
+        g.ibm_add_body(setups.circle_markers((5.0, 2.0, 0.0), 0.3, dh))
+        g.ibm_initialise()
+        g.ibm_write_tip(path, 0, 0, precision=10)
+        p0 = g.ibm_markers(0)["pos"][-1]
+        g.run(4)
+        g.ibm_write_tip(path, 0, 4, precision=10)
+        p4 = g.ibm_markers(0)["pos"][-1]
+        with pytest.raises(LumaGpuError, match="flexible bodies only"):
+            g.ibm_write_tip(tmp_path / "rigid.out", 1, 4)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "Timestep\tt\tTipX\tTipY\tTipZ" and len(lines) == 3
+    for line, t, p in ((lines[1], 0, p0), (lines[2], 4, p4)):
+        vals = line.split("\t")
+        assert vals[0] == str(t) and len(vals) == 5
+        assert float(vals[1]) == float(f"{t * dt:.10g}")
+        assert np.array_equal(np.array(vals[2:], float), np.array([float(f"{x:.10g}") for x in p]))
+    assert p4[1] != p0[1]
+
+
 def test_geometry_config_builds_luma_bodies(tmp_path):
     # ObjectManager::io_readInGeomConfig: prefab IBM bodies from a geometry.config; the markers
     # equal LUMA's generators (restated in setups) and the flexible filament gets its beam
