@@ -282,10 +282,11 @@ int lumagpu_io_lite(lumagpu_grid *g, const char *path, int rank, int precision);
  * dimension; the header line when t == 0. */
 int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, double time, int precision);
 /* ObjectManager::io_writeTipPositions (ObjectManager_ops_io.cpp:1220-1245, L_WRITE_TIP_POSITIONS)
- * for one flexible body: appends "t<TAB>t*dt<TAB>TipX<TAB>TipY<TAB>TipZ" of its last marker
+ * for one flexible body: appends "t<TAB>time<TAB>TipX<TAB>TipY<TAB>TipZ" of its last marker
  * (precision = L_OUTPUT_PRECISION, the stream's default format), the header line when t == 0;
- * the reference names the file Body_<id>_TipPositions.out. */
-int lumagpu_ibm_write_tip(lumagpu_grid *g, const char *path, int body, int t, int precision);
+ * time = t * the level-0 grid's dt as the reference writes it; the reference names the file
+ * Body_<id>_TipPositions.out. */
+int lumagpu_ibm_write_tip(lumagpu_grid *g, const char *path, int body, int t, double time, int precision);
 /* ObjectManager::io_restart (ObjectManager_ops_io.cpp:136-290) for the bodies of a hierarchy's
  * grids, in grid order: the body count, then per body "\t/\t" n "\t/\t" and its marker
  * positions (plus position0 for flexible bodies), at full precision (LUMA writes 6 digits).

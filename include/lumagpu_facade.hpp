@@ -210,6 +210,10 @@ class ObjectManager {
     void io_writeLiftDrag(GridObj &g, int body, const std::string &path, int precision = 8) {
         check(lumagpu_ibm_write_ld(g.handle(), path.c_str(), body, _Grids->t, _Grids->t * _Grids->dt, precision));
     }
+    // io_writeTipPositions (ObjectManager_ops_io.cpp:1220-1245) for one flexible body of grid g
+    void io_writeTipPositions(GridObj &g, int body, const std::string &path, int precision = 10) {
+        check(lumagpu_ibm_write_tip(g.handle(), path.c_str(), body, _Grids->t, _Grids->t * _Grids->dt, precision));
+    }
 
   private:
     explicit ObjectManager(GridObj *g) : _Grids(g) {}

@@ -84,7 +84,7 @@ _PROTOS = {
     "lumagpu_read_restart": [_vp, C.c_char_p, C.c_int],
     "lumagpu_io_lite": [_vp, C.c_char_p, C.c_int, C.c_int],
     "lumagpu_ibm_write_ld": [_vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int],
-    "lumagpu_ibm_write_tip": [_vp, C.c_char_p, C.c_int, C.c_int, C.c_int],
+    "lumagpu_ibm_write_tip": [_vp, C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int],
     "lumagpu_ibm_count": [_vp, _ip, C.c_int, _ip],
     "lumagpu_ibm_read_geometry": [C.POINTER(_vp), C.c_int, C.c_char_p, _dp, _ip],
     "lumagpu_ibm_write_restart": [C.POINTER(_vp), C.c_int, C.c_char_p],

@@ -235,9 +235,10 @@ int lumagpu_ibm_write_ld(lumagpu_grid *g, const char *path, int body, int t, dou
 }
 
 // ObjectManager::io_writeTipPositions (ObjectManager_ops_io.cpp:1220-1245) for one flexible
-// body: appends "t<TAB>t*dt<TAB>x<TAB>y<TAB>z" of its last marker in the stream's default
-// (%g-style) format at the given precision, after the header line when t == 0.
-int lumagpu_ibm_write_tip(lumagpu_grid *g, const char *path, int body, int t, int precision) {
+// body: appends "t<TAB>time<TAB>x<TAB>y<TAB>z" of its last marker in the stream's default
+// (%g-style) format at the given precision, after the header line when t == 0.  The reference
+// writes time = t * (level-0 grid's dt), whatever the body's level, so the caller passes it.
+int lumagpu_ibm_write_tip(lumagpu_grid *g, const char *path, int body, int t, double time, int precision) {
     if (!g || !path) return fail(LUMAGPU_EINVAL, "null argument");
     if (!g->ibm || body < 0 || body >= (int)g->ibm->bodies.size()) return fail(LUMAGPU_EINVAL, "no such IB body");
     if (precision < 1 || precision > 30) return fail(LUMAGPU_EINVAL, "output precision out of range");
@@ -247,7 +248,7 @@ int lumagpu_ibm_write_tip(lumagpu_grid *g, const char *path, int body, int t, in
     FILE *fp = std::fopen(path, "a");
     if (!fp) return fail(LUMAGPU_EINVAL, std::string("cannot open ") + path);
     if (t == 0) std::fprintf(fp, "Timestep\tt\tTipX\tTipY\tTipZ\n");
-    std::fprintf(fp, "%d\t%.*g\t%.*g\t%.*g\t%.*g\n", t, precision, t * g->d.dt, precision, m.pos[0], precision,
+    std::fprintf(fp, "%d\t%.*g\t%.*g\t%.*g\t%.*g\n", t, precision, time, precision, m.pos[0], precision,
                  m.pos[1], precision, m.pos[2]);
     const bool ok = std::ferror(fp) == 0;
     std::fclose(fp);

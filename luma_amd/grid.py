@@ -184,8 +184,8 @@ class GpuGrid:
     def ibm_write_ld(self, path, body, t, time, precision=8):
         call("lumagpu_ibm_write_ld", self.h, str(path).encode(), int(body), int(t), float(time), int(precision))
 
-    def ibm_write_tip(self, path, body, t, precision=10):
-        call("lumagpu_ibm_write_tip", self.h, str(path).encode(), int(body), int(t), int(precision))
+    def ibm_write_tip(self, path, body, t, time, precision=10):
+        call("lumagpu_ibm_write_tip", self.h, str(path).encode(), int(body), int(t), float(time), int(precision))
 
     def set_zslab(self, on=True):
         """z planes 0 and K-1 become halo planes exchanged around every odd step (one rank of a

@@ -355,13 +355,13 @@ def test_tip_positions_file(tmp_path):
         g.ibm_add_filament((1.5, 2.013, 0.0), 1.0, 0.05, 8.0, 8, True, density, E)
         g.ibm_add_body(setups.circle_markers((5.0, 2.0, 0.0), 0.3, dh))
         g.ibm_initialise()
-        g.ibm_write_tip(path, 0, 0, precision=10)
+        g.ibm_write_tip(path, 0, 0, 0.0, precision=10)
         p0 = g.ibm_markers(0)["pos"][-1]
         g.run(4)
-        g.ibm_write_tip(path, 0, 4, precision=10)
+        g.ibm_write_tip(path, 0, 4, 4 * dt, precision=10)
         p4 = g.ibm_markers(0)["pos"][-1]
         with pytest.raises(LumaGpuError, match="flexible bodies only"):
-            g.ibm_write_tip(tmp_path / "rigid.out", 1, 4)
+            g.ibm_write_tip(tmp_path / "rigid.out", 1, 4, 4 * dt)
     lines = path.read_text().splitlines()
     assert lines[0] == "Timestep\tt\tTipX\tTipY\tTipZ" and len(lines) == 3
     for line, t, p in ((lines[1], 0, p0), (lines[2], 4, p4)):
