@@ -159,7 +159,8 @@ int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time);
  * two single steps in FP64 and FP32-shifted storage, and within one unit in the last place of
  * the stored half per value in FP16-shifted.  Allocates a second population array (the kernel
  * reads one and writes the other); shape < 0 frees it and turns the two-step kernel off.
- * Grids that stop qualifying (labels or a force set later) step one step at a time. */
+ * Grids that stop qualifying (labels or a force set later) step one step at a time.  Long
+ * lumagpu_run calls replay a captured graph of two two-step launches. */
 int lumagpu_set_temporal_blocking(lumagpu_grid *g, int shape, int zchunk);
 /* L_REYNOLDS_RAMP (definitions.h:204): at the start of every step of this grid and of each grid
  * below it (LBM_multi_opt, GridObj_ops_lbm_optimised.cpp:39-42, before the children's

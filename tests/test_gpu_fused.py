@@ -122,3 +122,23 @@ def test_two_step_kernel_full_size_256():
     (ma, ra, ua), (mb, rb, ub) = out
     assert ma == mb
     assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32s"])
+def test_two_step_graph_replays_match_single_steps(prec):
+    # lumagpu_run replays a captured pair of two-step launches in long runs: several runs in a
+    # row, with downloads and reductions between them, each checked against single steps
+    case = _tgv((64, 16, 12))
+    print(f"[fused] graph replays {prec}", flush=True)
+    segments = (1, 20, 3, 17, 12)
+    with GpuGrid.from_case(case, prec, store_macro=True) as ref, \
+            GpuGrid.from_case(case, prec, store_macro=True) as g:
+        g.set_temporal_blocking(8 if prec == "fp32s" else 0, 5)   # two LDS copies need <= 4-byte storage
+        for n in segments:
+            ref.run(n)
+            g.run(n)
+            assert g.t == ref.t
+            assert np.array_equal(g.download_f(), ref.download_f())
+            (rg, ug), (rr, ur) = g.download_macro(), ref.download_macro()
+            assert np.array_equal(rg, rr) and np.array_equal(ug, ur)
+            assert g.moments() == ref.moments()
