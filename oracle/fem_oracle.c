@@ -467,3 +467,10 @@ int ora_fem_step_load(ofem *f, const double *R) {
     ora_fem_commit(f);
     return f->it;
 }
+
+/* Test access: the markers of the current state through updateIBMarkers (FEMBody.cpp:329-370):
+ * positions pos0 + N(zeta) U of each marker's parent element, velocities relaxed from Udot. */
+void ora_fem_markers(ofem *f, const double *mpos0, double *mpos, double *mvel, double *mvel_km1) {
+    update_markers(f, mpos, mpos0, mvel, mvel_km1);
+}
+int ora_fem_nib(const ofem *f) { return f->nib; }

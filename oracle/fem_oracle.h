@@ -37,5 +37,11 @@ int ora_fem_dynamic(ofem *f, const double *mforce, const double *meps, double *m
 void ora_fem_commit(ofem *f);
 void ora_fem_state(const ofem *f, double *U, double *Udot, int *it, double *res);
 int ora_fem_ndof(const ofem *f);
+/* test access (pinning tests) */
+void ora_fem_assemble(ofem *f, double *K, double *M);
+void ora_fem_set_state(ofem *f, const double *U, const double *Udot, const double *Udotdot);
+int ora_fem_step_load(ofem *f, const double *R);
+void ora_fem_markers(ofem *f, const double *mpos0, double *mpos, double *mvel, double *mvel_km1);
+int ora_fem_nib(const ofem *f);
 
 #endif

@@ -461,12 +461,13 @@ void ora_ibm_set_epsilon(ora_ibm *s, int ib, const double *eps) {
 void ora_ibm_get_markers(const ora_ibm *s, int ib, double *out) {
     for (int i = 0; i < s->b[ib].nm; ++i) {
         const omarker *m = &s->b[ib].m[i];
-        double *o = out + 16 * (size_t)i;
+        double *o = out + 19 * (size_t)i;
         for (int a = 0; a < 3; ++a) {
             o[a] = m->pos[a];
             o[3 + a] = m->vel[a];
             o[6 + a] = m->force[a];
             o[9 + a] = m->mom[a];
+            o[16 + a] = m->vel_km1[a];
         }
         o[12] = m->rho;
         o[13] = m->eps;

@@ -586,10 +586,17 @@ void ora_multi_opt(ora_grid *g, int subcycle) {
 }
 
 /* OpenMP timing mode for the CPU baseline: the same per-site arithmetic, sweeps split over
- * the x index like LUMA's own experimental pragma (:69-71).  Valid for grids without
- * regularised corner sites (whose serial order the parallel sweep cannot keep). */
+ * the x index like LUMA's own experimental pragma (:69-71).  Only an all-fluid grid runs in
+ * parallel: a regularised edge or corner site updates its neighbours on the fly (:1376) and a
+ * forced-equilibrium source under a ramp writes the source's u (:258-266), neither of which a
+ * parallel sweep orders as the serial one does, so every other grid steps serially. */
+static int all_fluid(const ora_grid *g) {
+    for (size_t id = 0; id < g->ncell; ++id)
+        if (g->LatTyp[id] != O_FLUID) return 0;
+    return 1;
+}
 void ora_run(ora_grid *g, int nsteps, int nthreads) {
-    if (nthreads <= 1 || g->child || g->hook || g->re_ramp > 0.0) {
+    if (nthreads <= 1 || g->child || g->hook || g->re_ramp > 0.0 || !all_fluid(g)) {
         for (int s = 0; s < nsteps; ++s) ora_multi_opt(g, 0);
         return;
     }

@@ -76,6 +76,7 @@ def _load():
             "ora_fem_set_state": ([_vp, _dp, _dp, _dp], None),
             "ora_fem_step_load": ([_vp, _dp], C.c_int),
             "ora_fem_state": ([_vp, _dp, _dp, _ip, _dp], None),
+            "ora_fem_markers": ([_vp, _dp, _dp, _dp, _dp], None), "ora_fem_nib": ([_vp], C.c_int),
             "ora_ibm_subiter": ([_vp, _ip, _dp], None),
             "ora_ibm_fem_state": ([_vp, C.c_int, _dp, _dp, _ip, _dp], C.c_int),
         }
@@ -270,10 +271,11 @@ class OracleIBM:
         self.lib.ora_ibm_set_epsilon(self.h, ib, e.ctypes.data_as(_dp))
 
     def markers(self, ib):
-        out = np.empty((self.nm[ib], 16))
+        out = np.empty((self.nm[ib], 19))
         self.lib.ora_ibm_get_markers(self.h, ib, out.ctypes.data_as(_dp))
         return {"pos": out[:, 0:3], "vel": out[:, 3:6], "force": out[:, 6:9], "mom": out[:, 9:12],
-                "rho": out[:, 12], "eps": out[:, 13], "ds": out[:, 14], "ns": out[:, 15].astype(int)}
+                "rho": out[:, 12], "eps": out[:, 13], "ds": out[:, 14], "ns": out[:, 15].astype(int),
+                "vel_km1": out[:, 16:19]}
 
     def support(self, ib, m):
         ijk = np.empty(27 * 3, np.int32)
@@ -325,6 +327,15 @@ class OracleFEM:
     def step(self, R=None):
         r = np.ascontiguousarray(R if R is not None else np.zeros(self.n), dtype=np.float64)
         return self.lib.ora_fem_step_load(self.h, r.ctypes.data_as(_dp))
+
+    def markers(self, pos0):
+        """Marker positions and velocities of the current state (updateIBMarkers)."""
+        p0 = np.ascontiguousarray(pos0, dtype=np.float64).reshape(-1, 3)
+        assert len(p0) == self.lib.ora_fem_nib(self.h)
+        pos, vel, vkm1 = np.zeros_like(p0), np.zeros_like(p0), np.zeros_like(p0)
+        self.lib.ora_fem_markers(self.h, p0.ctypes.data_as(_dp), pos.ctypes.data_as(_dp), vel.ctypes.data_as(_dp),
+                                 vkm1.ctypes.data_as(_dp))
+        return pos, vel
 
     def state(self):
         U, Ud = np.empty(self.n), np.empty(self.n)
