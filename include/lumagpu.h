@@ -65,7 +65,7 @@ typedef struct lumagpu_grid_desc {
     int level;            /* refinement level */
     double omega;         /* GridObj::omega */
     double rho_in;        /* L_RHOIN */
-    double rho0;          /* reference density of the shifted forms */
+    double rho0;          /* reference density of the shifted forms: must be 1 for them */
     double xmax, ymax, zmax;   /* GridManager::global_edges[e?Max][0] of level 0 */
     double wall_thk[6];        /* L_WALL_THICKNESS_{LEFT,RIGHT,BOTTOM,TOP,FRONT,BACK} */
     int coarse_lims[6];        /* CoarseLimsX/Y/Z[min,max] on the parent (level > 0) */
@@ -123,9 +123,28 @@ int lumagpu_tangent_seed_omega(lumagpu_grid *g, double d_omega);
 int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz);
 
 /* ---- time step ----------------------------------------------------------------------- */
-/* GridObj::LBM_multi_opt(int subcycle) (GridObj.h:188, GridObj_ops_lbm_optimised.cpp:36-193),
- * children first, without IBM work. */
+/* GridObj::LBM_multi_opt(int subcycle) (GridObj.h:188, GridObj_ops_lbm_optimised.cpp:36-193):
+ * the children's two sub-steps first, then this grid's step with its IB bodies' step
+ * (ObjectManager::ibm_apply) between the stream/macro and force/collide halves. */
 int lumagpu_step(lumagpu_grid *g, int subcycle);
+/* The step of one grid in the two halves LBM_multi_opt's sweeps form, so that a caller can run
+ * the IB stages (below), or code of its own, in between; the children must have stepped
+ * already (lumagpu_step(child, 0), lumagpu_step(child, 1)).  lumagpu_stream_bc_macro is
+ * sweep 1 (GridObj_ops_lbm_optimised.cpp:53-107: _LBM_resetForces on a level holding IB bodies,
+ * then stream, regularised BCs and _LBM_macro_opt): afterwards rho and u hold sweep 1's values
+ * at every fluid, eTransitionToFiner and regularised velocity-face site (edge/corner velocity
+ * and pressure sites receive theirs in the second half).  lumagpu_force_collide_swap is sweep 2
+ * (:125-172: _LBM_forceGrid_opt, _LBM_collide_opt, f.swap(fNew), ++t); it streams again in
+ * the same pass (the populations stay in place until then).  stream_bc_macro, the IB stages
+ * and force_collide_swap in that order equal lumagpu_step bit for bit; lumagpu_step and
+ * lumagpu_run are refused while a split step is open. */
+int lumagpu_stream_bc_macro(lumagpu_grid *g, int subcycle);
+int lumagpu_force_collide_swap(lumagpu_grid *g);
+/* GridObj::_LBM_resetForces (GridObj_ops_lbm_optimised.cpp:1327-1337): force_xyz = 0 at every
+ * site, or with L_GRAVITY_ON the gravity component rho * gravity * refinement_ratio (the other
+ * components left as they are); requires desc.forced.  lumagpu_stream_bc_macro already does
+ * this on a level holding IB bodies; this entry is for callers that sub-iterate themselves. */
+int lumagpu_reset_forces(lumagpu_grid *g);
 /* nsteps consecutive lumagpu_step(g, 0) calls */
 int lumagpu_run(lumagpu_grid *g, int nsteps);
 /* as lumagpu_run, timing with HIP events on the grid's stream: total wall time of the
@@ -209,6 +228,23 @@ int lumagpu_ibm_add_body(lumagpu_grid *g, int nmarkers, const double *pos, int f
 int lumagpu_ibm_add_filament(lumagpu_grid *g, const double *start, double length, double height, double depth,
                              double angle_deg, int n_elements, int clamped, double density, double youngs,
                              int *body);
+/* The IB stages of one grid's bodies, between lumagpu_stream_bc_macro and
+ * lumagpu_force_collide_swap (ESTATE elsewhere; no-ops on a grid without bodies):
+ *   lumagpu_ibm_apply            ObjectManager::ibm_apply(g, true) (ObjectManager_ops_ibm.cpp:33-54):
+ *                                all of the below, then for flexible bodies ibm_moveBodies and the
+ *                                FSI sub-iterations (ibm_subIterate, :111-185) -- what lumagpu_step runs
+ *   lumagpu_ibm_interpolate      ibm_interpolate (:498-591): rho and rho u at the markers from sweep 1's
+ *                                values at their supports (recomputed from the populations)
+ *   lumagpu_ibm_compute_force    ibm_computeForce (:598-614): F = 2 (rho u - rho U) per marker
+ *   lumagpu_ibm_spread           ibm_spread (:621-689) into force_xyz, in LUMA's accumulation order
+ *   lumagpu_ibm_update_macro     ibm_updateMacroscopic (:696-766): rho, u of the supports with F/2
+ * interpolate, compute_force, spread, update_macro in that order equal lumagpu_ibm_apply for rigid
+ * bodies (flexible bodies move only in lumagpu_ibm_apply). */
+int lumagpu_ibm_apply(lumagpu_grid *g);
+int lumagpu_ibm_interpolate(lumagpu_grid *g);
+int lumagpu_ibm_compute_force(lumagpu_grid *g);
+int lumagpu_ibm_spread(lumagpu_grid *g);
+int lumagpu_ibm_update_macro(lumagpu_grid *g);
 /* sub-iterations and residual of the last step's FSI loop (ObjectManager::timeav_subResidual) */
 int lumagpu_ibm_subiterations(const lumagpu_grid *g, int *iterations, double *residual);
 /* FSI sub-iteration counts of every step since lumagpu_ibm_initialise (the per-step values of

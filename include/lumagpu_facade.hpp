@@ -115,6 +115,19 @@ class GridObj {
         check(lumagpu_step(h_, subcycle));
         advance(1);
     }
+    // LBM_multi_opt in the halves its two sweeps form, for a caller that runs the IB stages
+    // (ObjectManager::ibm_interpolate ... ibm_updateMacroscopic) or code of its own in between;
+    // the sub-grids must have stepped first (their own LBM_multi_opt(0) and (1)).  Sweep 1:
+    // _LBM_resetForces on a level holding IB bodies, stream, regularised BCs, _LBM_macro_opt
+    // (GridObj_ops_lbm_optimised.cpp:53-107); sweep 2: _LBM_forceGrid_opt, _LBM_collide_opt, the
+    // swap and ++t (:125-172).  The host mirrors are not touched (sync() fills them).
+    void LBM_stream_bc_macro(int subcycle = 0) { check(lumagpu_stream_bc_macro(h_, subcycle)); }
+    void LBM_force_collide_swap() {
+        check(lumagpu_force_collide_swap(h_));
+        t += 1;
+    }
+    // _LBM_resetForces (:1327-1337): force_xyz = 0, or the gravity component with L_GRAVITY_ON
+    void _LBM_resetForces() { check(lumagpu_reset_forces(h_)); }
     // nsteps LBM_multi_opt calls of the top-level grid (replayed as a graph when possible)
     void LBM_multi_run(int nsteps) {
         check(lumagpu_run(h_, nsteps));
@@ -200,6 +213,16 @@ class ObjectManager {
             if (nb > 0) check(lumagpu_ibm_initialise(x->handle()));
         }
     }
+    // The IB step of the bodies on grid level `level` (one region per level), between a grid's
+    // LBM_stream_bc_macro and LBM_force_collide_swap.  ibm_apply (ObjectManager_ops_ibm.cpp:33-54,
+    // with doSubIterate = true as LBM_multi_opt calls it): all stages, the FEM move and the FSI
+    // sub-iterations of flexible bodies.  ibm_interpolate (:498-591), ibm_computeForce (:598-614),
+    // ibm_spread (:621-689), ibm_updateMacroscopic (:696-766): one stage each, for rigid bodies.
+    void ibm_apply(GridObj *g) { check(lumagpu_ibm_apply(g->handle())); }
+    void ibm_interpolate(int level) { check(lumagpu_ibm_interpolate(grid_at(level)->handle())); }
+    void ibm_computeForce(int level) { check(lumagpu_ibm_compute_force(grid_at(level)->handle())); }
+    void ibm_spread(int level) { check(lumagpu_ibm_spread(grid_at(level)->handle())); }
+    void ibm_updateMacroscopic(int level) { check(lumagpu_ibm_update_macro(grid_at(level)->handle())); }
     // FP64 sum of a body's marker forces (the terms of io_writeLiftDrag)
     std::array<double, 3> bodyForce(GridObj &g, int body) {
         std::array<double, 3> out{};
@@ -217,6 +240,12 @@ class ObjectManager {
 
   private:
     explicit ObjectManager(GridObj *g) : _Grids(g) {}
+    GridObj *grid_at(int level) {
+        for (GridObj *x = _Grids; x; x = x->subGrid.empty() ? nullptr : x->subGrid[0])
+            if (x->level == level) return x;
+        error_handler()("no grid on level " + std::to_string(level));
+        return _Grids;
+    }
     static ObjectManager *&instance() {
         static ObjectManager *m = nullptr;
         return m;

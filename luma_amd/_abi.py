@@ -63,6 +63,14 @@ _PROTOS = {
     "lumagpu_download_macro": [_vp, _dp, _dp],
     "lumagpu_set_force": [_vp, _dp],
     "lumagpu_step": [_vp, C.c_int],
+    "lumagpu_stream_bc_macro": [_vp, C.c_int],
+    "lumagpu_force_collide_swap": [_vp],
+    "lumagpu_reset_forces": [_vp],
+    "lumagpu_ibm_apply": [_vp],
+    "lumagpu_ibm_interpolate": [_vp],
+    "lumagpu_ibm_compute_force": [_vp],
+    "lumagpu_ibm_spread": [_vp],
+    "lumagpu_ibm_update_macro": [_vp],
     "lumagpu_run": [_vp, C.c_int],
     "lumagpu_run_timed": [_vp, C.c_int, _dp, _dp],
     "lumagpu_set_store_macro": [_vp, C.c_int],

@@ -458,7 +458,7 @@ def main(argv=None):
             print(json.dumps(r), flush=True)
         if "fp64" in res:
             parent, child = geo[0], geo[1]
-            mp = ~np.isin(parent.lattyp, [setups.SOLID, setups.REFINED, setups.EXTRAP_RIGHT])
+            mp = ~np.isin(parent.lattyp, [setups.SOLID, setups.REFINED])
             mc = child.lattyp != setups.SOLID
             for p in precs:
                 if p == "fp64":

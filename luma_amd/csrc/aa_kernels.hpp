@@ -19,9 +19,9 @@
 //     (:765-790) and _LBM_regularised_opt (:313-510), operand for operand.  The translation
 //     unit that instantiates it is compiled with -ffp-contract=off, so results match a
 //     non-FMA x86-64 build of LUMA bit for bit.
-//   * shifted (FP32 / FP16 storage): populations are stored as f_v - w_v*rho0 and the
-//     collision works on those deviations in FP32; moments and equilibrium are the same
-//     polynomials rearranged around rho0.
+//   * shifted (FP32 / FP16 storage): populations are stored as f_v - w_v*rho0 (rho0 = 1) and
+//     the collision works on those deviations in FP32; moments and equilibrium are the same
+//     polynomials rearranged around rho0, with every fused multiply-add written out.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -467,27 +467,56 @@ __device__ __forceinline__ void regularised_edge(T *fa, const int *n, const T *v
 // shifted (deviation) per-site arithmetic, FP32 compute (TF = float, or DualF in the shifted
 // tangent form)
 
+// Every fused operation of the shifted arithmetic below is written out (fmac, fma3, mul_split)
+// and the units that instantiate it are built without contraction, so the same site update
+// rounds the same way in every kernel that runs it (one- and two-step kernels, boundary and
+// IB kernels) whatever the compiler's scheduling.
+
+// c_V . x over the non-zero components of c_V only (c is -1, 0 or 1): the products with zero
+// would otherwise cost instructions, since 0 * x is not folded under IEEE semantics
+template <int Q, int V, class TF>
+__device__ __forceinline__ TF cdot(const TF *x) {
+    using L = Lattice<Q>;
+    constexpr int c0 = L::c(V, 0), c1 = L::c(V, 1), c2 = L::c(V, 2);
+    TF s = TF(0);
+    bool any = false;
+    if constexpr (c0 != 0) {
+        s = c0 > 0 ? x[0] : -x[0];
+        any = true;
+    }
+    if constexpr (c1 != 0) {
+        s = any ? (c1 > 0 ? s + x[1] : s - x[1]) : (c1 > 0 ? x[1] : -x[1]);
+        any = true;
+    }
+    if constexpr (c2 != 0) s = any ? (c2 > 0 ? s + x[2] : s - x[2]) : (c2 > 0 ? x[2] : -x[2]);
+    return s;
+}
+
 template <int Q, class TF, class FT>
 __device__ __forceinline__ void macro_shift(const TF *fin, const FT *F, bool fz_compat,
                                             TF &drho, TF *j) {
     using L = Lattice<Q>;
-    TF r = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+    // sums start from their first term (0 + x is not folded under IEEE semantics)
+    TF r = fin[0], jx = TF(0.f), jy = TF(0.f), jz = TF(0.f);
+    bool ax = false, ay = false, az = false;
 #pragma unroll
     for (int v = 0; v < Q; ++v) {
-        r += fin[v];
-        if (L::c(v, 0) == 1) jx += fin[v];
-        if (L::c(v, 0) == -1) jx -= fin[v];
-        if (L::c(v, 1) == 1) jy += fin[v];
-        if (L::c(v, 1) == -1) jy -= fin[v];
-        if (L::c(v, 2) == 1) jz += fin[v];
-        if (L::c(v, 2) == -1) jz -= fin[v];
+        if (v > 0) r += fin[v];
+        auto acc = [&](TF &s, bool &any, int c) {
+            if (c == 0) return;
+            s = any ? (c > 0 ? s + fin[v] : s - fin[v]) : (c > 0 ? fin[v] : -fin[v]);
+            any = true;
+        };
+        acc(jx, ax, L::c(v, 0));
+        acc(jy, ay, L::c(v, 1));
+        acc(jz, az, L::c(v, 2));
     }
     if (F) {
-        jx += 0.5f * TF(F[0]);
-        jy += 0.5f * TF(F[1]);
+        jx = fmac(0.5f, TF(F[0]), jx);
+        jy = fmac(0.5f, TF(F[1]), jy);
         if constexpr (L::D == 3) {
-            if (fz_compat) jx += 0.5f * TF(F[2]);
-            else jz += 0.5f * TF(F[2]);
+            if (fz_compat) jx = fmac(0.5f, TF(F[2]), jx);
+            else jz = fmac(0.5f, TF(F[2]), jz);
         }
     }
     drho = r;
@@ -513,65 +542,93 @@ __device__ __forceinline__ DualF mul_split(Split s, const DualF &x) {
     return DualF(fmaf(s.hi, x.v, s.lo * x.v), fmaf(s.hi, x.d, s.lo * x.d));
 }
 
-// BGK on deviations, with rho = rho0 + drho:
-//   feq_v - w_v rho0 = w_v (drho + rho P_v),  P_v = 3 c.u + 4.5 (c.u)^2 - 1.5 u.u,
-// evaluated as w_v (drho + rho0 P_v + drho P_v) so rho itself is never rounded;
-// Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).  One population at a time.
+// BGK on deviations from w_v (the storage of the shifted forms, reference density 1), with
+// rho = 1 + drho:
+//   feq_v - w_v = w_v s_v,  s_v = drho + (1 + drho) P_v,  P_v = 3 c.u + 4.5 (c.u)^2 - 1.5 u.u,
+// so rho itself is never rounded, and the relaxation f* = f + omega (feq - f), with w_v and
+// omega carried as float pairs (the constants enter exactly; the difference feq - f is small
+// near equilibrium, so the result rounds once at its own magnitude);
+// Guo term (1 - omega/2) w_v (3 (c - u).F + 9 (c.u)(c.F)).
+struct ShiftK {
+    Split om;
+    float gpre;
+};
+__device__ __forceinline__ ShiftK shift_k(const StepArgs &a) {
+    ShiftK k;
+    k.om = Split{a.om_hi, a.om_lo};
+    k.gpre = 1.0f - 0.5f * (a.om_hi + a.om_lo);
+    return k;
+}
+
 template <int Q, class TF = float>
 struct CollideShift {
-    TF drho, u[3], usq15, Fx, Fy, Fz, uF;
-    float rho0, gpre;
-    Split omega;
+    TF drho, u[3], nusq15, F[3], uF;
+    ShiftK k;
     bool forced;
     template <class FT>
-    __device__ __forceinline__ CollideShift(TF drho_, float rho0_, const TF *u_, const FT *F, Split om) {
+    __device__ __forceinline__ CollideShift(TF drho_, const TF *u_, const FT *Fs, const ShiftK &k_) {
         using L = Lattice<Q>;
         drho = drho_;
-        rho0 = rho0_;
         u[0] = u_[0];
         u[1] = u_[1];
         u[2] = u_[2];
-        omega = om;
-        usq15 = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-        gpre = 1.0f - 0.5f * (om.hi + om.lo);
-        forced = F != nullptr;
-        Fx = Fy = Fz = uF = 0.f;
+        k = k_;
+        // -1.5 u.u
+        nusq15 = -1.5f * fma3(u[0], u[0], fma3(u[1], u[1], u[2] * u[2]));
+        forced = Fs != nullptr;
+        F[0] = F[1] = F[2] = uF = 0.f;
         if (forced) {
-            Fx = TF(F[0]);
-            Fy = TF(F[1]);
-            if constexpr (L::D == 3) Fz = TF(F[2]);
-            uF = u[0] * Fx + u[1] * Fy + u[2] * Fz;
+            F[0] = TF(Fs[0]);
+            F[1] = TF(Fs[1]);
+            if constexpr (L::D == 3) F[2] = TF(Fs[2]);
+            uF = fma3(u[0], F[0], fma3(u[1], F[1], u[2] * F[2]));
         }
     }
+    // population V from its streamed value, with cu = c_V . u
     template <int V>
-    __device__ __forceinline__ void pair(TF fin_v, TF fin_o, TF &out_v, TF &out_o) const {
-        out_v = (*this)(V, fin_v);
-        out_o = (*this)(V + 1, fin_o);
-    }
-    __device__ __forceinline__ TF operator()(int v, TF fin) const {
-        using L = Lattice<Q>;
-        const float cx = float(L::c(v, 0)), cy = float(L::c(v, 1)), cz = float(L::c(v, 2));
-        const Split w = split_of(L::w(v));
-        const TF cu = cx * u[0] + cy * u[1] + cz * u[2];
-        const TF P = 3.0f * cu + 4.5f * cu * cu - usq15;
-        const TF feq = mul_split(w, drho + (rho0 * P + drho * P));
-        const TF dlt = feq - fin;
-        TF o = fmac(omega.hi, dlt, fmac(omega.lo, dlt, fin));
+    __device__ __forceinline__ TF one(TF fin, TF cu) const {
+        // P = (4.5 cu + 3) cu - 1.5 u.u
+        const TF P = fma3(fmac(4.5f, cu, TF(3.0f)), cu, nusq15);
+        const TF s = fma3(drho, P, drho + P);
+        constexpr Split w = split_of(Lattice<Q>::w(V));
+        const TF dlt = mul_split(w, s) - fin;
+        TF o = fmac(k.om.hi, dlt, fmac(k.om.lo, dlt, fin));
         if (forced) {
-            const TF cF = cx * Fx + cy * Fy + cz * Fz;
-            o += gpre * mul_split(w, 3.0f * (cF - uF) + 9.0f * cu * cF);
+            const TF cF = cdot<Q, V>(F);
+            o = fmac(k.gpre, mul_split(w, fma3(9.0f * cu, cF, 3.0f * (cF - uF))), o);
         }
         return o;
+    }
+    // V and its opposite V + 1 (c_{V+1} = -c_V): one c.u for both
+    template <int V>
+    __device__ __forceinline__ void pair(TF fin_v, TF fin_o, TF &out_v, TF &out_o) const {
+        const TF cu = cdot<Q, V>(u);
+        out_v = one<V>(fin_v, cu);
+        out_o = one<V + 1>(fin_o, -cu);
+    }
+    // single population (the rest population, and callers that loop over v)
+    template <int V>
+    __device__ __forceinline__ TF at(TF fin) const {
+        return one<V>(fin, cdot<Q, V>(u));
+    }
+    __device__ __forceinline__ TF operator()(int v, TF fin) const {
+        TF out = fin;
+        static_for<Q>([&](auto vc) {
+            constexpr int V = decltype(vc)::value;
+            if (v == V) out = at<V>(fin);
+        });
+        return out;
     }
 };
 
 template <int Q, class TF, class FT>
-__device__ __forceinline__ void collide_shift(const TF *fin, TF drho, float rho0,
-                                              const TF *u, const FT *F, Split omega,
+__device__ __forceinline__ void collide_shift(const TF *fin, TF drho, const TF *u, const FT *F, const ShiftK &k,
                                               TF *fout) {
-    const CollideShift<Q, TF> c(drho, rho0, u, F, omega);
-#pragma unroll
-    for (int v = 0; v < Q; ++v) fout[v] = c(v, fin[v]);
+    const CollideShift<Q, TF> c(drho, u, F, k);
+    static_for<Q>([&](auto vc) {
+        constexpr int V = decltype(vc)::value;
+        fout[V] = c.template at<V>(fin[V]);
+    });
 }
 
 // equilibrium of (rho, u) in the storage convention of P (absolute or shifted)
@@ -603,19 +660,17 @@ __device__ __forceinline__ auto make_collider(const typename P::T *fin, const ty
         using M = typename P::M;
         T drho, j[3], u[3];
         macro_shift<Q>(fin, F, a.fz_compat != 0, drho, j);
-        // u = j / (rho0 + drho) = (j - j*q)/rho0 with q = drho/(rho0 + drho): the rounding
-        // of rho0 + drho only reaches the small correction term
-        const float rho0 = float(a.rho0);
-        const T q = drho / (rho0 + drho);
-        const float r0inv = float(1.0 / a.rho0);
-        u[0] = (j[0] - j[0] * q) * r0inv;
-        u[1] = (j[1] - j[1] * q) * r0inv;
-        u[2] = (j[2] - j[2] * q) * r0inv;
-        rho_o = M(float(a.rho0) + drho);
+        // u = j / (1 + drho) = j - j q with q = drho / (1 + drho): the rounding of 1 + drho
+        // and of the reciprocal (one ulp) only reach the small correction term q
+        const T q = drho * recip(T(1.0f) + drho);
+        u[0] = fma3(-j[0], q, j[0]);
+        u[1] = fma3(-j[1], q, j[1]);
+        u[2] = fma3(-j[2], q, j[2]);
+        rho_o = M(1.0f + drho);
         u_o[0] = M(u[0]);
         u_o[1] = M(u[1]);
         u_o[2] = M(u[2]);
-        return CollideShift<Q, T>(drho, rho0, u, F, Split{a.om_hi, a.om_lo});
+        return CollideShift<Q, T>(drho, u, F, shift_k(a));
     }
 }
 
@@ -860,8 +915,8 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     const size_t cell = cell_of(x, y, z, nx, ny);
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
-    // sites LUMA's sweep 1 skips (:91-95); outlet sites are not read by any other site
-    if (ty == kSolid || ty == kRefined || ty == kExtrapRight || (!a.regularised && ty == kVelocity)) return;
+    // sites LUMA's sweep 1 skips (:91-95)
+    if (ty == kSolid || ty == kRefined || (!a.regularised && ty == kVelocity)) return;
     // edge and corner velocity sites and pressure sites run in k_edge_sites after this kernel
     if (ty == kPressure || (ty == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)) return;
     S *__restrict__ f = static_cast<S *>(a.f);
@@ -892,13 +947,30 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
             const CollideExact<Q, T> col{rr, {uu[0], uu[1], uu[2]}, omega_of<T>(a), T(a.cs2), T(a.den2), F};
             collide_pairs<Q>(col, fin, fout);
         } else {
-            const CollideShift<Q, T> col(rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo});
+            const CollideShift<Q, T> col(rr - 1.0f, uu, F, shift_k(a));
             collide_pairs<Q>(col, fin, fout);
         }
         rho = M(rr);
         u[0] = M(uu[0]);
         u[1] = M(uu[1]);
         u[2] = M(uu[2]);
+    } else if (ty == kExtrapRight) {
+        // outlet site: streamed (sources right of it are the x = 0 column, through the periodic
+        // wrap; sources on the outlet itself give the value two cells left, :246-251) and
+        // collided, but _LBM_macro_opt (:803-807) leaves its rho, u as they are: the collision
+        // uses the values its macro arrays hold (set at initialisation or by a restart)
+        rho = mv<M>(a.rho_out)[cell];
+        u[0] = mv<M>(a.u_out)[cell];
+        u[1] = mv<M>(a.u_out)[a.ncell + cell];
+        u[2] = (L::D == 3) ? mv<M>(a.u_out)[2 * a.ncell + cell] : M(0.0);
+        const T uu[3] = {T(u[0]), T(u[1]), T(u[2])};
+        if constexpr (!P::kShift) {
+            const CollideExact<Q, T> col{T(rho), {uu[0], uu[1], uu[2]}, omega_of<T>(a), T(a.cs2), T(a.den2), F};
+            collide_pairs<Q>(col, fin, fout);
+        } else {
+            const CollideShift<Q, T> col(T(rho - M(1.0)), uu, F, shift_k(a));
+            collide_pairs<Q>(col, fin, fout);
+        }
     } else {
         const auto col = make_collider<Q, P>(fin, F, a, rho, u);
         // TL2C sites are streamed but not collided (:144)
@@ -936,21 +1008,22 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     });
 }
 
-// Sweep-1 macroscopic values (stream + BC + macro, :72-107, force reset to zero) at a list of
-// sites, without touching the populations: the IB interpolation reads them.  Sites of other
+// Macroscopic values at a list of sites (every cell when cells is null), without touching the
+// populations, in the layout the next step kernel reads.  ib_force = 0: sweep 1's values
+// (stream + BC + macro, :72-107) with the force the step started from -- zero after an IB
+// reset, the field itself when the grid is not reset or gravity rewrote it; the IB
+// interpolation reads them.  ib_force = 1: ibm_updateMacroscopic (ObjectManager_ops_ibm.cpp
+// :696-766), _LBM_macro_opt again with the spread force at the flagged sites.  Sites of other
 // types keep the rho, u their arrays hold, as LUMA's arrays do.
 template <int Q, class P, bool ODD>
-__global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, const int *cells) {
+__device__ __forceinline__ void site_macro_one(const StepArgs &a, size_t cell, int ib_force) {
     using L = Lattice<Q>;
     using T = typename P::T;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nsites) return;
-    const size_t cell = (size_t)cells[i];
     const int x = (int)(cell % a.nx), y = (int)((cell / a.nx) % a.ny), z = (int)(cell / ((size_t)a.nx * a.ny));
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
     const bool macro_type = ty == kFluid || ty == kTL2F;
-    const bool vel_type = a.regularised && ty == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
+    const bool vel_type = !ib_force && a.regularised && ty == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
     if (!macro_type && !vel_type) return;
     Nbr nb;
     nb.init(x, y, z, a.nx, a.ny, a.nz);
@@ -966,11 +1039,12 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
         u[2] = typename P::M(uu[2]);
     } else {
         // sweep 1's macro adds F/2 of the force the step started from: zero after an IB reset,
-        // the gravity field when the reset writes gravity
+        // the gravity field when the reset writes gravity, the set field on a grid never reset
         using M = typename P::M;
         M Fv[3];
         const M *F = nullptr;
-        if (a.all_forced) {
+        const bool flagged = (fl & kForcedBit) != 0;
+        if (a.all_forced || (flagged && (ib_force || !a.sweep1_zero))) {
             const M *fx = mv<M>(a.force);
             Fv[0] = fx[cell];
             Fv[1] = fx[a.ncell + cell];
@@ -981,6 +1055,12 @@ __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, cons
         site_update<Q, P>(fin, fout, F, a, rho, u, false);
     }
     store_macro<L::D>(a, cell, rho, u);
+}
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, const int *cells, int ib_force) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsites) return;
+    site_macro_one<Q, P, ODD>(a, cells ? (size_t)cells[i] : (size_t)i, ib_force);
 }
 
 // Regularised sites that extrapolate from their inward neighbours: velocity sites on edges and
@@ -1082,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_edge_sites(StepArgs a, EdgeArgs e, int 
     if constexpr (!P::kShift) {
         collide_exact<Q, T>(fin, rr, uu, F, omega_of<T>(a), T(a.cs2), T(a.den2), fout);
     } else {
-        collide_shift<Q>(fin, rr - float(a.rho0), float(a.rho0), uu, F, Split{a.om_hi, a.om_lo}, fout);
+        collide_shift<Q>(fin, rr - 1.0f, uu, F, shift_k(a), fout);
     }
     if (store) {
         const M u[3] = {M(uu[0]), M(uu[1]), M(uu[2])};
@@ -1111,6 +1191,16 @@ __global__ void k_gravity_reset(StepArgs a, double *force, double gravity, doubl
     const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= a.ncell) return;
     mv<M>(force)[(size_t)dir * a.ncell + cell] = mv<M>(a.rho_out)[cell] * M(gravity) * M(rr);
+}
+
+// _LBM_resetForces (:1327-1337) without gravity: force_xyz = 0 at every site, and the force
+// bits (which make the step kernel read the field) cleared with it
+template <class M>
+__global__ void k_force_clear(StepArgs a, double *force, uint8_t *flags, int D) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.ncell) return;
+    for (int d = 0; d < D; ++d) mv<M>(force)[(size_t)d * a.ncell + cell] = M(0.0);
+    if (flags) flags[cell] &= (uint8_t)~kForcedBit;
 }
 
 // Side buffer for ExtrapolateRight sources (:246-251): xr[v][y, z] = f*_v(N-3, y, z), taken

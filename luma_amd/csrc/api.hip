@@ -172,14 +172,19 @@ void update_reynolds(lumagpu_grid *g) {
     g->d.omega = 1.0 / ((nu / (cs * cs)) + 0.5);
 }
 
-int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
-    if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
-    if (g->re_ramp > 0.0) update_reynolds(g);
-    // GridObj_ops_lbm_optimised.cpp:45-49: two sub-steps of the child first
+// The first half of one grid's LBM_multi_opt once its children have stepped: stream order
+// against the child and parent grids, the outlet side buffer, and _LBM_resetForces on a level
+// holding IB bodies (:53-57).  Sweep 1's stream, BC and macro happen in the step kernel, which
+// end_step launches; what the IB step reads of sweep 1 is recomputed at its supports.
+int begin_step(lumagpu_grid *g) {
+    // the outlet side buffer reads this grid's own populations only (the children read them
+    // too, but nobody writes them before this grid's step): it runs while the children step
+    if (g->has_outlet) {
+        hipError_t e = launch_extrap(g);
+        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
+    }
     if (g->child) {
         lumagpu_grid *c = g->child;
-        for (int sc = 0; sc < 2; ++sc)
-            if (int r = apply_step(c, sc, nullptr, nullptr)) return r;
         // the child reads this grid's populations, so order the two streams
         HIP_TRY(hipEventRecord(c->ev_done, c->stream));
         HIP_TRY(hipStreamWaitEvent(g->stream, c->ev_done, 0));
@@ -189,10 +194,6 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
         // this grid reads the parent's populations: wait for the parent's previous work
         HIP_TRY(hipEventRecord(g->parent->ev_ready, g->parent->stream));
         HIP_TRY(hipStreamWaitEvent(g->stream, g->parent->ev_ready, 0));
-    }
-    if (g->has_outlet) {
-        hipError_t e = launch_extrap(g);
-        if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
     }
     // _LBM_resetForces at the start of the step on a level holding IB bodies (:53-57); with
     // gravity it writes the gravity component from the previous step's rho, and every site
@@ -210,9 +211,12 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
             k_gravity_reset<Dual><<<nb, 256, 0, g->stream>>>(ga, g->force, g->gravity, rr, g->gravity_dir);
         HIP_TRY(hipGetLastError());
     }
-    // ObjectManager::ibm_apply between sweep 1 and sweep 2 (:120-121)
-    if (g->ibm)
-        if (int r = ibm_apply(g)) return r;
+    return 0;
+}
+
+// The second half: stream, BC, macro (with the IB force), Guo forcing and BGK collision in one
+// pass, the edge/corner and pressure sites around it, then the swap (AA parity) and ++t.
+int end_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0};
     StepArgs a = make_args(g);
     a.subcycle = subcycle;
@@ -240,6 +244,21 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     g->parity ^= 1;
     g->t += 1;
     return 0;
+}
+
+int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
+    if (g->half) return fail(LUMAGPU_ESTATE, "a split step is open on this grid: finish it with lumagpu_force_collide_swap");
+    if (g->re_ramp > 0.0) update_reynolds(g);
+    // GridObj_ops_lbm_optimised.cpp:45-49: two sub-steps of the child first
+    if (g->child)
+        for (int sc = 0; sc < 2; ++sc)
+            if (int r = apply_step(g->child, sc, nullptr, nullptr)) return r;
+    if (int r = begin_step(g)) return r;
+    // ObjectManager::ibm_apply between sweep 1 and sweep 2 (:120-121)
+    if (g->ibm)
+        if (int r = ibm_apply(g)) return r;
+    return end_step(g, subcycle, e0, e1);
 }
 
 }  // namespace
@@ -405,6 +424,11 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     if (!(d.omega > 0.0) || d.omega >= 2.0)
         return fail(LUMAGPU_EPHYS, "LBM relaxation frequency omega too large (must be 0 < omega < 2)");
     if (d.level < 0) return fail(LUMAGPU_EINVAL, "negative refinement level");
+    // the shifted forms store f - w with the reference density 1 (their collision is written
+    // for it); the plain forms do not use rho0
+    if ((d.precision == LUMAGPU_FP32S || d.precision == LUMAGPU_FP16S || d.precision == LUMAGPU_FP32T) &&
+        d.rho0 != 1.0)
+        return fail(LUMAGPU_EINVAL, "the shifted forms use the reference density rho0 = 1");
 
     lumagpu_grid *g = new lumagpu_grid();
     g->d = d;
@@ -965,9 +989,60 @@ int lumagpu_step(lumagpu_grid *g, int subcycle) {
     return apply_step(g, subcycle, nullptr, nullptr);
 }
 
+int lumagpu_stream_bc_macro(lumagpu_grid *g, int subcycle) {
+    if (int r = check_grid(g)) return r;
+    if (subcycle < 0 || subcycle > 1) return fail(LUMAGPU_EINVAL, "subcycle must be 0 or 1");
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
+    if (g->half) return fail(LUMAGPU_ESTATE, "a split step is already open on this grid");
+    if (g->re_ramp > 0.0) update_reynolds(g);
+    if (int r = begin_step(g)) return r;
+    // sweep 1's rho, u at every fluid, TL2F and regularised-face site, as LUMA's arrays hold them
+    // between the sweeps (edge/corner and pressure sites get theirs in the second half)
+    const StepArgs a = make_args(g);
+    hipError_t e = launch_sites(g, a, g->parity == 0, (int)g->ncell, nullptr, 0);
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("macro launch: ") + hipGetErrorString(e));
+    g->half = 1;
+    g->half_sub = subcycle;
+    return 0;
+}
+
+int lumagpu_force_collide_swap(lumagpu_grid *g) {
+    if (int r = check_grid(g)) return r;
+    if (!g->half) return fail(LUMAGPU_ESTATE, "no split step open: call lumagpu_stream_bc_macro first");
+    g->half = 0;
+    return end_step(g, g->half_sub, nullptr, nullptr);
+}
+
+int lumagpu_reset_forces(lumagpu_grid *g) {
+    if (int r = check_grid(g)) return r;
+    if (!g->d.forced) return fail(LUMAGPU_ESTATE, "grid was created without forcing (desc.forced = 0)");
+    const StepArgs a = make_args(g);
+    const unsigned nb = (unsigned)((g->ncell + 255) / 256);
+    if (g->gravity_on) {
+        // _LBM_resetForces (:1327-1337) with L_GRAVITY_ON: the gravity component from rho
+        const double rr = 1.0 / std::pow(2.0, (double)g->d.level);
+        if (g->melem == sizeof(double))
+            k_gravity_reset<double><<<nb, 256, 0, g->stream>>>(a, g->force, g->gravity, rr, g->gravity_dir);
+        else
+            k_gravity_reset<Dual><<<nb, 256, 0, g->stream>>>(a, g->force, g->gravity, rr, g->gravity_dir);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    // every component of every site zeroed, the force bits with it
+    if (g->melem == sizeof(double))
+        k_force_clear<double><<<nb, 256, 0, g->stream>>>(a, g->force, g->flags, g->D);
+    else
+        k_force_clear<Dual><<<nb, 256, 0, g->stream>>>(a, g->force, g->flags, g->D);
+    HIP_TRY(hipGetLastError());
+    for (auto &f : g->hflags) f &= (uint8_t)~kForcedBit;
+    ibm_forget_flags(g);
+    return 0;
+}
+
 int lumagpu_run(lumagpu_grid *g, int nsteps) {
     if (int r = check_grid(g)) return r;
     if (nsteps < 0) return fail(LUMAGPU_EINVAL, "negative step count");
+    if (g->half) return fail(LUMAGPU_ESTATE, "a split step is open on this grid: finish it with lumagpu_force_collide_swap");
     int s = 0;
     if (g->fuse_shape >= 0) {
         if (g->parity == 0 && nsteps >= 3) {

@@ -112,6 +112,19 @@ __device__ inline DualF fmac(float a, const DualF &x, const DualF &y) {
     return DualF(fmaf(a, x.v, y.v), fmaf(a, x.d, y.d));
 }
 
+// a b + c with one rounding of the value (fmaf); the derivative a' b + a b' + c'
+__device__ inline float fma3(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline DualF fma3(const DualF &a, const DualF &b, const DualF &c) {
+    return DualF(fmaf(a.v, b.v, c.v), fmaf(a.d, b.v, fmaf(a.v, b.d, c.d)));
+}
+
+// reciprocal, correctly rounded to within one ulp (v_rcp_f32); the derivative -x'/x^2
+__device__ inline float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline DualF recip(const DualF &x) {
+    const float r = __builtin_amdgcn_rcpf(x.v);
+    return DualF(r, -x.d * r * r);
+}
+
 // value part of every number type
 __host__ __device__ inline double value_of(double x) { return x; }
 __host__ __device__ inline double value_of(const Dual &x) { return x.v; }

@@ -46,16 +46,36 @@ __device__ __forceinline__ int fused_tile_of(int b, int nblocks) {
     return (b % 8) * (nblocks / 8) + b / 8;
 }
 
-template <class P, int TX, int TY, bool STORE, bool DB, bool O32 = false>
+// The 16 populations with a component in x or y, which the odd step takes from other columns
+// (the other three, c = (0, 0, +-1) and the rest population, come from the thread's own column).
+template <int V>
+constexpr int lateral_index() {
+    using L = Lattice<19>;
+    int k = 0;
+    for (int w = 0; w < V; ++w)
+        if (L::c(w, 0) != 0 || L::c(w, 1) != 0) ++k;
+    return (L::c(V, 0) != 0 || L::c(V, 1) != 0) ? k : -1;
+}
+
+// PF: the loads of plane p + 2 are issued at the start of plane p's iteration, into a second
+// register set, so they arrive while plane p + 1's even step and plane p's odd step compute
+// (one plane of load latency hidden per iteration instead of none).
+// WIN: the LDS copy holds, per lateral population v, only the TX x TY window the tile reads
+// (the value cell x + c_v of the tile pulls, stored at that cell's index): 16 * TX * TY values
+// instead of 19 * (TX + 2)(TY + 2), so two copies of a 32 x 16 tile take 64 KiB and two blocks
+// fit a CU; the odd step's LDS reads are then contiguous across the wave.
+template <class P, int TX, int TY, bool STORE, bool DB, bool O32 = false, bool PF = false, bool WIN = false>
 __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, void *dst_v, int zc) {
     using L = Lattice<19>;
     using S = typename P::S;
     using T = typename P::T;
     constexpr int Q = 19;
     constexpr int WX = FuseTile<TX, TY>::WX, C = FuseTile<TX, TY>::C;
+    constexpr int TT = TX * TY;
+    static_assert(!WIN || DB, "windowed LDS copies are double-buffered");
     // DB: two LDS copies used alternately, so a plane needs one barrier (between publishing and
     // reading) instead of two
-    __shared__ S shm[(DB ? 2 : 1) * Q * C];
+    __shared__ S shm[WIN ? 2 * 16 * TT : (DB ? 2 : 1) * Q * C];
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int ntx = nx / TX, nty = ny / TY, ntz = (nz + zc - 1) / zc;
@@ -92,17 +112,25 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
     const size_t col = (size_t)gx + (size_t)nx * (size_t)gy;
     const size_t plane = (size_t)nx * (size_t)ny;
 
-    // even step of plane z (own cell): post-collision populations in storage precision
-    auto even = [&](int z, S *e) {
+    // the 19 populations of plane z of this thread's column, as stored
+    auto load = [&](int z, S *raw) {
         const size_t cell = col + plane * (size_t)z;
-        T fin[Q];
         static_for<Q>([&](auto vc) {
             constexpr int v = decltype(vc)::value;
             if constexpr (O32)
-                fin[v] = ld<P, false>(reinterpret_cast<const S *>(
-                    reinterpret_cast<const char *>(src + (size_t)v * ps) + (uint32_t)((uint32_t)cell * (uint32_t)sizeof(S))));
+                raw[v] = *reinterpret_cast<const S *>(reinterpret_cast<const char *>(src + (size_t)v * ps) +
+                                                      (uint32_t)((uint32_t)cell * (uint32_t)sizeof(S)));
             else
-                fin[v] = ld<P, false>(src + (size_t)v * ps + cell);
+                raw[v] = src[(size_t)v * ps + cell];
+        });
+    };
+    // even step of a column cell from its loaded populations: post-collision populations in
+    // storage precision
+    auto even_from = [&](const S *raw, S *e) {
+        T fin[Q];
+        static_for<Q>([&](auto vc) {
+            constexpr int v = decltype(vc)::value;
+            fin[v] = static_cast<T>(raw[v]);
         });
         typename P::M rho, u[3];
         const auto c = make_collider<Q, P>(fin, nullptr, a, rho, u);
@@ -114,6 +142,11 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
             e[v + 1] = static_cast<S>(oo);
         });
         e[Q - 1] = static_cast<S>(c(Q - 1, fin[Q - 1]));
+    };
+    auto even = [&](int z, S *e) {
+        S raw[Q];
+        load(z, raw);
+        even_from(raw, e);
     };
     auto wrapz = [&](int z) { return z < 0 ? z + nz : (z >= nz ? z - nz : z); };
 
@@ -135,23 +168,52 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
         });
     }
 
-    for (int p = z0; p < z1; ++p) {
+    // one plane: the even step of plane p + 1 (its populations loaded into `cur`, or loaded
+    // here), publication, the odd step of plane p.  With PF the loads of plane p + 2 go into
+    // `nxt` first; the caller alternates two buffers, so no copy ties the loads to the loop end.
+    auto plane_step = [&](int p, const S *cur, S *nxt) {
         S e[Q];
-        even(wrapz(p + 1), e);
-        S *sh = DB ? shm + (p & 1) * (Q * C) : shm;
+        if constexpr (PF) {
+            // unconditionally (the last plane's load is spare): a branch here would make the
+            // wait for `cur` also wait for these loads
+            load(wrapz(p + 2), nxt);
+            // keep the loads ahead of the plane's work (the scheduler would otherwise sink them
+            // towards their use in the next plane)
+            __builtin_amdgcn_sched_barrier(0);
+            even_from(cur, e);
+        } else {
+            even(wrapz(p + 1), e);
+        }
+        S *sh = WIN ? shm + (p & 1) * (16 * TT) : (DB ? shm + (p & 1) * (Q * C) : shm);
         if constexpr (!DB) __syncthreads();   // the previous plane's odd step has read the LDS copy
+        // the own-column populations of the odd step (WIN): c_z = +1 of plane p - 1, c_z = -1 of
+        // plane p + 1, the rest population of plane p
+        S own4 = S(0), own5 = S(0), own18 = S(0);
+        if constexpr (WIN) {
+            own4 = rup_prev[4];
+            own5 = e[5];
+            own18 = rmid[18];
+        }
         static_for<Q>([&](auto vc) {
             constexpr int v = decltype(vc)::value;
             constexpr int cz = L::c(v, 2);
-            if constexpr (cz == -1) sh[v * C + lc] = e[v];
+            S val;
+            if constexpr (cz == -1) val = e[v];
             if constexpr (cz == 0) {
-                sh[v * C + lc] = rmid[v];
+                val = rmid[v];
                 rmid[v] = e[v];
             }
             if constexpr (cz == 1) {
-                sh[v * C + lc] = rup_prev[v];
+                val = rup_prev[v];
                 rup_prev[v] = rup[v];
                 rup[v] = e[v];
+            }
+            if constexpr (!WIN) {
+                sh[v * C + lc] = val;
+            } else if constexpr (lateral_index<v>() >= 0) {
+                // the tile cell that pulls this population from this cell
+                const int ix = lx - 1 + L::c(v, 0), iy = ly - 1 + L::c(v, 1);
+                if (ix >= 0 && ix < TX && iy >= 0 && iy < TY) sh[lateral_index<v>() * TT + iy * TX + ix] = val;
             }
         });
         __syncthreads();
@@ -160,7 +222,13 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
             static_for<Q>([&](auto vc) {
                 constexpr int v = decltype(vc)::value;
                 constexpr int cx = L::c(v, 0), cy = L::c(v, 1);
-                fin[v] = static_cast<T>(sh[v * C + lc - cy * WX - cx]);
+                if constexpr (!WIN) {
+                    fin[v] = static_cast<T>(sh[v * C + lc - cy * WX - cx]);
+                } else if constexpr (lateral_index<v>() >= 0) {
+                    fin[v] = static_cast<T>(sh[lateral_index<v>() * TT + (ly - 1) * TX + (lx - 1)]);
+                } else {
+                    fin[v] = static_cast<T>(v == 4 ? own4 : (v == 5 ? own5 : own18));
+                }
             });
             typename P::M rho, u[3];
             const auto c = make_collider<Q, P>(fin, nullptr, a, rho, u);
@@ -189,16 +257,29 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
             });
             put(std::integral_constant<int, Q - 1>{}, c(Q - 1, fin[Q - 1]));
         }
+    };
+    if constexpr (PF) {
+        S bufA[Q], bufB[Q];
+        load(wrapz(z0 + 1), bufA);
+        for (int p = z0; p < z1; p += 2) {
+            plane_step(p, bufA, bufB);
+            if (p + 1 < z1) plane_step(p + 1, bufB, bufA);
+        }
+    } else {
+        for (int p = z0; p < z1; ++p) plane_step(p, nullptr, nullptr);
     }
 }
 
-// Tile shapes the launcher offers; the grid must be a whole number of tiles in x and y.
+// Tile shapes the launcher offers; the grid must be a whole number of tiles in x and y.  Shape
+// codes: the tile (0-2), + 8 for two LDS copies, + 16 (with two copies) for loads one plane ahead.
 enum FusedShape : int { kFuse64x8 = 0, kFuse32x16 = 1, kFuse128x4 = 2 };
 
 inline bool fused_tile(int shape, int &tx, int &ty) {
     static const int t[3][2] = {{64, 8}, {32, 16}, {128, 4}};
     const int k = shape & 7;
-    if (shape < 0 || shape > 15 || k > 2 || (shape > 2 && shape < 8)) return false;
+    // tile (0-2), + 8 two LDS copies, + 16 loads one plane ahead, + 32 windowed copies; the
+    // last two need the two copies
+    if (shape < 0 || shape > 63 || k > 2 || (shape & 4) || ((shape & 48) && !(shape & 8))) return false;
     tx = t[k][0];
     ty = t[k][1];
     return true;
@@ -210,7 +291,7 @@ inline bool fused_shape_fits(int shape, int nx, int ny) {
     return nx % tx == 0 && ny % ty == 0 && nx >= tx && ny >= ty;
 }
 
-template <class P, int TX, int TY, bool DB>
+template <class P, int TX, int TY, bool DB, bool PF = false, bool WIN = false>
 hipError_t launch_fused2_t(const StepArgs &a, void *dst, int zc, bool store, hipStream_t s) {
     const int nblocks = (a.nx / TX) * (a.ny / TY) * ((a.nz + zc - 1) / zc);
     const dim3 g(nblocks), b(FuseTile<TX, TY>::C);
@@ -218,12 +299,12 @@ hipError_t launch_fused2_t(const StepArgs &a, void *dst, int zc, bool store, hip
     // (measured at 256^3 FP32, 64x8: 302 us per step against 318 with 64-bit addresses)
     const bool o32 = a.ps * sizeof(typename P::S) < (size_t(1) << 32);
     if (o32) {
-        if (store) k_aa_fused2<P, TX, TY, true, DB, true><<<g, b, 0, s>>>(a, dst, zc);
-        else k_aa_fused2<P, TX, TY, false, DB, true><<<g, b, 0, s>>>(a, dst, zc);
+        if (store) k_aa_fused2<P, TX, TY, true, DB, true, PF, WIN><<<g, b, 0, s>>>(a, dst, zc);
+        else k_aa_fused2<P, TX, TY, false, DB, true, PF, WIN><<<g, b, 0, s>>>(a, dst, zc);
         return hipGetLastError();
     }
-    if (store) k_aa_fused2<P, TX, TY, true, DB><<<g, b, 0, s>>>(a, dst, zc);
-    else k_aa_fused2<P, TX, TY, false, DB><<<g, b, 0, s>>>(a, dst, zc);
+    if (store) k_aa_fused2<P, TX, TY, true, DB, false, PF, WIN><<<g, b, 0, s>>>(a, dst, zc);
+    else k_aa_fused2<P, TX, TY, false, DB, false, PF, WIN><<<g, b, 0, s>>>(a, dst, zc);
     return hipGetLastError();
 }
 
@@ -241,6 +322,11 @@ hipError_t launch_fused2_p(int shape, const StepArgs &a, void *dst, int zc, bool
         if (shape == 8 + kFuse64x8) return launch_fused2_t<P, 64, 8, true>(a, dst, zc, store, s);
         if (shape == 8 + kFuse32x16) return launch_fused2_t<P, 32, 16, true>(a, dst, zc, store, s);
         if (shape == 8 + kFuse128x4) return launch_fused2_t<P, 128, 4, true>(a, dst, zc, store, s);
+        if (shape == 24 + kFuse64x8) return launch_fused2_t<P, 64, 8, true, true>(a, dst, zc, store, s);
+        if (shape == 24 + kFuse128x4) return launch_fused2_t<P, 128, 4, true, true>(a, dst, zc, store, s);
+        if (shape == 40 + kFuse64x8) return launch_fused2_t<P, 64, 8, true, false, true>(a, dst, zc, store, s);
+        if (shape == 40 + kFuse32x16) return launch_fused2_t<P, 32, 16, true, false, true>(a, dst, zc, store, s);
+        if (shape == 56 + kFuse32x16) return launch_fused2_t<P, 32, 16, true, true, true>(a, dst, zc, store, s);
     }
     return hipErrorInvalidValue;
 }

@@ -412,15 +412,19 @@ int upload(lumagpu_grid *g) {
     return 0;
 }
 
-hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int n, const int *cells) {
+}  // namespace
+
+hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int n, const int *cells, int ib_force) {
     switch (g->d.precision) {
-        case LUMAGPU_FP64: return launch_sites_f64(g->Q, a, odd, n, cells, g->stream);
-        case LUMAGPU_FP32S: return launch_sites_f32s(g->Q, a, odd, n, cells, g->stream);
-        case LUMAGPU_FP64T: return launch_sites_dual(g->Q, a, odd, n, cells, g->stream);
-        case LUMAGPU_FP32T: return launch_sites_dual32(g->Q, a, odd, n, cells, g->stream);
-        default: return launch_sites_f16s(g->Q, a, odd, n, cells, g->stream);
+        case LUMAGPU_FP64: return launch_sites_f64(g->Q, a, odd, n, cells, ib_force, g->stream);
+        case LUMAGPU_FP32S: return launch_sites_f32s(g->Q, a, odd, n, cells, ib_force, g->stream);
+        case LUMAGPU_FP64T: return launch_sites_dual(g->Q, a, odd, n, cells, ib_force, g->stream);
+        case LUMAGPU_FP32T: return launch_sites_dual32(g->Q, a, odd, n, cells, ib_force, g->stream);
+        default: return launch_sites_f16s(g->Q, a, odd, n, cells, ib_force, g->stream);
     }
 }
+
+namespace {
 
 IbmSystem *system_of(lumagpu_grid *g) {
     if (!g->ibm) {
@@ -445,6 +449,12 @@ int ensure_flags(lumagpu_grid *g) {
     return 0;
 }
 
+void ibm_forget_flags(lumagpu_grid *g) {
+    if (!g->ibm) return;
+    g->ibm->flagged.clear();
+    g->ibm->sites_dirty = 1;
+}
+
 int ibm_marker_count(const lumagpu_grid *g) {
     if (!g->ibm) return 0;
     int n = 0;
@@ -464,11 +474,11 @@ void ibm_destroy(lumagpu_grid *g) {
     g->ibm = nullptr;
 }
 
-// Interpolate, force and spread at the current supports.  Sites flagged by the previous
-// application and no longer supports are reset first (_LBM_resetForces before every
-// application, ObjectManager_ops_ibm.cpp:120-128 and GridObj_ops_lbm_optimised.cpp:55-58);
-// current supports are overwritten by the spread, which starts from zero.
-int apply_core(lumagpu_grid *g) {
+// Sites flagged by the previous application and no longer supports are reset
+// (_LBM_resetForces before every application, ObjectManager_ops_ibm.cpp:120-128 and
+// GridObj_ops_lbm_optimised.cpp:55-58); current supports are overwritten by the spread, which
+// starts from zero.
+int clear_gone(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
     // a gravity reset has rewritten every site's force: nothing to clear
     if (s->sites_dirty && !s->flagged.empty() && !g->all_forced) {
@@ -491,20 +501,74 @@ int apply_core(lumagpu_grid *g) {
             for (int c : gone) g->hflags[c] &= (uint8_t)~kForcedBit;
         }
     }
+    return 0;
+}
+
+// ibm_interpolate (:498-591) from the sweep-1 rho, u of the supports, recomputed here from the
+// populations (LUMA's u = u_n reset before each sub-iteration); mode 0 also computes the
+// marker forces (ibm_computeForce, :598-614) in the same pass
+int interpolate(lumagpu_grid *g, int mode) {
+    IbmSystem *s = g->ibm;
     StepArgs a = make_args(g);
-    hipError_t e = launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites);
+    hipError_t e = launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites, 0);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB site macro launch: ") + hipGetErrorString(e));
     IbmArgs b = ibm_args(g);
-    if (tangent_form(g)) {
-        k_ibm_interp_force<Dual><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b);
-        HIP_TRY(hipGetLastError());
-        k_ibm_spread<Dual><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
-    } else {
-        k_ibm_interp_force<double><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b);
-        HIP_TRY(hipGetLastError());
-        k_ibm_spread<double><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
-    }
+    if (tangent_form(g)) k_ibm_interp_force<Dual><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b, mode);
+    else k_ibm_interp_force<double><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b, mode);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ibm_computeForce (:598-614) from the values of the last interpolation
+int compute_force(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    IbmArgs b = ibm_args(g);
+    if (tangent_form(g)) k_ibm_interp_force<Dual><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b, 2);
+    else k_ibm_interp_force<double><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b, 2);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ibm_spread (:621-689): gone supports cleared, then every support site gathers its entries
+int spread(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    if (int r = clear_gone(g)) return r;
+    IbmArgs b = ibm_args(g);
+    if (tangent_form(g)) k_ibm_spread<Dual><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
+    else k_ibm_spread<double><<<(s->nsites + 127) / 128, 128, 0, g->stream>>>(b);
+    HIP_TRY(hipGetLastError());
+    if (s->sites_dirty) {
+        for (int c : s->sites) g->hflags[c] |= kForcedBit;
+        s->flagged = s->sites;
+        s->sites_dirty = 0;
+    }
+    return 0;
+}
+
+hipError_t launch_ibm_step(const lumagpu_grid *g, const StepArgs &a, const IbmArgs &b) {
+    const bool odd = g->parity == 0;
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_ibm_step_f64(g->Q, a, b, odd, g->stream);
+        case LUMAGPU_FP32S: return launch_ibm_step_f32s(g->Q, a, b, odd, g->stream);
+        case LUMAGPU_FP64T: return launch_ibm_step_dual(g->Q, a, b, odd, g->stream);
+        case LUMAGPU_FP32T: return launch_ibm_step_dual32(g->Q, a, b, odd, g->stream);
+        default: return launch_ibm_step_f16s(g->Q, a, b, odd, g->stream);
+    }
+}
+
+// Interpolate, force and spread at the current supports: one block for small body systems,
+// otherwise one kernel per stage.
+int apply_core(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    if (s->nsites > kIbmStepMaxSites || s->nm > kIbmStepMaxMarkers) {
+        if (int r = interpolate(g, 0)) return r;
+        return spread(g);
+    }
+    if (int r = clear_gone(g)) return r;
+    const StepArgs a = make_args(g);
+    const IbmArgs b = ibm_args(g);
+    hipError_t e = launch_ibm_step(g, a, b);
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB step launch: ") + hipGetErrorString(e));
     if (s->sites_dirty) {
         for (int c : s->sites) g->hflags[c] |= kForcedBit;
         s->flagged = s->sites;
@@ -719,7 +783,50 @@ int body_force(lumagpu_grid *g, int body, double *out, double *dout) {
 
 using namespace lumagpu;
 
+namespace {
+// The IB stages run between the halves of a split step, on bodies that ibm_initialise has set
+// up; 1 = no markers on the grid (nothing to do), 0 = ready, < 0 = error
+int stage_ready(lumagpu_grid *g, const char *what) {
+    if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");
+    if (!g->half)
+        return fail(LUMAGPU_ESTATE, std::string(what) + " runs between lumagpu_stream_bc_macro and "
+                                                        "lumagpu_force_collide_swap");
+    if (!g->ibm || g->ibm->nm == 0) return 1;
+    if (!g->ibm->initialised) return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
+    return 0;
+}
+}  // namespace
+
 extern "C" {
+
+int lumagpu_ibm_apply(lumagpu_grid *g) {
+    const int r = stage_ready(g, "lumagpu_ibm_apply");
+    return r == 1 ? 0 : (r ? r : ibm_apply(g));
+}
+
+int lumagpu_ibm_interpolate(lumagpu_grid *g) {
+    const int r = stage_ready(g, "lumagpu_ibm_interpolate");
+    return r == 1 ? 0 : (r ? r : interpolate(g, 1));
+}
+
+int lumagpu_ibm_compute_force(lumagpu_grid *g) {
+    const int r = stage_ready(g, "lumagpu_ibm_compute_force");
+    return r == 1 ? 0 : (r ? r : compute_force(g));
+}
+
+int lumagpu_ibm_spread(lumagpu_grid *g) {
+    const int r = stage_ready(g, "lumagpu_ibm_spread");
+    return r == 1 ? 0 : (r ? r : spread(g));
+}
+
+int lumagpu_ibm_update_macro(lumagpu_grid *g) {
+    const int r = stage_ready(g, "lumagpu_ibm_update_macro");
+    if (r) return r == 1 ? 0 : r;
+    IbmSystem *s = g->ibm;
+    const StepArgs a = make_args(g);
+    HIP_TRY(launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites, 1));
+    return 0;
+}
 
 int lumagpu_set_positions(lumagpu_grid *g, const double *x, const double *y, const double *z) {
     if (!g) return fail(LUMAGPU_EINVAL, "null grid handle");

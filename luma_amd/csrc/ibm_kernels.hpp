@@ -53,34 +53,43 @@ __device__ __forceinline__ const M *as(const A *p) {
 }
 
 // ibm_interpolate (:498-591) and ibm_computeForce (:598-614); one thread per marker, sums in
-// support order.  local_area = 1 (IBMarker.cpp:61).
+// support order.  local_area = 1 (IBMarker.cpp:61).  mode 0: both; 1: interpolation only;
+// 2: the force from the interpolated values held since the last interpolation.
 template <class M = double>
-__global__ void k_ibm_interp_force(IbmArgs b) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= b.nm) return;
+__device__ __forceinline__ void interp_force_one(const IbmArgs &b, int m, int mode) {
     const M *sdelta = as<M>(b.sdelta), *grho = as<M>(b.rho), *gu = as<M>(b.u), *vel = as<M>(b.vel);
+    M *interp = as<M>(b.interp), *force = as<M>(b.force);
     const M area = 1.0;
     M rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < b.ns[m]; ++k) {
-        const size_t cell = (size_t)b.scell[m * b.S + k];
-        const M dl = sdelta[m * b.S + k];
-        const M r = grho[cell];
-        rho += r * dl * area;
-        for (int d = 0; d < b.D; ++d) mom[d] += r * gu[d * b.ncell + cell] * dl * area;
+    if (mode != 2) {
+        for (int k = 0; k < b.ns[m]; ++k) {
+            const size_t cell = (size_t)b.scell[m * b.S + k];
+            const M dl = sdelta[m * b.S + k];
+            const M r = grho[cell];
+            rho += r * dl * area;
+            for (int d = 0; d < b.D; ++d) mom[d] += r * gu[d * b.ncell + cell] * dl * area;
+        }
+        interp[4 * m] = rho;
+        for (int d = 0; d < 3; ++d) interp[4 * m + 1 + d] = mom[d];
+    } else {
+        rho = interp[4 * m];
+        for (int d = 0; d < 3; ++d) mom[d] = interp[4 * m + 1 + d];
     }
-    M *interp = as<M>(b.interp), *force = as<M>(b.force);
-    interp[4 * m] = rho;
-    for (int d = 0; d < 3; ++d) interp[4 * m + 1 + d] = mom[d];
-    for (int d = 0; d < b.D; ++d) force[3 * m + d] = M(2.0) * (mom[d] - rho * vel[3 * m + d]) / M(1.0);
+    if (mode != 1)
+        for (int d = 0; d < b.D; ++d) force[3 * m + d] = M(2.0) * (mom[d] - rho * vel[3 * m + d]) / M(1.0);
+}
+template <class M = double>
+__global__ void k_ibm_interp_force(IbmArgs b, int mode) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= b.nm) return;
+    interp_force_one<M>(b, m, mode);
 }
 
 // ibm_spread (:621-689) as a per-site gather; the site's force starts from the reset value 0
 // (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
 // step kernel adds it in the macro (+F/2) and the Guo term.
 template <class M = double>
-__global__ void k_ibm_spread(IbmArgs b) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= b.nsites) return;
+__device__ __forceinline__ void spread_one(const IbmArgs &b, int s) {
     const M *sdelta = as<M>(b.sdelta), *force = as<M>(b.force), *eps = as<M>(b.eps), *ds = as<M>(b.ds);
     const size_t cell = (size_t)b.sites[s];
     M *fxyz = as<M>(b.fxyz);
@@ -97,6 +106,12 @@ __global__ void k_ibm_spread(IbmArgs b) {
     }
     for (int d = 0; d < b.D; ++d) fxyz[d * b.ncell + cell] = acc[d];
     b.flags[cell] |= kForcedBit;
+}
+template <class M = double>
+__global__ void k_ibm_spread(IbmArgs b) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= b.nsites) return;
+    spread_one<M>(b, s);
 }
 
 // Clear the force and flag of sites that stop being supports.

@@ -51,6 +51,8 @@ struct lumagpu_grid {
     int masked = 0;
     int zslab = 0;                    // planes 0 and K-1 are halo planes of a z-sharded box
     int has_state = 0;
+    int half = 0;                     // between lumagpu_stream_bc_macro and lumagpu_force_collide_swap
+    int half_sub = 0;                 // the subcycle of that split step
     int store_macro = 0;
     int parity = 0;
     int t = 0;
@@ -147,6 +149,8 @@ inline StepArgs make_args(const lumagpu_grid *g) {
 int ibm_apply(lumagpu_grid *g);
 void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
+// after a whole-field force reset: no site holds an IB force, the next spread flags its sites
+void ibm_forget_flags(lumagpu_grid *g);
 // the grid carries a flexible (FEM) body, whose step runs host work
 bool ibm_has_flexible(const lumagpu_grid *g);
 // markers of all IB bodies on the grid
@@ -155,5 +159,7 @@ int ibm_marker_count(const lumagpu_grid *g);
 int step_grid(lumagpu_grid *g, int subcycle);
 // ExtrapolateRight side buffer from the grid's current populations; defined in api.hip
 hipError_t outlet_prepass(const lumagpu_grid *g);
+// k_site_macro at a list of cells (every cell when cells is null); defined in ibm.hip
+hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int n, const int *cells, int ib_force);
 
 }  // namespace lumagpu

@@ -8,6 +8,7 @@
 
 #include "aa_kernels.hpp"
 #include "fused_kernels.hpp"
+#include "ibm_kernels.hpp"
 
 namespace lumagpu {
 
@@ -41,23 +42,58 @@ hipError_t launch_aux_dual(int op, int Q, const StepArgs &a, const double *p0, c
 hipError_t launch_aux_dual32(int op, int Q, const StepArgs &a, const double *p0, const double *p1,
                              double *out, int parity, int masked, hipStream_t s);
 
-// k_site_macro over a list of cells (IB support sites), in the layout the next step reads
-hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
-hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
-hipError_t launch_sites_f16s(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
-hipError_t launch_sites_dual(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
-hipError_t launch_sites_dual32(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s);
+// k_site_macro over a list of cells (IB support sites; every cell when cells is null), in the
+// layout the next step reads
+hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s);
+hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s);
+hipError_t launch_sites_f16s(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s);
+hipError_t launch_sites_dual(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s);
+hipError_t launch_sites_dual32(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s);
 
 template <class P>
-hipError_t launch_sites_p(int Q, const StepArgs &a, bool odd, int n, const int *cells, hipStream_t s) {
+hipError_t launch_sites_p(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const dim3 b(128), g((n + 127) / 128);
     if (Q == 9) {
-        if (odd) k_site_macro<9, P, true><<<g, b, 0, s>>>(a, n, cells);
-        else k_site_macro<9, P, false><<<g, b, 0, s>>>(a, n, cells);
+        if (odd) k_site_macro<9, P, true><<<g, b, 0, s>>>(a, n, cells, ib_force);
+        else k_site_macro<9, P, false><<<g, b, 0, s>>>(a, n, cells, ib_force);
     } else {
-        if (odd) k_site_macro<19, P, true><<<g, b, 0, s>>>(a, n, cells);
-        else k_site_macro<19, P, false><<<g, b, 0, s>>>(a, n, cells);
+        if (odd) k_site_macro<19, P, true><<<g, b, 0, s>>>(a, n, cells, ib_force);
+        else k_site_macro<19, P, false><<<g, b, 0, s>>>(a, n, cells, ib_force);
+    }
+    return hipGetLastError();
+}
+
+// One IB application of a small body system in one block (the site macro at the supports,
+// interpolation and force per marker, spread per support site, with block barriers between the
+// phases): the same per-thread work as the three kernels it replaces, one launch instead of
+// three on the launch-bound refined-cylinder and filament runs.
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(1024) void k_ibm_step(StepArgs a, IbmArgs b) {
+    using M = typename P::M;
+    for (int i = threadIdx.x; i < b.nsites; i += blockDim.x) site_macro_one<Q, P, ODD>(a, (size_t)b.sites[i], 0);
+    __syncthreads();
+    for (int m = threadIdx.x; m < b.nm; m += blockDim.x) interp_force_one<M>(b, m, 0);
+    __syncthreads();
+    for (int s = threadIdx.x; s < b.nsites; s += blockDim.x) spread_one<M>(b, s);
+}
+// body systems up to these sizes take the one-block path
+constexpr int kIbmStepMaxSites = 16384, kIbmStepMaxMarkers = 4096;
+
+hipError_t launch_ibm_step_f64(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
+hipError_t launch_ibm_step_f32s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
+hipError_t launch_ibm_step_f16s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
+hipError_t launch_ibm_step_dual(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
+hipError_t launch_ibm_step_dual32(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
+
+template <class P>
+hipError_t launch_ibm_step_p(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s) {
+    if (Q == 9) {
+        if (odd) k_ibm_step<9, P, true><<<1, 1024, 0, s>>>(a, b);
+        else k_ibm_step<9, P, false><<<1, 1024, 0, s>>>(a, b);
+    } else {
+        if (odd) k_ibm_step<19, P, true><<<1, 1024, 0, s>>>(a, b);
+        else k_ibm_step<19, P, false><<<1, 1024, 0, s>>>(a, b);
     }
     return hipGetLastError();
 }

@@ -198,6 +198,31 @@ class GpuGrid:
     def run(self, nsteps):
         call("lumagpu_run", self.h, int(nsteps))
 
+    # the step in the halves of LBM_multi_opt's two sweeps, with the IB stages between them
+    def stream_bc_macro(self, subcycle=0):
+        call("lumagpu_stream_bc_macro", self.h, int(subcycle))
+
+    def force_collide_swap(self):
+        call("lumagpu_force_collide_swap", self.h)
+
+    def reset_forces(self):
+        call("lumagpu_reset_forces", self.h)
+
+    def ibm_apply(self):
+        call("lumagpu_ibm_apply", self.h)
+
+    def ibm_interpolate(self):
+        call("lumagpu_ibm_interpolate", self.h)
+
+    def ibm_compute_force(self):
+        call("lumagpu_ibm_compute_force", self.h)
+
+    def ibm_spread(self):
+        call("lumagpu_ibm_spread", self.h)
+
+    def ibm_update_macro(self):
+        call("lumagpu_ibm_update_macro", self.h)
+
     def run_timed(self, nsteps):
         tot, ker = C.c_double(), C.c_double()
         call("lumagpu_run_timed", self.h, int(nsteps), C.byref(tot), C.byref(ker))

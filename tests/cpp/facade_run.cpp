@@ -2,9 +2,11 @@
 // drives GridObj and ObjectManager: a case written by the test (descriptor, labels, positions,
 // inlet, initial rho and u, LUMA layouts), an optional geometry.config, a number of steps,
 // then the post-collision f of the grid in LUMA's layout to an output file.
-//   facade_run <case.bin> <geometry.config or -> <steps> <out.bin> [<restart path>]
+//   facade_run <case.bin> <geometry.config or -> <steps> <out.bin> [<restart path> or -] [split]
 // With a restart path the state after the steps is also written as a restart file, read into
-// a second grid of the same descriptor, and that grid's f is appended to the output.
+// a second grid of the same descriptor, and that grid's f is appended to the output.  With
+// "split" the second half of the steps runs as LBM_stream_bc_macro, ObjectManager's four IB
+// stages and LBM_force_collide_swap instead of LBM_multi_opt.
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -111,13 +113,25 @@ int main(int argc, char **argv) {
             om->io_readInGeomConfig(argv[2], shift);
             om->ibm_initialise();
         }
+        const bool split = argc > 6 && std::strcmp(argv[6], "split") == 0;
         const int first = steps / 2;
         g.LBM_multi_run(first);
-        for (int s = first; s < steps; ++s) g.LBM_multi_opt();
+        for (int s = first; s < steps; ++s) {
+            if (!split) {
+                g.LBM_multi_opt();
+                continue;
+            }
+            g.LBM_stream_bc_macro(0);
+            om->ibm_interpolate(g.level);
+            om->ibm_computeForce(g.level);
+            om->ibm_spread(g.level);
+            om->ibm_updateMacroscopic(g.level);
+            g.LBM_force_collide_swap();
+        }
         g.sync();
         std::ofstream out(argv[4], std::ios::binary);
         out.write(reinterpret_cast<const char *>(g.f.data()), sizeof(double) * g.f.size());
-        if (argc > 5) {
+        if (argc > 5 && std::strcmp(argv[5], "-") != 0) {
             g.io_restart(luma::eWrite, argv[5]);
             luma::GridObj g2(c.d);
             fill(g2, c);
