@@ -23,3 +23,13 @@ def test_cpu_baseline_reports_the_fields_the_line_carries():
     assert cpu["unit"] == "MLUPS" and cpu["kind"] == "port" and cpu["value"] > 0
     assert cpu["cores"] >= 1 and "steps" in cpu["sample"]
     assert set(cpu["host"]) == {"nproc", "affinity", "cpu_model", "OMP_NUM_THREADS"}
+    assert "5 warm-up steps" in cpu["sample"] and cpu["luma_active_cell_ops_per_time"] > 0
+
+
+def test_kernel_source_hash_follows_the_kernel_sources():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    h = bench.kernel_source_hash()
+    assert len(h) == 16 and h == bench.kernel_source_hash()
+    assert all((ROOT / rel).exists() for rel in bench.KERNEL_SOURCES)

@@ -6,6 +6,7 @@ over 120 coarse steps (240 fine steps with two IB applications each); the full 1
 run is a measurement (luma_amd/cases.py), too long for the oracle.
 """
 import os
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -19,6 +20,7 @@ from test_gpu_ibm import compare_grids, compare_markers, oracle_grid
 pytestmark = pytest.mark.gpu
 
 THREADS = min(16, len(os.sched_getaffinity(0)))
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 def test_c1_cavity_256_full_length_fp64_bitwise():
@@ -58,3 +60,88 @@ def test_c3_cylinder_full_geometry_fp64_bitwise():
         mk = ib.markers(0)
         ref = (mk["force"] * (mk["eps"] * mk["ds"])[:, None]).sum(0)
         assert np.allclose(gc.ibm_body_force(0), ref, rtol=1e-12, atol=1e-16)
+
+
+def test_c4_arrangement_reduced_fp64_bitwise():
+    # C4's arrangement at reduced size: D3Q19 with a forced-equilibrium uniform inlet, an
+    # ExtrapolateRight outlet (streamed and collided, its rho and u left as initialised),
+    # periodic y and z, and a Fibonacci sphere with seeded N(0, 0.05 cell) marker noise
+    U, tau = cases.C4_U, cases.c4_tau()
+    case = setups.uniform_inflow_3d(48, 24, 24, U, tau)
+    pos = cases.c4_markers(n=113, centre=(12.3, 12.1, 11.9), diam=6.0)
+    print(f"[config] c4 arrangement 48x24x24, {len(pos)} markers, tau {tau:.4f}", flush=True)
+    og = oracle_grid(case)
+    ib = OracleIBM(og, 1.0)
+    ib.add_body(pos)
+    ib.initialise()
+    with GpuGrid.from_case(case, "fp64", store_macro=True) as g:
+        f0 = g.download_f()
+        g.ibm_add_body(pos)
+        g.ibm_initialise()
+        for _ in range(40):
+            og.step()
+            g.LBM_multi_opt()
+        compare_markers(g, ib)
+        # every site but the forced-equilibrium inlet, whose own populations no site reads
+        # (LUMA collides them from stale values; this build leaves them)
+        m = case.lattyp != setups.VELOCITY
+        fg, fo = g.download_f(), og.get_f()
+        assert np.array_equal(fg[m], fo[m]), np.abs(fg[m] - fo[m]).max()
+        (rg, ug), (ro, uo) = g.download_macro(), og.get_macro()
+        assert np.array_equal(rg, ro) and np.array_equal(ug, uo)
+    # the outlet plane is updated (it left its initial equilibrium) and equals the oracle's
+    assert np.abs(fg[-1] - f0[-1]).max() > 1e-8
+
+
+def test_c4_full_size_adjoint_short_horizon():
+    # C4 at full size (384 x 128 x 128, the committed 1,810-marker fixture) over a short horizon:
+    # T = 40 steps with checkpoints every 20; dJ/dU of the adjoint against central differences
+    # of the FP64 forward (h = 1e-4), bound 1e-4 relative
+    pos = np.load(GOLDEN / "c4_sphere_markers.npy")
+    assert pos.shape == (cases.C4_MARKERS, 3) and np.array_equal(pos, cases.c4_markers())
+    U, tau = cases.C4_U, cases.c4_tau()
+    print("[config] c4 full size adjoint, T 40, K 20", flush=True)
+    a = cases.c4_eval("fp64", U, tau, pos, 40, 20, True)
+    h = 1e-4
+    J = lambda U_: cases.c4_eval("fp64", U_, tau, pos, 40, 20, False)["J"]
+    fd = (J(U * (1 + h)) - J(U * (1 - h))) / (2 * h * U)
+    print(f"[config] c4 J {a['J']:.9e} dJ/dU adjoint {a['dJ_dU']:.9e} fd {fd:.9e}", flush=True)
+    assert abs(a["dJ_dU"] - fd) <= 1e-4 * abs(fd)
+
+
+def test_c3_fp32_shifted_tracks_fp64():
+    # the C3 geometry in the FP32-shifted form against the FP64 form (the oracle bit for bit):
+    # velocity relative L2 on both levels and the body force after 2,000 coarse steps
+    parent, child, pos = cases.c3_cylinder()
+    print("[config] c3 fp32s vs fp64, 2000 coarse steps", flush=True)
+    out = {}
+    for p in ("fp64", "fp32s"):
+        with GpuGrid.from_case(parent, p, store_macro=True) as gp, \
+                GpuGrid.from_case(child, p, store_macro=True) as gc:
+            gp.add_child(gc)
+            gc.ibm_add_body(pos)
+            gc.ibm_initialise()
+            gp.run(2000)
+            out[p] = (gp.download_macro()[1], gc.download_macro()[1], gc.ibm_body_force(0))
+    mp = ~np.isin(parent.lattyp, [setups.SOLID, setups.REFINED])
+    mc = child.lattyp != setups.SOLID
+    e_coarse = cases.rel_l2(out["fp32s"][0], out["fp64"][0], mp)
+    e_fine = cases.rel_l2(out["fp32s"][1], out["fp64"][1], mc)
+    e_force = np.abs(out["fp32s"][2] - out["fp64"][2]).max() / np.abs(out["fp64"][2]).max()
+    print(f"[config] c3 fp32s: u L2 coarse {e_coarse:.2e} fine {e_fine:.2e}, body force {e_force:.2e}", flush=True)
+    assert e_coarse < 1e-5 and e_fine < 1e-5 and e_force < 1e-5
+
+
+def test_tgv256_fp16_shifted_tracks_fp64():
+    # the metric box in FP16-shifted storage against FP64 after 500 steps
+    case = setups.taylor_green(256, u0=0.05, tau=0.55, dims=3)
+    print("[config] tgv 256^3 fp16s vs fp64, 500 steps", flush=True)
+    u = {}
+    for p in ("fp64", "fp16s"):
+        with GpuGrid.from_case(case, p, store_macro=True) as g:
+            g.run(500)
+            u[p] = g.download_macro()[1]
+    m = np.ones(u["fp64"].shape[:3], bool)
+    err = cases.rel_l2(u["fp16s"], u["fp64"], m)
+    print(f"[config] tgv fp16s u L2 {err:.2e}", flush=True)
+    assert err < 1e-3

@@ -107,3 +107,42 @@ def test_two_level_shifted_forms_track_fp64(prec):
         err = np.linalg.norm(res[prec][i] - res["fp64"][i]) / np.linalg.norm(res["fp64"][i])
         print(prec, "level", i, "rel L2 u error", err)
         assert err < (1e-5 if prec == "fp32s" else 5e-2)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_three_levels_fp64_bitwise(dims):
+    # GridObj_ops_lbm_optimised.cpp:45-49 recursively: the level-1 grid steps its own child twice
+    # per sub-step, so level 2 takes four steps per coarse step, each level explodes from and
+    # coalesces into the one above
+    if dims == 2:
+        l0 = setups.taylor_green((28, 24, 1), u0=0.04, tau=0.6, dims=2)
+        l1 = setups.add_refined_region(l0, (6, 19, 5, 17), 1.0, 1.0)
+        l2 = setups.add_refined_region(l1, (6, 19, 6, 17), 0.5, 0.5, parent_origin=l1.info["origin"], level=2)
+    else:
+        l0 = setups.taylor_green((16, 14, 14), u0=0.04, tau=0.6, dims=3)
+        l1 = setups.add_refined_region(l0, (4, 11, 3, 10, 3, 10), 1.0, 1.0)
+        l2 = setups.add_refined_region(l1, (4, 11, 4, 11, 4, 11), 0.5, 0.5, parent_origin=l1.info["origin"], level=2)
+    setups.velocity_field_on(l1, l0)
+    setups.velocity_field_on(l2, l1)
+    print(f"[refine] three levels {dims}D: {l0.N}x{l0.M}x{l0.K}, {l1.N}x{l1.M}x{l1.K}, {l2.N}x{l2.M}x{l2.K}", flush=True)
+    cases = (l0, l1, l2)
+    ors = [oracle_grid(c) for c in cases]
+    ors[0].add_child(ors[1])
+    ors[1].add_child(ors[2])
+    gs = [GpuGrid.from_case(c, "fp64", store_macro=True) for c in cases]
+    try:
+        gs[0].add_child(gs[1])
+        gs[1].add_child(gs[2])
+        for _ in range(12):
+            ors[0].step()
+            gs[0].LBM_multi_opt()
+        assert (gs[0].t, gs[1].t, gs[2].t) == (12, 24, 48)
+        for lev, (c, o, g) in enumerate(zip(cases, ors, gs)):
+            m = ~np.isin(c.lattyp, [setups.REFINED, setups.SOLID])
+            d = np.abs(g.download_f()[m] - o.get_f()[m]).max()
+            assert d == 0.0, f"level {lev}: f differs by {d}"
+            (rg, ug), (ro, uo) = g.download_macro(), o.get_macro()
+            assert np.array_equal(rg[m], ro[m]) and np.array_equal(ug[m], uo[m]), lev
+    finally:
+        for g in reversed(gs):
+            g.close()
