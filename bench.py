@@ -9,10 +9,15 @@ following neighbour step fused through LDS, reading and writing each population 
 steps), and the faster of the two FP32-shifted runs is the reported value.
 
 Timed: W warm-up steps, then exactly K steps between two stream synchronisations
-(wall clock), with a HIP event pair around every step launch on the grid's own stream
-(mean kernel duration -> achieved algorithmic GB/s).  Algorithmic bytes per cell per step
-are 2*Q*s (each population read once and written once): 304 B FP64, 152 B FP32-shifted,
-76 B FP16-shifted; a two-step launch counts them for both of its steps.
+(wall clock), with a HIP event pair around every launch on the grid's own stream (mean launch
+duration).  Roofline of the reported kernel: the least HBM traffic one launch needs -- every
+population read once and written once, 2*Q*s bytes per cell (304 B FP64, 152 B FP32-shifted,
+76 B FP16-shifted), whether the launch advances one step or two -- over the mean launch
+duration, against 8 TB/s.  For the two-step kernel that is the HBM bandwidth it uses; the
+per-step figure (2*Q*s per cell per step, both steps counted) is reported beside it as
+`cell_update_gbs`.  `traffic` is the HBM bytes per launch from rocprofv3's FETCH_SIZE and
+WRITE_SIZE counters (profiles/tgv256_step_pmc.json), given only when that file was made from
+the kernel sources of this tree.
 
 Also reported: the same step in all three forms, the velocity L2 error and kinetic-energy
 ratio of the shifted forms against FP64 after the configured 5,000 steps, and a CPU
@@ -38,6 +43,19 @@ METRIC = "MLUPS & %HBM roofline, D3Q19 256³: FP32-shifted vs FP64 at stated err
 PEAK_GBS = 8000.0            # MI355X HBM3E
 ELEM = {"fp64": 8, "fp32s": 4, "fp16s": 2}
 PMC_FILE = ROOT / "profiles" / "tgv256_step_pmc.json"
+# the sources and build flags that decide the step kernels' code
+KERNEL_SOURCES = ["luma_amd/csrc/aa_kernels.hpp", "luma_amd/csrc/fused_kernels.hpp", "luma_amd/csrc/lattice.hpp",
+                  "luma_amd/csrc/launch.hpp", "luma_amd/csrc/dual.hpp", "luma_amd/csrc/lbm_f32.hip",
+                  "luma_amd/csrc/lbm_f64.hip", "luma_amd/build.py"]
+
+
+def kernel_source_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update(rel.encode())
+        h.update((ROOT / rel).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def log(msg):
@@ -63,17 +81,23 @@ def time_form(case, form, steps, warmup, two_step=None):
         t1 = time.perf_counter()
     wall = t1 - t0
     bpc = 2 * case.Q * ELEM[form]
-    achieved = bpc * ncell / (ker_ms * 1e-3) / 1e9
+    spl = 2 if two_step is not None else 1
+    launch_ms = spl * ker_ms               # run_timed reports per step; a launch is spl steps
+    achieved = bpc * ncell / (launch_ms * 1e-3) / 1e9
     out = {
         "mlups": ncell * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms_mean": ker_ms,
+        "launch_ms_mean": launch_ms,
+        "steps_per_launch": spl,
         "bytes_per_cell_step": bpc,
+        "hbm_min_bytes_per_launch": bpc * ncell,
         "achieved_gbs": achieved,
         "roofline_frac": achieved / PEAK_GBS,
+        "cell_update_gbs": spl * achieved,
     }
     if two_step is not None:
-        out["two_step"] = {"shape": two_step[0], "z_chunk": two_step[1], "launch_ms_mean": 2 * ker_ms}
+        out["two_step"] = {"shape": two_step[0], "z_chunk": two_step[1]}
     return out
 
 
@@ -145,19 +169,21 @@ def error_run(case, nsteps, two_step=None):
     return out
 
 
-def cpu_baseline(case, target_s=15.0):
+def cpu_baseline(case, target_s=15.0, warmup=5):
     from oracle.oracle import OracleGrid
 
     threads = min(16, len(os.sched_getaffinity(0)))
     g = OracleGrid(case.dims, case.Q, case.N, case.M, case.K, omega=case.omega)
     g.init_feq(case.rho, case.u)
     t0 = time.perf_counter()
-    g.run(1, threads)
-    one = time.perf_counter() - t0
+    g.run(warmup, threads)
+    one = (time.perf_counter() - t0) / warmup
     n = int(max(2, min(50, target_s / max(one, 1e-3))))
     t0 = time.perf_counter()
+    c0 = time.process_time()
     g.run(n, threads)
     dt = time.perf_counter() - t0
+    cpu_s = time.process_time() - c0
     g.close()
     model = None
     try:
@@ -165,11 +191,18 @@ def cpu_baseline(case, target_s=15.0):
             model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
     except OSError:
         pass
+    # LUMA's own figure (main_lbm.cpp:455-457): GridManager::activeCellOps (GridManager.cpp:642-679,
+    # the level-0 cell count on one level) over the outer-loop time in ms, times 1000, where the
+    # outer-loop time comes from clock() (process CPU time, summed over threads) per step
+    ops = case.ncell
+    luma_fig = ops / ((cpu_s / n) * 1e3 * 1000)
     return {"value": case.ncell * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "luma_active_cell_ops_per_time": luma_fig,
             "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model,
                      "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")},
-            "sample": f"D3Q19 BGK TGV {case.N}^3 FP64, {n} steps after 1 warm-up: oracle restating "
-                      f"LUMA's LBM_multi_opt (AoS f/fNew pull, two sweeps), OpenMP over x"}
+            "sample": f"D3Q19 BGK TGV {case.N}^3 FP64, {n} steps after {warmup} warm-up steps: oracle restating "
+                      f"LUMA's LBM_multi_opt (AoS f/fNew pull, two sweeps), OpenMP over x; value from the "
+                      f"wall clock, luma_active_cell_ops_per_time as LUMA's log computes it (CPU time)"}
 
 
 def main():
@@ -182,7 +215,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--slab-steps", type=int, default=200, help="0 skips the per-GPU slab measurement")
     ap.add_argument("--forms", default="fp64,fp32s,fp16s")
-    ap.add_argument("--two-step", default="8,128",
+    ap.add_argument("--two-step", default="24,128",
                     help="FP32-shifted two-steps-per-launch kernel: tile shape code, z planes per block "
                          "('' skips it)")
     args = ap.parse_args()
@@ -211,8 +244,9 @@ def main():
             log(f"[bench] two-step kernel not available for this grid: {e}")
         if "fp32s_2step" in res:
             r2 = res["fp32s_2step"]
-            log(f"[bench] fp32s_2step: {r2['mlups']:.0f} MLUPS, launch {r2['two_step']['launch_ms_mean']:.3f} ms "
-                f"per two steps, {r2['achieved_gbs']:.0f} GB/s algorithmic ({100 * r2['roofline_frac']:.1f}% of HBM)")
+            log(f"[bench] fp32s_2step: {r2['mlups']:.0f} MLUPS, launch {r2['launch_ms_mean']:.3f} ms "
+                f"per two steps, {r2['achieved_gbs']:.0f} GB/s of HBM ({100 * r2['roofline_frac']:.1f}% of 8 TB/s), "
+                f"{r2['cell_update_gbs']:.0f} GB/s in per-step bytes")
     err = None
     if args.error_steps > 0 and "fp64" in forms:
         log(f"[bench] error run: {args.error_steps} steps per form")
@@ -240,16 +274,16 @@ def main():
         head = "fp32s_2step"
     form = head.split("_")[0]
     r = res[head]
-    traffic = None
+    # HBM bytes per launch of this kernel from the rocprofv3 counter runs, if they were made from
+    # this tree's kernel sources
+    traffic, stale = None, None
     if PMC_FILE.exists():
         pmc = json.loads(PMC_FILE.read_text())
-        traffic = pmc.get(head, {}).get("hbm_bytes_per_launch")
-    # roofline of the dominant kernel: algorithmic bytes of one launch (2 Q s per cell per step,
-    # times the steps one launch advances) over its mean duration
-    steps_per_launch = 2 if "two_step" in r else 1
-    launch_ms = r["kernel_ms_mean"] * steps_per_launch
-    # HBM bytes actually moved (rocprofv3 counters) per launch over the live launch duration
-    traffic_frac = None if traffic is None else traffic / (launch_ms * 1e-3) / (PEAK_GBS * 1e9)
+        stale = pmc.get("kernel_source_hash") != kernel_source_hash()
+        if not stale:
+            traffic = pmc.get(head, {}).get("hbm_bytes_per_launch")
+    steps_per_launch = r["steps_per_launch"]
+    launch_ms = r["launch_ms_mean"]
     line = {
         "metric": METRIC,
         "value": round(r["mlups"], 1),
@@ -271,8 +305,10 @@ def main():
                    "bytes_per_cell_step": r["bytes_per_cell_step"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["roofline_frac"], 4), "traffic": traffic,
-                     "traffic_frac": None if traffic_frac is None else round(traffic_frac, 4),
-                     "steps_per_launch": steps_per_launch, "launch_ms": round(launch_ms, 5)},
+                     "traffic_over_min": None if traffic is None else round(traffic / r["hbm_min_bytes_per_launch"], 4),
+                     "traffic_stale": stale, "min_bytes_per_launch": r["hbm_min_bytes_per_launch"],
+                     "steps_per_launch": steps_per_launch, "launch_ms": round(launch_ms, 5),
+                     "cell_update_gbs": round(r["cell_update_gbs"], 1)},
         "cpu_baseline": cpu,
         "forms": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in res.items()},
