@@ -78,6 +78,7 @@ struct StepArgs {
     size_t ps;                    // population stride in elements
     int nx, ny, nz;               // x = LUMA i (N_lim), y = j (M_lim), z = k (K_lim)
     const uint8_t *flags;         // per-cell flag byte (masked kernels only)
+    const uint32_t *plain;        // plain-site bitmap of a labelled grid (k_plain_bits), or null
     // The double arrays below hold P::M elements: FP64, or (value, derivative) pairs in the
     // tangent form; kernels address them through mv<M>.
     const double *force;          // force_xyz, SoA [d*ncell + cell], read where flagged
@@ -882,6 +883,50 @@ __device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z
     });
 }
 
+// gather_in for a plain site: every source is a regular pull, so no neighbour flag is read.
+template <int Q, class P, bool ODD>
+__device__ __forceinline__ void gather_plain(const StepArgs &a, size_t cell, const Nbr &nb, typename P::T *fin) {
+    using L = Lattice<Q>;
+    using S = typename P::S;
+    const S *__restrict__ f = static_cast<const S *>(a.f);
+    const size_t ps = a.ps;
+    static_for<Q>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+        if constexpr (ODD) fin[v] = ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + nb.at(-cx, -cy, -cz, a.nx, a.ny));
+        else fin[v] = ld<P>(f + (size_t)v * ps + cell);
+    });
+}
+
+// Plain-site bitmap of a labelled grid: bit (cell & 31) of word cell >> 5 is set when the site
+// is fluid and every stencil neighbour (with the periodic wrap the step kernels use) is fluid
+// or TL2F.  Each pull and push of such a site is then the regular one in every configuration
+// (no bounce-back, outlet, velocity, explode or coalesce source, no solid destination), and the
+// step kernel takes it without reading the neighbours' flag bytes.  One thread per word.
+template <int Q>
+__global__ void k_plain_bits(const uint8_t *flags, int nx, int ny, int nz, size_t ncell, uint32_t *plain) {
+    using L = Lattice<Q>;
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w * 32 >= ncell) return;
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; ++b) {
+        const size_t cell = w * 32 + b;
+        if (cell >= ncell) break;
+        if ((flags[cell] & kTypeMask) != kFluid) continue;
+        const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / ((size_t)nx * ny));
+        Nbr nb;
+        nb.init(x, y, z, nx, ny, nz);
+        bool ok = true;
+        static_for<Q>([&](auto vc) {
+            constexpr int v = decltype(vc)::value;
+            const int t = flags[nb.at(L::c(v, 0), L::c(v, 1), L::c(v, 2), nx, ny)] & kTypeMask;
+            ok = ok && (t == kFluid || t == kTL2F);
+        });
+        if (ok) bits |= 1u << b;
+    }
+    plain[w] = bits;
+}
+
 // Regularised velocity face for a gathered site: rho, u of the BC and the regularised
 // populations in storage convention.
 template <int Q, class P>
@@ -925,7 +970,9 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     nb.init(x, y, z, nx, ny, a.nz);
 
     T fin[Q], fout[Q];
-    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
+    const bool plain = a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u);
+    if (plain) gather_plain<Q, P, ODD>(a, cell, nb, fin);
+    else gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
 
     using M = typename P::M;
     const M *F = nullptr;
@@ -998,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
         constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD) {
             const size_t dst = nb.at(cx, cy, cz, nx, ny);
-            if ((a.flags[dst] & kTypeMask) == kSolid)
+            if (!plain && (a.flags[dst] & kTypeMask) == kSolid)
                 st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
             else
                 st<P, cx == 0>(f + (size_t)v * ps + dst, fout[v]);
@@ -1016,31 +1063,32 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
 // :696-766), _LBM_macro_opt again with the spread force at the flagged sites.  Sites of other
 // types keep the rho, u their arrays hold, as LUMA's arrays do.
 template <int Q, class P, bool ODD>
-__device__ __forceinline__ void site_macro_one(const StepArgs &a, size_t cell, int ib_force) {
+__device__ __forceinline__ bool site_macro_value(const StepArgs &a, size_t cell, int ib_force, typename P::M &rho,
+                                                 typename P::M *u) {
     using L = Lattice<Q>;
     using T = typename P::T;
+    using M = typename P::M;
     const int x = (int)(cell % a.nx), y = (int)((cell / a.nx) % a.ny), z = (int)(cell / ((size_t)a.nx * a.ny));
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
     const bool macro_type = ty == kFluid || ty == kTL2F;
     const bool vel_type = !ib_force && a.regularised && ty == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
-    if (!macro_type && !vel_type) return;
+    if (!macro_type && !vel_type) return false;
     Nbr nb;
     nb.init(x, y, z, a.nx, a.ny, a.nz);
     T fin[Q];
-    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
-    typename P::M rho, u[3];
+    if (a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u)) gather_plain<Q, P, ODD>(a, cell, nb, fin);
+    else gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
     if (vel_type) {
         T rr, uu[3];
         velocity_site<Q, P>(a, fl, y, fin, rr, uu);
-        rho = typename P::M(rr);
-        u[0] = typename P::M(uu[0]);
-        u[1] = typename P::M(uu[1]);
-        u[2] = typename P::M(uu[2]);
+        rho = M(rr);
+        u[0] = M(uu[0]);
+        u[1] = M(uu[1]);
+        u[2] = M(uu[2]);
     } else {
         // sweep 1's macro adds F/2 of the force the step started from: zero after an IB reset,
         // the gravity field when the reset writes gravity, the set field on a grid never reset
-        using M = typename P::M;
         M Fv[3];
         const M *F = nullptr;
         const bool flagged = (fl & kForcedBit) != 0;
@@ -1054,7 +1102,12 @@ __device__ __forceinline__ void site_macro_one(const StepArgs &a, size_t cell, i
         T fout[Q];
         site_update<Q, P>(fin, fout, F, a, rho, u, false);
     }
-    store_macro<L::D>(a, cell, rho, u);
+    return true;
+}
+template <int Q, class P, bool ODD>
+__device__ __forceinline__ void site_macro_one(const StepArgs &a, size_t cell, int ib_force) {
+    typename P::M rho, u[3];
+    if (site_macro_value<Q, P, ODD>(a, cell, ib_force, rho, u)) store_macro<Lattice<Q>::D>(a, cell, rho, u);
 }
 template <int Q, class P, bool ODD>
 __global__ __launch_bounds__(256) void k_site_macro(StepArgs a, int nsites, const int *cells, int ib_force) {

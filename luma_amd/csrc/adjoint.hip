@@ -344,7 +344,7 @@ extern "C" int lumagpu_adj_run(lumagpu_grid *g, int nsteps, int checkpoint_every
         return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
     if (!g->flags) {
         if (int r = ensure_flags(g)) return r;
-        HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+        if (int r = upload_flags(g)) return r;
     }
     if (g->d.precision == LUMAGPU_FP64)
         return adj_run<P64>(g, nsteps, checkpoint_every, dir, backward, res, dJ_dpos, lambda0);

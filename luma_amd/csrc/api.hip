@@ -402,6 +402,20 @@ int run_graph(lumagpu_grid *g, int reps) {
 
 namespace lumagpu {
 int step_grid(lumagpu_grid *g, int subcycle) { return apply_step(g, subcycle, nullptr, nullptr); }
+int upload_flags(lumagpu_grid *g) {
+    if (g->hflags.size() != g->ncell) return fail(LUMAGPU_ESTATE, "host flag array does not cover the grid");
+    if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
+    const size_t nw = (g->ncell + 31) / 32;
+    if (!g->plain && hipMalloc(&g->plain, nw * sizeof(uint32_t)) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "plain-site bitmap allocation failed");
+    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    const unsigned nb = (unsigned)((nw + 255) / 256);
+    if (g->Q == 9) k_plain_bits<9><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
+    else k_plain_bits<19><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 hipError_t outlet_prepass(const lumagpu_grid *g) { return launch_extrap(g); }
 }  // namespace lumagpu
 
@@ -481,6 +495,7 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->f);
     (void)hipFree(g->f2);
     (void)hipFree(g->flags);
+    (void)hipFree(g->plain);
     (void)hipFree(g->force);
     (void)hipFree(g->uin);
     (void)hipFree(g->rho);
@@ -621,10 +636,8 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
         HIP_TRY(hipMemcpyAsync(g->edge, etab.data(), etab.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
         g->nedge = ne;
     }
-    if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
-        return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
     g->hflags = std::move(nf);
-    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    if (int r = upload_flags(g)) return r;
     if (outlet && !g->xr &&
         hipMalloc(&g->xr, (size_t)g->Q * g->d.M * g->d.K * g->melem) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "outlet buffer allocation failed");
@@ -887,9 +900,7 @@ int lumagpu_set_force(lumagpu_grid *g, const double *force_xyz) {
                 }
                 g->hflags[cell] = (uint8_t)((g->hflags[cell] & ~kForcedBit) | (nz ? kForcedBit : 0));
             }
-    if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
-        return fail(LUMAGPU_ENOMEM, "flag array allocation failed");
-    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    if (int r = upload_flags(g)) return r;
     HIP_TRY(hipMemcpyAsync(g->force, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->masked = 1;

@@ -504,18 +504,25 @@ int clear_gone(lumagpu_grid *g) {
     return 0;
 }
 
+hipError_t launch_ibm_interp(const lumagpu_grid *g, const StepArgs &a, const IbmArgs &b, int mode) {
+    const bool odd = g->parity == 0;
+    switch (g->d.precision) {
+        case LUMAGPU_FP64: return launch_ibm_interp_f64(g->Q, a, b, odd, mode, g->stream);
+        case LUMAGPU_FP32S: return launch_ibm_interp_f32s(g->Q, a, b, odd, mode, g->stream);
+        case LUMAGPU_FP64T: return launch_ibm_interp_dual(g->Q, a, b, odd, mode, g->stream);
+        case LUMAGPU_FP32T: return launch_ibm_interp_dual32(g->Q, a, b, odd, mode, g->stream);
+        default: return launch_ibm_interp_f16s(g->Q, a, b, odd, mode, g->stream);
+    }
+}
+
 // ibm_interpolate (:498-591) from the sweep-1 rho, u of the supports, recomputed here from the
-// populations (LUMA's u = u_n reset before each sub-iteration); mode 0 also computes the
-// marker forces (ibm_computeForce, :598-614) in the same pass
+// populations (LUMA's u = u_n reset before each sub-iteration) and stored at the support sites;
+// mode 0 also computes the marker forces (ibm_computeForce, :598-614) in the same pass
 int interpolate(lumagpu_grid *g, int mode) {
-    IbmSystem *s = g->ibm;
-    StepArgs a = make_args(g);
-    hipError_t e = launch_sites(g, a, g->parity == 0, s->nsites, s->d_sites, 0);
-    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB site macro launch: ") + hipGetErrorString(e));
-    IbmArgs b = ibm_args(g);
-    if (tangent_form(g)) k_ibm_interp_force<Dual><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b, mode);
-    else k_ibm_interp_force<double><<<(s->nm + 127) / 128, 128, 0, g->stream>>>(b, mode);
-    HIP_TRY(hipGetLastError());
+    const StepArgs a = make_args(g);
+    const IbmArgs b = ibm_args(g);
+    const hipError_t e = launch_ibm_interp(g, a, b, mode);
+    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB interpolation launch: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -545,36 +552,10 @@ int spread(lumagpu_grid *g) {
     return 0;
 }
 
-hipError_t launch_ibm_step(const lumagpu_grid *g, const StepArgs &a, const IbmArgs &b) {
-    const bool odd = g->parity == 0;
-    switch (g->d.precision) {
-        case LUMAGPU_FP64: return launch_ibm_step_f64(g->Q, a, b, odd, g->stream);
-        case LUMAGPU_FP32S: return launch_ibm_step_f32s(g->Q, a, b, odd, g->stream);
-        case LUMAGPU_FP64T: return launch_ibm_step_dual(g->Q, a, b, odd, g->stream);
-        case LUMAGPU_FP32T: return launch_ibm_step_dual32(g->Q, a, b, odd, g->stream);
-        default: return launch_ibm_step_f16s(g->Q, a, b, odd, g->stream);
-    }
-}
-
-// Interpolate, force and spread at the current supports: one block for small body systems,
-// otherwise one kernel per stage.
+// Interpolate with forces, then spread, at the current supports.
 int apply_core(lumagpu_grid *g) {
-    IbmSystem *s = g->ibm;
-    if (s->nsites > kIbmStepMaxSites || s->nm > kIbmStepMaxMarkers) {
-        if (int r = interpolate(g, 0)) return r;
-        return spread(g);
-    }
-    if (int r = clear_gone(g)) return r;
-    const StepArgs a = make_args(g);
-    const IbmArgs b = ibm_args(g);
-    hipError_t e = launch_ibm_step(g, a, b);
-    if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("IB step launch: ") + hipGetErrorString(e));
-    if (s->sites_dirty) {
-        for (int c : s->sites) g->hflags[c] |= kForcedBit;
-        s->flagged = s->sites;
-        s->sites_dirty = 0;
-    }
-    return 0;
+    if (int r = interpolate(g, 0)) return r;
+    return spread(g);
 }
 
 namespace {
@@ -948,7 +929,7 @@ int lumagpu_ibm_initialise(lumagpu_grid *g) {
     for (auto &b : s->bodies) compute_ds(g, b);
     if (int r = find_epsilon_all(g)) return r;
     if (int r = upload(g)) return r;
-    HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    if (int r = upload_flags(g)) return r;
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->masked = 1;
     s->initialised = 1;

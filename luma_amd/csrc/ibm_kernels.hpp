@@ -54,20 +54,46 @@ __device__ __forceinline__ const M *as(const A *p) {
 
 // ibm_interpolate (:498-591) and ibm_computeForce (:598-614); one thread per marker, sums in
 // support order.  local_area = 1 (IBMarker.cpp:61).  mode 0: both; 1: interpolation only;
-// 2: the force from the interpolated values held since the last interpolation.
-template <class M = double>
+// 2: the force from the interpolated values held since the last interpolation.  The loads of a
+// chunk of supports are all issued before its ordered sum, so a marker waits on two memory
+// round trips per chunk instead of two per support.  DD: the dimension when known at compile
+// time (0: b.D).
+template <class M = double, int DD = 0>
 __device__ __forceinline__ void interp_force_one(const IbmArgs &b, int m, int mode) {
+    constexpr int CH = sizeof(M) == sizeof(double) ? 9 : 3;
+    const int D = DD ? DD : b.D;
     const M *sdelta = as<M>(b.sdelta), *grho = as<M>(b.rho), *gu = as<M>(b.u), *vel = as<M>(b.vel);
     M *interp = as<M>(b.interp), *force = as<M>(b.force);
     const M area = 1.0;
     M rho = 0.0, mom[3] = {0.0, 0.0, 0.0};
     if (mode != 2) {
-        for (int k = 0; k < b.ns[m]; ++k) {
-            const size_t cell = (size_t)b.scell[m * b.S + k];
-            const M dl = sdelta[m * b.S + k];
-            const M r = grho[cell];
-            rho += r * dl * area;
-            for (int d = 0; d < b.D; ++d) mom[d] += r * gu[d * b.ncell + cell] * dl * area;
+        const int n = b.ns[m];
+        const int *sc = b.scell + (size_t)m * b.S;
+        const M *sd = sdelta + (size_t)m * b.S;
+        for (int k0 = 0; k0 < n; k0 += CH) {
+            int cell[CH];
+            M dl[CH], r[CH], uv[CH][3];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int k = (k0 + j < n) ? k0 + j : k0;   // slots past the count repeat the first
+                cell[j] = sc[k];
+                dl[j] = sd[k];
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                r[j] = grho[cell[j]];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) uv[j][d] = (d < D) ? gu[d * b.ncell + (size_t)cell[j]] : M(0.0);
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                if (k0 + j < n) {
+                    rho += r[j] * dl[j] * area;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        if (d < D) mom[d] += r[j] * uv[j][d] * dl[j] * area;
+                }
+            }
         }
         interp[4 * m] = rho;
         for (int d = 0; d < 3; ++d) interp[4 * m + 1 + d] = mom[d];
@@ -76,7 +102,7 @@ __device__ __forceinline__ void interp_force_one(const IbmArgs &b, int m, int mo
         for (int d = 0; d < 3; ++d) mom[d] = interp[4 * m + 1 + d];
     }
     if (mode != 1)
-        for (int d = 0; d < b.D; ++d) force[3 * m + d] = M(2.0) * (mom[d] - rho * vel[3 * m + d]) / M(1.0);
+        for (int d = 0; d < D; ++d) force[3 * m + d] = M(2.0) * (mom[d] - rho * vel[3 * m + d]) / M(1.0);
 }
 template <class M = double>
 __global__ void k_ibm_interp_force(IbmArgs b, int mode) {
@@ -87,24 +113,44 @@ __global__ void k_ibm_interp_force(IbmArgs b, int mode) {
 
 // ibm_spread (:621-689) as a per-site gather; the site's force starts from the reset value 0
 // (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
-// step kernel adds it in the macro (+F/2) and the Guo term.
-template <class M = double>
+// step kernel adds it in the macro (+F/2) and the Guo term.  Entries are read in chunks whose
+// loads are issued together, then summed in entry order.
+template <class M = double, int DD = 0>
 __device__ __forceinline__ void spread_one(const IbmArgs &b, int s) {
+    constexpr int CH = sizeof(M) == sizeof(double) ? 6 : 3;
+    const int D = DD ? DD : b.D;
     const M *sdelta = as<M>(b.sdelta), *force = as<M>(b.force), *eps = as<M>(b.eps), *ds = as<M>(b.ds);
     const size_t cell = (size_t)b.sites[s];
     M *fxyz = as<M>(b.fxyz);
     M acc[3] = {0.0, 0.0, 0.0};
     if (b.accumulate)
-        for (int d = 0; d < b.D; ++d) acc[d] = fxyz[d * b.ncell + cell];
-    for (int e = b.sptr[s]; e < b.sptr[s + 1]; ++e) {
-        const int code = b.sent[e];
-        const int m = code / b.S;
-        const M dl = sdelta[code];
-        const M volWidth = eps[m];
-        const M volDepth = (b.D == 3) ? ds[m] : M(1.0);
-        for (int d = 0; d < b.D; ++d) acc[d] -= dl * force[3 * m + d] * volWidth * volDepth * ds[m];
+        for (int d = 0; d < D; ++d) acc[d] = fxyz[d * b.ncell + cell];
+    const int e0 = b.sptr[s], e1 = b.sptr[s + 1];
+    for (int eb = e0; eb < e1; eb += CH) {
+        int code[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) code[j] = b.sent[(eb + j < e1) ? eb + j : eb];
+        M dl[CH], fm[CH][3], vw[CH], dsm[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int m = code[j] / b.S;
+            dl[j] = sdelta[code[j]];
+            vw[j] = eps[m];
+            dsm[j] = ds[m];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fm[j][d] = (d < D) ? force[3 * m + d] : M(0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            if (eb + j < e1) {
+                const M volDepth = (D == 3) ? dsm[j] : M(1.0);
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    if (d < D) acc[d] -= dl[j] * fm[j][d] * vw[j] * volDepth * dsm[j];
+            }
+        }
     }
-    for (int d = 0; d < b.D; ++d) fxyz[d * b.ncell + cell] = acc[d];
+    for (int d = 0; d < D; ++d) fxyz[d * b.ncell + cell] = acc[d];
     b.flags[cell] |= kForcedBit;
 }
 template <class M = double>

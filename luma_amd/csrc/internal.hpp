@@ -23,6 +23,7 @@ struct lumagpu_grid {
     void *f2 = nullptr;               // second population array of the two-step (temporal-blocking) kernel
     int fuse_shape = -1, fuse_zc = 0; // its tile shape (FusedShape, -1: off) and z planes per block
     uint8_t *flags = nullptr;
+    uint32_t *plain = nullptr;        // plain-site bitmap, rebuilt with every flag upload
     double *force = nullptr;
     double *uin = nullptr;
     double *rho = nullptr, *u = nullptr;
@@ -81,6 +82,7 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.ny = g->d.M;
     a.nz = g->d.K;
     a.flags = g->flags;
+    a.plain = g->plain;
     a.force = g->force;
     a.uin = g->uin;
     a.rho_out = g->rho;
@@ -149,6 +151,8 @@ inline StepArgs make_args(const lumagpu_grid *g) {
 int ibm_apply(lumagpu_grid *g);
 void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
+// copy the host flag bytes to the device and rebuild the plain-site bitmap (api.hip)
+int upload_flags(lumagpu_grid *g);
 // after a whole-field force reset: no site holds an IB force, the next spread flags its sites
 void ibm_forget_flags(lumagpu_grid *g);
 // the grid carries a flexible (FEM) body, whose step runs host work

@@ -64,36 +64,82 @@ hipError_t launch_sites_p(int Q, const StepArgs &a, bool odd, int n, const int *
     return hipGetLastError();
 }
 
-// One IB application of a small body system in one block (the site macro at the supports,
-// interpolation and force per marker, spread per support site, with block barriers between the
-// phases): the same per-thread work as the three kernels it replaces, one launch instead of
-// three on the launch-bound refined-cylinder and filament runs.
-template <int Q, class P, bool ODD>
-__global__ __launch_bounds__(1024) void k_ibm_step(StepArgs a, IbmArgs b) {
-    using M = typename P::M;
-    for (int i = threadIdx.x; i < b.nsites; i += blockDim.x) site_macro_one<Q, P, ODD>(a, (size_t)b.sites[i], 0);
-    __syncthreads();
-    for (int m = threadIdx.x; m < b.nm; m += blockDim.x) interp_force_one<M>(b, m, 0);
-    __syncthreads();
-    for (int s = threadIdx.x; s < b.nsites; s += blockDim.x) spread_one<M>(b, s);
+// ibm_interpolate (:498-591) with the sweep-1 macro of each support computed where it is
+// needed: one lane per (marker, support), a marker's S supports in consecutive lanes of one
+// wave (7 markers per wave in 2D, 2 in 3D).  Each lane computes its site's rho, u from the
+// populations exactly as the site-macro pass does (and stores them, so the macro arrays hold
+// what that pass writes; lanes sharing a site store the same bits), then the group sums its
+// products in support order through lane shuffles and the first lane writes the interpolated
+// values and, for mode 0, the marker force (ibm_computeForce, :598-614).  Every memory access
+// of a marker is issued in parallel across its lanes instead of one support after another.
+__device__ __forceinline__ double shfl_m(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ Dual shfl_m(const Dual &v, int src) {
+    return Dual(__shfl(v.v, src, 64), __shfl(v.d, src, 64));
 }
-// body systems up to these sizes take the one-block path
-constexpr int kIbmStepMaxSites = 16384, kIbmStepMaxMarkers = 4096;
+template <int Q, class P, bool ODD>
+__global__ __launch_bounds__(256) void k_ibm_interp_sites(StepArgs a, IbmArgs b, int mode) {
+    using L = Lattice<Q>;
+    using M = typename P::M;
+    constexpr int D = L::D, S = (D == 3) ? 27 : 9, MPW = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / S, k = lane - grp * S;
+    const int m = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * MPW + grp;
+    const bool live = grp < MPW && m < b.nm;
+    const int n = live ? b.ns[m] : 0;
+    const M area = 1.0;
+    M pr = M(0.0), pm[3] = {M(0.0), M(0.0), M(0.0)};
+    if (k < n) {
+        const size_t cell = (size_t)b.scell[m * S + k];
+        const M dl = as<M>(b.sdelta)[m * S + k];
+        M r, u[3];
+        if (site_macro_value<Q, P, ODD>(a, cell, 0, r, u)) {
+            store_macro<D>(a, cell, r, u);
+        } else {   // sites of other types keep the rho, u their arrays hold
+            r = as<M>(b.rho)[cell];
+            for (int d = 0; d < D; ++d) u[d] = as<M>(b.u)[d * b.ncell + cell];
+        }
+        pr = r * dl * area;
+        for (int d = 0; d < D; ++d) pm[d] = r * u[d] * dl * area;
+    }
+    M rho = 0.0, mom[3] = {M(0.0), M(0.0), M(0.0)};
+    const int base = grp * S;
+    for (int kk = 0; kk < S; ++kk) {
+        const M tr = shfl_m(pr, base + kk);
+        M tm[3];
+        for (int d = 0; d < D; ++d) tm[d] = shfl_m(pm[d], base + kk);
+        if (kk < n) {
+            rho += tr;
+            for (int d = 0; d < D; ++d) mom[d] += tm[d];
+        }
+    }
+    if (!live || k != 0) return;
+    M *interp = as<M>(b.interp);
+    interp[4 * m] = rho;
+    for (int d = 0; d < 3; ++d) interp[4 * m + 1 + d] = mom[d];
+    if (mode == 0) {
+        const M *vel = as<M>(b.vel);
+        M *force = as<M>(b.force);
+        for (int d = 0; d < D; ++d) force[3 * m + d] = M(2.0) * (mom[d] - rho * vel[3 * m + d]) / M(1.0);
+    }
+}
 
-hipError_t launch_ibm_step_f64(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
-hipError_t launch_ibm_step_f32s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
-hipError_t launch_ibm_step_f16s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
-hipError_t launch_ibm_step_dual(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
-hipError_t launch_ibm_step_dual32(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s);
+hipError_t launch_ibm_interp_f64(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s);
+hipError_t launch_ibm_interp_f32s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s);
+hipError_t launch_ibm_interp_f16s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s);
+hipError_t launch_ibm_interp_dual(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s);
+hipError_t launch_ibm_interp_dual32(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s);
 
 template <class P>
-hipError_t launch_ibm_step_p(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s) {
+hipError_t launch_ibm_interp_p(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s) {
+    if (b.nm <= 0) return hipSuccess;
+    const int mpb = 4 * (64 / (Q == 9 ? 9 : 27));   // markers per 256-thread block
+    const dim3 g((unsigned)((b.nm + mpb - 1) / mpb)), blk(256);
     if (Q == 9) {
-        if (odd) k_ibm_step<9, P, true><<<1, 1024, 0, s>>>(a, b);
-        else k_ibm_step<9, P, false><<<1, 1024, 0, s>>>(a, b);
+        if (odd) k_ibm_interp_sites<9, P, true><<<g, blk, 0, s>>>(a, b, mode);
+        else k_ibm_interp_sites<9, P, false><<<g, blk, 0, s>>>(a, b, mode);
     } else {
-        if (odd) k_ibm_step<19, P, true><<<1, 1024, 0, s>>>(a, b);
-        else k_ibm_step<19, P, false><<<1, 1024, 0, s>>>(a, b);
+        if (odd) k_ibm_interp_sites<19, P, true><<<g, blk, 0, s>>>(a, b, mode);
+        else k_ibm_interp_sites<19, P, false><<<g, blk, 0, s>>>(a, b, mode);
     }
     return hipGetLastError();
 }

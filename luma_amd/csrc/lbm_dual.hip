@@ -14,8 +14,8 @@ hipError_t launch_aux_dual(int op, int Q, const StepArgs &a, const double *p0, c
     return launch_aux_p<PDual>(op, Q, a, p0, p1, out, parity, masked, s);
 }
 
-hipError_t launch_ibm_step_dual(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s) {
-    return launch_ibm_step_p<PDual>(Q, a, b, odd, s);
+hipError_t launch_ibm_interp_dual(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s) {
+    return launch_ibm_interp_p<PDual>(Q, a, b, odd, mode, s);
 }
 hipError_t launch_sites_dual(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s) {
     return launch_sites_p<PDual>(Q, a, odd, n, cells, ib_force, s);

@@ -24,11 +24,11 @@ hipError_t launch_aux_f16s(int op, int Q, const StepArgs &a, const double *p0, c
     return launch_aux_p<P16S>(op, Q, a, p0, p1, out, parity, masked, s);
 }
 
-hipError_t launch_ibm_step_f32s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s) {
-    return launch_ibm_step_p<P32S>(Q, a, b, odd, s);
+hipError_t launch_ibm_interp_f32s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s) {
+    return launch_ibm_interp_p<P32S>(Q, a, b, odd, mode, s);
 }
-hipError_t launch_ibm_step_f16s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s) {
-    return launch_ibm_step_p<P16S>(Q, a, b, odd, s);
+hipError_t launch_ibm_interp_f16s(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s) {
+    return launch_ibm_interp_p<P16S>(Q, a, b, odd, mode, s);
 }
 hipError_t launch_sites_f32s(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s) {
     return launch_sites_p<P32S>(Q, a, odd, n, cells, ib_force, s);

@@ -16,8 +16,8 @@ hipError_t launch_aux_f64(int op, int Q, const StepArgs &a, const double *p0, co
     return launch_aux_p<P64>(op, Q, a, p0, p1, out, parity, masked, s);
 }
 
-hipError_t launch_ibm_step_f64(int Q, const StepArgs &a, const IbmArgs &b, bool odd, hipStream_t s) {
-    return launch_ibm_step_p<P64>(Q, a, b, odd, s);
+hipError_t launch_ibm_interp_f64(int Q, const StepArgs &a, const IbmArgs &b, bool odd, int mode, hipStream_t s) {
+    return launch_ibm_interp_p<P64>(Q, a, b, odd, mode, s);
 }
 hipError_t launch_sites_f64(int Q, const StepArgs &a, bool odd, int n, const int *cells, int ib_force, hipStream_t s) {
     return launch_sites_p<P64>(Q, a, odd, n, cells, ib_force, s);
