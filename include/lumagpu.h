@@ -206,6 +206,14 @@ int lumagpu_sync(lumagpu_grid *g);
  * mass, momentum and kinetic energy of the post-collision state; LUMA has no counterpart
  * (its logs hold per-marker forces only, ObjectManager_ops_io.cpp:65-128). */
 int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]);
+/* lumagpu_moments without any allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Writes the same LUMAGPU_NUM_MOMENTS doubles, bit for bit, into
+ * out_dev (device memory of the caller) asynchronously on the grid's stream (lumagpu_sync
+ * waits).  Its per-block partial sums live in a static device array of the library, one per
+ * GPU, sized for at most 2^18 row blocks (ceil(N/256) * M * K), ~10 MB; larger grids are
+ * refused with LUMAGPU_EINVAL.  Calls that overlap on one GPU share that array and race. */
+int lumagpu_moments_noalloc(lumagpu_grid *g, double *out_dev);
 
 /* ---- immersed boundary (ObjectManager, LUMA/src/ObjectManager_ops_ibm.cpp) ------------
  * Bodies live on one grid; their step (interpolate, force, spread, macroscopic update,

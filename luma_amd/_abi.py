@@ -78,6 +78,7 @@ _PROTOS = {
     "lumagpu_get_time": [_vp, _ip],
     "lumagpu_sync": [_vp],
     "lumagpu_moments": [_vp, _dp],
+    "lumagpu_moments_noalloc": [_vp, _vp],
     "lumagpu_set_positions": [_vp, _dp, _dp, _dp],
     "lumagpu_ibm_add_body": [_vp, C.c_int, _dp, C.c_int, _ip],
     "lumagpu_ibm_initialise": [_vp],

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -400,6 +401,31 @@ int run_graph(lumagpu_grid *g, int reps) {
 
 }  // namespace
 
+namespace {
+// Per-block partial sums of lumagpu_moments_noalloc: device memory fixed at load time, one copy
+// per GPU, at most kScratchBlocks row blocks.  Host code never hands the symbol itself to a
+// kernel (in host code it is a stand-in, not a device address): it takes each GPU's address
+// from hipGetSymbolAddress once, with that GPU current, and passes that.
+constexpr size_t kScratchBlocks = size_t(1) << 18;
+static __device__ __align__(256) double scratch[kScratchBlocks * kMomentCount];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on(int device, double **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(LUMAGPU_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(LUMAGPU_EHIP, "hipGetSymbolAddress returned no address for the reduction scratch");
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+}  // namespace
+
 namespace lumagpu {
 int step_grid(lumagpu_grid *g, int subcycle) { return apply_step(g, subcycle, nullptr, nullptr); }
 int upload_flags(lumagpu_grid *g) {
@@ -460,6 +486,7 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         lumagpu_grid_destroy(g);
         return fail(code, m);
     };
+    if (hipGetDevice(&g->device) != hipSuccess) return cleanup(LUMAGPU_EHIP, "hipGetDevice failed");
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(LUMAGPU_EHIP, "hipStreamCreate failed");
     if (hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming) != hipSuccess ||
@@ -1152,6 +1179,27 @@ int lumagpu_moments(lumagpu_grid *g, double out[LUMAGPU_NUM_MOMENTS]) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, g->moments_out, kMomentCount * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int lumagpu_moments_noalloc(lumagpu_grid *g, double *out_dev) {
+    if (int r = check_grid(g)) return r;
+    if (!out_dev) return fail(LUMAGPU_EINVAL, "null output");
+    if (!g->has_state) return fail(LUMAGPU_ESTATE, "no state");
+    const dim3 gr = cell_grid(g->d.N, g->d.M, g->d.K);
+    const size_t nblocks = (size_t)gr.x * gr.y * gr.z;
+    if (nblocks > kScratchBlocks)
+        return fail(LUMAGPU_EINVAL, "grid too large for lumagpu_moments_noalloc: " + std::to_string(nblocks) +
+                                        " row blocks of 256 cells, at most " + std::to_string(kScratchBlocks));
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != g->device) return fail(LUMAGPU_EINVAL, "call lumagpu_moments_noalloc with the grid's GPU current");
+    double *partials = nullptr;
+    if (int r = scratch_on(dev, &partials)) return r;
+    // every block writes its own partials and the sum writes every output: nothing to clear
+    HIP_TRY(launch_aux(g, kAuxMoments, nullptr, nullptr, partials));
+    k_sum_partials<<<1, 256, 0, g->stream>>>(partials, nblocks, out_dev);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

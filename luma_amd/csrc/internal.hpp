@@ -32,6 +32,7 @@ struct lumagpu_grid {
     size_t stage_bytes = 0;
     double *moments_out = nullptr;
     hipStream_t stream = nullptr;
+    int device = 0;                   // the GPU the grid was created on
     hipEvent_t ev_done = nullptr;     // recorded on this grid's stream after its sub-steps
     hipEvent_t ev_ready = nullptr;    // recorded before a child reads this grid's populations
     lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
