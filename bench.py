@@ -16,8 +16,9 @@ population read once and written once, 2*Q*s bytes per cell (304 B FP64, 152 B F
 duration, against 8 TB/s.  For the two-step kernel that is the HBM bandwidth it uses; the
 per-step figure (2*Q*s per cell per step, both steps counted) is reported beside it as
 `cell_update_gbs`.  `traffic` is the HBM bytes per launch from rocprofv3's FETCH_SIZE and
-WRITE_SIZE counters (profiles/tgv256_step_pmc.json), given only when that file was made from
-the kernel sources of this tree.
+WRITE_SIZE counters (profiles/tgv256_step_pmc.json, written by `bench.py --pmc-summary` from
+two counter runs of this script), given only when that file was made from the kernel sources of
+this tree.
 
 Also reported: the same step in all three forms, the velocity L2 error and kinetic-energy
 ratio of the shifted forms against FP64 after the configured 5,000 steps, and a CPU
@@ -205,7 +206,69 @@ def cpu_baseline(case, target_s=15.0, warmup=5):
                       f"wall clock, luma_active_cell_ops_per_time as LUMA's log computes it (CPU time)"}
 
 
+def pmc_summary(fetch_dir: str, write_dir: str, out_path: str, n: int = 256) -> dict:
+    """HBM bytes per launch of the step kernels from two rocprofv3 counter runs of this script
+    (one with --pmc FETCH_SIZE, one with --pmc WRITE_SIZE; counters in KiB per dispatch).  Each
+    counter is scaled by what it reads for a kernel of known traffic in the same run: k_moments
+    reads every population slot once (19*s B/cell), k_init_feq writes every slot once."""
+    import csv
+    import re
+
+    def load(d, counter):
+        vals = {}
+        for f in sorted(Path(d).rglob("*counter_collection.csv")):
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") == counter:
+                        vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]) * 1024.0)
+        if not vals:
+            raise SystemExit(f"no {counter} rows under {d}")
+        return vals
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    cells = n ** 3
+    forms = {"fp64": "P64", "fp32s": "P32S", "fp16s": "P16S"}
+
+    def mean_of(vals, pattern):
+        xs = [v for k, vs in vals.items() if re.search(pattern, k) for v in vs]
+        return sum(xs) / len(xs) if xs else None
+
+    def factor(vals, kernel):
+        fs = []
+        for form, P in forms.items():
+            m = mean_of(vals, rf"{kernel}<19, (lumagpu::)?{P}\b")
+            if m is not None:
+                fs.append(m / (19 * ELEM[form] * cells))
+        if not fs:
+            raise SystemExit(f"no {kernel} dispatches to calibrate against")
+        return sum(fs) / len(fs)
+
+    ff, wf = factor(fetch, "k_moments"), factor(write, "k_init_feq")
+    out = {"kernel_source_hash": kernel_source_hash(), "workload": f"D3Q19 TGV {n}^3", "fetch_factor": round(ff, 4),
+           "write_factor": round(wf, 4),
+           "method": "FETCH_SIZE / fetch_factor + WRITE_SIZE / write_factor, means over dispatches"}
+    for form, P in forms.items():
+        pat = rf"k_aa_periodic<19, (lumagpu::)?{P}, (true|false), false"
+        fm, wm = mean_of(fetch, pat), mean_of(write, pat)
+        if fm is not None and wm is not None:
+            out[form] = {"hbm_bytes_per_launch": round(fm / ff + wm / wf), "min_bytes_per_launch": 2 * 19 * ELEM[form] * cells,
+                         "steps_per_launch": 1}
+    pat = r"k_aa_fused2<(lumagpu::)?P32S"
+    fm, wm = mean_of(fetch, pat), mean_of(write, pat)
+    if fm is not None and wm is not None:
+        kn = sorted(k for k in fetch if re.search(pat, k))
+        out["fp32s_2step"] = {"hbm_bytes_per_launch": round(fm / ff + wm / wf), "min_bytes_per_launch": 2 * 19 * 4 * cells,
+                              "steps_per_launch": 2, "kernels": kn}
+    Path(out_path).write_text(json.dumps(out, indent=1) + "\n")
+    return out
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
+        if len(sys.argv) != 5:
+            sys.exit("usage: bench.py --pmc-summary FETCH_DIR WRITE_DIR OUT_JSON")
+        print(json.dumps(pmc_summary(*sys.argv[2:5])))
+        return
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)

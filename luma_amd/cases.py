@@ -164,6 +164,26 @@ def c4_eval(precision, U, tau, pos, steps, every, backward, shape=(384, 128, 128
     return out
 
 
+def run_c4_forward(precision: str, steps: int, body: bool = True, warmup: int = 20):
+    """Forward-step throughput of the C4 arrangement (384 x 128 x 128, forced-equilibrium inlet,
+    ExtrapolateRight outlet, the 1,810-marker sphere when ``body``): ``steps`` LBM_multi_opt calls
+    after ``warmup``, between two synchronisations."""
+    case = c4_case()
+    with GpuGrid.from_case(case, precision) as g:
+        if body:
+            g.ibm_add_body(c4_markers())
+            g.ibm_initialise()
+        g.run(warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(steps)
+        g.sync()
+        wall = time.perf_counter() - t0
+    cells = case.N * case.M * case.K
+    return {"config": "c4_forward", "precision": precision, "body": body, "steps": steps, "wall_s": wall,
+            "ms_per_step": 1e3 * wall / steps, "mlups": cells * steps / wall / 1e6}
+
+
 def run_c4(precisions=("fp64", "fp32s"), steps=4000, every=200, h=1e-4, fd_markers=(0, 605, 1210),
            shape=(384, 128, 128)):
     pos = c4_markers()
@@ -380,6 +400,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--sample", type=int, default=20)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--forward", action="store_true",
+                    help="c4: forward-step throughput with and without the sphere instead of the adjoint check")
     ap.add_argument("--grad", action="store_true",
                     help="c5: gradient of the mean tip amplitude with respect to bending stiffness")
     ap.add_argument("--check-steps", type=int, default=2000,
@@ -422,6 +444,16 @@ def main(argv=None):
                 if p != "fp64":
                     report[f"tip_max_diff_cells_{p}_vs_fp64"] = float(np.abs(series[p] - series["fp64"]).max() / dhf)
         print(json.dumps({k: v for k, v in report.items() if k != "runs"}), flush=True)
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump(report, fh, indent=1)
+        return 0
+    if a.config == "c4" and a.forward:
+        for p in precs:
+            for body in (False, True):
+                r = run_c4_forward(p, a.steps or 400, body)
+                report["runs"].append(r)
+                print(json.dumps(r), flush=True)
         if a.out:
             with open(a.out, "w") as fh:
                 json.dump(report, fh, indent=1)

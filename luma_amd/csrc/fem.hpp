@@ -13,6 +13,7 @@
 // and derivative; math functions are called unqualified so a Dual finds its own).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <functional>
@@ -29,6 +30,9 @@ class FemBeamT {
     using Mat6 = std::array<std::array<Sc, 6>, 6>;
     using Vec6 = std::array<Sc, 6>;
     using Solver = std::function<int(int, std::vector<Sc> &, std::vector<Sc> &)>;
+    // half-bandwidth of the assembled matrices: an element couples the six DOFs of its two
+    // nodes, so no entry lies more than five places from the diagonal
+    static constexpr int kBand = 5;
 
     struct Element {
         Sc length0 = 0, length = 0, area = 0, E = 0, density = 0, angle = 0, I = 0;
@@ -137,12 +141,20 @@ class FemBeamT {
         e.T[2][2] = e.T[5][5] = 1.0;
     }
 
+    // Positions set_rotation can make nonzero: the two 3 x 3 blocks, each an in-plane 2 x 2
+    // rotation and a 1.  The others hold +0 from Mat6{}, and the products below skip them: a +0
+    // factor gives a zero term, and adding a zero to a sum started at +0 leaves it as it is, so
+    // the results equal the full 6 x 6 sums bit for bit.
+    static constexpr bool tnz(int i, int j) {
+        return (i < 3) == (j < 3) && ((i % 3 < 2 && j % 3 < 2) || (i % 3 == 2 && j % 3 == 2));
+    }
     // T^T v
     static Vec6 rotate_back(const Mat6 &T, const Vec6 &v) {
         Vec6 out{};
         for (int r = 0; r < 6; ++r) {
             out[r] = 0.0;
-            for (int c = 0; c < 6; ++c) out[r] += T[c][r] * v[c];
+            for (int c = 0; c < 6; ++c)
+                if (tnz(c, r)) out[r] += T[c][r] * v[c];
         }
         return out;
     }
@@ -152,12 +164,14 @@ class FemBeamT {
         for (int r = 0; r < 6; ++r)
             for (int c = 0; c < 6; ++c) {
                 TL[r][c] = 0.0;
-                for (int k = 0; k < 6; ++k) TL[r][c] += T[k][r] * L[k][c];
+                for (int k = 0; k < 6; ++k)
+                    if (tnz(k, r)) TL[r][c] += T[k][r] * L[k][c];
             }
         for (int r = 0; r < 6; ++r)
             for (int c = 0; c < 6; ++c) {
                 G[r][c] = 0.0;
-                for (int k = 0; k < 6; ++k) G[r][c] += TL[r][k] * T[k][c];
+                for (int k = 0; k < 6; ++k)
+                    if (tnz(k, c)) G[r][c] += TL[r][k] * T[k][c];
             }
         return G;
     }
@@ -320,10 +334,18 @@ class FemBeamT {
     }
 
     // FEMBody::newtonRaphsonIterator (:183-218) with setNewmark (:264-293)
+    // The matrices are banded (kBand): entries off the band are written once, as +0, and never
+    // again, so the loops below visit the band only.  A skipped term is a product with a +0
+    // entry added to a sum started at +0, which leaves the sum as it is: the results equal the
+    // full loops' bit for bit.
     int newton_iteration(const Solver &solve) {
         std::fill(F_.begin(), F_.end(), 0.0);
-        std::fill(M_.begin(), M_.end(), 0.0);
-        std::fill(K_.begin(), K_.end(), 0.0);
+        for (const Element &e : els_)
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j) {
+                    M_[(size_t)(e.dof0 + i) * ndof_ + (e.dof0 + j)] = 0.0;
+                    K_[(size_t)(e.dof0 + i) * ndof_ + (e.dof0 + j)] = 0.0;
+                }
         for (size_t el = 0; el < els_.size(); ++el) {
             internal_force(els_[el], (int)el);
             mass(els_[el]);
@@ -332,17 +354,20 @@ class FemBeamT {
         const double a0 = 1.0 / (kAlpha * (dt_ * dt_)), a2 = 1.0 / (kAlpha * dt_), a3 = 1.0 / (2.0 * kAlpha) - 1.0;
         std::vector<Sc> meff(ndof_), mf(ndof_, Sc(0.0));
         for (int i = 0; i < ndof_; i++) meff[i] = a0 * (Un_[i] - U_[i]) + a2 * V_[i] + a3 * A_[i];
+        auto lo = [&](int r) { return std::max(0, r - kBand); };
+        auto hi = [&](int r) { return std::min(ndof_ - 1, r + kBand); };
         for (int r = 0; r < ndof_; ++r)
-            for (int c = 0; c < ndof_; ++c) mf[r] += M_[(size_t)r * ndof_ + c] * meff[c];
+            for (int c = lo(r); c <= hi(r); ++c) mf[r] += M_[(size_t)r * ndof_ + c] * meff[c];
         for (int i = 0; i < ndof_; i++) {
             F_[i] = R_[i] - F_[i] + mf[i];
-            for (int j = 0; j < ndof_; j++) K_[(size_t)i * ndof_ + j] += a0 * M_[(size_t)i * ndof_ + j];
+            for (int j = lo(i); j <= hi(i); j++) K_[(size_t)i * ndof_ + j] += a0 * M_[(size_t)i * ndof_ + j];
         }
         // GridUtils::solveLinearSystem(K, F, BC): the block past the fixed DOFs
         const int m = ndof_ - bc_;
-        std::vector<Sc> a((size_t)m * m), x(m);
+        std::vector<Sc> a((size_t)m * m, Sc(0.0)), x(m);
         for (int i = 0; i < m; ++i) {
-            for (int j = 0; j < m; ++j) a[(size_t)i * m + j] = K_[(size_t)(bc_ + i) * ndof_ + (bc_ + j)];
+            for (int j = std::max(0, i - kBand); j <= std::min(m - 1, i + kBand); ++j)
+                a[(size_t)i * m + j] = K_[(size_t)(bc_ + i) * ndof_ + (bc_ + j)];
             x[i] = F_[bc_ + i];
         }
         if (int r = solve(m, a, x)) return r;

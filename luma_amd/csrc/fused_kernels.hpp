@@ -261,10 +261,15 @@ __global__ __launch_bounds__((TX + 2) * (TY + 2)) void k_aa_fused2(StepArgs a, v
     if constexpr (PF) {
         S bufA[Q], bufB[Q];
         load(wrapz(z0 + 1), bufA);
-        for (int p = z0; p < z1; p += 2) {
+        // pairs of planes with no branch inside the loop body (a conditional second plane merges
+        // a path with bufB's loads still in flight into the loop header, whose wait would then
+        // cover every outstanding load and store), then an odd last plane
+        const int zp = z0 + ((z1 - z0) & ~1);
+        for (int p = z0; p < zp; p += 2) {
             plane_step(p, bufA, bufB);
-            if (p + 1 < z1) plane_step(p + 1, bufB, bufA);
+            plane_step(p + 1, bufB, bufA);
         }
+        if (zp < z1) plane_step(zp, bufA, bufB);
     } else {
         for (int p = z0; p < z1; ++p) plane_step(p, nullptr, nullptr);
     }
@@ -323,6 +328,7 @@ hipError_t launch_fused2_p(int shape, const StepArgs &a, void *dst, int zc, bool
         if (shape == 8 + kFuse32x16) return launch_fused2_t<P, 32, 16, true>(a, dst, zc, store, s);
         if (shape == 8 + kFuse128x4) return launch_fused2_t<P, 128, 4, true>(a, dst, zc, store, s);
         if (shape == 24 + kFuse64x8) return launch_fused2_t<P, 64, 8, true, true>(a, dst, zc, store, s);
+        if (shape == 24 + kFuse32x16) return launch_fused2_t<P, 32, 16, true, true>(a, dst, zc, store, s);
         if (shape == 24 + kFuse128x4) return launch_fused2_t<P, 128, 4, true, true>(a, dst, zc, store, s);
         if (shape == 40 + kFuse64x8) return launch_fused2_t<P, 64, 8, true, false, true>(a, dst, zc, store, s);
         if (shape == 40 + kFuse32x16) return launch_fused2_t<P, 32, 16, true, false, true>(a, dst, zc, store, s);

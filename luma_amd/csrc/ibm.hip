@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -254,6 +255,74 @@ int lu_solve_t(int n, std::vector<Sc> &a, std::vector<Sc> &x) {
 int lu_solve(int n, std::vector<double> &a, std::vector<double> &x) { return lu_solve_t(n, a, x); }
 int lu_solve(int n, std::vector<Dual> &a, std::vector<Dual> &x) { return lu_solve_t(n, a, x); }
 
+// lu_solve_t for a matrix A whose entries more than kl below or ku above the diagonal are zero
+// (a beam's effective stiffness: kl = ku = 5).  The loops skip only positions that hold exact
+// zeros -- pivot candidates past row j + kl, multipliers past it, update columns past
+// j + kl + ku (the reach of U under row pivoting), the same reach in the first substitution --
+// and every operation on the other entries is lu_solve_t's, in its order.  A skipped term
+// changes no value: a zero never wins the pivot search, the rank-1 update already passes over
+// zero pivot-row entries, and subtracting a zero product leaves a nonzero sum as it is.  (A
+// zero solution component may come out as -0 where lu_solve_t gives +0 or the reverse; the
+// beam adds the solution to displacements that are never -0 and squares it for the residual,
+// so no result sees the difference.)  The L multipliers of earlier columns move down with later
+// row swaps, so after any swap the second substitution runs in full.
+template <class Sc>
+int lu_solve_band_t(int n, int kl, int ku, std::vector<Sc> &a, std::vector<Sc> &x) {
+    auto M = [&](int i, int j) -> Sc & { return a[(size_t)j * n + i]; };
+    std::vector<int> piv(n), rows;
+    rows.reserve(kl);
+    bool swapped = false;
+    for (int j = 0; j < n; ++j) {
+        const int ilast = std::min(n - 1, j + kl);
+        int p = j;
+        double best = std::fabs(value_of(M(j, j)));
+        for (int i = j + 1; i <= ilast; ++i)
+            if (std::fabs(value_of(M(i, j))) > best) {
+                best = std::fabs(value_of(M(i, j)));
+                p = i;
+            }
+        piv[j] = p;
+        if (value_of(M(p, j)) == 0.0) return fail(LUMAGPU_EPHYS, "singular beam system");
+        if (p != j) {
+            swapped = true;
+            for (int k = 0; k < n; ++k) std::swap(M(j, k), M(p, k));
+        }
+        const Sc r = Sc(1.0) / M(j, j);
+        rows.clear();
+        for (int i = j + 1; i <= ilast; ++i) {
+            M(i, j) *= r;
+            if (!is_zero(M(i, j))) rows.push_back(i);
+        }
+        const int klast = std::min(n - 1, j + kl + ku);
+        for (int k = j + 1; k <= klast; ++k) {
+            const Sc t = M(j, k);
+            if (!is_zero(t))
+                for (int i : rows) M(i, k) -= M(i, j) * t;
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        Sc t = x[j];
+        for (int i = std::max(0, j - (kl + ku)); i < j; ++i) t -= M(i, j) * x[i];
+        x[j] = t / M(j, j);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        Sc t = x[j];
+        const int ilast = swapped ? n - 1 : std::min(n - 1, j + kl);
+        for (int i = j + 1; i <= ilast; ++i) t -= M(i, j) * x[i];
+        x[j] = t;
+    }
+    for (int j = n - 1; j >= 0; --j)
+        if (piv[j] != j) std::swap(x[j], x[piv[j]]);
+    return 0;
+}
+
+int lu_solve_band(int n, int kl, int ku, std::vector<double> &a, std::vector<double> &x) {
+    return lu_solve_band_t(n, kl, ku, a, x);
+}
+int lu_solve_band(int n, int kl, int ku, std::vector<Dual> &a, std::vector<Dual> &x) {
+    return lu_solve_band_t(n, kl, ku, a, x);
+}
+
 namespace {
 
 // ibm_findEpsilon, :773-920: A_IJ = ds_J * sum_s delta_I(s) delta(x_J - x_s) A_I, A eps = 1.
@@ -360,8 +429,17 @@ int upload(lumagpu_grid *g) {
     };
     std::vector<int> scell((size_t)nm * S, 0), ns(nm);
     std::vector<double> sdelta((size_t)nm * S * w, 0.0), vel((size_t)nm * 3 * w), eps(nm * w), ds(nm * w);
-    std::vector<int> site_of(g->ncell, -1), order;
+    std::vector<int> &site_of = s->site_of, order;
+    if (site_of.size() != g->ncell) site_of.assign(g->ncell, -1);
     std::vector<std::vector<int>> entries;
+    // back to all -1 on every exit
+    struct Reset {
+        std::vector<int> &so;
+        const std::vector<int> &cells;
+        ~Reset() {
+            for (int c : cells) so[c] = -1;
+        }
+    } reset{site_of, order};
     for (const auto &b : s->bodies)
         for (size_t mi = 0; mi < b.m.size(); ++mi) {
             const Marker &m = b.m[mi];
@@ -394,18 +472,55 @@ int upload(lumagpu_grid *g) {
     }
     // Interpolated values and marker forces persist across re-uploads (a marker keeps them
     // when its body moves); they start at zero when the marker set changes size.
+    struct Part {
+        void **dev;
+        const void *src;
+        size_t bytes, off;
+    };
+    Part parts[] = {{(void **)&s->d_scell, scell.data(), scell.size() * sizeof(int), 0},
+                    {(void **)&s->d_ns, ns.data(), ns.size() * sizeof(int), 0},
+                    {(void **)&s->d_sites, order.data(), order.size() * sizeof(int), 0},
+                    {(void **)&s->d_sptr, sptr.data(), sptr.size() * sizeof(int), 0},
+                    {(void **)&s->d_sent, sent.data(), sent.size() * sizeof(int), 0},
+                    {(void **)&s->d_sdelta, sdelta.data(), sdelta.size() * sizeof(double), 0},
+                    {(void **)&s->d_vel, vel.data(), vel.size() * sizeof(double), 0},
+                    {(void **)&s->d_eps, eps.data(), eps.size() * sizeof(double), 0},
+                    {(void **)&s->d_ds, ds.data(), ds.size() * sizeof(double), 0}};
+    size_t total = 0;
+    for (Part &q : parts) {
+        q.off = total;
+        total += (q.bytes + 255) & ~size_t(255);
+    }
+    if (!s->pack_ev && hipEventCreateWithFlags(&s->pack_ev, hipEventDisableTiming) != hipSuccess)
+        return fail(LUMAGPU_EHIP, "IB staging event creation failed");
+    if (total > s->pack_cap) {
+        // kernels still queued may read the old arena
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (s->d_pack) (void)hipFree(s->d_pack);
+        if (s->h_pack) (void)hipHostFree(s->h_pack);
+        s->d_pack = s->h_pack = nullptr;
+        s->pack_cap = 0;
+        const size_t cap = 2 * total;
+        if (hipMalloc(&s->d_pack, cap) != hipSuccess || hipHostMalloc(&s->h_pack, cap, hipHostMallocDefault) != hipSuccess)
+            return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
+        s->pack_cap = cap;
+    } else {
+        // the previous copy out of the staging buffer has finished
+        HIP_TRY(hipEventSynchronize(s->pack_ev));
+    }
+    for (const Part &q : parts) {
+        if (q.bytes) std::memcpy(static_cast<char *>(s->h_pack) + q.off, q.src, q.bytes);
+        *q.dev = static_cast<char *>(s->d_pack) + q.off;
+    }
+    if (total) HIP_TRY(hipMemcpyAsync(s->d_pack, s->h_pack, total, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipEventRecord(s->pack_ev, g->stream));
     int r = 0;
     size_t *c = s->cap;
-    if ((r = dev_upload(&s->d_scell, c + 0, scell, g->stream)) || (r = dev_upload(&s->d_ns, c + 1, ns, g->stream)) ||
-        (r = dev_upload(&s->d_sites, c + 2, order, g->stream)) || (r = dev_upload(&s->d_sptr, c + 3, sptr, g->stream)) ||
-        (r = dev_upload(&s->d_sent, c + 4, sent, g->stream)) || (r = dev_upload(&s->d_sdelta, c + 5, sdelta, g->stream)) ||
-        (r = dev_upload(&s->d_vel, c + 6, vel, g->stream)) || (r = dev_upload(&s->d_eps, c + 7, eps, g->stream)) ||
-        (r = dev_upload(&s->d_ds, c + 8, ds, g->stream)) || (r = dev_resize(&s->d_interp, c + 9, (size_t)nm * 4 * w, g->stream)) ||
+    if ((r = dev_resize(&s->d_interp, c + 9, (size_t)nm * 4 * w, g->stream)) ||
         (r = dev_resize(&s->d_force, c + 10, (size_t)nm * 3 * w, g->stream)))
         return r;
     if (!s->d_out && hipMalloc(&s->d_out, 6 * sizeof(double)) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
-    HIP_TRY(hipStreamSynchronize(g->stream));
     if (order != s->sites) s->sites_dirty = 1;
     s->sites = order;
     s->nsites = (int)order.size();
@@ -482,22 +597,37 @@ int clear_gone(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
     // a gravity reset has rewritten every site's force: nothing to clear
     if (s->sites_dirty && !s->flagged.empty() && !g->all_forced) {
-        std::vector<char> now(g->ncell, 0);
-        for (int c : s->sites) now[c] = 1;
+        // site_of is all -1 between uses: mark the current sites, collect, unmark
+        std::vector<int> &mark = s->site_of;
+        if (mark.size() != g->ncell) mark.assign(g->ncell, -1);
+        for (int c : s->sites) mark[c] = 0;
         std::vector<int> gone;
         for (int c : s->flagged)
-            if (!now[c]) gone.push_back(c);
+            if (mark[c] < 0) gone.push_back(c);
+        for (int c : s->sites) mark[c] = -1;
         if (!gone.empty()) {
-            int *d_gone = nullptr;
-            size_t n_gone = 0;
-            if (int r = dev_upload(&d_gone, &n_gone, gone, g->stream)) return r;
+            if (gone.size() > s->gone_cap) {
+                HIP_TRY(hipStreamSynchronize(g->stream));
+                if (s->d_gone) (void)hipFree(s->d_gone);
+                if (s->h_gone) (void)hipHostFree(s->h_gone);
+                s->d_gone = s->h_gone = nullptr;
+                s->gone_cap = 0;
+                const size_t cap = 2 * gone.size();
+                if (hipMalloc(&s->d_gone, cap * sizeof(int)) != hipSuccess ||
+                    hipHostMalloc(&s->h_gone, cap * sizeof(int), hipHostMallocDefault) != hipSuccess)
+                    return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
+                s->gone_cap = cap;
+            }
+            std::memcpy(s->h_gone, gone.data(), gone.size() * sizeof(int));
+            HIP_TRY(hipMemcpyAsync(s->d_gone, s->h_gone, gone.size() * sizeof(int), hipMemcpyHostToDevice, g->stream));
             IbmArgs b = ibm_args(g);
             const int nb = ((int)gone.size() + 127) / 128;
-            if (tangent_form(g)) k_ibm_clear<Dual><<<nb, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
-            else k_ibm_clear<double><<<nb, 128, 0, g->stream>>>(b, (int)gone.size(), d_gone);
+            if (tangent_form(g)) k_ibm_clear<Dual><<<nb, 128, 0, g->stream>>>(b, (int)gone.size(), s->d_gone);
+            else k_ibm_clear<double><<<nb, 128, 0, g->stream>>>(b, (int)gone.size(), s->d_gone);
             HIP_TRY(hipGetLastError());
+            // h_gone is free again once the copy has run; the spread that follows needs the clear
+            // ordered before it, which the stream gives
             HIP_TRY(hipStreamSynchronize(g->stream));
-            (void)hipFree(d_gone);
             for (int c : gone) g->hflags[c] &= (uint8_t)~kForcedBit;
         }
     }
@@ -570,7 +700,7 @@ int move_bodies_tangent(lumagpu_grid *g) {
     HIP_TRY(hipMemcpyAsync(force.data(), s->d_force, force.size() * sizeof(Dual), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     const FemBeamT<Dual>::Solver solver = [](int n, std::vector<Dual> &a, std::vector<Dual> &x) {
-        return lu_solve(n, a, x);
+        return lu_solve_band(n, FemBeamT<Dual>::kBand, FemBeamT<Dual>::kBand, a, x);
     };
     for (auto &b : s->bodies) {
         if (!b.flexible || !b.femd) continue;
@@ -614,10 +744,21 @@ int move_bodies_tangent(lumagpu_grid *g) {
 int move_bodies(lumagpu_grid *g) {
     if (tangent_form(g)) return move_bodies_tangent(g);
     IbmSystem *s = g->ibm;
-    std::vector<double> force((size_t)s->nm * 3);
-    HIP_TRY(hipMemcpyAsync(force.data(), s->d_force, force.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    const size_t nf = (size_t)s->nm * 3;
+    if (nf > s->h_force_cap) {
+        if (s->h_force) (void)hipHostFree(s->h_force);
+        s->h_force = nullptr;
+        s->h_force_cap = 0;
+        if (hipHostMalloc(&s->h_force, nf * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            return fail(LUMAGPU_ENOMEM, "IB force buffer allocation failed");
+        s->h_force_cap = nf;
+    }
+    HIP_TRY(hipMemcpyAsync(s->h_force, s->d_force, nf * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
-    const FemBeam::Solver solver = [](int n, std::vector<double> &a, std::vector<double> &x) { return lu_solve(n, a, x); };
+    const double *force = s->h_force;
+    const FemBeam::Solver solver = [](int n, std::vector<double> &a, std::vector<double> &x) {
+        return lu_solve_band(n, FemBeam::kBand, FemBeam::kBand, a, x);
+    };
     for (auto &b : s->bodies) {
         if (!b.flexible || !b.fem) continue;
         const size_t n = b.m.size();
@@ -631,7 +772,7 @@ int move_bodies(lumagpu_grid *g) {
                 vkm1[3 * i + a] = b.m[i].vel_km1[a];
             }
         }
-        if (int r = b.fem->dynamic(force.data() + 3 * (size_t)b.first, eps.data(), pos.data(), pos0.data(), vel.data(),
+        if (int r = b.fem->dynamic(force + 3 * (size_t)b.first, eps.data(), pos.data(), pos0.data(), vel.data(),
                                    vkm1.data(), solver))
             return r;
         for (size_t i = 0; i < n; ++i)

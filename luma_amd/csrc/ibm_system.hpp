@@ -65,11 +65,21 @@ struct IbmSystem {
     int S = 9;
     int initialised = 0;
     int universal = 0;          // L_UNIVERSAL_EPSILON_CALC: one epsilon system over all bodies
-    // device copies
+    // device copies: the marker and support arrays point into one arena, filled from a pinned
+    // staging buffer by one copy per upload (a moving filament re-uploads every FSI
+    // sub-iteration); the arena only grows
     int *d_scell = nullptr, *d_ns = nullptr, *d_sites = nullptr, *d_sptr = nullptr, *d_sent = nullptr;
     double *d_sdelta = nullptr, *d_vel = nullptr, *d_eps = nullptr, *d_ds = nullptr;
+    void *d_pack = nullptr, *h_pack = nullptr;
+    size_t pack_cap = 0;        // bytes of d_pack and h_pack
+    hipEvent_t pack_ev = nullptr;   // recorded after the last copy out of h_pack
     double *d_interp = nullptr, *d_force = nullptr, *d_out = nullptr;
-    size_t cap[11] = {};        // element counts of the device buffers above, in declaration order
+    size_t cap[11] = {};        // element counts of d_interp (9) and d_force (10)
+    double *h_force = nullptr;  // pinned download buffer of the marker forces
+    size_t h_force_cap = 0;
+    int *d_gone = nullptr, *h_gone = nullptr;   // cells to clear (clear_gone), device and pinned host
+    size_t gone_cap = 0;
+    std::vector<int> site_of;   // per cell: index of the support site, -1 elsewhere (kept all -1 between uses)
     int nsites = 0;
     std::vector<int> sites;     // host copy of the distinct support cells
     std::vector<int> flagged;   // cells whose force and flag the device currently holds
@@ -81,10 +91,11 @@ struct IbmSystem {
     size_t pinned_pos = 0;
 
     ~IbmSystem() {
-        for (void *p : {(void *)d_scell, (void *)d_ns, (void *)d_sites, (void *)d_sptr, (void *)d_sent,
-                        (void *)d_sdelta, (void *)d_vel, (void *)d_eps, (void *)d_ds, (void *)d_interp,
-                        (void *)d_force, (void *)d_out})
+        for (void *p : {d_pack, (void *)d_interp, (void *)d_force, (void *)d_out, (void *)d_gone})
             if (p) (void)hipFree(p);
+        for (void *p : {h_pack, (void *)h_force, (void *)h_gone})
+            if (p) (void)hipHostFree(p);
+        if (pack_ev) (void)hipEventDestroy(pack_ev);
     }
 };
 
@@ -112,5 +123,8 @@ int apply_core(lumagpu_grid *g);
 // A x = rhs, row-major A, LAPACK dgetf2 + dgetrs('T') operation order
 int lu_solve(int n, std::vector<double> &a, std::vector<double> &x);
 int lu_solve(int n, std::vector<Dual> &a, std::vector<Dual> &x);
+// the same solve for a matrix with kl sub- and ku super-diagonals (zeros elsewhere)
+int lu_solve_band(int n, int kl, int ku, std::vector<double> &a, std::vector<double> &x);
+int lu_solve_band(int n, int kl, int ku, std::vector<Dual> &a, std::vector<Dual> &x);
 
 }  // namespace lumagpu

@@ -29,15 +29,17 @@ def _run(case, prec, nsteps, fuse=None, parity_first=False):
 
 CASES = [(p, s, z) for p in ("fp64", "fp32s", "fp16s") for s, z in ((0, 7), (1, 5), (2, 64), (0, 128))]
 CASES += [(p, s, z) for p in ("fp32s", "fp16s") for s, z in ((8, 6), (9, 7), (10, 128), (10, 3))]
-# loads one plane ahead (24, 26), windowed LDS copies (40, 41) and both (57)
-CASES += [(p, s, z) for p in ("fp32s", "fp16s") for s, z in ((24, 7), (26, 128), (40, 6), (41, 5), (57, 3), (57, 128))]
+# loads one plane ahead (24, 25, 26; zc 7 and 5 leave an odd last plane pair), windowed LDS
+# copies (40, 41) and both (57)
+CASES += [(p, s, z) for p in ("fp32s", "fp16s")
+          for s, z in ((24, 7), (24, 6), (25, 5), (25, 128), (26, 128), (40, 6), (41, 5), (57, 3), (57, 128))]
 
 
 @pytest.mark.parametrize("prec,shape,zc", CASES)
 def test_two_step_kernel_matches_single_steps(prec, shape, zc):
     # 128 x 32 x 20: two or four tiles in x, two to eight in y, z chunks that do not divide the
     # depth (zc = 7, 5, 6, 3) and one chunk over the whole depth (zc = 64, 128); shapes 8 and 10
-    # alternate two LDS copies, 24 and 26 load a plane ahead, 40 and 41 keep only the tile's
+    # alternate two LDS copies, 24 to 26 load a plane ahead, 40 and 41 keep only the tile's
     # windows in LDS, 57 both
     case = _tgv((128, 32, 20))
     print(f"[fused] {prec} shape={shape} zc={zc}", flush=True)
