@@ -82,8 +82,10 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out);
 int lumagpu_grid_destroy(lumagpu_grid *g);
 
 /* GridObj::LBM_addSubGrid / LBM_initSubGrid (GridObj.cpp:92-118, GridObj_init_grids.cpp:390-536):
- * attach `child` (level + 1, extents 2*(coarse_lims max - min + 1)) as the refined region of
- * `parent`.  Stepping the parent then steps the child twice first, as LBM_multi_opt does. */
+ * attach `child` (level + 1, extents 2*(coarse_lims max - min + 1)) as a refined region of
+ * `parent` (GridObj::subGrid; up to 16 per grid, disjoint, else LUMAGPU_EINVAL /
+ * LUMAGPU_EUNSUPPORTED).  Stepping the parent then steps each child twice first, in the order
+ * they were added, as LBM_multi_opt does. */
 int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child);
 
 /* ---- state upload / download --------------------------------------------------------- */

@@ -73,6 +73,18 @@ struct P32T {
     static constexpr bool kShift = true;
 };
 
+// One refined region of a grid (GridObj::subGrid[i]): its populations (coalesce source, read
+// in the post-local layout), extents, CoarseLims{X,Y,Z}[eMinimum/eMaximum] on this grid, and
+// macro arrays (TL2F copy, :850-891).
+struct ChildRef {
+    const void *f;
+    size_t ps;
+    int nx, ny, nz;
+    int lo[3], hi[3];
+    double *rho, *u;
+    size_t ncell;
+};
+
 struct StepArgs {
     void *f;                      // population array
     size_t ps;                    // population stride in elements
@@ -105,12 +117,8 @@ struct StepArgs {
     int pnx, pny, pnz, pparity;
     const uint8_t *pflags;
     int clx, cly, clz;            // this grid's CoarseLims{X,Y,Z}[eMinimum] on the parent
-    const void *cf;               // child population array (coalesce source), post-local layout
-    size_t cps;
-    int cnx, cny, cnz;
-    int cclx, ccly, cclz;         // the child's CoarseLims{X,Y,Z}[eMinimum] on this grid
-    double *crho, *cu;            // child macro arrays (TL2F copy, :850-891)
-    size_t cncell;
+    const ChildRef *kids;         // GridObj::subGrid: the refined regions (coalesce sources), device table
+    int nkids;
     int subcycle;
     // --- ExtrapolateRight sources: xr[v*(ny*nz) + y + ny*z] = f*_v two cells left of (N-1, y, z)
     const double *xr;
@@ -827,6 +835,18 @@ __device__ __forceinline__ typename P::T post_of(const void *fv, size_t ps, int 
     return ld<P, false>(f + (size_t)v * ps + dst);
 }
 
+// GridUtils::getSubGrid (GridUtils.cpp:1761-1799): the refined region whose coarse limits hold
+// site (x, y, z), or null
+template <int D>
+__device__ __forceinline__ const ChildRef *child_at(const StepArgs &a, int x, int y, int z) {
+    for (int k = 0; k < a.nkids; ++k) {
+        const ChildRef &c = a.kids[k];
+        if (x >= c.lo[0] && x <= c.hi[0] && y >= c.lo[1] && y <= c.hi[1] && (D == 2 || (z >= c.lo[2] && z <= c.hi[2])))
+            return &c;
+    }
+    return nullptr;
+}
+
 // Incoming (streamed, pre-collision) populations of one site: _LBM_stream_opt (:206-297) with
 // LUMA's source precedence: solid (HWBB), ExtrapolateRight, forced-equilibrium velocity,
 // explode from the parent (subcycle 0), coalesce from the child, regular pull.
@@ -839,6 +859,8 @@ __device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z
     const S *__restrict__ f = static_cast<const S *>(a.f);
     const size_t ps = a.ps;
     const int nx = a.nx, ny = a.ny;
+    // the region a TL2F site coalesces from
+    const ChildRef *kid = (a.nkids && ty == kTL2F) ? child_at<L::D>(a, x, y, z) : nullptr;
     static_for<Q>([&](auto vc) {
         constexpr int v = decltype(vc)::value;
         constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
@@ -862,16 +884,16 @@ __device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z
             const int sx = nb.xs[1 - cx], sy = nb.ys[1 - cy], sz = nb.zs[1 - cz];
             fin[v] = post_of<Q, P>(a.pf, a.pps, a.pnx, a.pny, a.pnz, a.pflags, a.pparity, (sx >> 1) + a.clx,
                                    (sy >> 1) + a.cly, (L::D == 3) ? (sz >> 1) + a.clz : 0, v);
-        } else if (a.cf && sty == kRefined && ty == kTL2F) {
+        } else if (kid && sty == kRefined) {
             // coalesce, :591-636: average of the child's values over this cell's 2^D children
-            const S *cfp = static_cast<const S *>(a.cf);
-            const int fx = 2 * (x - a.cclx), fy = 2 * (y - a.ccly), fz = (L::D == 3) ? 2 * (z - a.cclz) : 0;
+            const S *cfp = static_cast<const S *>(kid->f);
+            const int fx = 2 * (x - kid->lo[0]), fy = 2 * (y - kid->lo[1]), fz = (L::D == 3) ? 2 * (z - kid->lo[2]) : 0;
             T acc = T(0);
             for (int ii = 0; ii < 2; ++ii)
                 for (int jj = 0; jj < 2; ++jj)
                     for (int kk = 0; kk < (L::D == 3 ? 2 : 1); ++kk)
-                        acc += ld<P, false>(cfp + (size_t)L::opp(v) * a.cps +
-                                            cell_of(fx + ii, fy + jj, fz + kk, a.cnx, a.cny));
+                        acc += ld<P, false>(cfp + (size_t)L::opp(v) * kid->ps +
+                                            cell_of(fx + ii, fy + jj, fz + kk, kid->nx, kid->ny));
             if constexpr (L::D == 3) acc /= T(8.0);
             else acc /= T(4.0);
             fin[v] = acc;
@@ -1027,15 +1049,16 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     }
     if constexpr (STORE) {
         if (ty == kFluid || ty == kTL2F || ty == kVelocity) store_macro<L::D>(a, cell, rho, u);
-        if (ty == kTL2F && a.crho) {
+        const ChildRef *kid = (ty == kTL2F && a.nkids) ? child_at<L::D>(a, x, y, z) : nullptr;
+        if (kid && kid->rho) {
             // child TL cluster gets this site's rho, u (:850-891)
-            const int fx = 2 * (x - a.cclx), fy = 2 * (y - a.ccly), fz = (L::D == 3) ? 2 * (z - a.cclz) : 0;
+            const int fx = 2 * (x - kid->lo[0]), fy = 2 * (y - kid->lo[1]), fz = (L::D == 3) ? 2 * (z - kid->lo[2]) : 0;
             for (int ii = 0; ii < 2; ++ii)
                 for (int jj = 0; jj < 2; ++jj)
                     for (int kk = 0; kk < (L::D == 3 ? 2 : 1); ++kk) {
-                        const size_t cc = cell_of(fx + ii, fy + jj, fz + kk, a.cnx, a.cny);
-                        mv<M>(a.crho)[cc] = rho;
-                        for (int d = 0; d < L::D; ++d) mv<M>(a.cu)[d * a.cncell + cc] = u[d];
+                        const size_t cc = cell_of(fx + ii, fy + jj, fz + kk, kid->nx, kid->ny);
+                        mv<M>(kid->rho)[cc] = rho;
+                        for (int d = 0; d < L::D; ++d) mv<M>(kid->u)[d * kid->ncell + cc] = u[d];
                     }
         }
     }

@@ -65,7 +65,7 @@ hipError_t launch_adj_stream(const lumagpu_grid *g, const StepArgs &a, const Adj
 int check_adjoint_grid(const lumagpu_grid *g) {
     if (g->d.precision != LUMAGPU_FP64 && g->d.precision != LUMAGPU_FP32S)
         return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs in the FP64 and FP32-shifted forms");
-    if (g->parent || g->child) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs on a single grid level");
+    if (g->parent || !g->kids.empty()) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs on a single grid level");
     if (g->zslab || g->nedge) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs or edge sites");
     if (g->gravity_on) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover the gravity reset");
     if (g->ramp_time > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a velocity ramp");

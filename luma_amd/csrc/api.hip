@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -184,8 +185,7 @@ int begin_step(lumagpu_grid *g) {
         hipError_t e = launch_extrap(g);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
     }
-    if (g->child) {
-        lumagpu_grid *c = g->child;
+    for (lumagpu_grid *c : g->kids) {
         // the child reads this grid's populations, so order the two streams
         HIP_TRY(hipEventRecord(c->ev_done, c->stream));
         HIP_TRY(hipStreamWaitEvent(g->stream, c->ev_done, 0));
@@ -251,10 +251,10 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     if (!g->has_state) return fail(LUMAGPU_ESTATE, "step before the populations were initialised");
     if (g->half) return fail(LUMAGPU_ESTATE, "a split step is open on this grid: finish it with lumagpu_force_collide_swap");
     if (g->re_ramp > 0.0) update_reynolds(g);
-    // GridObj_ops_lbm_optimised.cpp:45-49: two sub-steps of the child first
-    if (g->child)
+    // GridObj_ops_lbm_optimised.cpp:45-49: two sub-steps of each child first, child by child
+    for (lumagpu_grid *c : g->kids)
         for (int sc = 0; sc < 2; ++sc)
-            if (int r = apply_step(g->child, sc, nullptr, nullptr)) return r;
+            if (int r = apply_step(c, sc, nullptr, nullptr)) return r;
     if (int r = begin_step(g)) return r;
     // ObjectManager::ibm_apply between sweep 1 and sweep 2 (:120-121)
     if (g->ibm)
@@ -271,7 +271,7 @@ namespace {
 // neighbour (odd) step, so it starts at parity 1 (the local step's input layout) and leaves the
 // grid there.
 bool fused_ready(const lumagpu_grid *g) {
-    return g->fuse_shape >= 0 && g->f2 && !g->masked && !g->zslab && !g->parent && !g->child &&
+    return g->fuse_shape >= 0 && g->f2 && !g->masked && !g->zslab && !g->parent && g->kids.empty() &&
            ibm_marker_count(g) == 0 && !(g->re_ramp > 0.0) && g->parity == 1 && g->has_state;
 }
 
@@ -334,17 +334,19 @@ int run_fused_graph(lumagpu_grid *g, int reps) {
 // no host work inside a step anywhere in the hierarchy (flexible bodies download forces and
 // run the FEM on the host every step)
 bool graph_ok(const lumagpu_grid *g) {
-    for (const lumagpu_grid *x = g; x; x = x->child) {
-        if (ibm_has_flexible(x)) return false;
-        if (x->ramp_time > 0.0) return false;   // the ramp factors change every step
-        if (x->re_ramp > 0.0) return false;     // so does omega under a Reynolds ramp
-        if (x->fuse_shape >= 0) return false;   // the two-step kernel alternates two arrays
-    }
-    return !g->parent;
+    bool ok = !g->parent;
+    for_each_level(g, [&](const lumagpu_grid *x) {
+        if (ibm_has_flexible(x)) ok = false;
+        if (x->ramp_time > 0.0) ok = false;   // the ramp factors change every step
+        if (x->re_ramp > 0.0) ok = false;     // so does omega under a Reynolds ramp
+        if (x->fuse_shape >= 0) ok = false;   // the two-step kernel alternates two arrays
+    });
+    return ok;
 }
 
 void add_time(lumagpu_grid *g, int steps) {
-    for (lumagpu_grid *x = g; x; x = x->child, steps *= 2) x->t += steps;
+    g->t += steps;
+    for (lumagpu_grid *k : g->kids) add_time(k, 2 * steps);
 }
 
 // Capture two steps of g (and its descendants) into a graph and launch it reps times.
@@ -352,13 +354,13 @@ void add_time(lumagpu_grid *g, int steps) {
 // restored), or an error code.
 int run_graph(lumagpu_grid *g, int reps) {
     std::vector<int> t0;
-    for (lumagpu_grid *x = g; x; x = x->child) t0.push_back(x->t);
+    for_each_level(g, [&](lumagpu_grid *x) { t0.push_back(x->t); });
     auto restore = [&]() {
-        int i = 0;
-        for (lumagpu_grid *x = g; x; x = x->child) {
+        size_t i = 0;
+        for_each_level(g, [&](lumagpu_grid *x) {
             x->t = t0[i++];
             x->parity = 0;
-        }
+        });
     };
     if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         (void)hipGetLastError();
@@ -383,15 +385,17 @@ int run_graph(lumagpu_grid *g, int reps) {
     // the graph runs on g's stream: order it after whatever the descendants' streams hold,
     // and the descendants' later work after it
     hipError_t e = hipSuccess;
-    for (lumagpu_grid *x = g->child; x && e == hipSuccess; x = x->child) {
+    for_each_level(g, [&](lumagpu_grid *x) {
+        if (x == g || e != hipSuccess) return;
         e = hipEventRecord(x->ev_done, x->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, x->ev_done, 0);
-    }
+    });
     for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(exec, g->stream);
-    for (lumagpu_grid *x = g->child; x && e == hipSuccess; x = x->child) {
+    for_each_level(g, [&](lumagpu_grid *x) {
+        if (x == g || e != hipSuccess) return;
         e = hipEventRecord(g->ev_ready, g->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(x->stream, g->ev_ready, 0);
-    }
+    });
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
@@ -428,6 +432,34 @@ int scratch_on(int device, double **out) {
 
 namespace lumagpu {
 int step_grid(lumagpu_grid *g, int subcycle) { return apply_step(g, subcycle, nullptr, nullptr); }
+// the device table of g's refined regions, rewritten whenever the list changes
+int upload_kids(lumagpu_grid *g) {
+    (void)hipStreamSynchronize(g->stream);   // no queued step reads the old table
+    (void)hipFree(g->d_kids);
+    g->d_kids = nullptr;
+    if (g->kids.empty()) return 0;
+    std::vector<ChildRef> h(g->kids.size());
+    for (size_t i = 0; i < g->kids.size(); ++i) {
+        const lumagpu_grid *c = g->kids[i];
+        ChildRef &r = h[i];
+        r.f = c->f;
+        r.ps = c->ps;
+        r.nx = c->d.N;
+        r.ny = c->d.M;
+        r.nz = c->d.K;
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = c->d.coarse_lims[2 * a];
+            r.hi[a] = c->d.coarse_lims[2 * a + 1];
+        }
+        r.rho = c->rho;
+        r.u = c->u;
+        r.ncell = c->ncell;
+    }
+    if (hipMalloc(&g->d_kids, h.size() * sizeof(ChildRef)) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "refined-region table allocation failed");
+    HIP_TRY(hipMemcpy(g->d_kids, h.data(), h.size() * sizeof(ChildRef), hipMemcpyHostToDevice));
+    return 0;
+}
 int upload_flags(lumagpu_grid *g) {
     if (g->hflags.size() != g->ncell) return fail(LUMAGPU_ESTATE, "host flag array does not cover the grid");
     if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
@@ -534,8 +566,14 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->edge);
     (void)hipFree(g->erho);
     ibm_destroy(g);
-    if (g->parent && g->parent->child == g) g->parent->child = nullptr;
-    if (g->child && g->child->parent == g) g->child->parent = nullptr;
+    if (g->parent) {
+        auto &pk = g->parent->kids;
+        pk.erase(std::remove(pk.begin(), pk.end(), g), pk.end());
+        (void)upload_kids(g->parent);
+    }
+    for (lumagpu_grid *k : g->kids)
+        if (k->parent == g) k->parent = nullptr;
+    (void)hipFree(g->d_kids);
     if (g->ev_done) (void)hipEventDestroy(g->ev_done);
     if (g->ev_ready) (void)hipEventDestroy(g->ev_ready);
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -689,7 +727,7 @@ extern "C" {
 int lumagpu_checkpoint_save(lumagpu_grid *g, lumagpu_checkpoint **out) {
     if (int r = check_grid(g)) return r;
     if (!out) return fail(LUMAGPU_EINVAL, "null output");
-    if (g->parent || g->child) return fail(LUMAGPU_EUNSUPPORTED, "checkpoints cover single-level grids");
+    if (g->parent || !g->kids.empty()) return fail(LUMAGPU_EUNSUPPORTED, "checkpoints cover single-level grids");
     if (ibm_has_flexible(g)) return fail(LUMAGPU_EUNSUPPORTED, "checkpoints do not hold flexible-body state");
     auto *ck = new lumagpu_checkpoint();
     ck->owner = g;
@@ -744,7 +782,7 @@ int lumagpu_set_zslab(lumagpu_grid *g, int on) {
     }
     if (g->d.dims != 3 || g->Q != 19 || g->d.K < 3)
         return fail(LUMAGPU_EINVAL, "a z-slab needs a D3Q19 grid with at least one interior and two halo planes");
-    if (g->masked || g->d.forced || g->parent || g->child || g->ibm)
+    if (g->masked || g->d.forced || g->parent || !g->kids.empty() || g->ibm)
         return fail(LUMAGPU_EUNSUPPORTED, "z-slab halo exchange is implemented for unlabelled, unforced single-level grids");
     if (g->melem != sizeof(double)) return fail(LUMAGPU_EUNSUPPORTED, "z-slabs run in the FP64 and shifted forms");
     if (g->parity != 0) return fail(LUMAGPU_ESTATE, "switch to a z-slab between step pairs");
@@ -755,8 +793,9 @@ int lumagpu_set_zslab(lumagpu_grid *g, int on) {
 int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child) {
     if (int r = check_grid(parent)) return r;
     if (int r = check_grid(child)) return r;
-    if (parent->child) return fail(LUMAGPU_EUNSUPPORTED, "one refined region per grid is supported");
     if (child->parent) return fail(LUMAGPU_ESTATE, "grid already has a parent");
+    if (parent->kids.size() >= kMaxKids)
+        return fail(LUMAGPU_EUNSUPPORTED, "at most " + std::to_string(kMaxKids) + " refined regions per grid");
     const lumagpu_grid_desc &p = parent->d, &c = child->d;
     if (c.level != p.level + 1) return fail(LUMAGPU_EINVAL, "child level must be parent level + 1");
     if (c.precision != p.precision || c.Q != p.Q) return fail(LUMAGPU_EINVAL, "child and parent must share lattice and precision");
@@ -767,6 +806,14 @@ int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child) {
         const int lo = c.coarse_lims[2 * a], hi = c.coarse_lims[2 * a + 1];
         if (lo < 0 || hi >= pext[a] || hi < lo || ext[a] != 2 * (hi - lo + 1))
             return fail(LUMAGPU_EINVAL, "child extents do not match its coarse limits on the parent");
+    }
+    // regions of one grid are disjoint (GridUtils::getSubGrid takes the first that holds a site)
+    for (const lumagpu_grid *o : parent->kids) {
+        bool apart = false;
+        for (int a = 0; a < c.dims; ++a)
+            apart = apart || c.coarse_lims[2 * a + 1] < o->d.coarse_lims[2 * a] ||
+                    o->d.coarse_lims[2 * a + 1] < c.coarse_lims[2 * a];
+        if (!apart) return fail(LUMAGPU_EINVAL, "refined regions of one grid overlap");
     }
     // Only transition-layer sites may read refined ones (coalesce).  Elsewhere LUMA would pull
     // the refined sites' stale, still-collided populations (:144, :289-292), which this build
@@ -791,8 +838,13 @@ int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child) {
                                                 "keep the refined region off the domain boundaries");
                 }
     }
-    parent->child = child;
+    parent->kids.push_back(child);
     child->parent = parent;
+    if (int r = upload_kids(parent)) {
+        parent->kids.pop_back();
+        child->parent = nullptr;
+        return r;
+    }
     return 0;
 }
 
@@ -977,13 +1029,15 @@ int lumagpu_set_reynolds_ramp(lumagpu_grid *g, double re, double ramp_time) {
     if (int r = check_grid(g)) return r;
     if (!(ramp_time >= 0.0)) return fail(LUMAGPU_EINVAL, "Reynolds ramp time must be >= 0");
     if (ramp_time > 0.0 && !(re > 0.0)) return fail(LUMAGPU_EINVAL, "a Reynolds ramp needs L_RE > 0");
-    for (lumagpu_grid *x = g; x; x = x->child)
-        if (ramp_time > 0.0 && !(x->d.dt > 0.0 && x->d.dh > 0.0))
-            return fail(LUMAGPU_EINVAL, "a Reynolds ramp needs desc.dt and desc.dh > 0 on every level");
-    for (lumagpu_grid *x = g; x; x = x->child) {
+    bool dims_ok = true;
+    for_each_level(g, [&](const lumagpu_grid *x) {
+        if (ramp_time > 0.0 && !(x->d.dt > 0.0 && x->d.dh > 0.0)) dims_ok = false;
+    });
+    if (!dims_ok) return fail(LUMAGPU_EINVAL, "a Reynolds ramp needs desc.dt and desc.dh > 0 on every level");
+    for_each_level(g, [&](lumagpu_grid *x) {
         x->re = re;
         x->re_ramp = ramp_time;
-    }
+    });
     return 0;
 }
 

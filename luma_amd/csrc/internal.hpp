@@ -36,7 +36,8 @@ struct lumagpu_grid {
     hipEvent_t ev_done = nullptr;     // recorded on this grid's stream after its sub-steps
     hipEvent_t ev_ready = nullptr;    // recorded before a child reads this grid's populations
     lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
-    lumagpu_grid *child = nullptr;    // GridObj::subGrid[0] (non-owning; one region per level)
+    std::vector<lumagpu_grid *> kids; // GridObj::subGrid (non-owning), stepped in this order
+    lumagpu::ChildRef *d_kids = nullptr;   // device table of the kids for the step kernels
     double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
     int has_outlet = 0;
     int nedge = 0;                    // regularised velocity sites on edges and corners
@@ -129,22 +130,17 @@ inline StepArgs make_args(const lumagpu_grid *g) {
         a.cly = g->d.coarse_lims[2];
         a.clz = g->d.coarse_lims[4];
     }
-    if (g->child) {
-        const lumagpu_grid *c = g->child;
-        a.cf = c->f;
-        a.cps = c->ps;
-        a.cnx = c->d.N;
-        a.cny = c->d.M;
-        a.cnz = c->d.K;
-        a.cclx = c->d.coarse_lims[0];
-        a.ccly = c->d.coarse_lims[2];
-        a.cclz = c->d.coarse_lims[4];
-        a.crho = c->rho;
-        a.cu = c->u;
-        a.cncell = c->ncell;
-    }
+    a.kids = g->d_kids;
+    a.nkids = (int)g->kids.size();
     a.xr = g->xr;
     return a;
+}
+
+// g and every grid below it, parents before their children
+template <class G, class Fn>
+void for_each_level(G *g, Fn &&fn) {
+    fn(g);
+    for (lumagpu_grid *k : g->kids) for_each_level(static_cast<G *>(k), fn);
 }
 
 // IB step between the stream/macro and the force/collide halves (ObjectManager::ibm_apply);
@@ -154,6 +150,10 @@ void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
 // copy the host flag bytes to the device and rebuild the plain-site bitmap (api.hip)
 int upload_flags(lumagpu_grid *g);
+// refined regions per grid (GridObj::subGrid)
+constexpr size_t kMaxKids = 16;
+// rewrite the device table of g's refined regions (api.hip)
+int upload_kids(lumagpu_grid *g);
 // after a whole-field force reset: no site holds an IB force, the next spread flags its sites
 void ibm_forget_flags(lumagpu_grid *g);
 // the grid carries a flexible (FEM) body, whose step runs host work

@@ -86,7 +86,7 @@ class GpuGrid:
     def add_child(self, child: "GpuGrid"):
         """GridObj::LBM_addSubGrid: the child is stepped twice inside every step of this grid."""
         call("lumagpu_grid_add_child", self.h, child.h)
-        self._child = child
+        self._kids = getattr(self, "_kids", []) + [child]
 
     # state ----------------------------------------------------------------------------
     def set_labels(self, lattyp, xpos=None, ypos=None, zpos=None):

@@ -51,7 +51,8 @@ struct ora_grid {
     double *XPos, *YPos, *ZPos;
     int t;
     ora_grid *parent;
-    ora_grid *child;   /* one region per level, as the configs use */
+    ora_grid *kids[ORA_MAX_KIDS];   /* GridObj::subGrid, in the order they step */
+    int nkids;
     ora_ibm_hook hook;
     void *hook_ctx;
     int copy_u_n;
@@ -141,7 +142,7 @@ ora_grid *ora_create(const ora_desc *d) {
 
 void ora_destroy(ora_grid *g) {
     if (!g) return;
-    if (g->child) ora_destroy(g->child);
+    for (int k = 0; k < g->nkids; ++k) ora_destroy(g->kids[k]);
     free(g->f); free(g->fNew); free(g->force_i); free(g->u); free(g->u_n);
     free(g->force_xyz); free(g->rho); free(g->LatTyp);
     free(g->ux_in); free(g->uy_in); free(g->uz_in);
@@ -150,10 +151,22 @@ void ora_destroy(ora_grid *g) {
 }
 
 int ora_add_child(ora_grid *parent, ora_grid *child) {
-    if (parent->child) return -1;
-    parent->child = child;
+    if (parent->nkids >= ORA_MAX_KIDS) return -1;
+    parent->kids[parent->nkids++] = child;
     child->parent = parent;
     return 0;
+}
+
+/* ---- GridUtils::getSubGrid: GridUtils.cpp:1761-1799 ----------------------------------
+ * The region whose edges hold the site's position; for the sites that ask (TL2F sites, which
+ * lie inside their region) that is the region whose coarse limits hold the site's indices. */
+static const ora_grid *sub_grid(const ora_grid *g, int i, int j, int k) {
+    for (int n = 0; n < g->nkids; ++n) {
+        const int *L = g->kids[n]->d.coarse_lims;
+        if (i >= L[0] && i <= L[1] && j >= L[2] && j <= L[3] && (g->D == 2 || (k >= L[4] && k <= L[5])))
+            return g->kids[n];
+    }
+    return NULL;
 }
 
 void ora_set_labels(ora_grid *g, const int *lt) { memcpy(g->LatTyp, lt, g->ncell * sizeof(int)); }
@@ -246,8 +259,7 @@ static void explode(ora_grid *g, int id, int v, int sx, int sy, int sz) {
 }
 
 /* ---- _LBM_coalesce_opt: GridObj_ops_lbm_optimised.cpp:591-636 ------------------------ */
-static void coalesce(ora_grid *g, int i, int j, int k, int id, int v) {
-    const ora_grid *C = g->child;
+static void coalesce(ora_grid *g, const ora_grid *C, int i, int j, int k, int id, int v) {
     int cM = C->d.M, cK = C->d.K, Q = g->Q;
     int ci[3];
     fine_indices(g, i, C->d.coarse_lims[0], j, C->d.coarse_lims[2], k, C->d.coarse_lims[4], ci);
@@ -285,8 +297,8 @@ static void stream_site(ora_grid *g, int i, int j, int k, int id, int type_local
             g->fNew[v + id * Q] = ora_equilibrium(g, sid, v);
         } else if (g->parent && st == O_TL2C && subcycle == 0) {
             explode(g, id, v, sx, sy, sz);
-        } else if (g->child && st == O_REFINED && type_local == O_TL2F) {
-            coalesce(g, i, j, k, id, v);
+        } else if (g->nkids && st == O_REFINED && type_local == O_TL2F && sub_grid(g, i, j, k)) {
+            coalesce(g, sub_grid(g, i, j, k), i, j, k, id, v);
         } else {
             g->fNew[v + id * Q] = g->f[v + sid * Q];
         }
@@ -320,9 +332,9 @@ static void macro_site(ora_grid *g, int i, int j, int k, int id, int type_local)
         if (D == 3) u[2] = rz / rho_t;
         g->rho[id] = rho_t;
     }
-    if (type_local == O_TL2F && g->child) {
+    if (type_local == O_TL2F && g->nkids && sub_grid(g, i, j, k)) {
         /* :850-891 copy u, rho into the child TL cluster */
-        ora_grid *C = g->child;
+        ora_grid *C = (ora_grid *)sub_grid(g, i, j, k);
         int ci[3];
         fine_indices(g, i, C->d.coarse_lims[0], j, C->d.coarse_lims[2], k, C->d.coarse_lims[4], ci);
         for (int ii = 0; ii < 2; ++ii)
@@ -574,9 +586,9 @@ void ora_sweep2(ora_grid *g) {
 /* ---- LBM_multi_opt: GridObj_ops_lbm_optimised.cpp:36-193 ---------------------------- */
 void ora_multi_opt(ora_grid *g, int subcycle) {
     if (g->re_ramp > 0.0) update_reynolds(g);
-    if (g->child) {
-        ora_multi_opt(g->child, 0);
-        ora_multi_opt(g->child, 1);
+    for (int n = 0; n < g->nkids; ++n) {
+        ora_multi_opt(g->kids[n], 0);
+        ora_multi_opt(g->kids[n], 1);
     }
     if (g->hook) ora_reset_forces(g);
     ora_sweep1(g, subcycle);
@@ -596,7 +608,7 @@ static int all_fluid(const ora_grid *g) {
     return 1;
 }
 void ora_run(ora_grid *g, int nsteps, int nthreads) {
-    if (nthreads <= 1 || g->child || g->hook || g->re_ramp > 0.0 || !all_fluid(g)) {
+    if (nthreads <= 1 || g->nkids || g->hook || g->re_ramp > 0.0 || !all_fluid(g)) {
         for (int s = 0; s < nsteps; ++s) ora_multi_opt(g, 0);
         return;
     }

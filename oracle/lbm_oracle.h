@@ -40,7 +40,8 @@ typedef struct ora_grid ora_grid;
 
 ora_grid *ora_create(const ora_desc *d);
 void ora_destroy(ora_grid *g);                     /* destroys children too */
-int ora_add_child(ora_grid *parent, ora_grid *child);
+#define ORA_MAX_KIDS 16                           /* refined regions per grid */
+int ora_add_child(ora_grid *parent, ora_grid *child);   /* -1 past ORA_MAX_KIDS */
 
 /* Arrays in LUMA layout; all are copied in. */
 void ora_set_labels(ora_grid *g, const int *lattyp);

@@ -135,7 +135,7 @@ class OracleGrid:
     def add_child(self, child: "OracleGrid"):
         assert self.lib.ora_add_child(self.h, child.h) == 0
         child._owned = False   # destroyed with the parent
-        self._child = child
+        self._kids = getattr(self, "_kids", []) + [child]
 
     def set_labels(self, lattyp):
         a = np.ascontiguousarray(lattyp, dtype=np.int32)

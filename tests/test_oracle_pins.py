@@ -260,6 +260,57 @@ def test_two_level_step_is_mirror_symmetric():
     assert np.abs(ca - a[1].get_f()[::-1][..., mirror_v]).max() > 1e-4   # the state is not symmetric itself
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_two_regions_each_step_as_if_alone(dims):
+    # Two refined regions of one grid (L_NUM_REGIONS = 2, GridUtils::getSubGrid) far enough apart
+    # that in three coarse steps neither reaches the other: each region's fine state, and the
+    # coarse state around it, equal those of a run with that region alone, bit for bit.  A
+    # transition site that coalesced from, or copied into, the wrong region breaks this.
+    N, M, K = (30, 12, 1) if dims == 2 else (26, 10, 8)
+    la = (3, 7, 2, 6) + ((2, 5) if dims == 3 else ())
+    lb = (N - 9, N - 5, 3, 7) + ((1, 4) if dims == 3 else ())
+    rng = np.random.default_rng(5)
+    rho = 1 + 0.01 * rng.standard_normal((N, M, K))
+    u = 0.03 * rng.standard_normal((N, M, K, dims))
+
+    def hierarchy(regions):
+        parent = setups.taylor_green((N, M, K), u0=0.0, tau=0.7, dims=dims)
+        parent.xpos, parent.ypos = np.arange(N) + 0.5, np.arange(M) + 0.5
+        parent.zpos = (np.arange(K) + 0.5) if dims == 3 else np.zeros(1)
+        kids = [setups.add_refined_region(parent, l, 1.0, 1.0) for l in regions]
+
+        def mk(c, r, v):
+            g = OracleGrid(dims, parent.Q, c.N, c.M, c.K, omega=c.omega, regularised=False, fz_compat=False,
+                           level=c.info.get("level", 0), coarse_lims=c.info.get("coarse_lims", (0,) * 6))
+            g.set_labels(c.lattyp)
+            g.set_positions(c.xpos, c.ypos, c.zpos)
+            g.init_feq(r, v)
+            return g
+        op = mk(parent, rho, u)
+        oks = []
+        for c in kids:
+            crng = np.random.default_rng(c.info["coarse_lims"][0])
+            oc = mk(c, 1 + 0.01 * crng.standard_normal((c.N, c.M, c.K)), 0.03 * crng.standard_normal((c.N, c.M, c.K, dims)))
+            op.add_child(oc)
+            oks.append(oc)
+        for _ in range(3):
+            op.step()
+        return op, oks, parent
+
+    both, kids2, parent2 = hierarchy([la, lb])
+    for idx, lims in ((0, la), (1, lb)):
+        alone, kids1, parent1 = hierarchy([lims])
+        assert np.array_equal(kids2[idx].get_f(), kids1[0].get_f()), idx
+        # coarse sites within two cells of the region, outside both regions' refined interiors
+        near = np.zeros((N, M, K), bool)
+        sl = [slice(max(lims[2 * a] - 2, 0), lims[2 * a + 1] + 3) for a in range(dims)] + ([slice(None)] if dims == 2 else [])
+        near[tuple(sl)] = True
+        live = near & np.isin(parent2.lattyp, [FLUID, setups.TL2F])
+        assert live.sum() > 20
+        assert np.array_equal(both.get_f()[live], alone.get_f()[live]), idx
+    assert not np.array_equal(kids2[0].get_f()[:4, :4], kids2[1].get_f()[:4, :4])
+
+
 # ---- corotational beam at large rotation: FEMElement.cpp:273-302, FEMBody.cpp:183-325 -------
 
 # Bisshopp & Drucker (1945), Q. Appl. Math. 3:272, cantilever under a dead tip load P,
