@@ -124,6 +124,12 @@ struct StepArgs {
     const double *xr;
     // --- z-slab with halo planes: the step updates planes zoff .. nz-1-zoff only ------------
     int zoff;
+    // --- refined sites read by sites other than TL2F (a region touching a velocity, pressure or
+    // outlet boundary): LUMA never updates a refined site, so those reads see its initial
+    // populations, kept here in storage form at stale[v * nstale + stale_idx[cell]]
+    const void *stale;
+    const int *stale_idx;
+    int nstale;
 };
 
 // ----------------------------------------------------------------------------------------
@@ -897,6 +903,9 @@ __device__ __forceinline__ void gather_in(const StepArgs &a, int x, int y, int z
             if constexpr (L::D == 3) acc /= T(8.0);
             else acc /= T(4.0);
             fin[v] = acc;
+        } else if (sty == kRefined && a.stale && a.stale_idx[src] >= 0) {
+            // a refined source of any other site: the populations it was initialised with
+            fin[v] = ld<P, false>(static_cast<const S *>(a.stale) + (size_t)v * a.nstale + a.stale_idx[src]);
         } else if constexpr (ODD) {
             fin[v] = ld<P, cx == 0>(f + (size_t)L::opp(v) * ps + src);
         } else {
@@ -1288,9 +1297,13 @@ __global__ void k_extrap_right(StepArgs a, int parity, double *xr) {
     if (y >= a.ny) return;
     const size_t plane = (size_t)a.ny * a.nz;
     using M = typename P::M;
+    const size_t src = cell_of(a.nx - 3, y, z, a.nx, a.ny);
+    // a refined site there holds the populations it was initialised with (see StepArgs::stale)
+    const int si = (a.stale && (a.flags[src] & kTypeMask) == kRefined) ? a.stale_idx[src] : -1;
     for (int v = 0; v < Q; ++v)
         mv<M>(xr)[(size_t)v * plane + y + (size_t)a.ny * z] =
-            M(post_of<Q, P>(a.f, a.ps, a.nx, a.ny, a.nz, a.flags, parity, a.nx - 3, y, z, v));
+            si >= 0 ? M(ld<P, false>(static_cast<const typename P::S *>(a.stale) + (size_t)v * a.nstale + si))
+                  : M(post_of<Q, P>(a.f, a.ps, a.nx, a.ny, a.nz, a.flags, parity, a.nx - 3, y, z, v));
 }
 
 // ----------------------------------------------------------------------------------------

@@ -460,6 +460,97 @@ int upload_kids(lumagpu_grid *g) {
     HIP_TRY(hipMemcpy(g->d_kids, h.data(), h.size() * sizeof(ChildRef), hipMemcpyHostToDevice));
     return 0;
 }
+namespace {
+// f_v of the listed cells from the post-local layout (population v of a cell in its slot
+// opp(v)), into out[v * n + k]
+template <class W, int Q>
+__global__ void k_copy_slots(const W *f, size_t ps, const int *cells, int n, W *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int v = 0; v < Q; ++v) out[(size_t)v * n + k] = f[(size_t)Lattice<Q>::opp(v) * ps + (size_t)cells[k]];
+}
+struct Elem16 {
+    uint64_t a, b;
+};
+template <class W>
+void copy_slots(const lumagpu_grid *g) {
+    const unsigned nb = (unsigned)((g->nstale + 127) / 128);
+    const W *f = static_cast<const W *>(g->f);
+    W *out = static_cast<W *>(g->stale);
+    if (g->Q == 9) k_copy_slots<W, 9><<<nb, 128, 0, g->stream>>>(f, g->ps, g->stale_cells, g->nstale, out);
+    else k_copy_slots<W, 19><<<nb, 128, 0, g->stream>>>(f, g->ps, g->stale_cells, g->nstale, out);
+}
+}  // namespace
+
+int capture_stale(lumagpu_grid *g) {
+    if (!g->nstale || !g->has_state) return 0;
+    if (g->parity != 0) return fail(LUMAGPU_ESTATE, "refined-site capture needs the post-local layout");
+    switch (g->elem) {
+        case 2: copy_slots<uint16_t>(g); break;
+        case 4: copy_slots<uint32_t>(g); break;
+        case 8: copy_slots<uint64_t>(g); break;
+        case 16: copy_slots<Elem16>(g); break;
+        default: return fail(LUMAGPU_EUNSUPPORTED, "unexpected population element size");
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// refined sites read by a site that does not coalesce (anything but TL2F, solid or refined),
+// with the periodic wrap of the step kernels
+static int build_stale(lumagpu_grid *g) {
+    const int N = g->d.N, M = g->d.M, K = g->d.K;
+    auto ty = [&](int i, int j, int k) {
+        i = (i + N) % N; j = (j + M) % M; k = (k + K) % K;
+        return g->hflags[(size_t)i + (size_t)N * ((size_t)j + (size_t)M * k)] & kTypeMask;
+    };
+    std::vector<int> idx(g->ncell, -1), cells;
+    for (int k = 0; k < K; ++k)
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < N; ++i) {
+                const int t = ty(i, j, k);
+                if (t == kTL2F || t == kRefined || t == kSolid) continue;
+                // an outlet site's source two cells left (its side buffer, :246-251)
+                if (t == kExtrapRight && i >= 2 && ty(i - 2, j, k) == kRefined) {
+                    const size_t c = (size_t)(i - 2) + (size_t)N * ((size_t)j + (size_t)M * k);
+                    if (idx[c] < 0) {
+                        idx[c] = (int)cells.size();
+                        cells.push_back((int)c);
+                    }
+                }
+                for (int dk = (g->D == 3 ? -1 : 0); dk <= (g->D == 3 ? 1 : 0); ++dk)
+                    for (int dj = -1; dj <= 1; ++dj)
+                        for (int di = -1; di <= 1; ++di) {
+                            if (ty(i + di, j + dj, k + dk) != kRefined) continue;
+                            const int ii = (i + di + N) % N, jj = (j + dj + M) % M, kk = (k + dk + K) % K;
+                            const size_t c = (size_t)ii + (size_t)N * ((size_t)jj + (size_t)M * kk);
+                            if (idx[c] < 0) {
+                                idx[c] = (int)cells.size();
+                                cells.push_back((int)c);
+                            }
+                        }
+            }
+    // the same sites as before (a flag upload that changed only force bits): keep what was captured
+    if (cells == g->stale_list) return 0;
+    g->stale_list = cells;
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipFree(g->stale);
+    (void)hipFree(g->stale_idx);
+    (void)hipFree(g->stale_cells);
+    g->stale = nullptr;
+    g->stale_idx = g->stale_cells = nullptr;
+    g->nstale = 0;
+    if (cells.empty()) return 0;
+    if (hipMalloc(&g->stale, (size_t)g->Q * cells.size() * g->elem) != hipSuccess ||
+        hipMalloc(&g->stale_idx, g->ncell * sizeof(int)) != hipSuccess ||
+        hipMalloc(&g->stale_cells, cells.size() * sizeof(int)) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "refined-site buffer allocation failed");
+    HIP_TRY(hipMemcpy(g->stale_idx, idx.data(), g->ncell * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->stale_cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+    g->nstale = (int)cells.size();
+    return capture_stale(g);
+}
+
 int upload_flags(lumagpu_grid *g) {
     if (g->hflags.size() != g->ncell) return fail(LUMAGPU_ESTATE, "host flag array does not cover the grid");
     if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
@@ -472,7 +563,7 @@ int upload_flags(lumagpu_grid *g) {
     if (g->Q == 9) k_plain_bits<9><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
     else k_plain_bits<19><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
     HIP_TRY(hipGetLastError());
-    return 0;
+    return build_stale(g);
 }
 hipError_t outlet_prepass(const lumagpu_grid *g) { return launch_extrap(g); }
 }  // namespace lumagpu
@@ -555,6 +646,9 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->f2);
     (void)hipFree(g->flags);
     (void)hipFree(g->plain);
+    (void)hipFree(g->stale);
+    (void)hipFree(g->stale_idx);
+    (void)hipFree(g->stale_cells);
     (void)hipFree(g->force);
     (void)hipFree(g->uin);
     (void)hipFree(g->rho);
@@ -815,29 +909,6 @@ int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child) {
                     o->d.coarse_lims[2 * a + 1] < c.coarse_lims[2 * a];
         if (!apart) return fail(LUMAGPU_EINVAL, "refined regions of one grid overlap");
     }
-    // Only transition-layer sites may read refined ones (coalesce).  Elsewhere LUMA would pull
-    // the refined sites' stale, still-collided populations (:144, :289-292), which this build
-    // does not keep, so such hierarchies are refused.
-    if (parent->hflags.size() == parent->ncell) {
-        const int N = p.N, M = p.M, K = p.K;
-        auto ty = [&](int i, int j, int k) {
-            i = (i + N) % N; j = (j + M) % M; k = (k + K) % K;
-            return parent->hflags[(size_t)i + (size_t)N * ((size_t)j + (size_t)M * k)] & kTypeMask;
-        };
-        for (int k = 0; k < K; ++k)
-            for (int j = 0; j < M; ++j)
-                for (int i = 0; i < N; ++i) {
-                    const int t = ty(i, j, k);
-                    if (t == kTL2F || t == kRefined || t == kSolid) continue;
-                    for (int dk = (p.dims == 3 ? -1 : 0); dk <= (p.dims == 3 ? 1 : 0); ++dk)
-                        for (int dj = -1; dj <= 1; ++dj)
-                            for (int di = -1; di <= 1; ++di)
-                                if (ty(i + di, j + dj, k + dk) == kRefined)
-                                    return fail(LUMAGPU_EUNSUPPORTED,
-                                                "a site other than eTransitionToFiner borders a refined site; "
-                                                "keep the refined region off the domain boundaries");
-                }
-    }
     parent->kids.push_back(child);
     child->parent = parent;
     if (int r = upload_kids(parent)) {
@@ -872,9 +943,10 @@ int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u
     HIP_TRY(hipMemcpyAsync(du, u, g->ncell * g->D * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIP_TRY(launch_aux(g, kAuxInitFeq, drho, du, nullptr));
     HIP_TRY(macro_from_luma(g, drho, du));
-    HIP_TRY(hipStreamSynchronize(g->stream));
     g->parity = 0;
     g->has_state = 1;
+    if (int r = capture_stale(g)) return r;
+    HIP_TRY(hipStreamSynchronize(g->stream));
     return 0;
 }
 
@@ -885,9 +957,10 @@ int lumagpu_upload_f(lumagpu_grid *g, const double *f) {
     if (int r = ensure_stage(g, nb)) return r;
     HIP_TRY(hipMemcpyAsync(g->stage, f, nb, hipMemcpyHostToDevice, g->stream));
     HIP_TRY(launch_aux(g, kAuxUpload, g->stage, nullptr, nullptr));
-    HIP_TRY(hipStreamSynchronize(g->stream));
     g->parity = 0;
     g->has_state = 1;
+    if (int r = capture_stale(g)) return r;
+    HIP_TRY(hipStreamSynchronize(g->stream));
     return 0;
 }
 

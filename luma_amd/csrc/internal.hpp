@@ -24,6 +24,12 @@ struct lumagpu_grid {
     int fuse_shape = -1, fuse_zc = 0; // its tile shape (FusedShape, -1: off) and z planes per block
     uint8_t *flags = nullptr;
     uint32_t *plain = nullptr;        // plain-site bitmap, rebuilt with every flag upload
+    // refined sites that sites other than TL2F read, and their initial populations
+    void *stale = nullptr;            // [Q][nstale] storage elements
+    int *stale_idx = nullptr;         // [ncell] index into stale, -1 elsewhere
+    int *stale_cells = nullptr;       // [nstale] their cells
+    std::vector<int> stale_list;      // host copy of stale_cells
+    int nstale = 0;
     double *force = nullptr;
     double *uin = nullptr;
     double *rho = nullptr, *u = nullptr;
@@ -133,6 +139,9 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     a.kids = g->d_kids;
     a.nkids = (int)g->kids.size();
     a.xr = g->xr;
+    a.stale = g->stale;
+    a.stale_idx = g->stale_idx;
+    a.nstale = g->nstale;
     return a;
 }
 
@@ -150,6 +159,9 @@ void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
 // copy the host flag bytes to the device and rebuild the plain-site bitmap (api.hip)
 int upload_flags(lumagpu_grid *g);
+// copy the initial populations of the refined sites other sites read (api.hip); the grid is
+// in the post-local layout (parity 0) right after its state was set
+int capture_stale(lumagpu_grid *g);
 // refined regions per grid (GridObj::subGrid)
 constexpr size_t kMaxKids = 16;
 // rewrite the device table of g's refined regions (api.hip)

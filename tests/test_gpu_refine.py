@@ -81,14 +81,37 @@ def test_two_level_cavity_reynolds_ramp_fp64_bitwise():
     run_pair(parent, child, 40, re_ramp=(parent.info["re"], 25 * parent.info["dt"]))
 
 
-def test_refined_patch_touching_the_lid_is_refused():
-    # LUMA's lid sites would pull the refined sites' stale populations; refused, with a reason
-    from luma_amd._abi import LumaGpuError
+def test_refined_patch_touching_the_lid_fp64_bitwise():
+    # The lid's velocity sites pull from refined sites, which LUMA never updates: they stream
+    # the populations those sites were initialised with (f = fNew at initialisation,
+    # GridObj_init_grids.cpp:333, and neither sweep visits eRefined, :91-95, :125-156)
     parent = setups.lid_driven_cavity(32, re=1280.0 * 32 / 256)
     child = setups.add_refined_region(parent, (10, 21, 22, 31), parent.info["dh"], parent.info["dt"])
-    with GpuGrid.from_case(parent, "fp64") as gp, GpuGrid.from_case(child, "fp64") as gc:
-        with pytest.raises(LumaGpuError, match="borders a refined site"):
-            gp.add_child(gc)
+    run_pair(parent, child, 60)
+
+
+@pytest.mark.parametrize("where", ["inlet", "outlet"])
+def test_refined_patch_touching_inlet_walls_or_outlet_fp64_bitwise(where):
+    # A channel region spanning the full height (both solid walls): from the inlet column on, or
+    # up to the column before the ExtrapolateRight outlet, whose sources two cells left are then
+    # refined sites (the outlet's side buffer takes their initial populations as well)
+    parent = setups.channel_2d(32, 20, 8.0, 0.04, 20.0, outlet=True)
+    lims = (0, 13, 0, 19) if where == "inlet" else (14, 30, 0, 19)
+    child = setups.add_refined_region(parent, lims, parent.info["dh"], parent.info["dt"])
+    setups.velocity_field_on(child, parent)
+    if where == "outlet":
+        assert (parent.lattyp[29, 1:-1] == setups.REFINED).all()
+    run_pair(parent, child, 40)
+
+
+def test_region_over_the_outlet_column_is_refused():
+    # its child would hold a two-column ExtrapolateRight layer; the outlet sources are built for
+    # the last x column only
+    from luma_amd._abi import LumaGpuError
+    parent = setups.channel_2d(32, 20, 8.0, 0.04, 20.0, outlet=True)
+    child = setups.add_refined_region(parent, (14, 31, 0, 19), parent.info["dh"], parent.info["dt"])
+    with pytest.raises(LumaGpuError, match="last x column"):
+        GpuGrid.from_case(child, "fp64").close()
 
 
 @pytest.mark.parametrize("prec", ["fp32s", "fp16s"])
