@@ -980,13 +980,10 @@ __device__ __forceinline__ void velocity_site(const StepArgs &a, unsigned fl, in
 // labelled grid: solid (HWBB), velocity (regularised or forced equilibrium), ExtrapolateRight
 // sources, refinement transition layers, body force.
 template <int Q, class P, bool ODD, bool STORE>
-__global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
+__device__ __forceinline__ void masked_site(const StepArgs &a, int x, int y, int z) {
     using L = Lattice<Q>;
     using S = typename P::S;
     using T = typename P::T;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x >= a.nx) return;
     const int nx = a.nx, ny = a.ny;
     const size_t cell = cell_of(x, y, z, nx, ny);
     const unsigned fl = a.flags[cell];
@@ -1085,6 +1082,74 @@ __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
             st<P>(f + (size_t)L::opp(v) * ps + cell, fout[v]);
         }
     });
+}
+
+template <int Q, class P, bool ODD, bool STORE>
+__global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.nx) return;
+    masked_site<Q, P, ODD, STORE>(a, x, blockIdx.y, blockIdx.z);
+}
+
+// The same step for a list of sites: on large labelled grids the plain sites run in
+// k_aa_plain and this kernel takes the rest (boundary, transition and bounce-back neighbours).
+template <int Q, class P, bool ODD, bool STORE>
+__global__ __launch_bounds__(256) void k_aa_masked_list(StepArgs a, const int *cells, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cell = cells[i];
+    const int x = cell % a.nx, y = (cell / a.nx) % a.ny, z = cell / (a.nx * a.ny);
+    masked_site<Q, P, ODD, STORE>(a, x, y, z);
+}
+
+// The step of the plain sites of a labelled grid (k_plain_bits), every other site left to
+// k_aa_masked_list: the unlabelled kernel's pulls and pushes plus the body force where the
+// site carries one, with the arithmetic of masked_site.  It keeps the register use (and so the
+// occupancy) of the unlabelled kernel, which the boundary branches of masked_site would not.
+template <int Q, class P, bool ODD, bool STORE>
+__global__ __launch_bounds__(256) void k_aa_plain(StepArgs a) {
+    using L = Lattice<Q>;
+    using S = typename P::S;
+    using T = typename P::T;
+    using M = typename P::M;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    const int nx = a.nx, ny = a.ny;
+    const size_t cell = cell_of(x, y, z, nx, ny);
+    if (!((a.plain[cell >> 5] >> (cell & 31)) & 1u)) return;
+    S *__restrict__ f = static_cast<S *>(a.f);
+    const size_t ps = a.ps;
+    Nbr nb;
+    nb.init(x, y, z, nx, ny, a.nz);
+    T fin[Q];
+    gather_plain<Q, P, ODD>(a, cell, nb, fin);
+    auto put = [&](auto vc, T o) {
+        constexpr int v = decltype(vc)::value;
+        constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
+        if constexpr (ODD) st<P, cx == 0>(f + (size_t)v * ps + nb.at(cx, cy, cz, nx, ny), o);
+        else st<P>(f + (size_t)L::opp(v) * ps + cell, o);
+    };
+    auto step = [&](const M *F) {
+        M rho, u[3];
+        const auto col = make_collider<Q, P>(fin, F, a, rho, u);
+        if constexpr (STORE) store_macro<L::D>(a, cell, rho, u);
+        static_for<(Q - 1) / 2>([&](auto pc) {
+            constexpr int v = 2 * decltype(pc)::value;
+            T ov, oo;
+            col.template pair<v>(fin[v], fin[v + 1], ov, oo);
+            put(std::integral_constant<int, v>{}, ov);
+            put(std::integral_constant<int, v + 1>{}, oo);
+        });
+        put(std::integral_constant<int, Q - 1>{}, col(Q - 1, fin[Q - 1]));
+    };
+    if (a.all_forced || (a.flags[cell] & kForcedBit)) {
+        const M *fx = mv<M>(a.force);
+        const M Fv[3] = {fx[cell], fx[a.ncell + cell], (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0)};
+        step(Fv);
+    } else {
+        step(nullptr);
+    }
 }
 
 // Macroscopic values at a list of sites (every cell when cells is null), without touching the

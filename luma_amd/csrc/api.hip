@@ -218,7 +218,7 @@ int begin_step(lumagpu_grid *g) {
 // The second half: stream, BC, macro (with the IB force), Guo forcing and BGK collision in one
 // pass, the edge/corner and pressure sites around it, then the swap (AA parity) and ++t.
 int end_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
-    LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0};
+    LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0, g->mlist, g->nmlist};
     StepArgs a = make_args(g);
     a.subcycle = subcycle;
     if (g->nedge) {
@@ -551,6 +551,59 @@ static int build_stale(lumagpu_grid *g) {
     return capture_stale(g);
 }
 
+// Large labelled grids step in two launches: k_aa_plain over the grid for the plain sites
+// (fluid, every stencil neighbour fluid or TL2F: the k_plain_bits predicate) and the labelled
+// kernel over the list of the other sites it steps.  Small grids, whose steps are launch-bound,
+// keep one launch.
+constexpr size_t kSplitMinCells = size_t(1) << 20;
+static int build_mlist(lumagpu_grid *g) {
+    if (g->ncell < kSplitMinCells || g->zslab) {
+        g->nmlist = -1;
+        return 0;
+    }
+    std::vector<uint8_t> types(g->ncell);
+    for (size_t c = 0; c < g->ncell; ++c) types[c] = g->hflags[c] & kTypeMask;
+    if (types == g->mlist_types && g->nmlist >= 0) return 0;   // only force bits changed
+    const int N = g->d.N, M = g->d.M, K = g->d.K;
+    auto ty = [&](int i, int j, int k) {
+        i = (i + N) % N; j = (j + M) % M; k = (k + K) % K;
+        return (int)types[(size_t)i + (size_t)N * ((size_t)j + (size_t)M * k)];
+    };
+    std::vector<int> list;
+    for (int k = 0; k < K; ++k)
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < N; ++i) {
+                const size_t c = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
+                const int t = types[c];
+                const unsigned fl = g->hflags[c];
+                // the sites k_aa_masked leaves alone
+                if (t == kSolid || t == kRefined || (!g->d.regularised && t == kVelocity)) continue;
+                if (t == kPressure || (t == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)) continue;
+                bool plain = t == kFluid;
+                for (int dk = (g->D == 3 ? -1 : 0); plain && dk <= (g->D == 3 ? 1 : 0); ++dk)
+                    for (int dj = -1; plain && dj <= 1; ++dj)
+                        for (int di = -1; plain && di <= 1; ++di) {
+                            if (g->D == 3 && di != 0 && dj != 0 && dk != 0) continue;   // D3Q19: no corners
+                            const int tn = ty(i + di, j + dj, k + dk);
+                            plain = tn == kFluid || tn == kTL2F;
+                        }
+                if (!plain) list.push_back((int)c);
+            }
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipFree(g->mlist);
+    g->mlist = nullptr;
+    g->nmlist = -1;
+    g->mlist_types.clear();
+    if (!list.empty()) {
+        if (hipMalloc(&g->mlist, list.size() * sizeof(int)) != hipSuccess)
+            return fail(LUMAGPU_ENOMEM, "site list allocation failed");
+        HIP_TRY(hipMemcpy(g->mlist, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    g->nmlist = (int)list.size();
+    g->mlist_types = std::move(types);
+    return 0;
+}
+
 int upload_flags(lumagpu_grid *g) {
     if (g->hflags.size() != g->ncell) return fail(LUMAGPU_ESTATE, "host flag array does not cover the grid");
     if (!g->flags && hipMalloc(&g->flags, g->ncell) != hipSuccess)
@@ -563,6 +616,7 @@ int upload_flags(lumagpu_grid *g) {
     if (g->Q == 9) k_plain_bits<9><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
     else k_plain_bits<19><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
     HIP_TRY(hipGetLastError());
+    if (int r = build_mlist(g)) return r;
     return build_stale(g);
 }
 hipError_t outlet_prepass(const lumagpu_grid *g) { return launch_extrap(g); }
@@ -649,6 +703,7 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->stale);
     (void)hipFree(g->stale_idx);
     (void)hipFree(g->stale_cells);
+    (void)hipFree(g->mlist);
     (void)hipFree(g->force);
     (void)hipFree(g->uin);
     (void)hipFree(g->rho);

@@ -29,6 +29,10 @@ struct lumagpu_grid {
     int *stale_idx = nullptr;         // [ncell] index into stale, -1 elsewhere
     int *stale_cells = nullptr;       // [nstale] their cells
     std::vector<int> stale_list;      // host copy of stale_cells
+    // large labelled grids: the sites the plain-site kernel leaves to the labelled one
+    int *mlist = nullptr;
+    int nmlist = -1;                  // < 0: one labelled launch over the grid
+    std::vector<uint8_t> mlist_types; // the site types mlist was built from
     int nstale = 0;
     double *force = nullptr;
     double *uin = nullptr;

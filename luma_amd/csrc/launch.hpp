@@ -17,6 +17,9 @@ struct LaunchSpec {
     bool masked;    // labelled kernel
     bool odd;       // neighbour-access step
     bool store;     // write rho, u
+    // labelled grid stepped as plain sites (k_aa_plain) plus a list of the others
+    const int *mlist = nullptr;
+    int nmlist = -1;   // < 0: one k_aa_masked launch over the grid
 };
 
 // Each returns hipSuccess or the launch error.
@@ -204,7 +207,26 @@ inline dim3 cell_grid(int nx, int ny, int nz) {
 template <int Q, class P>
 hipError_t launch_step_q(const LaunchSpec &ls, const StepArgs &a, hipStream_t s) {
     const dim3 b = cell_block(a.nx), g = cell_grid(a.nx, a.ny, a.nz - 2 * a.zoff);
-    if (ls.masked) {
+    if (ls.masked && ls.nmlist >= 0) {
+        // AA steps touch only a site's own set of slots, so the two disjoint site sets can run
+        // one after the other
+        const dim3 lg((unsigned)((ls.nmlist + 255) / 256));
+        if (ls.odd) {
+            if (ls.store) k_aa_plain<Q, P, true, true><<<g, b, 0, s>>>(a);
+            else k_aa_plain<Q, P, true, false><<<g, b, 0, s>>>(a);
+            if (ls.nmlist > 0) {
+                if (ls.store) k_aa_masked_list<Q, P, true, true><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
+                else k_aa_masked_list<Q, P, true, false><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
+            }
+        } else {
+            if (ls.store) k_aa_plain<Q, P, false, true><<<g, b, 0, s>>>(a);
+            else k_aa_plain<Q, P, false, false><<<g, b, 0, s>>>(a);
+            if (ls.nmlist > 0) {
+                if (ls.store) k_aa_masked_list<Q, P, false, true><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
+                else k_aa_masked_list<Q, P, false, false><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
+            }
+        }
+    } else if (ls.masked) {
         if (ls.odd) {
             if (ls.store) k_aa_masked<Q, P, true, true><<<g, b, 0, s>>>(a);
             else k_aa_masked<Q, P, true, false><<<g, b, 0, s>>>(a);

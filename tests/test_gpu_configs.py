@@ -93,6 +93,40 @@ def test_c4_arrangement_reduced_fp64_bitwise():
     assert np.abs(fg[-1] - f0[-1]).max() > 1e-8
 
 
+def test_large_labelled_grids_two_launch_step_fp64_bitwise():
+    # Grids of 2^20 cells and more step their plain sites in k_aa_plain and the rest from a site
+    # list: a 1024^2 cavity (regularised lid and corners, macro arrays written) and a 128 x 96 x 96
+    # uniform-inflow channel with an IB sphere (forced plain sites) against the oracle
+    cav = setups.lid_driven_cavity(1024, re=1280.0 * 1024 / 256)
+    print("[config] cavity 1024x1024, 4 steps", flush=True)
+    og = oracle_grid(cav)
+    with GpuGrid.from_case(cav, "fp64", store_macro=True) as g:
+        for _ in range(4):
+            og.step()
+            g.LBM_multi_opt()
+        compare_grids(cav, g, og, "cavity 1024")
+    U, tau = cases.C4_U, cases.c4_tau()
+    case = setups.uniform_inflow_3d(128, 96, 96, U, tau)
+    pos = cases.c4_markers(n=400, centre=(40.2, 48.1, 47.9), diam=16.0)
+    print(f"[config] uniform inflow 128x96x96, {len(pos)} markers, 3 steps", flush=True)
+    og = oracle_grid(case)
+    ib = OracleIBM(og, 1.0)
+    ib.add_body(pos)
+    ib.initialise()
+    with GpuGrid.from_case(case, "fp64", store_macro=True) as g:
+        g.ibm_add_body(pos)
+        g.ibm_initialise()
+        for _ in range(3):
+            og.step()
+            g.LBM_multi_opt()
+        compare_markers(g, ib)
+        m = case.lattyp != setups.VELOCITY
+        fg, fo = g.download_f(), og.get_f()
+        assert np.array_equal(fg[m], fo[m]), np.abs(fg[m] - fo[m]).max()
+        (rg, ug), (ro, uo) = g.download_macro(), og.get_macro()
+        assert np.array_equal(rg, ro) and np.array_equal(ug, uo)
+
+
 def test_c4_full_size_adjoint_short_horizon():
     # C4 at full size (384 x 128 x 128, the committed 1,810-marker fixture) over a short horizon:
     # T = 40 steps with checkpoints every 20; dJ/dU of the adjoint against central differences
