@@ -186,12 +186,15 @@ int begin_step(lumagpu_grid *g) {
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
     }
     for (lumagpu_grid *c : g->kids) {
-        // the child reads this grid's populations, so order the two streams
-        HIP_TRY(hipEventRecord(c->ev_done, c->stream));
-        HIP_TRY(hipStreamWaitEvent(g->stream, c->ev_done, 0));
+        // the child reads this grid's populations, so order the two streams (one stream, as
+        // lumagpu_grid_add_child sets up, orders them already)
+        if (c->stream != g->stream) {
+            HIP_TRY(hipEventRecord(c->ev_done, c->stream));
+            HIP_TRY(hipStreamWaitEvent(g->stream, c->ev_done, 0));
+        }
         if (c->parity != 0) return fail(LUMAGPU_ESTATE, "child grid out of step with its parent");
     }
-    if (g->parent) {
+    if (g->parent && g->parent->stream != g->stream) {
         // this grid reads the parent's populations: wait for the parent's previous work
         HIP_TRY(hipEventRecord(g->parent->ev_ready, g->parent->stream));
         HIP_TRY(hipStreamWaitEvent(g->stream, g->parent->ev_ready, 0));
@@ -386,13 +389,13 @@ int run_graph(lumagpu_grid *g, int reps) {
     // and the descendants' later work after it
     hipError_t e = hipSuccess;
     for_each_level(g, [&](lumagpu_grid *x) {
-        if (x == g || e != hipSuccess) return;
+        if (x->stream == g->stream || e != hipSuccess) return;
         e = hipEventRecord(x->ev_done, x->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, x->ev_done, 0);
     });
     for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(exec, g->stream);
     for_each_level(g, [&](lumagpu_grid *x) {
-        if (x == g || e != hipSuccess) return;
+        if (x->stream == g->stream || e != hipSuccess) return;
         e = hipEventRecord(g->ev_ready, g->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(x->stream, g->ev_ready, 0);
     });
@@ -664,8 +667,9 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
         return fail(code, m);
     };
     if (hipGetDevice(&g->device) != hipSuccess) return cleanup(LUMAGPU_EHIP, "hipGetDevice failed");
-    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&g->own_stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(LUMAGPU_EHIP, "hipStreamCreate failed");
+    g->stream = g->own_stream;
     if (hipEventCreateWithFlags(&g->ev_done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming) != hipSuccess)
         return cleanup(LUMAGPU_EHIP, "hipEventCreate failed");
@@ -720,12 +724,15 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
         pk.erase(std::remove(pk.begin(), pk.end(), g), pk.end());
         (void)upload_kids(g->parent);
     }
-    for (lumagpu_grid *k : g->kids)
+    for (lumagpu_grid *k : g->kids) {
         if (k->parent == g) k->parent = nullptr;
+        // a detached subtree goes back to its own streams
+        for_each_level(k, [](lumagpu_grid *x) { x->stream = x->own_stream; });
+    }
     (void)hipFree(g->d_kids);
     if (g->ev_done) (void)hipEventDestroy(g->ev_done);
     if (g->ev_ready) (void)hipEventDestroy(g->ev_ready);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
     delete g;
     return 0;
 }
@@ -971,6 +978,12 @@ int lumagpu_grid_add_child(lumagpu_grid *parent, lumagpu_grid *child) {
         child->parent = nullptr;
         return r;
     }
+    // The hierarchy's levels step one after another (each child reads its parent and the parent
+    // its children), so they share the root's stream: one in-order queue instead of an event
+    // hop between queues at every level change.
+    HIP_TRY(hipStreamSynchronize(child->stream));
+    HIP_TRY(hipStreamSynchronize(parent->stream));
+    for_each_level(child, [&](lumagpu_grid *x) { x->stream = parent->stream; });
     return 0;
 }
 

@@ -41,7 +41,8 @@ struct lumagpu_grid {
     double *stage = nullptr;       // LUMA-layout staging (f AoS sized)
     size_t stage_bytes = 0;
     double *moments_out = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // the stream this grid's work goes to: the root's, in a hierarchy
+    hipStream_t own_stream = nullptr; // created with the grid
     int device = 0;                   // the GPU the grid was created on
     hipEvent_t ev_done = nullptr;     // recorded on this grid's stream after its sub-steps
     hipEvent_t ev_ready = nullptr;    // recorded before a child reads this grid's populations
