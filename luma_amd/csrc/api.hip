@@ -352,7 +352,31 @@ void add_time(lumagpu_grid *g, int steps) {
     for (lumagpu_grid *k : g->kids) add_time(k, 2 * steps);
 }
 
-// Capture two steps of g (and its descendants) into a graph and launch it reps times.
+// Everything the two captured steps pass to their kernels, level by level: while it stays the
+// same, the graph captured from them is the graph a new capture would give.
+std::vector<uint8_t> graph_key(lumagpu_grid *g) {
+    std::vector<uint8_t> key;
+    auto put = [&](const void *p, size_t n) {
+        const uint8_t *b = static_cast<const uint8_t *>(p);
+        key.insert(key.end(), b, b + n);
+    };
+    for_each_level(g, [&](lumagpu_grid *x) {
+        const StepArgs a = make_args(x);
+        put(&a, sizeof a);
+        const int misc[] = {x->Q, x->masked, x->store_macro, x->nmlist, x->nedge, x->zslab, x->has_outlet,
+                            x->gravity_on, x->gravity_dir, (int)x->kids.size(), x->parity, (int)x->d.precision,
+                            x->d.level, x->d.regularised};
+        put(misc, sizeof misc);
+        const void *ptrs[] = {x->mlist, x->edge, x->erho, x->stream, x->force};
+        put(ptrs, sizeof ptrs);
+        put(&x->gravity, sizeof x->gravity);
+        ibm_graph_key(x, key);
+    });
+    return key;
+}
+
+// Capture two steps of g (and its descendants) into a graph and launch it reps times; the
+// executable graph is kept and launched again by later runs while its key holds.
 // Returns 0 when the steps ran, 1 when the capture was not possible (nothing ran, host state
 // restored), or an error code.
 int run_graph(lumagpu_grid *g, int reps) {
@@ -365,26 +389,35 @@ int run_graph(lumagpu_grid *g, int reps) {
             x->parity = 0;
         });
     };
-    if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipGetLastError();
-        return 1;
-    }
-    int rc = 0;
-    for (int k = 0; k < 2 && !rc; ++k) rc = apply_step(g, 0, nullptr, nullptr);
-    hipGraph_t graph = nullptr;
-    const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
-    restore();
-    if (rc || ec != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipGetLastError();
-        return rc ? rc : 1;
-    }
-    hipGraphExec_t exec = nullptr;
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    std::vector<uint8_t> key = graph_key(g);
+    if (!g->gexec || key != g->gkey) {
+        if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return 1;
+        }
+        int rc = 0;
+        for (int k = 0; k < 2 && !rc; ++k) rc = apply_step(g, 0, nullptr, nullptr);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+        restore();
+        if (rc || ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return rc ? rc : 1;
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        (void)hipGetLastError();
-        return 1;
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            return 1;
+        }
+        if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+        g->gexec = exec;
+        // the key after the capture: its host-side effects (IB supports settling) are in it
+        g->gkey = graph_key(g);
     }
+    hipGraphExec_t exec = g->gexec;
     // the graph runs on g's stream: order it after whatever the descendants' streams hold,
     // and the descendants' later work after it
     hipError_t e = hipSuccess;
@@ -399,8 +432,6 @@ int run_graph(lumagpu_grid *g, int reps) {
         e = hipEventRecord(g->ev_ready, g->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(x->stream, g->ev_ready, 0);
     });
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
     add_time(g, 2 * reps);
     return 0;
@@ -732,6 +763,7 @@ int lumagpu_grid_destroy(lumagpu_grid *g) {
     (void)hipFree(g->d_kids);
     if (g->ev_done) (void)hipEventDestroy(g->ev_done);
     if (g->ev_ready) (void)hipEventDestroy(g->ev_ready);
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
     if (g->own_stream) (void)hipStreamDestroy(g->own_stream);
     delete g;
     return 0;

@@ -68,7 +68,8 @@ void delta_grad(const double *x, const double *s, double dh, double dil, int D, 
 
 IbmArgs ibm_args(lumagpu_grid *g) {
     IbmSystem *s = g->ibm;
-    IbmArgs b{};
+    IbmArgs b;
+    std::memset(&b, 0, sizeof b);
     b.nm = s->nm;
     b.S = s->S;
     b.D = g->D;
@@ -91,6 +92,21 @@ IbmArgs ibm_args(lumagpu_grid *g) {
     b.sent = s->d_sent;
     b.accumulate = g->all_forced;
     return b;
+}
+
+void ibm_graph_key(const lumagpu_grid *g, std::vector<uint8_t> &key) {
+    const IbmSystem *s = g->ibm;
+    auto put = [&](const void *p, size_t n) {
+        const uint8_t *b = static_cast<const uint8_t *>(p);
+        key.insert(key.end(), b, b + n);
+    };
+    const int present = s && s->nm > 0;
+    put(&present, sizeof present);
+    if (!present) return;
+    const IbmArgs b = ibm_args(const_cast<lumagpu_grid *>(g));
+    put(&b, sizeof b);
+    const int state[3] = {s->sites_dirty, (int)s->flagged.size(), s->initialised};
+    put(state, sizeof state);
 }
 
 namespace {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,9 @@ struct lumagpu_grid {
     int *mlist = nullptr;
     int nmlist = -1;                  // < 0: one labelled launch over the grid
     std::vector<uint8_t> mlist_types; // the site types mlist was built from
+    // the captured two-step graph of this hierarchy (root grids), reused while its key holds
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> gkey;
     int nstale = 0;
     double *force = nullptr;
     double *uin = nullptr;
@@ -88,7 +92,8 @@ int fail(int code, const std::string &msg);
     } while (0)
 
 inline StepArgs make_args(const lumagpu_grid *g) {
-    StepArgs a{};
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);   // padding included: the bytes key the cached step graph
     a.f = g->f;
     a.ps = g->ps;
     a.nx = g->d.N;
@@ -162,6 +167,8 @@ void for_each_level(G *g, Fn &&fn) {
 int ibm_apply(lumagpu_grid *g);
 void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
+// append what the IB step of g passes to its kernels (ibm.hip), for the step graph's key
+void ibm_graph_key(const lumagpu_grid *g, std::vector<uint8_t> &key);
 // copy the host flag bytes to the device and rebuild the plain-site bitmap (api.hip)
 int upload_flags(lumagpu_grid *g);
 // copy the initial populations of the refined sites other sites read (api.hip); the grid is

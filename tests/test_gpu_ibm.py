@@ -295,11 +295,16 @@ def test_flexible_filament_on_refined_level_fp64_bitwise():
 
 
 def test_graph_replayed_runs_match_eager_steps():
-    # lumagpu_run replays a captured pair of steps (child sub-steps and IB included); replays
-    # interleaved with other stream work must equal the same number of eager steps bit for bit
+    # lumagpu_run replays a captured pair of steps (child sub-steps and IB included) and keeps the
+    # graph for later runs while what its kernels are passed stays the same; replays interleaved
+    # with other stream work, and after a change of marker velocities (new values, same arrays),
+    # must equal the same number of eager steps bit for bit
     parent = setups.channel_2d(64, 32, 8.0, 0.04, 80.0)
     child = setups.add_refined_region(parent, (8, 31, 6, 25), parent.info["dh"], parent.info["dt"])
     pos = setups.circle_markers((2.5 + 0.01, 2.0 + 0.13, 0.0), 0.5, child.info["dh"])
+    vel = np.zeros((len(pos), 3))
+    vel[:, 1] = 1e-3
+    runs = (37, 21, 9, 9)
     out = []
     for mode in ("run", "eager"):
         print(f"[graph] {mode}", flush=True)
@@ -307,17 +312,21 @@ def test_graph_replayed_runs_match_eager_steps():
             gp.add_child(gc)
             gc.ibm_add_body(pos)
             gc.ibm_initialise()
+            for i, n in enumerate(runs):
+                if mode == "run":
+                    gp.run(n)
+                else:
+                    for _ in range(n):
+                        gp.LBM_multi_opt()
+                if i == 0:
+                    m1 = gp.moments()
+            gc.ibm_set_velocity(0, vel)
             if mode == "run":
-                gp.run(37)
-                m1 = gp.moments()
-                gp.run(21)
-                assert gp.t == 58 and gc.t == 116
+                gp.run(10)
             else:
-                for _ in range(37):
+                for _ in range(10):
                     gp.LBM_multi_opt()
-                m1 = gp.moments()
-                for _ in range(21):
-                    gp.LBM_multi_opt()
+            assert gp.t == sum(runs) + 10 and gc.t == 2 * gp.t
             out.append((m1, gp.download_f(), gc.download_f(), gc.ibm_markers(0)["force"]))
     (ma, pa, ca, fa), (mb, pb, cb, fb) = out
     assert ma == mb
