@@ -122,6 +122,9 @@ struct StepArgs {
     int subcycle;
     // --- ExtrapolateRight sources: xr[v*(ny*nz) + y + ny*z] = f*_v two cells left of (N-1, y, z)
     const double *xr;
+    // the same buffer for the next step, written by this step's sites at x = N-3 with their
+    // post-collision populations as stored (null: the next step's pre-pass rebuilds it)
+    double *xr_next;
     // --- z-slab with halo planes: the step updates planes zoff .. nz-1-zoff only ------------
     int zoff;
     // --- refined sites read by sites other than TL2F (a region touching a velocity, pressure or
@@ -1069,6 +1072,10 @@ __device__ __forceinline__ void masked_site(const StepArgs &a, int x, int y, int
         }
     }
 
+    if (a.xr_next && x == nx - 3) {
+        const size_t row = (size_t)y + (size_t)ny * z, plane = (size_t)ny * a.nz;
+        for (int v = 0; v < Q; ++v) mv<M>(a.xr_next)[(size_t)v * plane + row] = M(static_cast<T>(static_cast<S>(fout[v])));
+    }
     static_for<Q>([&](auto vc) {
         constexpr int v = decltype(vc)::value;
         constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
@@ -1124,11 +1131,15 @@ __global__ __launch_bounds__(256) void k_aa_plain(StepArgs a) {
     nb.init(x, y, z, nx, ny, a.nz);
     T fin[Q];
     gather_plain<Q, P, ODD>(a, cell, nb, fin);
+    // the outlet side buffer of the next step (see StepArgs::xr_next)
+    M *xr_row = (a.xr_next && x == nx - 3) ? mv<M>(a.xr_next) + (size_t)y + (size_t)ny * z : nullptr;
+    const size_t xr_plane = (size_t)ny * a.nz;
     auto put = [&](auto vc, T o) {
         constexpr int v = decltype(vc)::value;
         constexpr int cx = L::c(v, 0), cy = L::c(v, 1), cz = L::c(v, 2);
         if constexpr (ODD) st<P, cx == 0>(f + (size_t)v * ps + nb.at(cx, cy, cz, nx, ny), o);
         else st<P>(f + (size_t)L::opp(v) * ps + cell, o);
+        if (xr_row) xr_row[(size_t)v * xr_plane] = M(static_cast<T>(static_cast<S>(o)));
     };
     auto step = [&](const M *F) {
         M rho, u[3];

@@ -151,6 +151,7 @@ int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsb
 template <class P>
 int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int backward, lumagpu_adj_result *res,
             double *dpos, double *lambda0) {
+    g->xr_ready = 0;   // the populations change outside the step kernels
     using T = typename P::T;
     IbmSystem *s = g->ibm;
     const bool ib = s && s->nm > 0;
@@ -245,6 +246,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         const int t0 = seg * every, t1 = std::min(nsteps, t0 + every);
         HIP_TRY(hipMemcpyAsync(g->f, ck[seg].p, fbytes, hipMemcpyDeviceToDevice, g->stream));
         g->parity = ckpar[seg];
+        g->xr_ready = 0;
         for (int t = t0; t < t1 && !rc; ++t) {
             HIP_TRY(hipMemcpyAsync(tape[t - t0].p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
             tpar[t - t0] = g->parity;
@@ -290,6 +292,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     HIP_TRY(hipMemcpyAsync(g->f, fin.p, fbytes, hipMemcpyDeviceToDevice, g->stream));
     g->parity = fin_par;
     g->t = fin_t;
+    g->xr_ready = 0;
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
     std::vector<T> hl;

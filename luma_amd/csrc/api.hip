@@ -151,12 +151,13 @@ hipError_t macro_to_luma(lumagpu_grid *g, double *rho, double *u, bool tan) {
 
 hipError_t launch_extrap(const lumagpu_grid *g) {
     const StepArgs a = make_args(g);
+    double *xr = const_cast<double *>(a.xr);
     switch (g->d.precision) {
-        case LUMAGPU_FP64: return launch_aux_f64(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
-        case LUMAGPU_FP32S: return launch_aux_f32s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
-        case LUMAGPU_FP64T: return launch_aux_dual(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
-        case LUMAGPU_FP32T: return launch_aux_dual32(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
-        default: return launch_aux_f16s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, g->xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP64: return launch_aux_f64(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP32S: return launch_aux_f32s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP64T: return launch_aux_dual(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
+        case LUMAGPU_FP32T: return launch_aux_dual32(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
+        default: return launch_aux_f16s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
     }
 }
 
@@ -181,7 +182,8 @@ void update_reynolds(lumagpu_grid *g) {
 int begin_step(lumagpu_grid *g) {
     // the outlet side buffer reads this grid's own populations only (the children read them
     // too, but nobody writes them before this grid's step): it runs while the children step
-    if (g->has_outlet) {
+    // the previous step wrote this step's outlet sources unless the state changed since
+    if (g->has_outlet && !g->xr_ready) {
         hipError_t e = launch_extrap(g);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("outlet launch: ") + hipGetErrorString(e));
     }
@@ -237,6 +239,12 @@ int end_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     hipError_t e = launch_step(g, ls, a);
     if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("step launch: ") + hipGetErrorString(e));
     if (e1) HIP_TRY(hipEventRecord(e1, g->stream));
+    if (a.xr_next) {
+        g->xr_sel ^= 1;
+        g->xr_ready = 1;
+    } else {
+        g->xr_ready = 0;
+    }
     if (halo) {
         e = launch_zslab(g, a, 1);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("halo launch: ") + hipGetErrorString(e));
@@ -365,7 +373,7 @@ std::vector<uint8_t> graph_key(lumagpu_grid *g) {
         put(&a, sizeof a);
         const int misc[] = {x->Q, x->masked, x->store_macro, x->nmlist, x->nedge, x->zslab, x->has_outlet,
                             x->gravity_on, x->gravity_dir, (int)x->kids.size(), x->parity, (int)x->d.precision,
-                            x->d.level, x->d.regularised};
+                            x->d.level, x->d.regularised, x->xr_sel, x->xr_ready, x->xr_fold};
         put(misc, sizeof misc);
         const void *ptrs[] = {x->mlist, x->edge, x->erho, x->stream, x->force};
         put(ptrs, sizeof ptrs);
@@ -892,10 +900,29 @@ int lumagpu_set_labels(lumagpu_grid *g, const int *lattyp, const double *xpos, c
     g->hflags = std::move(nf);
     if (int r = upload_flags(g)) return r;
     if (outlet && !g->xr &&
-        hipMalloc(&g->xr, (size_t)g->Q * g->d.M * g->d.K * g->melem) != hipSuccess)
+        hipMalloc(&g->xr, 2 * (size_t)g->Q * g->d.M * g->d.K * g->melem) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "outlet buffer allocation failed");
     HIP_TRY(hipStreamSynchronize(g->stream));
     g->has_outlet = outlet;
+    // the step kernels can write the next step's outlet sources when every outlet row's source
+    // two cells left is a site they step (collided, or streamed only for TL2C)
+    g->xr_fold = 0;
+    g->xr_ready = 0;
+    if (outlet && g->d.N >= 3) {
+        bool ok = true;
+        const int N = g->d.N, M = g->d.M, K = g->d.K;
+        for (int k = 0; k < K && ok; ++k)
+            for (int j = 0; j < M && ok; ++j) {
+                const size_t row = (size_t)N * ((size_t)j + (size_t)M * k);
+                if ((g->hflags[row + N - 1] & kTypeMask) != kExtrapRight) continue;
+                const unsigned fl = g->hflags[row + N - 3];
+                const int t = fl & kTypeMask;
+                const bool stepped = t == kFluid || t == kTL2F || t == kTL2C ||
+                                     (t == kVelocity && g->d.regularised && ((fl >> kNormalShift) & 7) != kNormalEdge);
+                ok = stepped;
+            }
+        g->xr_fold = ok ? 1 : 0;
+    }
     g->masked = any || g->d.forced;
     return 0;
 }
@@ -944,6 +971,7 @@ int lumagpu_checkpoint_restore(lumagpu_grid *g, const lumagpu_checkpoint *ck) {
     if (!ck) return fail(LUMAGPU_EINVAL, "null checkpoint");
     if (ck->owner != g) return fail(LUMAGPU_EINVAL, "checkpoint belongs to another grid");
     HIP_TRY(hipMemcpyAsync(g->f, ck->f, ck->fbytes, hipMemcpyDeviceToDevice, g->stream));
+    g->xr_ready = 0;
     HIP_TRY(hipMemcpyAsync(g->rho, ck->rho, g->ncell * g->melem, hipMemcpyDeviceToDevice, g->stream));
     HIP_TRY(hipMemcpyAsync(g->u, ck->u, g->ncell * g->D * g->melem, hipMemcpyDeviceToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -1045,6 +1073,7 @@ int lumagpu_init_equilibrium(lumagpu_grid *g, const double *rho, const double *u
     HIP_TRY(macro_from_luma(g, drho, du));
     g->parity = 0;
     g->has_state = 1;
+    g->xr_ready = 0;
     if (int r = capture_stale(g)) return r;
     HIP_TRY(hipStreamSynchronize(g->stream));
     return 0;
@@ -1059,6 +1088,7 @@ int lumagpu_upload_f(lumagpu_grid *g, const double *f) {
     HIP_TRY(launch_aux(g, kAuxUpload, g->stage, nullptr, nullptr));
     g->parity = 0;
     g->has_state = 1;
+    g->xr_ready = 0;
     if (int r = capture_stale(g)) return r;
     HIP_TRY(hipStreamSynchronize(g->stream));
     return 0;

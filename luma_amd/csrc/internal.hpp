@@ -53,8 +53,12 @@ struct lumagpu_grid {
     lumagpu_grid *parent = nullptr;   // GridObj::parentGrid (non-owning)
     std::vector<lumagpu_grid *> kids; // GridObj::subGrid (non-owning), stepped in this order
     lumagpu::ChildRef *d_kids = nullptr;   // device table of the kids for the step kernels
-    double *xr = nullptr;             // ExtrapolateRight side buffer, Q*M*K
+    double *xr = nullptr;             // ExtrapolateRight side buffers, 2 x Q*M*K (this step's, the next's)
     int has_outlet = 0;
+    int xr_sel = 0;                   // which of the two is this step's
+    int xr_ready = 0;                 // this step's holds the sources (written by the previous step)
+    int xr_fold = 0;                  // the step kernels write the next step's (every outlet row's x = N-3
+                                      // site is one they step)
     int nedge = 0;                    // regularised velocity sites on edges and corners
     int *edge = nullptr;              // [4*nedge]: cells, packed normals, neighbour x+n, x+2n
     double *erho = nullptr;           // [8*nedge] neighbour rho, u of the current step (edge/pressure)
@@ -148,7 +152,9 @@ inline StepArgs make_args(const lumagpu_grid *g) {
     }
     a.kids = g->d_kids;
     a.nkids = (int)g->kids.size();
-    a.xr = g->xr;
+    a.xr = g->xr ? g->xr + (size_t)g->xr_sel * g->Q * g->d.M * g->d.K * (g->melem / sizeof(double)) : nullptr;
+    a.xr_next = (g->xr && g->xr_fold) ? g->xr + (size_t)(g->xr_sel ^ 1) * g->Q * g->d.M * g->d.K * (g->melem / sizeof(double))
+                                       : nullptr;
     a.stale = g->stale;
     a.stale_idx = g->stale_idx;
     a.nstale = g->nstale;
