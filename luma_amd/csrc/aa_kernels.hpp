@@ -32,6 +32,7 @@
 #include <utility>
 
 #include "dual.hpp"
+#include "ibm_kernels.hpp"
 #include "lattice.hpp"
 
 namespace lumagpu {
@@ -133,7 +134,15 @@ struct StepArgs {
     const void *stale;
     const int *stale_idx;
     int nstale;
+    // --- ibm_spread done by the step kernel (rigid bodies whose supports did not move): a
+    // forced site with ib_site_of[cell] >= 0 sums its spread force (spread_sum, k_ibm_spread's
+    // order), writes it to force_xyz and uses it.  Marker arrays hold P::M elements.
+    const int *ib_site_of;
+    const int *ib_sptr, *ib_sent;
+    const double *ib_sdelta, *ib_force, *ib_eps, *ib_ds;
+    int ib_S, ib_accumulate;
 };
+
 
 // ----------------------------------------------------------------------------------------
 // storage helpers
@@ -173,6 +182,28 @@ __device__ __forceinline__ M *mv(double *p) {
 template <class M>
 __device__ __forceinline__ const M *mv(const double *p) {
     return reinterpret_cast<const M *>(p);
+}
+
+// The body force of a forced site: spread here (see StepArgs::ib_site_of) or read from force_xyz
+template <int D, class M>
+__device__ __forceinline__ void site_force(const StepArgs &a, size_t cell, unsigned fl, M Fv[3]) {
+    const int sidx = (a.ib_site_of && (fl & kForcedBit)) ? a.ib_site_of[cell] : -1;
+    M *fx = mv<M>(const_cast<double *>(a.force));
+    if (sidx >= 0) {
+        M acc[3] = {M(0.0), M(0.0), M(0.0)};
+        if (a.ib_accumulate)
+            for (int d = 0; d < D; ++d) acc[d] = fx[d * a.ncell + cell];
+        spread_sum<M, D>(sidx, a.ib_S, a.ib_sptr, a.ib_sent, mv<M>(a.ib_sdelta), mv<M>(a.ib_force), mv<M>(a.ib_eps),
+                         mv<M>(a.ib_ds), acc);
+        for (int d = 0; d < D; ++d) fx[d * a.ncell + cell] = acc[d];
+        Fv[0] = acc[0];
+        Fv[1] = acc[1];
+        Fv[2] = (D == 3) ? acc[2] : M(0.0);
+    } else {
+        Fv[0] = fx[cell];
+        Fv[1] = fx[a.ncell + cell];
+        Fv[2] = (D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
+    }
 }
 
 // Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1.
@@ -1009,10 +1040,7 @@ __device__ __forceinline__ void masked_site(const StepArgs &a, int x, int y, int
     const M *F = nullptr;
     M Fv[3];
     if (a.all_forced || (fl & kForcedBit)) {
-        const M *fx = mv<M>(a.force);
-        Fv[0] = fx[cell];
-        Fv[1] = fx[a.ncell + cell];
-        Fv[2] = (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0);
+        site_force<L::D>(a, cell, fl, Fv);
         F = Fv;
     }
 
@@ -1154,9 +1182,10 @@ __global__ __launch_bounds__(256) void k_aa_plain(StepArgs a) {
         });
         put(std::integral_constant<int, Q - 1>{}, col(Q - 1, fin[Q - 1]));
     };
-    if (a.all_forced || (a.flags[cell] & kForcedBit)) {
-        const M *fx = mv<M>(a.force);
-        const M Fv[3] = {fx[cell], fx[a.ncell + cell], (L::D == 3) ? fx[2 * a.ncell + cell] : M(0.0)};
+    const unsigned fl = a.flags[cell];
+    if (a.all_forced || (fl & kForcedBit)) {
+        M Fv[3];
+        site_force<L::D>(a, cell, fl, Fv);
         step(Fv);
     } else {
         step(nullptr);

@@ -226,6 +226,8 @@ int end_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0, g->mlist, g->nmlist};
     StepArgs a = make_args(g);
     a.subcycle = subcycle;
+    if (g->ib_fold) ibm_fold_args(g, a);
+    g->ib_fold = 0;
     if (g->nedge) {
         hipError_t e = launch_edges(g, a, 0);
         if (e != hipSuccess) return fail(LUMAGPU_EHIP, std::string("edge launch: ") + hipGetErrorString(e));
@@ -269,7 +271,7 @@ int apply_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     if (int r = begin_step(g)) return r;
     // ObjectManager::ibm_apply between sweep 1 and sweep 2 (:120-121)
     if (g->ibm)
-        if (int r = ibm_apply(g)) return r;
+        if (int r = ibm_apply(g, true)) return r;
     return end_step(g, subcycle, e0, e1);
 }
 

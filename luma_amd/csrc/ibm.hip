@@ -94,6 +94,8 @@ IbmArgs ibm_args(lumagpu_grid *g) {
     return b;
 }
 
+int build_site_map(lumagpu_grid *g);
+
 void ibm_graph_key(const lumagpu_grid *g, std::vector<uint8_t> &key) {
     const IbmSystem *s = g->ibm;
     auto put = [&](const void *p, size_t n) {
@@ -105,8 +107,9 @@ void ibm_graph_key(const lumagpu_grid *g, std::vector<uint8_t> &key) {
     if (!present) return;
     const IbmArgs b = ibm_args(const_cast<lumagpu_grid *>(g));
     put(&b, sizeof b);
-    const int state[3] = {s->sites_dirty, (int)s->flagged.size(), s->initialised};
+    const int state[4] = {s->sites_dirty, (int)s->flagged.size(), s->initialised, s->site_map_ok};
     put(state, sizeof state);
+    put(&s->d_site_of, sizeof s->d_site_of);
 }
 
 namespace {
@@ -537,7 +540,12 @@ int upload(lumagpu_grid *g) {
         return r;
     if (!s->d_out && hipMalloc(&s->d_out, 6 * sizeof(double)) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "IB buffer allocation failed");
-    if (order != s->sites) s->sites_dirty = 1;
+    if (order != s->sites) {
+        s->sites_dirty = 1;
+        s->site_map_ok = 0;
+    }
+    if (!s->d_site_of && !ibm_has_flexible(g) && hipMalloc(&s->d_site_of, g->ncell * sizeof(int)) != hipSuccess)
+        return fail(LUMAGPU_ENOMEM, "IB site map allocation failed");
     s->sites = order;
     s->nsites = (int)order.size();
     return 0;
@@ -694,6 +702,8 @@ int spread(lumagpu_grid *g) {
         for (int c : s->sites) g->hflags[c] |= kForcedBit;
         s->flagged = s->sites;
         s->sites_dirty = 0;
+        if (!ibm_has_flexible(g))
+            if (int r = build_site_map(g)) return r;
     }
     return 0;
 }
@@ -702,6 +712,42 @@ int spread(lumagpu_grid *g) {
 int apply_core(lumagpu_grid *g) {
     if (int r = interpolate(g, 0)) return r;
     return spread(g);
+}
+
+namespace {
+__global__ void k_site_map_fill(int *map, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) map[i] = -1;
+}
+__global__ void k_site_map_set(int *map, const int *sites, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) map[sites[i]] = i;
+}
+}  // namespace
+
+// the device cell -> support index map of a rigid system (built once its supports settle;
+// upload allocated it)
+int build_site_map(lumagpu_grid *g) {
+    IbmSystem *s = g->ibm;
+    if (!s->d_site_of) return 0;
+    k_site_map_fill<<<(unsigned)((g->ncell + 255) / 256), 256, 0, g->stream>>>(s->d_site_of, g->ncell);
+    if (s->nsites > 0) k_site_map_set<<<(s->nsites + 255) / 256, 256, 0, g->stream>>>(s->d_site_of, s->d_sites, s->nsites);
+    HIP_TRY(hipGetLastError());
+    s->site_map_ok = 1;
+    return 0;
+}
+
+void ibm_fold_args(const lumagpu_grid *g, StepArgs &a) {
+    const IbmSystem *s = g->ibm;
+    a.ib_site_of = s->d_site_of;
+    a.ib_sptr = s->d_sptr;
+    a.ib_sent = s->d_sent;
+    a.ib_sdelta = s->d_sdelta;
+    a.ib_force = s->d_force;
+    a.ib_eps = s->d_eps;
+    a.ib_ds = s->d_ds;
+    a.ib_S = s->S;
+    a.ib_accumulate = g->all_forced;
 }
 
 namespace {
@@ -828,13 +874,22 @@ double vel_diff(const lumagpu_grid *g) {
 // step kernel, which recomputes u with +F/2 wherever the force bit is set; every application
 // recomputes the sweep-1 rho, u at its supports from the populations, which is LUMA's
 // u = u_n reset.
-int ibm_apply(lumagpu_grid *g) {
+int ibm_apply(lumagpu_grid *g, bool fold_ok) {
     IbmSystem *s = g->ibm;
     if (!s || s->nm == 0) return 0;
     if (!s->initialised) return fail(LUMAGPU_ESTATE, "IB bodies added but lumagpu_ibm_initialise not called");
     if (g->all_forced && ibm_has_flexible(g))
         return fail(LUMAGPU_EUNSUPPORTED, "gravity with flexible bodies is not built (each FSI sub-iteration "
                                           "re-resets gravity from the updated density)");
+    // Rigid bodies whose supports are where the last spread left them: the step kernel that
+    // follows sums the spread force at each support itself (one launch fewer per step).  Not with
+    // gravity, whose reset value the edge-site pre-pass reads from force_xyz before the step.
+    if (fold_ok && !ibm_has_flexible(g) && !s->sites_dirty && !g->all_forced && s->site_map_ok &&
+        s->flagged == s->sites) {
+        if (int r = interpolate(g, 0)) return r;
+        g->ib_fold = 1;
+        return 0;
+    }
     if (int r = apply_core(g)) return r;
     bool flexible = false;
     for (const auto &b : s->bodies) flexible = flexible || b.flexible;

@@ -111,29 +111,22 @@ __global__ void k_ibm_interp_force(IbmArgs b, int mode) {
     interp_force_one<M>(b, m, mode);
 }
 
-// ibm_spread (:621-689) as a per-site gather; the site's force starts from the reset value 0
-// (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
-// step kernel adds it in the macro (+F/2) and the Guo term.  Entries are read in chunks whose
-// loads are issued together, then summed in entry order.
-template <class M = double, int DD = 0>
-__device__ __forceinline__ void spread_one(const IbmArgs &b, int s) {
+// The sum of ibm_spread (:621-689) at support site s: acc starts from the site's reset value
+// and takes every (marker, slot) entry of the site in LUMA's accumulation order.  Entries are
+// read in chunks whose loads are issued together, then summed in entry order.
+template <class M, int D>
+__device__ __forceinline__ void spread_sum(int s, int S, const int *sptr, const int *sent, const M *sdelta,
+                                           const M *force, const M *eps, const M *ds, M acc[3]) {
     constexpr int CH = sizeof(M) == sizeof(double) ? 6 : 3;
-    const int D = DD ? DD : b.D;
-    const M *sdelta = as<M>(b.sdelta), *force = as<M>(b.force), *eps = as<M>(b.eps), *ds = as<M>(b.ds);
-    const size_t cell = (size_t)b.sites[s];
-    M *fxyz = as<M>(b.fxyz);
-    M acc[3] = {0.0, 0.0, 0.0};
-    if (b.accumulate)
-        for (int d = 0; d < D; ++d) acc[d] = fxyz[d * b.ncell + cell];
-    const int e0 = b.sptr[s], e1 = b.sptr[s + 1];
+    const int e0 = sptr[s], e1 = sptr[s + 1];
     for (int eb = e0; eb < e1; eb += CH) {
         int code[CH];
 #pragma unroll
-        for (int j = 0; j < CH; ++j) code[j] = b.sent[(eb + j < e1) ? eb + j : eb];
+        for (int j = 0; j < CH; ++j) code[j] = sent[(eb + j < e1) ? eb + j : eb];
         M dl[CH], fm[CH][3], vw[CH], dsm[CH];
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
-            const int m = code[j] / b.S;
+            const int m = code[j] / S;
             dl[j] = sdelta[code[j]];
             vw[j] = eps[m];
             dsm[j] = ds[m];
@@ -150,6 +143,23 @@ __device__ __forceinline__ void spread_one(const IbmArgs &b, int s) {
             }
         }
     }
+}
+
+// ibm_spread as a per-site gather; the site's force starts from the reset value 0
+// (_LBM_resetForces, GridObj_ops_lbm_optimised.cpp:1327-1337) and the site is flagged so the
+// step kernel adds it in the macro (+F/2) and the Guo term.
+template <class M = double, int DD = 0>
+__device__ __forceinline__ void spread_one(const IbmArgs &b, int s) {
+    const size_t cell = (size_t)b.sites[s];
+    M *fxyz = as<M>(b.fxyz);
+    M acc[3] = {0.0, 0.0, 0.0};
+    const int D = DD ? DD : b.D;
+    if (b.accumulate)
+        for (int d = 0; d < D; ++d) acc[d] = fxyz[d * b.ncell + cell];
+    if (D == 3)
+        spread_sum<M, 3>(s, b.S, b.sptr, b.sent, as<M>(b.sdelta), as<M>(b.force), as<M>(b.eps), as<M>(b.ds), acc);
+    else
+        spread_sum<M, 2>(s, b.S, b.sptr, b.sent, as<M>(b.sdelta), as<M>(b.force), as<M>(b.eps), as<M>(b.ds), acc);
     for (int d = 0; d < D; ++d) fxyz[d * b.ncell + cell] = acc[d];
     b.flags[cell] |= kForcedBit;
 }

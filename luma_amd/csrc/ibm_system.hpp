@@ -80,6 +80,8 @@ struct IbmSystem {
     int *d_gone = nullptr, *h_gone = nullptr;   // cells to clear (clear_gone), device and pinned host
     size_t gone_cap = 0;
     std::vector<int> site_of;   // per cell: index of the support site, -1 elsewhere (kept all -1 between uses)
+    int *d_site_of = nullptr;   // device copy for the step kernel's spread (rigid systems), -1 elsewhere
+    int site_map_ok = 0;        // d_site_of matches the current sites
     int nsites = 0;
     std::vector<int> sites;     // host copy of the distinct support cells
     std::vector<int> flagged;   // cells whose force and flag the device currently holds
@@ -91,7 +93,7 @@ struct IbmSystem {
     size_t pinned_pos = 0;
 
     ~IbmSystem() {
-        for (void *p : {d_pack, (void *)d_interp, (void *)d_force, (void *)d_out, (void *)d_gone})
+        for (void *p : {d_pack, (void *)d_interp, (void *)d_force, (void *)d_out, (void *)d_gone, (void *)d_site_of})
             if (p) (void)hipFree(p);
         for (void *p : {h_pack, (void *)h_force, (void *)h_gone})
             if (p) (void)hipHostFree(p);

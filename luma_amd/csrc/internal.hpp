@@ -35,6 +35,7 @@ struct lumagpu_grid {
     int nmlist = -1;                  // < 0: one labelled launch over the grid
     std::vector<uint8_t> mlist_types; // the site types mlist was built from
     // the captured two-step graph of this hierarchy (root grids), reused while its key holds
+    int ib_fold = 0;                  // the step kernel does this step's IB spread
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> gkey;
     int nstale = 0;
@@ -170,7 +171,10 @@ void for_each_level(G *g, Fn &&fn) {
 
 // IB step between the stream/macro and the force/collide halves (ObjectManager::ibm_apply);
 // defined in ibm.hip
-int ibm_apply(lumagpu_grid *g);
+// fold_ok: the spread may be left to the step kernel that follows (see StepArgs::ib_site_of)
+int ibm_apply(lumagpu_grid *g, bool fold_ok = false);
+// the step kernel's spread arguments when the last IB application left the spread to it
+void ibm_fold_args(const lumagpu_grid *g, StepArgs &a);
 void ibm_destroy(lumagpu_grid *g);
 int ensure_flags(lumagpu_grid *g);
 // append what the IB step of g passes to its kernels (ibm.hip), for the step graph's key
