@@ -1012,14 +1012,20 @@ __device__ __forceinline__ void velocity_site(const StepArgs &a, unsigned fl, in
 }
 
 // labelled grid: solid (HWBB), velocity (regularised or forced equilibrium), ExtrapolateRight
-// sources, refinement transition layers, body force.
-template <int Q, class P, bool ODD, bool STORE>
+// sources, refinement transition layers, body force.  SPEC: issue the regular pulls before the
+// site's flags are known (most sites are plain, and their step then waits on one memory round
+// trip less); sites that turn out otherwise gather again.
+template <int Q, class P, bool ODD, bool STORE, bool SPEC = false>
 __device__ __forceinline__ void masked_site(const StepArgs &a, int x, int y, int z) {
     using L = Lattice<Q>;
     using S = typename P::S;
     using T = typename P::T;
     const int nx = a.nx, ny = a.ny;
     const size_t cell = cell_of(x, y, z, nx, ny);
+    Nbr nb;
+    nb.init(x, y, z, nx, ny, a.nz);
+    T fin[Q], fout[Q];
+    if constexpr (SPEC) gather_plain<Q, P, ODD>(a, cell, nb, fin);
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
     // sites LUMA's sweep 1 skips (:91-95)
@@ -1028,13 +1034,10 @@ __device__ __forceinline__ void masked_site(const StepArgs &a, int x, int y, int
     if (ty == kPressure || (ty == kVelocity && ((fl >> kNormalShift) & 7) == kNormalEdge)) return;
     S *__restrict__ f = static_cast<S *>(a.f);
     const size_t ps = a.ps;
-    Nbr nb;
-    nb.init(x, y, z, nx, ny, a.nz);
 
-    T fin[Q], fout[Q];
     const bool plain = a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u);
-    if (plain) gather_plain<Q, P, ODD>(a, cell, nb, fin);
-    else gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
+    if (!plain) gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
+    else if constexpr (!SPEC) gather_plain<Q, P, ODD>(a, cell, nb, fin);
 
     using M = typename P::M;
     const M *F = nullptr;
@@ -1123,7 +1126,7 @@ template <int Q, class P, bool ODD, bool STORE>
 __global__ __launch_bounds__(256) void k_aa_masked(StepArgs a) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= a.nx) return;
-    masked_site<Q, P, ODD, STORE>(a, x, blockIdx.y, blockIdx.z);
+    masked_site<Q, P, ODD, STORE, true>(a, x, blockIdx.y, blockIdx.z);
 }
 
 // The same step for a list of sites: on large labelled grids the plain sites run in
@@ -1206,16 +1209,17 @@ __device__ __forceinline__ bool site_macro_value(const StepArgs &a, size_t cell,
     using T = typename P::T;
     using M = typename P::M;
     const int x = (int)(cell % a.nx), y = (int)((cell / a.nx) % a.ny), z = (int)(cell / ((size_t)a.nx * a.ny));
+    Nbr nb;
+    nb.init(x, y, z, a.nx, a.ny, a.nz);
+    // regular pulls issued before the flags arrive (see masked_site's SPEC)
+    T fin[Q];
+    gather_plain<Q, P, ODD>(a, cell, nb, fin);
     const unsigned fl = a.flags[cell];
     const int ty = fl & kTypeMask;
     const bool macro_type = ty == kFluid || ty == kTL2F;
     const bool vel_type = !ib_force && a.regularised && ty == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
     if (!macro_type && !vel_type) return false;
-    Nbr nb;
-    nb.init(x, y, z, a.nx, a.ny, a.nz);
-    T fin[Q];
-    if (a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u)) gather_plain<Q, P, ODD>(a, cell, nb, fin);
-    else gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
+    if (!(a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u))) gather_in<Q, P, ODD>(a, x, y, z, cell, nb, ty, fin);
     if (vel_type) {
         T rr, uu[3];
         velocity_site<Q, P>(a, fl, y, fin, rr, uu);
