@@ -71,11 +71,12 @@ def c3_cylinder(nx: int = 1024, ny: int = 256, patch=C3_PATCH, centre=C3_CENTRE,
 def strouhal(cl: np.ndarray, dstep: float, diameter: float, u0: float, tail: float = 0.5):
     """St = f D / U0 with f the dominant frequency of the lift signal over its last ``tail``
     fraction (FFT peak of the mean-removed, Hann-windowed signal, refined by a parabola
-    through the peak and its two neighbours).  Returns (St, period in steps)."""
+    through the peak and its two neighbours).  Returns (St, period in steps), or (None, None)
+    for a window too short or flat to the last bits (a steady wake)."""
     x = cl[int(len(cl) * (1.0 - tail)):]
-    x = (x - x.mean()) * np.hanning(len(x))
-    if len(x) < 16 or not np.any(x):
+    if len(x) < 16 or np.ptp(x) <= 1e-12 * max(1.0, float(np.abs(x).max())):
         return None, None
+    x = (x - x.mean()) * np.hanning(len(x))
     p = np.abs(np.fft.rfft(x)) ** 2
     k = int(np.argmax(p[1:])) + 1
     if 1 <= k < len(p) - 1:
