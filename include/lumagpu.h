@@ -175,8 +175,10 @@ int lumagpu_set_velocity_ramp(lumagpu_grid *g, double ramp_time);
  * box without labels, force, refinement or bodies: lumagpu_run and lumagpu_run_timed then
  * advance such a grid two steps per kernel launch (even and odd AA step fused through LDS, a
  * TX x TY tile of columns per block marching `zchunk` z planes; shape 0 = 64 x 8, 1 = 32 x 16,
- * 2 = 128 x 4, plus 8 for two alternating LDS copies (shapes 8, 9, 10, FP32- and FP16-shifted
- * only); N and M whole numbers of tiles), bit for bit the same populations and macro arrays as
+ * 2 = 128 x 4, plus 8 for two alternating LDS copies (FP32- and FP16-shifted only), plus 16
+ * (with 8) for loads one plane ahead, plus 32 (with 8) for LDS copies holding only the tile's
+ * read windows; 24 with zchunk 128 is the fastest at 256^3; N and M whole numbers of tiles),
+ * bit for bit the same populations and macro arrays as
  * two single steps in FP64 and FP32-shifted storage, and within one unit in the last place of
  * the stored half per value in FP16-shifted.  Allocates a second population array (the kernel
  * reads one and writes the other); shape < 0 frees it and turns the two-step kernel off.

@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(256) void k_aa_masked_list(StepArgs a, const int *c
 // k_aa_masked_list: the unlabelled kernel's pulls and pushes plus the body force where the
 // site carries one, with the arithmetic of masked_site.  It keeps the register use (and so the
 // occupancy) of the unlabelled kernel, which the boundary branches of masked_site would not.
-template <int Q, class P, bool ODD, bool STORE>
+template <int Q, class P, bool ODD, bool STORE, bool FORCED>
 __global__ __launch_bounds__(256) void k_aa_plain(StepArgs a) {
     using L = Lattice<Q>;
     using S = typename P::S;
@@ -1185,14 +1185,20 @@ __global__ __launch_bounds__(256) void k_aa_plain(StepArgs a) {
         });
         put(std::integral_constant<int, Q - 1>{}, col(Q - 1, fin[Q - 1]));
     };
-    const unsigned fl = a.flags[cell];
-    if (a.all_forced || (fl & kForcedBit)) {
-        M Fv[3];
-        site_force<L::D>(a, cell, fl, Fv);
-        step(Fv);
-    } else {
-        step(nullptr);
+    // FORCED: grids that may carry a body force (IB bodies, gravity, a set force field).  The
+    // force path (with the spread sum of settled rigid bodies) doubles the kernel's registers,
+    // so grids without a force take the variant without it, at the unlabelled kernel's
+    // registers and occupancy.
+    if constexpr (FORCED) {
+        const unsigned fl = a.flags[cell];
+        if (a.all_forced || (fl & kForcedBit)) {
+            M Fv[3];
+            site_force<L::D>(a, cell, fl, Fv);
+            step(Fv);
+            return;
+        }
     }
+    step(nullptr);
 }
 
 // Macroscopic values at a list of sites (every cell when cells is null), without touching the

@@ -224,6 +224,7 @@ int begin_step(lumagpu_grid *g) {
 // pass, the edge/corner and pressure sites around it, then the swap (AA parity) and ++t.
 int end_step(lumagpu_grid *g, int subcycle, hipEvent_t e0, hipEvent_t e1) {
     LaunchSpec ls{g->Q, g->masked != 0, g->parity == 0, g->store_macro != 0, g->mlist, g->nmlist};
+    ls.forced = g->all_forced || g->ibm || g->host_forced;
     StepArgs a = make_args(g);
     a.subcycle = subcycle;
     if (g->ib_fold) ibm_fold_args(g, a);
@@ -375,7 +376,8 @@ std::vector<uint8_t> graph_key(lumagpu_grid *g) {
         put(&a, sizeof a);
         const int misc[] = {x->Q, x->masked, x->store_macro, x->nmlist, x->nedge, x->zslab, x->has_outlet,
                             x->gravity_on, x->gravity_dir, (int)x->kids.size(), x->parity, (int)x->d.precision,
-                            x->d.level, x->d.regularised, x->xr_sel, x->xr_ready, x->xr_fold};
+                            x->d.level, x->d.regularised, x->xr_sel, x->xr_ready, x->xr_fold,
+                            x->host_forced, x->ibm != nullptr};
         put(misc, sizeof misc);
         const void *ptrs[] = {x->mlist, x->edge, x->erho, x->stream, x->force};
         put(ptrs, sizeof ptrs);
@@ -656,6 +658,7 @@ int upload_flags(lumagpu_grid *g) {
     if (!g->plain && hipMalloc(&g->plain, nw * sizeof(uint32_t)) != hipSuccess)
         return fail(LUMAGPU_ENOMEM, "plain-site bitmap allocation failed");
     HIP_TRY(hipMemcpyAsync(g->flags, g->hflags.data(), g->ncell, hipMemcpyHostToDevice, g->stream));
+    g->host_forced = std::any_of(g->hflags.begin(), g->hflags.end(), [](uint8_t f) { return (f & kForcedBit) != 0; });
     const unsigned nb = (unsigned)((nw + 255) / 256);
     if (g->Q == 9) k_plain_bits<9><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
     else k_plain_bits<19><<<nb, 256, 0, g->stream>>>(g->flags, g->d.N, g->d.M, g->d.K, g->ncell, g->plain);
@@ -696,7 +699,7 @@ int lumagpu_grid_create(const lumagpu_grid_desc *desc, lumagpu_grid **out) {
     g->Q = d.Q;
     g->D = d.dims;
     g->ncell = (size_t)d.N * d.M * d.K;
-    g->ps = (g->ncell + 255) / 256 * 256;
+    g->ps = (g->ncell + 255) / 256 * 256 + kPlanePad;
     const bool tan = d.precision == LUMAGPU_FP64T || d.precision == LUMAGPU_FP32T;
     g->elem = d.precision == LUMAGPU_FP64T                                  ? 16
               : (d.precision == LUMAGPU_FP64 || d.precision == LUMAGPU_FP32T) ? 8

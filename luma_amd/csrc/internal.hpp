@@ -25,6 +25,7 @@ struct lumagpu_grid {
     int fuse_shape = -1, fuse_zc = 0; // its tile shape (FusedShape, -1: off) and z planes per block
     uint8_t *flags = nullptr;
     uint32_t *plain = nullptr;        // plain-site bitmap, rebuilt with every flag upload
+    int host_forced = 0;              // some host flag carries the force bit (at the last upload)
     // refined sites that sites other than TL2F read, and their initial populations
     void *stale = nullptr;            // [Q][nstale] storage elements
     int *stale_idx = nullptr;         // [ncell] index into stale, -1 elsewhere

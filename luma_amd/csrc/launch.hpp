@@ -20,6 +20,7 @@ struct LaunchSpec {
     // labelled grid stepped as plain sites (k_aa_plain) plus a list of the others
     const int *mlist = nullptr;
     int nmlist = -1;   // < 0: one k_aa_masked launch over the grid
+    bool forced = true;   // some site may carry a body force (IB bodies, gravity, a set force)
 };
 
 // Each returns hipSuccess or the launch error.
@@ -195,9 +196,21 @@ enum AuxOp : int {
     kAuxDownloadTan = 5, // out = derivative part of f AoS (tangent form)
 };
 
+// Cells between consecutive population planes beyond the grid (rounded to 256): the planes'
+// bases then do not all fall on one large power-of-two stride, which the 19 streams of a step
+// kernel read and write in step.  Measured at 256^3 (planes of 64 MiB in FP32) against no
+// padding: one-step kernels FP64 +4.7%, FP32-shifted +1.8%, FP16-shifted +4.9%, two-step
+// kernel +0.5-0.9%; 3D channel 384 x 128 x 128 +2-3% in FP32-shifted.  (4096 cells lost 1.7% on
+// the two-step kernel; 256 and 8192 were between.)
+constexpr size_t kPlanePad = 2048;
+
+// One block per row segment: 256 threads, or the widest of 192 and 128 that divides the row
+// when 256 does not (a 384-cell row in two blocks of 192 rather than 256 + a half-idle 256).
 inline dim3 cell_block(int nx) {
-    int bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
-    return dim3(bx, 1, 1);
+    if (nx < 256) return dim3(((nx + 63) / 64) * 64, 1, 1);
+    for (int bx : {256, 192, 128})
+        if (nx % bx == 0) return dim3(bx, 1, 1);
+    return dim3(256, 1, 1);
 }
 inline dim3 cell_grid(int nx, int ny, int nz) {
     const dim3 b = cell_block(nx);
@@ -212,15 +225,25 @@ hipError_t launch_step_q(const LaunchSpec &ls, const StepArgs &a, hipStream_t s)
         // one after the other
         const dim3 lg((unsigned)((ls.nmlist + 255) / 256));
         if (ls.odd) {
-            if (ls.store) k_aa_plain<Q, P, true, true><<<g, b, 0, s>>>(a);
-            else k_aa_plain<Q, P, true, false><<<g, b, 0, s>>>(a);
+            if (ls.forced) {
+                if (ls.store) k_aa_plain<Q, P, true, true, true><<<g, b, 0, s>>>(a);
+                else k_aa_plain<Q, P, true, false, true><<<g, b, 0, s>>>(a);
+            } else {
+                if (ls.store) k_aa_plain<Q, P, true, true, false><<<g, b, 0, s>>>(a);
+                else k_aa_plain<Q, P, true, false, false><<<g, b, 0, s>>>(a);
+            }
             if (ls.nmlist > 0) {
                 if (ls.store) k_aa_masked_list<Q, P, true, true><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
                 else k_aa_masked_list<Q, P, true, false><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
             }
         } else {
-            if (ls.store) k_aa_plain<Q, P, false, true><<<g, b, 0, s>>>(a);
-            else k_aa_plain<Q, P, false, false><<<g, b, 0, s>>>(a);
+            if (ls.forced) {
+                if (ls.store) k_aa_plain<Q, P, false, true, true><<<g, b, 0, s>>>(a);
+                else k_aa_plain<Q, P, false, false, true><<<g, b, 0, s>>>(a);
+            } else {
+                if (ls.store) k_aa_plain<Q, P, false, true, false><<<g, b, 0, s>>>(a);
+                else k_aa_plain<Q, P, false, false, false><<<g, b, 0, s>>>(a);
+            }
             if (ls.nmlist > 0) {
                 if (ls.store) k_aa_masked_list<Q, P, false, true><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
                 else k_aa_masked_list<Q, P, false, false><<<lg, 256, 0, s>>>(a, ls.mlist, ls.nmlist);
