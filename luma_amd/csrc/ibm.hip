@@ -160,15 +160,24 @@ int find_support(const lumagpu_grid *g, Body &b, int ib) {
             m.ns++;
         };
         add(in, jn, kn, nearpos);
+        // the cage test per axis, once per offset (each axis' test depends on that offset only)
+        double ex[11], ey[11], ez[11];
+        bool cx[11], cy[11], cz[11];
+        for (int o = 0; o < 11; ++o) {
+            ex[o] = nearpos[0] + (o - 5) * dh;
+            ey[o] = nearpos[1] + (o - 5) * dh;
+            ez[o] = (D == 3) ? nearpos[2] + (o - 5) * dh : 0.0;
+            cx[o] = std::fabs(m.pos[0] - ex[o]) / dh < 1.5 * m.dilation;
+            cy[o] = std::fabs(m.pos[1] - ey[o]) / dh < 1.5 * m.dilation;
+            cz[o] = D != 3 || std::fabs(m.pos[2] - ez[o]) / dh < 1.5 * m.dilation;
+        }
         for (int i = in - 5; i <= in + 5; i++)
             for (int j = jn - 5; j <= jn + 5; j++)
                 for (int k = (D == 3 ? kn - 5 : 0); k <= (D == 3 ? kn + 5 : 0); k++) {
-                    double est[3] = {nearpos[0] + (i - in) * dh, nearpos[1] + (j - jn) * dh, 0.0};
-                    if (D == 3) est[2] = nearpos[2] + (k - kn) * dh;
-                    bool cage = (std::fabs(m.pos[0] - est[0]) / dh < 1.5 * m.dilation) &&
-                                (std::fabs(m.pos[1] - est[1]) / dh < 1.5 * m.dilation);
-                    if (D == 3) cage = cage && (std::fabs(m.pos[2] - est[2]) / dh < 1.5 * m.dilation);
-                    if (!cage || !within_domain(g, est)) continue;
+                    const int oi = i - in + 5, oj = j - jn + 5, ok = (D == 3) ? k - kn + 5 : 5;
+                    if (!(cx[oi] && cy[oj] && cz[ok])) continue;
+                    double est[3] = {ex[oi], ey[oj], (D == 3) ? ez[ok] : 0.0};
+                    if (!within_domain(g, est)) continue;
                     if (i == in && j == jn && (D != 3 || k == kn)) continue;
                     if (m.ns >= kSmax) return fail(LUMAGPU_EINVAL, "IB support stencil overflow");
                     if (i < 0 || j < 0 || k < 0 || i >= N || j >= M || k >= K)
@@ -182,20 +191,75 @@ int find_support(const lumagpu_grid *g, Body &b, int ib) {
 
 // ibm_computeDs, :927-981; on a tangent-form grid also d ds along the marker tangents
 // (through the nearest marker: ds = |x_m - x_n| / dh)
+// Markers in the order of their coordinate on the axis of the largest extent, for sweeps that
+// visit only the markers within a distance of one marker on that axis.
+struct AxisOrder {
+    int axis = 0;
+    std::vector<int> idx, rank;
+};
+AxisOrder axis_order(const std::vector<double> &p, size_t stride, size_t n, int D) {
+    AxisOrder o;
+    double ext = -1.0;
+    for (int a = 0; a < D; ++a) {
+        double lo = HUGE_VAL, hi = -HUGE_VAL;
+        for (size_t i = 0; i < n; ++i) {
+            lo = std::min(lo, p[stride * i + a]);
+            hi = std::max(hi, p[stride * i + a]);
+        }
+        if (n && hi - lo > ext) {
+            ext = hi - lo;
+            o.axis = a;
+        }
+    }
+    o.idx.resize(n);
+    o.rank.resize(n);
+    for (size_t i = 0; i < n; ++i) o.idx[i] = (int)i;
+    std::stable_sort(o.idx.begin(), o.idx.end(),
+                     [&](int x, int y) { return p[stride * x + o.axis] < p[stride * y + o.axis]; });
+    for (size_t k = 0; k < n; ++k) o.rank[o.idx[k]] = (int)k;
+    return o;
+}
+
 void compute_ds(const lumagpu_grid *g, Body &b) {
     const double dh = g->d.dh;
-    for (size_t mi = 0; mi < b.m.size(); ++mi) {
-        double ds = 10.0;
+    const size_t nm = b.m.size();
+    // The result of the all-pairs scan (the smallest distance, at the lowest marker index among
+    // equal distances) from a sweep outwards along the sorted axis: the squared offset on that
+    // axis is a lower bound of the rounded squared distance and grows along the sweep, so the
+    // sweep stops once it exceeds the best squared distance (with a margin far above the
+    // rounding that could make two different sums give one distance), and the root and
+    // division are taken only for sums within that bound.
+    std::vector<double> p(3 * nm);
+    for (size_t i = 0; i < nm; ++i)
+        for (int a = 0; a < 3; ++a) p[3 * i + a] = b.m[i].pos[a];
+    const AxisOrder ord = axis_order(p, 3, nm, g->D);
+    const int ax = ord.axis;
+    for (size_t mi = 0; mi < nm; ++mi) {
+        double ds = 10.0, best = HUGE_VAL;
         size_t arg = mi;
-        for (size_t n = 0; n < b.m.size(); ++n) {
-            if (n == mi) continue;
+        const double *pm = &p[3 * mi];
+        auto visit = [&](size_t n) {
+            const double da = pm[ax] - p[3 * n + ax];
+            const double bound = best * (1.0 + 1e-12);
+            if (std::pow(da, 2) > bound) return false;
             double acc = 0.0;
-            for (int a = 0; a < 3; ++a) acc += std::pow(b.m[mi].pos[a] - b.m[n].pos[a], 2);
+            for (int a = 0; a < 3; ++a) acc += std::pow(pm[a] - p[3 * n + a], 2);
+            if (acc > bound) return true;
             const double dist = std::sqrt(acc) / dh;
             if (dist < ds) {
                 ds = dist;
                 arg = n;
+                best = acc;
+            } else if (dist == ds && arg != mi && n < arg) {
+                arg = n;
+                best = std::min(best, acc);
             }
+            return true;
+        };
+        const int r = ord.rank[mi];
+        for (int k = r + 1; k < (int)nm && visit((size_t)ord.idx[k]); ++k) {
+        }
+        for (int k = r - 1; k >= 0 && visit((size_t)ord.idx[k]); --k) {
         }
         Marker &m = b.m[mi];
         m.ds = ds;
@@ -353,14 +417,38 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
     const bool tan = tangent_form(g);
     std::vector<double> A((size_t)n * n, 0.0), rhs(n, 1.0), dA;
     if (tan) dA.assign((size_t)n * n, 0.0);
+    // positions and dilations side by side for the all-pairs test (a marker record spans ~1.6 KB)
+    std::vector<double> pd(4 * (size_t)n);
+    for (int I = 0; I < n; ++I) {
+        for (int a = 0; a < 3; ++a) pd[4 * (size_t)I + a] = b.m[I].pos[a];
+        pd[4 * (size_t)I + 3] = b.m[I].dilation;
+    }
+    // the markers J near I come from a sweep along the sorted axis that stops past the largest
+    // reach any J can have on it; every entry is computed on its own, so the visiting order
+    // leaves A as the all-pairs loop makes it
+    const AxisOrder ord = axis_order(pd, 4, (size_t)n, D);
+    double dmax = 0.0;
+    for (int J = 0; J < n; ++J) dmax = std::max(dmax, pd[4 * (size_t)J + 3]);
+    std::vector<int> near;
     for (int I = 0; I < n; ++I) {
         const Marker &mI = b.m[I];
-        for (int J = 0; J < n; ++J) {
-            const Marker &mJ = b.m[J];
+        const double *qI = &pd[4 * (size_t)I];
+        const double reach = (1.5 * qI[3] + 1.5 * dmax + 0.01) * dh;
+        near.clear();
+        for (int k = ord.rank[I]; k < n && std::fabs(pd[4 * (size_t)ord.idx[k] + ord.axis] - qI[ord.axis]) <= reach; ++k)
+            near.push_back(ord.idx[k]);
+        for (int k = ord.rank[I] - 1; k >= 0 && std::fabs(pd[4 * (size_t)ord.idx[k] + ord.axis] - qI[ord.axis]) <= reach; --k)
+            near.push_back(ord.idx[k]);
+        for (int J : near) {
             // every support of I lies within 1.5 dilations of I per axis, so a marker J further
             // than 1.5 + 1.5 dilations (+ margin) away on some axis has a zero delta at all of
-            // them; skipping those exact zero terms leaves the sum unchanged
-            if (far_apart(mI, mJ, D, dh)) continue;
+            // them; skipping those exact zero terms leaves the sum unchanged (far_apart's test)
+            const double *qJ = &pd[4 * (size_t)J];
+            const double lim = (1.5 * qI[3] + 1.5 * qJ[3] + 0.01) * dh;
+            bool far = false;
+            for (int a = 0; a < D; ++a) far = far || std::fabs(qJ[a] - qI[a]) > lim;
+            if (far) continue;
+            const Marker &mJ = b.m[J];
             double acc = 0.0;
             for (int k = 0; k < mI.ns; ++k) {
                 double dJ;
