@@ -350,7 +350,8 @@ int lu_solve(int n, std::vector<Dual> &a, std::vector<Dual> &x) { return lu_solv
 // so no result sees the difference.)  The L multipliers of earlier columns move down with later
 // row swaps, so after any swap the second substitution runs in full.
 template <class Sc>
-int lu_solve_band_t(int n, int kl, int ku, std::vector<Sc> &a, std::vector<Sc> &x) {
+int lu_solve_band_t(int n, int kl, int ku, std::vector<Sc> &a, std::vector<Sc> &x,
+                    const char *singular = "singular beam system") {
     auto M = [&](int i, int j) -> Sc & { return a[(size_t)j * n + i]; };
     std::vector<int> piv(n), rows;
     rows.reserve(kl);
@@ -365,7 +366,7 @@ int lu_solve_band_t(int n, int kl, int ku, std::vector<Sc> &a, std::vector<Sc> &
                 p = i;
             }
         piv[j] = p;
-        if (value_of(M(p, j)) == 0.0) return fail(LUMAGPU_EPHYS, "singular beam system");
+        if (value_of(M(p, j)) == 0.0) return fail(LUMAGPU_EPHYS, singular);
         if (p != j) {
             swapped = true;
             for (int k = 0; k < n; ++k) std::swap(M(j, k), M(p, k));
@@ -430,6 +431,7 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
     double dmax = 0.0;
     for (int J = 0; J < n; ++J) dmax = std::max(dmax, pd[4 * (size_t)J + 3]);
     std::vector<int> near;
+    int kl = 0, ku = 0;
     for (int I = 0; I < n; ++I) {
         const Marker &mI = b.m[I];
         const double *qI = &pd[4 * (size_t)I];
@@ -462,6 +464,11 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
                 acc += mI.delta[k] * dJ * mI.area;
             }
             A[(size_t)I * n + J] = acc * mJ.ds;
+            if (A[(size_t)I * n + J] != 0.0) {
+                // the band of the transpose, which lu_solve_t factors: row J, column I
+                kl = std::max(kl, J - I);
+                ku = std::max(ku, I - J);
+            }
             if (tan) {
                 double dacc = 0.0;
                 for (int k = 0; k < mI.ns; ++k) {
@@ -477,7 +484,10 @@ int find_epsilon(const lumagpu_grid *g, Body &b) {
     }
     std::vector<double> A0;
     if (tan) A0 = A;
-    if (int r = lu_solve(n, A, rhs)) return r;
+    // a chain of markers (a filament) gives a matrix banded in marker order: the banded solver,
+    // which skips only the exact zeros outside the band, otherwise the dense one
+    const bool band = 4 * (kl + ku) < n;
+    if (int r = band ? lu_solve_band_t(n, kl, ku, A, rhs, "singular IB epsilon system") : lu_solve(n, A, rhs)) return r;
     for (int i = 0; i < n; ++i) b.m[i].eps = rhs[i];
     if (tan) {
         std::vector<double> drhs(n, 0.0);
