@@ -75,13 +75,24 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     for (size_t c = 0; c < g->hflags.size(); ++c) {
         const int ty = g->hflags[c] & kTypeMask;
         if (ty == kFluid || ty == kSolid || ty == kExtrapRight) continue;
-        if (ty == kVelocity && !g->d.regularised) continue;
-        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint covers fluid, solid, forced-equilibrium velocity and "
-                                          "ExtrapolateRight sites only");
+        // forced-equilibrium velocity sources, or regularised velocity faces (edge and corner
+        // sites are refused above)
+        if (ty == kVelocity) continue;
+        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint covers fluid, solid, velocity and ExtrapolateRight sites only");
     }
-    if (g->ibm)
+    if (g->ibm) {
         for (const auto &b : g->ibm->bodies)
             if (b.flexible) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint covers rigid IB bodies only");
+        // the interpolation's adjoint spreads to fluid support sites
+        for (const auto &b : g->ibm->bodies)
+            for (const auto &m : b.m)
+                for (int k = 0; k < m.ns; ++k) {
+                    const size_t c = (size_t)m.sijk[k][0] +
+                                     (size_t)g->d.N * ((size_t)m.sijk[k][1] + (size_t)g->d.M * m.sijk[k][2]);
+                    if (c < g->hflags.size() && (g->hflags[c] & kTypeMask) == kVelocity)
+                        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover IB supports on velocity sites");
+                }
+    }
     return 0;
 }
 
@@ -202,7 +213,9 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     if (int r = lam.alloc((size_t)g->Q * nc * sizeof(T))) return r;
     if (int r = gbar.alloc((size_t)g->Q * nc * sizeof(T))) return r;
     if (int r = fbar.alloc(3 * nc * sizeof(double))) return r;
-    if (int r = part.alloc(2 * (size_t)nblk * sizeof(double))) return r;
+    // omega (collision), inlet scale (streaming from forced-equilibrium sources), inlet scale
+    // (regularised faces)
+    if (int r = part.alloc(3 * (size_t)nblk * sizeof(double))) return r;
     HIP_TRY(hipMemsetAsync(lam.p, 0, (size_t)g->Q * nc * sizeof(T), g->stream));
     HIP_TRY(hipMemsetAsync(fbar.p, 0, 3 * nc * sizeof(double), g->stream));
     AdjArgs<T> ab{static_cast<T *>(lam.p), static_cast<T *>(gbar.p), static_cast<double *>(fbar.p),
@@ -280,7 +293,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 }
                 if (e == hipSuccess) e = launch_adj_stream<P>(g, a, ab);
                 if (e == hipSuccess) {
-                    k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ab.part, nblk, 2, acc + 1);
+                    k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ab.part, nblk, 3, acc + 1);
                     e = hipGetLastError();
                 }
                 if (e != hipSuccess) rc = fail(LUMAGPU_EHIP, std::string("adjoint launch: ") + hipGetErrorString(e));
@@ -310,7 +323,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         HIP_TRY(hipMemcpyAsync(hd.data(), dsbar.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     }
     HIP_TRY(hipStreamSynchronize(g->stream));
-    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1], hacc[2]};
+    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1], hacc[2] + hacc[3]};
     if (lambda0) {
         // LUMA layout f[v + Q*id], id = k + j*K + i*K*M
         const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q;

@@ -49,19 +49,68 @@ __device__ __forceinline__ double block_sum(double v) {
     return sh[0];
 }
 
+// A regularised velocity face site (not an edge or corner) of a grid stepped with
+// regularised boundaries.
+__device__ __forceinline__ bool regularised_face_site(const StepArgs &a, unsigned fl) {
+    return a.regularised && (fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
+}
+
+// Transposed Jacobian of a regularised velocity face site's update (regularised_face, :313-510,
+// then BGK, :765-790) at its streamed populations g (absolute): one forward-mode (Dual) pass per
+// population, one for omega and one for the inlet profile scale (u_in -> s u_in at s = 1).
+// gbar_v = sum_w lam_w d f'_w / d g_v; omb and inb the two parameter derivatives.  The site
+// carries no body force (the adjoint refuses IB supports on velocity sites).
+template <int Q>
+__device__ void adj_regularised_face(const StepArgs &a, unsigned fl, int y, const double *g, const double *lam,
+                                     double *gbar, double &omb, double &inb) {
+    using L = Lattice<Q>;
+    const int ncode = (fl >> kNormalShift) & 7;
+    const double r = a.ramp_reg;
+    const double vin0[3] = {a.uin[y] * r, a.uin[a.ny + y] * r, (L::D == 3) ? a.uin[2 * a.ny + y] * r : 0.0};
+    for (int s = 0; s < Q + 2; ++s) {
+        Dual ga[Q];
+        for (int v = 0; v < Q; ++v) ga[v] = Dual(g[v], v == s ? 1.0 : 0.0);
+        Dual vin[3];
+        for (int d = 0; d < 3; ++d) vin[d] = Dual(vin0[d], s == Q + 1 ? vin0[d] : 0.0);
+        Dual rr, uu[3];
+        regularised_face<Q, Dual>(ga, ncode, vin, Dual(a.rho_in), Dual(a.cs2), Dual(a.den2), rr, uu);
+        const CollideExact<Q, Dual> col{rr, {uu[0], uu[1], uu[2]}, Dual(a.omega, s == Q ? 1.0 : 0.0),
+                                        Dual(a.cs2), Dual(a.den2), nullptr};
+        double acc = 0.0;
+        for (int w = 0; w < Q; ++w) acc += lam[w] * col(w, ga[w]).d;
+        if (s < Q) gbar[s] = acc;
+        else if (s == Q) omb = acc;
+        else inb = acc;
+    }
+}
+
 // Transposed collision Jacobian at every fluid site: gbar = (d f'/d g)^T lam, fbar =
-// (d f'/d F)^T lam at forced sites, and the omega derivative sum_v lam_v d f'_v / d omega.
+// (d f'/d F)^T lam at forced sites, and the omega derivative sum_v lam_v d f'_v / d omega; at
+// regularised velocity face sites the same through adj_regularised_face, which also gives the
+// inlet-scale derivative (part[2 nblk + block]).
 template <int Q, class P, bool ODD>
 __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typename P::T> b) {
     using L = Lattice<Q>;
     using T = typename P::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
-    double om_site = 0.0;
+    double om_site = 0.0, in_site = 0.0;
     if (x < a.nx) {
         const size_t cell = cell_of(x, y, z, a.nx, a.ny);
         const unsigned fl = a.flags[cell];
-        if ((fl & kTypeMask) == kFluid) {
+        if (regularised_face_site(a, fl)) {
+            Nbr nb;
+            nb.init(x, y, z, a.nx, a.ny, a.nz);
+            T fin[Q];
+            gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kVelocity, fin);
+            double g[Q], lam[Q], gb[Q];
+            for (int v = 0; v < Q; ++v) {
+                g[v] = double(fin[v]) + (P::kShift ? L::w(v) * a.rho0 : 0.0);
+                lam[v] = double(b.lam[(size_t)v * a.ncell + cell]);
+            }
+            adj_regularised_face<Q>(a, fl, y, g, lam, gb, om_site, in_site);
+            for (int v = 0; v < Q; ++v) b.gbar[(size_t)v * a.ncell + cell] = T(gb[v]);
+        } else if ((fl & kTypeMask) == kFluid) {
             Nbr nb;
             nb.init(x, y, z, a.nx, a.ny, a.nz);
             T fin[Q];
@@ -143,11 +192,15 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typenam
     }
     const double s = block_sum(om_site);
     if (threadIdx.x == 0) b.part[block_linear()] = s;
+    __syncthreads();
+    const double si = block_sum(in_site);
+    if (threadIdx.x == 0) b.part[2 * b.nblk + block_linear()] = si;
 }
 
 // Reverse streaming: lam_w(x) gathers the adjoint of every streamed population that read
-// f_w(x) -- the pull by x + c_w, the bounce-back at x itself when x + c_w is solid, and the
-// pull through an ExtrapolateRight source two cells to the right.  Also the derivative with
+// f_w(x) -- the pull by x + c_w (a fluid or regularised velocity face site), the bounce-back at
+// x itself when x + c_w is solid, and the pull through an ExtrapolateRight source two cells to
+// the right.  Also the derivative with
 // respect to a scale s of the inlet profile (u_in -> s u_in) of the forced-equilibrium
 // velocity sources, d feq_v(rho_in, s u_in) / ds at s = 1.
 template <int Q, class P>
@@ -160,8 +213,9 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
     if (x < a.nx) {
         const int nx = a.nx, ny = a.ny;
         const size_t cell = cell_of(x, y, z, nx, ny);
-        const int ty = a.flags[cell] & kTypeMask;
-        if (ty == kFluid) {
+        const unsigned fl = a.flags[cell];
+        const int ty = fl & kTypeMask;
+        if (ty == kFluid || regularised_face_site(a, fl)) {
             Nbr nb;
             nb.init(x, y, z, nx, ny, a.nz);
             const bool xr = (x + 2 < nx) && ((a.flags[cell + 2] & kTypeMask) == kExtrapRight);
@@ -171,9 +225,10 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
             for (int w = 0; w < Q; ++w) {
                 const int cx = L::c(w, 0), cy = L::c(w, 1), cz = L::c(w, 2);
                 const size_t yn = nb.at(cx, cy, cz, nx, ny);
-                const int tn = a.flags[yn] & kTypeMask;
+                const unsigned fn = a.flags[yn];
+                const int tn = fn & kTypeMask;
                 T acc = T(0);
-                if (tn == kFluid) acc = b.gbar[(size_t)w * a.ncell + yn];
+                if (tn == kFluid || regularised_face_site(a, fn)) acc = b.gbar[(size_t)w * a.ncell + yn];
                 else if (tn == kSolid) acc = b.gbar[(size_t)L::opp(w) * a.ncell + cell];
                 if (xr) {
                     const size_t yp = ns.at(cx, cy, cz, nx, ny);

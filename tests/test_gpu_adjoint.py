@@ -1,9 +1,10 @@
 """Discrete adjoint of the step (lumagpu_adj_run) against central finite differences of the
 forward step, whose FP64 form equals the oracle bit for bit (the other GPU tests): gradients
 of the time-averaged IB drag with respect to the relaxation frequency, the inlet velocity
-scale, the marker positions and the initial populations, in 2D (channel with forced-
-equilibrium inlet, ExtrapolateRight outlet, solid walls, IB cylinder) and 3D (periodic box,
-IB sphere); checkpointing with a partial last segment; FP32-shifted adjoint against FP64.
+scale, the marker positions and the initial populations, in 2D (channel with a forced-
+equilibrium or a regularised velocity inlet, ExtrapolateRight outlet, solid walls, IB cylinder)
+and 3D (periodic box, IB sphere); checkpointing with a partial last segment; FP32-shifted
+adjoint against FP64.
 Bound: 1e-4 relative (central differences, step sizes in each test)."""
 import numpy as np
 import pytest
@@ -18,6 +19,14 @@ RTOL = 1e-4
 
 def channel_case():
     c = setups.channel_2d(48, 24, 8.0, 0.04, 20.0, regularised=False)
+    pos = setups.circle_markers((1.52, 1.49, 0.0), 0.3, c.info["dh"])
+    return c, pos
+
+
+def channel_reg_case():
+    # the same channel with LUMA's regularised velocity inlet (Latt 2008 faces; the inlet's
+    # ends meet the solid walls, so there are no edge sites)
+    c = setups.channel_2d(48, 24, 8.0, 0.04, 20.0, regularised=True)
     pos = setups.circle_markers((1.52, 1.49, 0.0), 0.3, c.info["dh"])
     return c, pos
 
@@ -70,7 +79,8 @@ def close(a, b, rtol=RTOL, scale=None):
     return abs(a - b) <= rtol * s
 
 
-@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (sphere_case, 24, 10), (inflow_case, 30, 12)])
+@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (channel_reg_case, 60, 16), (sphere_case, 24, 10),
+                                       (inflow_case, 30, 12)])
 def test_adjoint_matches_central_differences(build, T, K):
     case, pos = build()
     print(f"[adjoint] {case.name} T={T} K={K} markers={len(pos)}", flush=True)
@@ -93,10 +103,11 @@ def test_adjoint_matches_central_differences(build, T, K):
         print("dJ/dinlet adjoint", adj["dJ_dinlet"], "fd", fd, flush=True)
         assert close(adj["dJ_dinlet"], fd)
 
-    # initial populations along a random direction on the fluid sites
+    # initial populations along a random direction on the fluid and velocity sites (a
+    # forced-equilibrium source's own populations are never read: both sides give 0 there)
     rng = np.random.default_rng(11)
     fluid = np.ones(case.N * case.M * case.K, bool) if case.lattyp is None else \
-        (case.lattyp == setups.FLUID).reshape(-1)
+        np.isin(case.lattyp, (setups.FLUID, setups.VELOCITY)).reshape(-1)
     dirn = rng.standard_normal(f0.shape) * fluid.reshape(f0.shape[:3])[..., None]
     h = 1e-6
     fd = (J_of(case, pos, T, f0=f0 + h * dirn) - J_of(case, pos, T, f0=f0 - h * dirn)) / (2 * h)
