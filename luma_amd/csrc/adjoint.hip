@@ -12,6 +12,7 @@
 // ds (nearest-marker spacing, :927-981) on the positions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -66,7 +67,7 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     if (g->d.precision != LUMAGPU_FP64 && g->d.precision != LUMAGPU_FP32S)
         return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs in the FP64 and FP32-shifted forms");
     if (g->parent || !g->kids.empty()) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs on a single grid level");
-    if (g->zslab || g->nedge) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs or edge sites");
+    if (g->zslab) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs");
     if (g->gravity_on) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover the gravity reset");
     if (g->ramp_time > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a velocity ramp");
     if (g->re_ramp > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a Reynolds ramp");
@@ -176,8 +177,10 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         if (int r = b.alloc(fbytes)) return r;
     DevBuf fin, accJ;
     if (int r = fin.alloc(fbytes)) return r;
-    if (int r = accJ.alloc(4 * sizeof(double))) return r;
-    HIP_TRY(hipMemsetAsync(accJ.p, 0, 4 * sizeof(double), g->stream));
+    // J; omega (fluid and face sites), inlet scale (sources), inlet scale (faces); omega and
+    // inlet scale (edge and corner sites)
+    if (int r = accJ.alloc(6 * sizeof(double))) return r;
+    HIP_TRY(hipMemsetAsync(accJ.p, 0, 6 * sizeof(double), g->stream));
 
     // forward with checkpoints
     for (int t = 0; t < nsteps; ++t) {
@@ -194,7 +197,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     }
     HIP_TRY(hipMemcpyAsync(fin.p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
     const int fin_par = g->parity, fin_t = g->t;
-    double hacc[4] = {0, 0, 0, 0};
+    double hacc[6] = {0, 0, 0, 0, 0, 0};
     if (!backward) {
         HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
@@ -252,6 +255,48 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         aj.invT = invT;
     }
     double *acc = static_cast<double *>(accJ.p);
+    // edge and corner velocity sites: per-site density adjoints and parameter derivatives, and
+    // the neighbour cells they extrapolate from with their entries (2 * site + 0 for x + n,
+    // + 1 for x + 2n), in ascending order
+    EdgeArgs ea{};
+    ea.n = g->nedge;
+    ea.cell = g->edge;
+    ea.nrm = g->edge + g->nedge;
+    ea.n1 = g->edge + 2 * g->nedge;
+    ea.n2 = g->edge + 3 * g->nedge;
+    ea.erho = g->erho;
+    DevBuf dbar, epart, enb_cell, enb_ptr, enb_ent;
+    int n_enb = 0;
+    if (g->nedge > 0) {
+        const int ne = g->nedge;
+        std::vector<int> he(4 * (size_t)ne);
+        HIP_TRY(hipMemcpyAsync(he.data(), g->edge, he.size() * sizeof(int), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        std::vector<std::pair<int, int>> pairs;   // (cell, code)
+        for (int i = 0; i < ne; ++i) {
+            pairs.emplace_back(he[2 * (size_t)ne + i], 2 * i);
+            pairs.emplace_back(he[3 * (size_t)ne + i], 2 * i + 1);
+        }
+        std::sort(pairs.begin(), pairs.end());
+        std::vector<int> cells, ptr{0}, ent;
+        for (size_t k = 0; k < pairs.size(); ++k) {
+            if (k == 0 || pairs[k].first != pairs[k - 1].first) {
+                if (k) ptr.push_back((int)ent.size());
+                cells.push_back(pairs[k].first);
+            }
+            ent.push_back(pairs[k].second);
+        }
+        ptr.push_back((int)ent.size());
+        n_enb = (int)cells.size();
+        if (int r = dbar.alloc(ne * sizeof(double))) return r;
+        if (int r = epart.alloc(2 * (size_t)ne * sizeof(double))) return r;
+        if (int r = enb_cell.alloc(cells.size() * sizeof(int))) return r;
+        if (int r = enb_ptr.alloc(ptr.size() * sizeof(int))) return r;
+        if (int r = enb_ent.alloc(ent.size() * sizeof(int))) return r;
+        HIP_TRY(hipMemcpy(enb_cell.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(enb_ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(enb_ent.p, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
 
     void *const f_own = g->f;
     int rc = 0;
@@ -276,7 +321,31 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
             if (!rc && ib) rc = apply_core(g);
             const StepArgs a = make_args(g);
             if (!rc) {
-                hipError_t e = launch_adj_collide<P>(g, a, ab);
+                // the edge sites' extrapolation densities of this step
+                hipError_t e = g->nedge ? launch_edges(g, a, 0) : hipSuccess;
+                if (e == hipSuccess) e = launch_adj_collide<P>(g, a, ab);
+                if (e == hipSuccess && g->nedge) {
+                    const dim3 eb(64), eg((unsigned)((g->nedge + 63) / 64));
+                    double *db = static_cast<double *>(dbar.p), *ep = static_cast<double *>(epart.p);
+                    if (g->Q == 9) {
+                        if (g->parity == 0) k_adj_edges<9, P, true><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
+                        else k_adj_edges<9, P, false><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
+                    } else {
+                        if (g->parity == 0) k_adj_edges<19, P, true><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
+                        else k_adj_edges<19, P, false><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
+                    }
+                    e = hipGetLastError();
+                    if (e == hipSuccess) {
+                        k_adj_edge_nbrs<T><<<(n_enb + 127) / 128, 128, 0, g->stream>>>(
+                            n_enb, static_cast<const int *>(enb_cell.p), static_cast<const int *>(enb_ptr.p),
+                            static_cast<const int *>(enb_ent.p), db, g->Q, nc, ab.gbar);
+                        e = hipGetLastError();
+                    }
+                    if (e == hipSuccess) {
+                        k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ep, g->nedge, 2, acc + 4);
+                        e = hipGetLastError();
+                    }
+                }
                 if (e == hipSuccess && ib) {
                     IbmArgs ia = ibm_args(g);
                     k_adj_ibm_marker<<<(s->nm + 127) / 128, 128, 0, g->stream>>>(ia, aj);
@@ -323,7 +392,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         HIP_TRY(hipMemcpyAsync(hd.data(), dsbar.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     }
     HIP_TRY(hipStreamSynchronize(g->stream));
-    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1], hacc[2] + hacc[3]};
+    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1] + hacc[4], hacc[2] + hacc[3] + hacc[5]};
     if (lambda0) {
         // LUMA layout f[v + Q*id], id = k + j*K + i*K*M
         const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q;

@@ -50,9 +50,12 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // A regularised velocity face site (not an edge or corner) of a grid stepped with
-// regularised boundaries.
+// regularised boundaries; with the edge and corner sites, every regularised velocity site.
 __device__ __forceinline__ bool regularised_face_site(const StepArgs &a, unsigned fl) {
     return a.regularised && (fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
+}
+__device__ __forceinline__ bool regularised_velocity_site(const StepArgs &a, unsigned fl) {
+    return a.regularised && (fl & kTypeMask) == kVelocity;
 }
 
 // Transposed Jacobian of a regularised velocity face site's update (regularised_face, :313-510,
@@ -197,8 +200,77 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typenam
     if (threadIdx.x == 0) b.part[2 * b.nblk + block_linear()] = si;
 }
 
+// Edge and corner velocity sites (k_edge_sites: regularised_edge with the density
+// 2 rho(x+n) - rho(x+2n) extrapolated from the neighbours' sweep-1 values, then BGK), one
+// thread per site: the site's own gbar from forward-mode passes over its populations, and
+// dbar[i] = sum_w lam_w d f'_w / d rho_ext, which reaches the neighbours' streamed
+// populations through d rho / d g_v = 1 (k_adj_edge_nbrs); omega and inlet-scale derivatives
+// per site in epart[i] and epart[n + i].
+template <int Q, class P, bool ODD>
+__global__ void k_adj_edges(StepArgs a, EdgeArgs e, AdjArgs<typename P::T> b, double *dbar, double *epart) {
+    using L = Lattice<Q>;
+    using T = typename P::T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e.n) return;
+    const size_t cell = (size_t)e.cell[i];
+    const int nx = a.nx, ny = a.ny;
+    const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / ((size_t)nx * ny));
+    const int pk = e.nrm[i];
+    const int n[3] = {(pk & 3) - 1, ((pk >> 2) & 3) - 1, ((pk >> 4) & 3) - 1};
+    Nbr nb;
+    nb.init(x, y, z, nx, ny, a.nz);
+    T fin[Q];
+    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kVelocity, fin);
+    double g[Q], lam[Q];
+    for (int v = 0; v < Q; ++v) {
+        g[v] = double(fin[v]) + (P::kShift ? L::w(v) * a.rho0 : 0.0);
+        lam[v] = double(b.lam[(size_t)v * a.ncell + cell]);
+    }
+    const double *er1 = e.erho + 8 * (size_t)i, *er2 = er1 + 4;
+    const double dens0 = 2.0 * er1[0] - er2[0];
+    const double r = a.ramp_reg;
+    const double vin0[3] = {a.uin[y] * r, a.uin[ny + y] * r, (L::D == 3) ? a.uin[2 * ny + y] * r : 0.0};
+    // seeds: populations 0..Q-1, then the extrapolated density, omega, the inlet scale
+    double omb = 0.0, inb = 0.0;
+    for (int s = 0; s < Q + 3; ++s) {
+        Dual fa[Q];
+        for (int v = 0; v < Q; ++v) fa[v] = Dual(g[v], v == s ? 1.0 : 0.0);
+        Dual vin[3];
+        for (int d = 0; d < 3; ++d) vin[d] = Dual(vin0[d], s == Q + 2 ? vin0[d] : 0.0);
+        Dual rr, uu[3], fout[Q];
+        regularised_edge<Q, Dual>(fa, n, vin, Dual(dens0, s == Q ? 1.0 : 0.0), Dual(a.cs2), Dual(a.den2), rr, uu);
+        collide_exact<Q, Dual>(fa, rr, uu, static_cast<const Dual *>(nullptr), Dual(a.omega, s == Q + 1 ? 1.0 : 0.0),
+                               Dual(a.cs2), Dual(a.den2), fout);
+        double acc = 0.0;
+        for (int w = 0; w < Q; ++w) acc += lam[w] * fout[w].d;
+        if (s < Q) b.gbar[(size_t)s * a.ncell + cell] = T(acc);
+        else if (s == Q) dbar[i] = acc;
+        else if (s == Q + 1) omb = acc;
+        else inb = acc;
+    }
+    epart[i] = omb;
+    epart[e.n + i] = inb;
+}
+
+// The extrapolation's reach into the neighbours: gbar_v(x) += sum over the edge sites that
+// extrapolate from x of coef * dbar (coef 2 for the neighbour x + n, -1 for x + 2n), the same
+// for every v; one thread per neighbour cell, its entries in a fixed order (no atomics).
+template <class T>
+__global__ void k_adj_edge_nbrs(int nn, const int *ncell, const int *ptr, const int *ent, const double *dbar, int Q,
+                                size_t ncell_total, T *gbar) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    double s = 0.0;
+    for (int j = ptr[k]; j < ptr[k + 1]; ++j) {
+        const int code = ent[j];   // 2 * edge + (0: x + n, 1: x + 2n)
+        s += ((code & 1) ? -1.0 : 2.0) * dbar[code >> 1];
+    }
+    const size_t cell = (size_t)ncell[k];
+    for (int v = 0; v < Q; ++v) gbar[(size_t)v * ncell_total + cell] += T(s);
+}
+
 // Reverse streaming: lam_w(x) gathers the adjoint of every streamed population that read
-// f_w(x) -- the pull by x + c_w (a fluid or regularised velocity face site), the bounce-back at
+// f_w(x) -- the pull by x + c_w (a fluid or regularised velocity site), the bounce-back at
 // x itself when x + c_w is solid, and the pull through an ExtrapolateRight source two cells to
 // the right.  Also the derivative with
 // respect to a scale s of the inlet profile (u_in -> s u_in) of the forced-equilibrium
@@ -215,7 +287,7 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
         const size_t cell = cell_of(x, y, z, nx, ny);
         const unsigned fl = a.flags[cell];
         const int ty = fl & kTypeMask;
-        if (ty == kFluid || regularised_face_site(a, fl)) {
+        if (ty == kFluid || regularised_velocity_site(a, fl)) {
             Nbr nb;
             nb.init(x, y, z, nx, ny, a.nz);
             const bool xr = (x + 2 < nx) && ((a.flags[cell + 2] & kTypeMask) == kExtrapRight);
@@ -228,7 +300,7 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
                 const unsigned fn = a.flags[yn];
                 const int tn = fn & kTypeMask;
                 T acc = T(0);
-                if (tn == kFluid || regularised_face_site(a, fn)) acc = b.gbar[(size_t)w * a.ncell + yn];
+                if (tn == kFluid || regularised_velocity_site(a, fn)) acc = b.gbar[(size_t)w * a.ncell + yn];
                 else if (tn == kSolid) acc = b.gbar[(size_t)L::opp(w) * a.ncell + cell];
                 if (xr) {
                     const size_t yp = ns.at(cx, cy, cz, nx, ny);

@@ -99,6 +99,9 @@ hipError_t launch_zslab(const lumagpu_grid *g, const StepArgs &a, int phase) {
     }
 }
 
+}  // namespace
+
+namespace lumagpu {
 hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
     EdgeArgs e{};
     e.n = g->nedge;
@@ -117,6 +120,9 @@ hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
         default: return launch_edges_f16s(g->Q, a, e, odd, phase, store, g->stream);
     }
 }
+}  // namespace lumagpu
+
+namespace {
 
 // LUMA-layout rho, u <-> the grid's macro arrays (Duals with zero derivative in the tangent
 // form); macro_to_luma takes the values, or with tan the derivatives

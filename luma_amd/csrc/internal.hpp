@@ -199,6 +199,9 @@ int ibm_marker_count(const lumagpu_grid *g);
 int step_grid(lumagpu_grid *g, int subcycle);
 // ExtrapolateRight side buffer from the grid's current populations; defined in api.hip
 hipError_t outlet_prepass(const lumagpu_grid *g);
+// edge/corner and pressure sites: phase 0 the neighbours' sweep-1 rho, u (erho), phase 1 the
+// sites' step; defined in api.hip
+hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase);
 // k_site_macro at a list of cells (every cell when cells is null); defined in ibm.hip
 hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int n, const int *cells, int ib_force);
 

@@ -2,9 +2,10 @@
 forward step, whose FP64 form equals the oracle bit for bit (the other GPU tests): gradients
 of the time-averaged IB drag with respect to the relaxation frequency, the inlet velocity
 scale, the marker positions and the initial populations, in 2D (channel with a forced-
-equilibrium or a regularised velocity inlet, ExtrapolateRight outlet, solid walls, IB cylinder)
-and 3D (periodic box, IB sphere); checkpointing with a partial last segment; FP32-shifted
-adjoint against FP64.
+equilibrium or a regularised velocity inlet, ExtrapolateRight outlet, solid walls, IB cylinder;
+a box of regularised velocity walls meeting at corner sites, IB cylinder) and 3D (periodic box,
+IB sphere; uniform inflow); checkpointing with a partial last segment; FP32-shifted adjoint
+against FP64.
 Bound: 1e-4 relative (central differences, step sizes in each test)."""
 import numpy as np
 import pytest
@@ -28,6 +29,16 @@ def channel_reg_case():
     # ends meet the solid walls, so there are no edge sites)
     c = setups.channel_2d(48, 24, 8.0, 0.04, 20.0, regularised=True)
     pos = setups.circle_markers((1.52, 1.49, 0.0), 0.3, c.info["dh"])
+    return c, pos
+
+
+def corner_box_case():
+    # regularised velocity walls left, right and on top, meeting at corner sites (the density
+    # extrapolated from two inward neighbours), solid bottom; an IB cylinder inside
+    import dataclasses
+    c = dataclasses.replace(setups.velocity_corner_box(32, 24, u_in=0.02, u_lid=0.05, tau=0.6), forced=True)
+    dh = c.info["dh"]
+    pos = setups.circle_markers((16.3 * dh, 11.6 * dh, 0.0), 3.0 * dh, dh)
     return c, pos
 
 
@@ -79,8 +90,8 @@ def close(a, b, rtol=RTOL, scale=None):
     return abs(a - b) <= rtol * s
 
 
-@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (channel_reg_case, 60, 16), (sphere_case, 24, 10),
-                                       (inflow_case, 30, 12)])
+@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (channel_reg_case, 60, 16), (corner_box_case, 40, 12),
+                                       (sphere_case, 24, 10), (inflow_case, 30, 12)])
 def test_adjoint_matches_central_differences(build, T, K):
     case, pos = build()
     print(f"[adjoint] {case.name} T={T} K={K} markers={len(pos)}", flush=True)
