@@ -78,8 +78,9 @@ int check_adjoint_grid(const lumagpu_grid *g) {
         if (ty == kFluid || ty == kSolid || ty == kExtrapRight) continue;
         // forced-equilibrium velocity sources, or regularised velocity faces (edge and corner
         // sites are refused above)
-        if (ty == kVelocity) continue;
-        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint covers fluid, solid, velocity and ExtrapolateRight sites only");
+        if (ty == kVelocity || ty == kPressure) continue;
+        return fail(LUMAGPU_EUNSUPPORTED,
+                    "the adjoint covers fluid, solid, velocity, pressure and ExtrapolateRight sites only");
     }
     if (g->ibm) {
         for (const auto &b : g->ibm->bodies)
@@ -90,8 +91,10 @@ int check_adjoint_grid(const lumagpu_grid *g) {
                 for (int k = 0; k < m.ns; ++k) {
                     const size_t c = (size_t)m.sijk[k][0] +
                                      (size_t)g->d.N * ((size_t)m.sijk[k][1] + (size_t)g->d.M * m.sijk[k][2]);
-                    if (c < g->hflags.size() && (g->hflags[c] & kTypeMask) == kVelocity)
-                        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover IB supports on velocity sites");
+                    const int t = c < g->hflags.size() ? (g->hflags[c] & kTypeMask) : kFluid;
+                    if (t == kVelocity || t == kPressure)
+                        return fail(LUMAGPU_EUNSUPPORTED,
+                                    "the adjoint does not cover IB supports on velocity or pressure sites");
                 }
     }
     return 0;
@@ -265,7 +268,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     ea.n1 = g->edge + 2 * g->nedge;
     ea.n2 = g->edge + 3 * g->nedge;
     ea.erho = g->erho;
-    DevBuf dbar, epart, enb_cell, enb_ptr, enb_ent;
+    DevBuf dbar, tbar, epart, enb_cell, enb_ptr, enb_ent;
     int n_enb = 0;
     if (g->nedge > 0) {
         const int ne = g->nedge;
@@ -289,6 +292,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         ptr.push_back((int)ent.size());
         n_enb = (int)cells.size();
         if (int r = dbar.alloc(ne * sizeof(double))) return r;
+        if (int r = tbar.alloc(3 * (size_t)ne * sizeof(double))) return r;
         if (int r = epart.alloc(2 * (size_t)ne * sizeof(double))) return r;
         if (int r = enb_cell.alloc(cells.size() * sizeof(int))) return r;
         if (int r = enb_ptr.alloc(ptr.size() * sizeof(int))) return r;
@@ -326,21 +330,27 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 if (e == hipSuccess) e = launch_adj_collide<P>(g, a, ab);
                 if (e == hipSuccess && g->nedge) {
                     const dim3 eb(64), eg((unsigned)((g->nedge + 63) / 64));
-                    double *db = static_cast<double *>(dbar.p), *ep = static_cast<double *>(epart.p);
+                    double *db = static_cast<double *>(dbar.p), *tb = static_cast<double *>(tbar.p);
+                    double *ep = static_cast<double *>(epart.p);
+                    const int *nc_p = static_cast<const int *>(enb_cell.p), *np_p = static_cast<const int *>(enb_ptr.p);
+                    const int *ne_p = static_cast<const int *>(enb_ent.p);
+                    const dim3 nbb(128), nbg((unsigned)((n_enb + 127) / 128));
                     if (g->Q == 9) {
-                        if (g->parity == 0) k_adj_edges<9, P, true><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
-                        else k_adj_edges<9, P, false><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
-                    } else {
-                        if (g->parity == 0) k_adj_edges<19, P, true><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
-                        else k_adj_edges<19, P, false><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, ep);
-                    }
-                    e = hipGetLastError();
-                    if (e == hipSuccess) {
-                        k_adj_edge_nbrs<T><<<(n_enb + 127) / 128, 128, 0, g->stream>>>(
-                            n_enb, static_cast<const int *>(enb_cell.p), static_cast<const int *>(enb_ptr.p),
-                            static_cast<const int *>(enb_ent.p), db, g->Q, nc, ab.gbar);
+                        if (g->parity == 0) k_adj_edges<9, P, true><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, tb, ep);
+                        else k_adj_edges<9, P, false><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, tb, ep);
                         e = hipGetLastError();
+                        if (e == hipSuccess)
+                            k_adj_edge_nbrs<9, T><<<nbg, nbb, 0, g->stream>>>(n_enb, nc_p, np_p, ne_p, db, tb, g->erho, nc,
+                                                                             ab.gbar);
+                    } else {
+                        if (g->parity == 0) k_adj_edges<19, P, true><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, tb, ep);
+                        else k_adj_edges<19, P, false><<<eg, eb, 0, g->stream>>>(a, ea, ab, db, tb, ep);
+                        e = hipGetLastError();
+                        if (e == hipSuccess)
+                            k_adj_edge_nbrs<19, T><<<nbg, nbb, 0, g->stream>>>(n_enb, nc_p, np_p, ne_p, db, tb, g->erho,
+                                                                              nc, ab.gbar);
                     }
+                    if (e == hipSuccess) e = hipGetLastError();
                     if (e == hipSuccess) {
                         k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ep, g->nedge, 2, acc + 4);
                         e = hipGetLastError();

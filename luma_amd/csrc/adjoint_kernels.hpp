@@ -50,12 +50,14 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // A regularised velocity face site (not an edge or corner) of a grid stepped with
-// regularised boundaries; with the edge and corner sites, every regularised velocity site.
+// regularised boundaries; with the edge and corner sites and the pressure faces, every
+// regularised boundary site (each streams like a fluid site and is pulled like one).
 __device__ __forceinline__ bool regularised_face_site(const StepArgs &a, unsigned fl) {
     return a.regularised && (fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge;
 }
-__device__ __forceinline__ bool regularised_velocity_site(const StepArgs &a, unsigned fl) {
-    return a.regularised && (fl & kTypeMask) == kVelocity;
+__device__ __forceinline__ bool regularised_boundary_site(const StepArgs &a, unsigned fl) {
+    const int ty = fl & kTypeMask;
+    return a.regularised && (ty == kVelocity || ty == kPressure);
 }
 
 // Transposed Jacobian of a regularised velocity face site's update (regularised_face, :313-510,
@@ -200,14 +202,17 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typenam
     if (threadIdx.x == 0) b.part[2 * b.nblk + block_linear()] = si;
 }
 
-// Edge and corner velocity sites (k_edge_sites: regularised_edge with the density
-// 2 rho(x+n) - rho(x+2n) extrapolated from the neighbours' sweep-1 values, then BGK), one
-// thread per site: the site's own gbar from forward-mode passes over its populations, and
-// dbar[i] = sum_w lam_w d f'_w / d rho_ext, which reaches the neighbours' streamed
-// populations through d rho / d g_v = 1 (k_adj_edge_nbrs); omega and inlet-scale derivatives
-// per site in epart[i] and epart[n + i].
+// Sites that extrapolate from their two inward neighbours (k_edge_sites), one thread per site:
+// edge and corner velocity sites (regularised_edge with the density 2 rho(x+n) - rho(x+2n) from
+// the neighbours' sweep-1 values) and pressure faces (regularised_pressure with the tangential
+// velocity 2 u(x+n) - u(x+2n)), then BGK.  The site's own gbar comes from forward-mode passes
+// over its populations; the extrapolated quantities' adjoints (dbar[i], or tbar[3 i + d] for
+// the velocity components) reach the neighbours' streamed populations through
+// d rho / d g_v = 1 and d u_d / d g_v = (c_vd - u_d) / rho (k_adj_edge_nbrs); omega and
+// inlet-scale derivatives per site in epart[i] and epart[n + i].
 template <int Q, class P, bool ODD>
-__global__ void k_adj_edges(StepArgs a, EdgeArgs e, AdjArgs<typename P::T> b, double *dbar, double *epart) {
+__global__ void k_adj_edges(StepArgs a, EdgeArgs e, AdjArgs<typename P::T> b, double *dbar, double *tbar,
+                            double *epart) {
     using L = Lattice<Q>;
     using T = typename P::T;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,50 +232,75 @@ __global__ void k_adj_edges(StepArgs a, EdgeArgs e, AdjArgs<typename P::T> b, do
         lam[v] = double(b.lam[(size_t)v * a.ncell + cell]);
     }
     const double *er1 = e.erho + 8 * (size_t)i, *er2 = er1 + 4;
+    const bool pressure = (a.flags[cell] & kTypeMask) == kPressure;
+    const int ncode = (a.flags[cell] >> kNormalShift) & 7;
     const double dens0 = 2.0 * er1[0] - er2[0];
+    const double vt0[3] = {2.0 * er1[1] - er2[1], 2.0 * er1[2] - er2[2], (L::D == 3) ? 2.0 * er1[3] - er2[3] : 0.0};
     const double r = a.ramp_reg;
     const double vin0[3] = {a.uin[y] * r, a.uin[ny + y] * r, (L::D == 3) ? a.uin[2 * ny + y] * r : 0.0};
-    // seeds: populations 0..Q-1, then the extrapolated density, omega, the inlet scale
-    double omb = 0.0, inb = 0.0;
-    for (int s = 0; s < Q + 3; ++s) {
+    // seeds: populations 0..Q-1, then omega, then the extrapolated density and the inlet scale
+    // (velocity edges) or the three extrapolated velocity components (pressure faces)
+    double omb = 0.0, inb = 0.0, db = 0.0, tb[3] = {0.0, 0.0, 0.0};
+    const int ns = Q + (pressure ? 4 : 3);
+    for (int s = 0; s < ns; ++s) {
         Dual fa[Q];
         for (int v = 0; v < Q; ++v) fa[v] = Dual(g[v], v == s ? 1.0 : 0.0);
-        Dual vin[3];
-        for (int d = 0; d < 3; ++d) vin[d] = Dual(vin0[d], s == Q + 2 ? vin0[d] : 0.0);
         Dual rr, uu[3], fout[Q];
-        regularised_edge<Q, Dual>(fa, n, vin, Dual(dens0, s == Q ? 1.0 : 0.0), Dual(a.cs2), Dual(a.den2), rr, uu);
-        collide_exact<Q, Dual>(fa, rr, uu, static_cast<const Dual *>(nullptr), Dual(a.omega, s == Q + 1 ? 1.0 : 0.0),
+        if (pressure) {
+            Dual vt[3];
+            for (int d = 0; d < 3; ++d) vt[d] = Dual(vt0[d], s == Q + 1 + d ? 1.0 : 0.0);
+            regularised_pressure<Q, Dual>(fa, ncode, vt, Dual(a.rho_p), Dual(a.cs2), Dual(a.den2), rr, uu);
+        } else {
+            Dual vin[3];
+            for (int d = 0; d < 3; ++d) vin[d] = Dual(vin0[d], s == Q + 2 ? vin0[d] : 0.0);
+            regularised_edge<Q, Dual>(fa, n, vin, Dual(dens0, s == Q + 1 ? 1.0 : 0.0), Dual(a.cs2), Dual(a.den2), rr,
+                                      uu);
+        }
+        collide_exact<Q, Dual>(fa, rr, uu, static_cast<const Dual *>(nullptr), Dual(a.omega, s == Q ? 1.0 : 0.0),
                                Dual(a.cs2), Dual(a.den2), fout);
         double acc = 0.0;
         for (int w = 0; w < Q; ++w) acc += lam[w] * fout[w].d;
         if (s < Q) b.gbar[(size_t)s * a.ncell + cell] = T(acc);
-        else if (s == Q) dbar[i] = acc;
-        else if (s == Q + 1) omb = acc;
+        else if (s == Q) omb = acc;
+        else if (pressure) tb[s - Q - 1] = acc;
+        else if (s == Q + 1) db = acc;
         else inb = acc;
     }
+    dbar[i] = db;
+    for (int d = 0; d < 3; ++d) tbar[3 * (size_t)i + d] = tb[d];
     epart[i] = omb;
     epart[e.n + i] = inb;
 }
 
-// The extrapolation's reach into the neighbours: gbar_v(x) += sum over the edge sites that
-// extrapolate from x of coef * dbar (coef 2 for the neighbour x + n, -1 for x + 2n), the same
-// for every v; one thread per neighbour cell, its entries in a fixed order (no atomics).
-template <class T>
-__global__ void k_adj_edge_nbrs(int nn, const int *ncell, const int *ptr, const int *ent, const double *dbar, int Q,
-                                size_t ncell_total, T *gbar) {
+// The extrapolation's reach into the neighbours: gbar_v(x) += sum over the sites that
+// extrapolate from x of coef * (dbar + sum_d tbar_d (c_vd - u_d(x)) / rho(x)) (coef 2 for the
+// neighbour x + n, -1 for x + 2n; rho, u the neighbour's sweep-1 values in erho); one thread per
+// neighbour cell, its entries in a fixed order (no atomics).
+template <int Q, class T>
+__global__ void k_adj_edge_nbrs(int nn, const int *ncell, const int *ptr, const int *ent, const double *dbar,
+                                const double *tbar, const double *erho, size_t ncell_total, T *gbar) {
+    using L = Lattice<Q>;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nn) return;
-    double s = 0.0;
+    double s[Q];
+    for (int v = 0; v < Q; ++v) s[v] = 0.0;
     for (int j = ptr[k]; j < ptr[k + 1]; ++j) {
-        const int code = ent[j];   // 2 * edge + (0: x + n, 1: x + 2n)
-        s += ((code & 1) ? -1.0 : 2.0) * dbar[code >> 1];
+        const int code = ent[j];   // 2 * site + (0: x + n, 1: x + 2n)
+        const int i = code >> 1;
+        const double coef = (code & 1) ? -1.0 : 2.0;
+        const double *er = erho + 4 * (size_t)code;   // this neighbour's rho, u
+        for (int v = 0; v < Q; ++v) {
+            double t = dbar[i];
+            for (int d = 0; d < L::D; ++d) t += tbar[3 * (size_t)i + d] * (L::c(v, d) - er[1 + d]) / er[0];
+            s[v] += coef * t;
+        }
     }
     const size_t cell = (size_t)ncell[k];
-    for (int v = 0; v < Q; ++v) gbar[(size_t)v * ncell_total + cell] += T(s);
+    for (int v = 0; v < Q; ++v) gbar[(size_t)v * ncell_total + cell] += T(s[v]);
 }
 
 // Reverse streaming: lam_w(x) gathers the adjoint of every streamed population that read
-// f_w(x) -- the pull by x + c_w (a fluid or regularised velocity site), the bounce-back at
+// f_w(x) -- the pull by x + c_w (a fluid or regularised boundary site), the bounce-back at
 // x itself when x + c_w is solid, and the pull through an ExtrapolateRight source two cells to
 // the right.  Also the derivative with
 // respect to a scale s of the inlet profile (u_in -> s u_in) of the forced-equilibrium
@@ -287,7 +317,7 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
         const size_t cell = cell_of(x, y, z, nx, ny);
         const unsigned fl = a.flags[cell];
         const int ty = fl & kTypeMask;
-        if (ty == kFluid || regularised_velocity_site(a, fl)) {
+        if (ty == kFluid || regularised_boundary_site(a, fl)) {
             Nbr nb;
             nb.init(x, y, z, nx, ny, a.nz);
             const bool xr = (x + 2 < nx) && ((a.flags[cell + 2] & kTypeMask) == kExtrapRight);
@@ -300,7 +330,7 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
                 const unsigned fn = a.flags[yn];
                 const int tn = fn & kTypeMask;
                 T acc = T(0);
-                if (tn == kFluid || regularised_velocity_site(a, fn)) acc = b.gbar[(size_t)w * a.ncell + yn];
+                if (tn == kFluid || regularised_boundary_site(a, fn)) acc = b.gbar[(size_t)w * a.ncell + yn];
                 else if (tn == kSolid) acc = b.gbar[(size_t)L::opp(w) * a.ncell + cell];
                 if (xr) {
                     const size_t yp = ns.at(cx, cy, cz, nx, ny);

@@ -2,7 +2,8 @@
 forward step, whose FP64 form equals the oracle bit for bit (the other GPU tests): gradients
 of the time-averaged IB drag with respect to the relaxation frequency, the inlet velocity
 scale, the marker positions and the initial populations, in 2D (channel with a forced-
-equilibrium or a regularised velocity inlet, ExtrapolateRight outlet, solid walls, IB cylinder;
+equilibrium or a regularised velocity inlet, ExtrapolateRight or regularised pressure outlet,
+solid walls, IB cylinder;
 a box of regularised velocity walls meeting at corner sites, IB cylinder) and 3D (periodic box,
 IB sphere; uniform inflow); checkpointing with a partial last segment; FP32-shifted adjoint
 against FP64.
@@ -28,6 +29,14 @@ def channel_reg_case():
     # the same channel with LUMA's regularised velocity inlet (Latt 2008 faces; the inlet's
     # ends meet the solid walls, so there are no edge sites)
     c = setups.channel_2d(48, 24, 8.0, 0.04, 20.0, regularised=True)
+    pos = setups.circle_markers((1.52, 1.49, 0.0), 0.3, c.info["dh"])
+    return c, pos
+
+
+def channel_pressure_case():
+    # regularised inlet and LUMA's default outlet, a regularised pressure face (its tangential
+    # velocity extrapolated from the two inward neighbours)
+    c = setups.channel_2d(48, 24, 8.0, 0.04, 20.0, regularised=True, outlet="pressure")
     pos = setups.circle_markers((1.52, 1.49, 0.0), 0.3, c.info["dh"])
     return c, pos
 
@@ -90,7 +99,8 @@ def close(a, b, rtol=RTOL, scale=None):
     return abs(a - b) <= rtol * s
 
 
-@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (channel_reg_case, 60, 16), (corner_box_case, 40, 12),
+@pytest.mark.parametrize("build,T,K", [(channel_case, 60, 16), (channel_reg_case, 60, 16),
+                                       (channel_pressure_case, 60, 16), (corner_box_case, 40, 12),
                                        (sphere_case, 24, 10), (inflow_case, 30, 12)])
 def test_adjoint_matches_central_differences(build, T, K):
     case, pos = build()
@@ -118,7 +128,7 @@ def test_adjoint_matches_central_differences(build, T, K):
     # forced-equilibrium source's own populations are never read: both sides give 0 there)
     rng = np.random.default_rng(11)
     fluid = np.ones(case.N * case.M * case.K, bool) if case.lattyp is None else \
-        np.isin(case.lattyp, (setups.FLUID, setups.VELOCITY)).reshape(-1)
+        np.isin(case.lattyp, (setups.FLUID, setups.VELOCITY, setups.PRESSURE)).reshape(-1)
     dirn = rng.standard_normal(f0.shape) * fluid.reshape(f0.shape[:3])[..., None]
     h = 1e-6
     fd = (J_of(case, pos, T, f0=f0 + h * dirn) - J_of(case, pos, T, f0=f0 - h * dirn)) / (2 * h)
