@@ -371,8 +371,9 @@ int lumagpu_checkpoint_free(lumagpu_checkpoint *ck);
  *   populations in LUMA's layout f[v + Q*id]; NULL to skip).
  * Covered: FP64 and FP32-shifted forms, one grid level, fluid / solid / forced-equilibrium
  * velocity / regularised velocity (faces, edges, corners) / regularised pressure face /
- * ExtrapolateRight sites, body force, rigid IB bodies (supports on fluid sites).  Refused
- * (LUMAGPU_EUNSUPPORTED): refinement, z-slabs, ramps, gravity, flexible bodies.  Memory: ceil(T/K) + K + 1
+ * ExtrapolateRight sites, body force, rigid IB bodies (supports on fluid sites), a velocity
+ * ramp.  Refused (LUMAGPU_EUNSUPPORTED): refinement, z-slabs, a Reynolds ramp, gravity,
+ * flexible bodies.  Memory: ceil(T/K) + K + 1
  * population arrays plus three adjoint arrays. */
 typedef struct lumagpu_adj_result {
     double J;

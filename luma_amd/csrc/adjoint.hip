@@ -69,7 +69,6 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     if (g->parent || !g->kids.empty()) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs on a single grid level");
     if (g->zslab) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs");
     if (g->gravity_on) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover the gravity reset");
-    if (g->ramp_time > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a velocity ramp");
     if (g->re_ramp > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a Reynolds ramp");
     if (g->ibm && g->ibm->universal)
         return fail(LUMAGPU_EUNSUPPORTED, "the adjoint chains epsilon body by body, not the universal system");
@@ -173,6 +172,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     const size_t fbytes = (size_t)g->elem * g->Q * g->ps;
     const double invT = 1.0 / nsteps;
     const int nck = (nsteps + every - 1) / every;
+    const int t_start = g->t;   // step t of the run runs at grid time t_start + t (velocity ramp)
 
     std::vector<DevBuf> ck(nck);
     std::vector<int> ckpar(nck);
@@ -308,6 +308,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         const int t0 = seg * every, t1 = std::min(nsteps, t0 + every);
         HIP_TRY(hipMemcpyAsync(g->f, ck[seg].p, fbytes, hipMemcpyDeviceToDevice, g->stream));
         g->parity = ckpar[seg];
+        g->t = t_start + t0;
         g->xr_ready = 0;
         for (int t = t0; t < t1 && !rc; ++t) {
             HIP_TRY(hipMemcpyAsync(tape[t - t0].p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
@@ -317,6 +318,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         for (int t = t1 - 1; t >= t0 && !rc; --t) {
             // the step's input state stands in for the grid's populations while it is reversed
             g->f = tape[t - t0].p;
+            g->t = t_start + t;
             g->parity = tpar[t - t0];
             if (g->has_outlet) {
                 hipError_t e = outlet_prepass(g);

@@ -342,8 +342,11 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
                 for (int v = 0; v < Q; ++v) {
                     const size_t src = nb.at(-L::c(v, 0), -L::c(v, 1), -L::c(v, 2), nx, ny);
                     if ((a.flags[src] & kTypeMask) != kVelocity) continue;
-                    const int js = nb.ys[1 - L::c(v, 1)];
-                    const double us[3] = {a.uin[js], a.uin[ny + js], (L::D == 3) ? a.uin[2 * ny + js] : 0.0};
+                    // the source's velocity as the step's gather_in takes it (with a velocity ramp,
+                    // the ramped profile of the pulling site's row)
+                    const int js = a.ramp_on ? y : nb.ys[1 - L::c(v, 1)];
+                    const double rf = a.ramp_feq;
+                    const double us[3] = {a.uin[js] * rf, a.uin[ny + js] * rf, (L::D == 3) ? a.uin[2 * ny + js] * rf : 0.0};
                     const double cu = L::c(v, 0) * us[0] + L::c(v, 1) * us[1] + L::c(v, 2) * us[2];
                     // d feq_v / du . u_in = w rho_in (3 c.u + 9 (c.u)^2 - 3 u.u)
                     const double usq = us[0] * us[0] + us[1] * us[1] + us[2] * us[2];

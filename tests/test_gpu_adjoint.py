@@ -72,7 +72,7 @@ def inflow_case():
     return c, pos
 
 
-def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None):
+def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None, ramp=None):
     c = case
     if omega is not None:
         import dataclasses
@@ -84,6 +84,8 @@ def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None):
         g.set_inlet(c.ux_in * inlet_scale, c.uy_in * inlet_scale, c.uz_in * inlet_scale)
     if f0 is not None:
         g.upload_f(f0)
+    if ramp is not None:
+        g.set_velocity_ramp(ramp)
     g.ibm_add_body(pos)
     g.ibm_initialise()
     return g
@@ -148,6 +150,36 @@ def test_adjoint_matches_central_differences(build, T, K):
             fd = (J_of(case, pp, T) - J_of(case, pm, T)) / (2 * h)
             print(f"dJ/dx[{m},{a}] adjoint", adj["dJ_dpos"][m, a], "fd", fd, flush=True)
             assert close(adj["dJ_dpos"][m, a], fd, rtol=RTOL, scale=1e-2 * gscale)
+
+
+@pytest.mark.parametrize("regularised", [False, True])
+def test_adjoint_with_velocity_ramp(regularised):
+    # L_VELOCITY_RAMP over the first 30 of the 60 steps: every reversed step takes the ramp
+    # factor of its own time (forced-equilibrium sources and regularised faces)
+    case = channel_case()[0] if not regularised else channel_reg_case()[0]
+    pos = channel_case()[1]
+    ramp = 30 * case.info["dt"]
+    T, K = 60, 16
+    print(f"[adjoint] velocity ramp regularised={regularised}", flush=True)
+    with make_grid(case, pos, ramp=ramp) as g:
+        f0 = g.download_f()
+        adj = g.adj_run(T, K, initial=True)
+    h = 1e-5 * case.omega
+    fd = (J_of(case, pos, T, omega=case.omega + h, ramp=ramp) - J_of(case, pos, T, omega=case.omega - h, ramp=ramp)) / (2 * h)
+    print("dJ/domega adjoint", adj["dJ_domega"], "fd", fd, flush=True)
+    assert close(adj["dJ_domega"], fd)
+    h = 1e-5
+    fd = (J_of(case, pos, T, inlet_scale=1 + h, ramp=ramp) - J_of(case, pos, T, inlet_scale=1 - h, ramp=ramp)) / (2 * h)
+    print("dJ/dinlet adjoint", adj["dJ_dinlet"], "fd", fd, flush=True)
+    assert close(adj["dJ_dinlet"], fd)
+    rng = np.random.default_rng(5)
+    m = np.isin(case.lattyp, (setups.FLUID, setups.VELOCITY)).reshape(f0.shape[:3])
+    dirn = rng.standard_normal(f0.shape) * m[..., None]
+    h = 1e-6
+    fd = (J_of(case, pos, T, f0=f0 + h * dirn, ramp=ramp) - J_of(case, pos, T, f0=f0 - h * dirn, ramp=ramp)) / (2 * h)
+    pred = float((adj["lambda0"] * dirn).sum())
+    print("dJ/df0 . d adjoint", pred, "fd", fd, flush=True)
+    assert close(pred, fd)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
