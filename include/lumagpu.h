@@ -228,7 +228,7 @@ int lumagpu_moments_noalloc(lumagpu_grid *g, double *out_dev);
  * every marker of the grid's bodies, in body order (ibm_universalEpsilonGather /
  * ibm_universalEpsilonScatter, ObjectManager_ops_ibm.cpp:775-786, :905-909, :1067-1141), for
  * bodies whose supports overlap.  Set before lumagpu_ibm_initialise; 0 = per body (default).
- * The adjoint refuses a grid in this mode. */
+ * The adjoint chains the positions through the same joint system. */
 int lumagpu_ibm_set_universal_epsilon(lumagpu_grid *g, int on);
 /* Body<IBMarker>::addMarker (Body.h:144-255) + IBMarker ctor (IBMarker.cpp:56-86):
  * a body of n markers at pos[3*m + a]; epsilon 0, ds 1, markerVel 0 until initialised. */

@@ -70,8 +70,6 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     if (g->zslab) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs");
     if (g->gravity_on) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover the gravity reset");
     if (g->re_ramp > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a Reynolds ramp");
-    if (g->ibm && g->ibm->universal)
-        return fail(LUMAGPU_EUNSUPPORTED, "the adjoint chains epsilon body by body, not the universal system");
     for (size_t c = 0; c < g->hflags.size(); ++c) {
         const int ty = g->hflags[c] & kTypeMask;
         if (ty == kFluid || ty == kSolid || ty == kExtrapRight) continue;
@@ -99,30 +97,31 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     return 0;
 }
 
-// epsilon and ds chains of one body: xg += dJ/dx through eps (A eps = 1) and ds
-int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsbar, const double *dsbar_dev,
-                        double *xg) {
-    const int n = (int)b.m.size(), D = g->D;
+// The epsilon chain of one system (a body's markers, or with the universal system every marker
+// of the grid in body order): xg += dJ/dx through eps (A eps = 1), and dsbar += the part of
+// dJ/d ds that enters through A's columns.
+int epsilon_chain(const lumagpu_grid *g, const std::vector<const Marker *> &ms, const double *epsbar, double *dsbar,
+                  double *xg) {
+    const int n = (int)ms.size(), D = g->D;
     const double dh = g->d.dh;
-    std::vector<double> dsbar(dsbar_dev, dsbar_dev + n);
     // A_IJ = ds_J sum_k delta(x_I; s^I_k) delta(x_J; s^I_k) area_I; solve A^T mu = epsbar
     std::vector<double> A((size_t)n * n, 0.0);
     for (int I = 0; I < n; ++I)
         for (int J = 0; J < n; ++J) {
-            if (far_apart(b.m[I], b.m[J], D, dh)) continue;
+            if (far_apart(*ms[I], *ms[J], D, dh)) continue;
             double acc = 0.0;
-            for (int k = 0; k < b.m[I].ns; ++k)
-                acc += b.m[I].delta[k] * delta_at(b.m[J].pos, b.m[I].sxyz[k], dh, b.m[J].dilation, D) * b.m[I].area;
-            A[(size_t)I * n + J] = acc * b.m[J].ds;
+            for (int k = 0; k < ms[I]->ns; ++k)
+                acc += ms[I]->delta[k] * delta_at(ms[J]->pos, ms[I]->sxyz[k], dh, ms[J]->dilation, D) * ms[I]->area;
+            A[(size_t)I * n + J] = acc * ms[J]->ds;
         }
     std::vector<double> At((size_t)n * n), mu(epsbar, epsbar + n);
     for (int I = 0; I < n; ++I)
         for (int J = 0; J < n; ++J) At[(size_t)J * n + I] = A[(size_t)I * n + J];
     if (int r = lu_solve(n, At, mu)) return r;
     for (int I = 0; I < n; ++I) {
-        const Marker &mI = b.m[I];
+        const Marker &mI = *ms[I];
         for (int J = 0; J < n; ++J) {
-            const Marker &mJ = b.m[J];
+            const Marker &mJ = *ms[J];
             const double Abar = -mu[I] * mJ.eps;
             if (Abar == 0.0 || far_apart(mI, mJ, D, dh)) continue;
             dsbar[J] += Abar * A[(size_t)I * n + J] / mJ.ds;
@@ -138,7 +137,13 @@ int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsb
             }
         }
     }
-    // ds_m = min(10, min_n |x_m - x_n| / dh)
+    return 0;
+}
+
+// The ds chain of one body (ibm_computeDs is per body): ds_m = min(10, min_n |x_m - x_n| / dh)
+void ds_chain(const lumagpu_grid *g, const Body &b, const double *dsbar, double *xg) {
+    const int n = (int)b.m.size(), D = g->D;
+    const double dh = g->d.dh;
     for (int m = 0; m < n; ++m) {
         double best = 10.0;
         int arg = -1;
@@ -159,6 +164,26 @@ int body_position_chain(const lumagpu_grid *g, const Body &b, const double *epsb
             xg[3 * arg + a] -= dsbar[m] * gdir;
         }
     }
+}
+
+// dJ/dx of every marker through epsilon and ds, per body or through the universal system
+int position_chains(const lumagpu_grid *g, const double *epsbar, const double *dsbar_dev, double *xg) {
+    const IbmSystem *s = g->ibm;
+    std::vector<double> dsbar(dsbar_dev, dsbar_dev + s->nm);
+    if (s->universal) {
+        std::vector<const Marker *> ms;
+        for (const auto &b : s->bodies)
+            for (const auto &m : b.m) ms.push_back(&m);
+        if (int r = epsilon_chain(g, ms, epsbar, dsbar.data(), xg)) return r;
+    } else {
+        for (const auto &b : s->bodies) {
+            std::vector<const Marker *> ms;
+            for (const auto &m : b.m) ms.push_back(&m);
+            if (int r = epsilon_chain(g, ms, epsbar + b.first, dsbar.data() + b.first, xg + 3 * (size_t)b.first))
+                return r;
+        }
+    }
+    for (const auto &b : s->bodies) ds_chain(g, b, dsbar.data() + b.first, xg + 3 * (size_t)b.first);
     return 0;
 }
 
@@ -418,9 +443,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     }
     if (ib && dpos) {
         for (size_t i = 0; i < hx.size(); ++i) dpos[i] = hx[i];
-        for (const auto &b : s->bodies)
-            if (int r = body_position_chain(g, b, he.data() + b.first, hd.data() + b.first, dpos + 3 * (size_t)b.first))
-                return r;
+        if (int r = position_chains(g, he.data(), hd.data(), dpos)) return r;
     }
     return 0;
 }

@@ -182,6 +182,43 @@ def test_adjoint_with_velocity_ramp(regularised):
     assert close(pred, fd)
 
 
+def test_adjoint_universal_epsilon():
+    # L_UNIVERSAL_EPSILON_CALC: two cylinders whose supports overlap share one epsilon system;
+    # the position gradient goes through that joint system (and ds body by body)
+    case, pa = channel_case()
+    dh = case.info["dh"]
+    pb = setups.circle_markers((2.1, 1.5, 0.0), 0.2, dh)
+    T, K = 50, 16
+
+    def grid(a, b):
+        g = GpuGrid.from_case(case, "fp64", store_macro=True)
+        g.ibm_set_universal_epsilon(True)
+        g.ibm_add_body(a)
+        g.ibm_add_body(b)
+        g.ibm_initialise()
+        return g
+
+    print(f"[adjoint] universal epsilon, {len(pa)} + {len(pb)} markers", flush=True)
+    with grid(pa, pb) as g:
+        adj = g.adj_run(T, K, positions=True)
+    grad = adj["dJ_dpos"]
+    gscale = np.abs(grad).max()
+    for body, m in ((0, 3), (1, 0), (1, len(pb) // 2)):
+        idx = m if body == 0 else len(pa) + m
+        for a in range(2):
+            h = 1e-5 * dh
+            pos = [pa.copy(), pb.copy()]
+            pos[body][m, a] += h
+            with grid(*pos) as g:
+                jp = g.adj_run(T, T, backward=False)["J"]
+            pos[body][m, a] -= 2 * h
+            with grid(*pos) as g:
+                jm = g.adj_run(T, T, backward=False)["J"]
+            fd = (jp - jm) / (2 * h)
+            print(f"dJ/dx[{body}:{m},{a}] adjoint", grad[idx, a], "fd", fd, flush=True)
+            assert close(grad[idx, a], fd, rtol=RTOL, scale=1e-2 * gscale)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     case, pos = channel_case()
     res = []
