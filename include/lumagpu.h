@@ -364,7 +364,8 @@ int lumagpu_checkpoint_free(lumagpu_checkpoint *ck);
  * (as T lumagpu_step calls; it is left at the end of them), keeping a copy of the populations
  * every checkpoint_every steps; with backward = 1 it then runs the discrete adjoint of those
  * steps, recomputing each segment from its checkpoint, and returns
- *   res->J, res->dJ_domega (relaxation frequency), res->dJ_dinlet (derivative with respect to
+ *   res->J, res->dJ_domega (relaxation frequency; under a Reynolds ramp the omega after the
+ *   ramp, which L_RE sets for every step), res->dJ_dinlet (derivative with respect to
  *   a common scale s of the inlet profile, u_in -> s u_in, at s = 1),
  *   dJ_dpos[3*m + a] (marker positions of all bodies in marker order, through the delta
  *   weights, epsilon and ds; NULL to skip) and lambda0 (dJ/df of the initial post-collision
@@ -372,7 +373,7 @@ int lumagpu_checkpoint_free(lumagpu_checkpoint *ck);
  * Covered: FP64 and FP32-shifted forms, one grid level, fluid / solid / forced-equilibrium
  * velocity / regularised velocity (faces, edges, corners) / regularised pressure face /
  * ExtrapolateRight sites, body force, rigid IB bodies (supports on fluid sites), a velocity
- * ramp.  Refused (LUMAGPU_EUNSUPPORTED): refinement, z-slabs, a Reynolds ramp, gravity,
+ * ramp, a Reynolds ramp.  Refused (LUMAGPU_EUNSUPPORTED): refinement, z-slabs, gravity,
  * flexible bodies.  Memory: ceil(T/K) + K + 1
  * population arrays plus three adjoint arrays. */
 typedef struct lumagpu_adj_result {

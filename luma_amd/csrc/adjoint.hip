@@ -69,12 +69,10 @@ int check_adjoint_grid(const lumagpu_grid *g) {
     if (g->parent || !g->kids.empty()) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint runs on a single grid level");
     if (g->zslab) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover z-slabs");
     if (g->gravity_on) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover the gravity reset");
-    if (g->re_ramp > 0.0) return fail(LUMAGPU_EUNSUPPORTED, "the adjoint does not cover a Reynolds ramp");
     for (size_t c = 0; c < g->hflags.size(); ++c) {
         const int ty = g->hflags[c] & kTypeMask;
         if (ty == kFluid || ty == kSolid || ty == kExtrapRight) continue;
-        // forced-equilibrium velocity sources, or regularised velocity faces (edge and corner
-        // sites are refused above)
+        // forced-equilibrium velocity sources, regularised velocity and pressure sites
         if (ty == kVelocity || ty == kPressure) continue;
         return fail(LUMAGPU_EUNSUPPORTED,
                     "the adjoint covers fluid, solid, velocity, pressure and ExtrapolateRight sites only");
@@ -225,6 +223,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     }
     HIP_TRY(hipMemcpyAsync(fin.p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
     const int fin_par = g->parity, fin_t = g->t;
+    const double fin_omega = g->d.omega;
     double hacc[6] = {0, 0, 0, 0, 0, 0};
     if (!backward) {
         HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
@@ -350,6 +349,13 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 if (e != hipSuccess) rc = fail(LUMAGPU_EHIP, std::string("adjoint outlet: ") + hipGetErrorString(e));
             }
             if (!rc && ib) rc = apply_core(g);
+            // a Reynolds ramp: this step's omega, and its omega partial weighted onto the omega
+            // after the ramp
+            double w_om = 1.0;
+            if (g->re_ramp > 0.0) {
+                update_reynolds(g);
+                w_om = reynolds_omega_weight(g);
+            }
             const StepArgs a = make_args(g);
             if (!rc) {
                 // the edge sites' extrapolation densities of this step
@@ -379,7 +385,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                     }
                     if (e == hipSuccess) e = hipGetLastError();
                     if (e == hipSuccess) {
-                        k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ep, g->nedge, 2, acc + 4);
+                        k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ep, g->nedge, 2, w_om, acc + 4);
                         e = hipGetLastError();
                     }
                 }
@@ -399,7 +405,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 }
                 if (e == hipSuccess) e = launch_adj_stream<P>(g, a, ab);
                 if (e == hipSuccess) {
-                    k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ab.part, nblk, 3, acc + 1);
+                    k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ab.part, nblk, 3, w_om, acc + 1);
                     e = hipGetLastError();
                 }
                 if (e != hipSuccess) rc = fail(LUMAGPU_EHIP, std::string("adjoint launch: ") + hipGetErrorString(e));
@@ -411,6 +417,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     HIP_TRY(hipMemcpyAsync(g->f, fin.p, fbytes, hipMemcpyDeviceToDevice, g->stream));
     g->parity = fin_par;
     g->t = fin_t;
+    g->d.omega = fin_omega;
     g->xr_ready = 0;
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));

@@ -453,12 +453,14 @@ static __global__ void k_adj_objective(IbmArgs ib, double dx, double dy, double 
 }
 
 // acc[k] += sum of part[k*n .. k*n + n), k = 0..nk-1; one block of 256, fixed order
-static __global__ __launch_bounds__(256) void k_adj_sum_partials(const double *part, int n, int nk, double *acc) {
+static __global__ __launch_bounds__(256) void k_adj_sum_partials(const double *part, int n, int nk, double w0,
+                                                                  double *acc) {
     for (int k = 0; k < nk; ++k) {
         double s = 0.0;
         for (int i = threadIdx.x; i < n; i += 256) s += part[(size_t)k * n + i];
         s = block_sum(s);
-        if (threadIdx.x == 0) acc[k] += s;
+        // the first partial (omega) scaled by this step's d omega_t / d omega (1 without a ramp)
+        if (threadIdx.x == 0) acc[k] += k == 0 ? w0 * s : s;
         __syncthreads();
     }
 }

@@ -120,6 +120,33 @@ hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase) {
         default: return launch_edges_f16s(g->Q, a, e, odd, phase, store, g->stream);
     }
 }
+
+// _LBM_updateReynolds (GridObj_ops_lbm_optimised.cpp:1313-1321) at the start of
+// LBM_multi_opt (:39-42): Re = L_RE * getReynoldsRampCoefficient((t + 1) dt)
+// (GridUtils.cpp:1825-1833: 1 - cos(pi t / T) up to T, 1 after), nu = nud2nulbm(1 / Re)
+// (GridUnits.h:128-131), omega = 1 / (nu / cs^2 + 0.5)
+void update_reynolds(lumagpu_grid *g) {
+    const double t = (g->t + 1) * g->d.dt;
+    double coef = 1.0;
+    if (t <= g->re_ramp) coef = 1.0 - std::cos(3.14159265358979323846 * t / g->re_ramp);
+    const double re = g->re * coef;
+    const double nu = ((1.0 / re) * g->d.dt) / (g->d.dh * g->d.dh);
+    const double cs = 1.0 / std::sqrt(3.0);
+    g->d.omega = 1.0 / ((nu / (cs * cs)) + 0.5);
+}
+
+// d omega_t / d omega_end for the step update_reynolds sets up: omega_t = 1 / (nu_t / cs^2 + 0.5)
+// with nu_t = c / (L_RE coef_t), and omega_end, the omega after the ramp (coef = 1), follows
+// L_RE the same way, so d omega_t / d omega_end = omega_t^2 nu_t / (omega_end^2 nu_end)
+double reynolds_omega_weight(const lumagpu_grid *g) {
+    const double t = (g->t + 1) * g->d.dt;
+    double coef = 1.0;
+    if (t <= g->re_ramp) coef = 1.0 - std::cos(3.14159265358979323846 * t / g->re_ramp);
+    const double cs2 = 1.0 / 3.0, c = g->d.dt / (g->d.dh * g->d.dh);
+    const double nt = c / (g->re * coef), ne = c / g->re;
+    const double ot = 1.0 / (nt / cs2 + 0.5), oe = 1.0 / (ne / cs2 + 0.5);
+    return (ot * ot * nt) / (oe * oe * ne);
+}
 }  // namespace lumagpu
 
 namespace {
@@ -165,20 +192,6 @@ hipError_t launch_extrap(const lumagpu_grid *g) {
         case LUMAGPU_FP32T: return launch_aux_dual32(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
         default: return launch_aux_f16s(kAuxExtrapRight, g->Q, a, nullptr, nullptr, xr, g->parity, 1, g->stream);
     }
-}
-
-// _LBM_updateReynolds (GridObj_ops_lbm_optimised.cpp:1313-1321) at the start of
-// LBM_multi_opt (:39-42): Re = L_RE * getReynoldsRampCoefficient((t + 1) dt)
-// (GridUtils.cpp:1825-1833: 1 - cos(pi t / T) up to T, 1 after), nu = nud2nulbm(1 / Re)
-// (GridUnits.h:128-131), omega = 1 / (nu / cs^2 + 0.5)
-void update_reynolds(lumagpu_grid *g) {
-    const double t = (g->t + 1) * g->d.dt;
-    double coef = 1.0;
-    if (t <= g->re_ramp) coef = 1.0 - std::cos(3.14159265358979323846 * t / g->re_ramp);
-    const double re = g->re * coef;
-    const double nu = ((1.0 / re) * g->d.dt) / (g->d.dh * g->d.dh);
-    const double cs = 1.0 / std::sqrt(3.0);
-    g->d.omega = 1.0 / ((nu / (cs * cs)) + 0.5);
 }
 
 // The first half of one grid's LBM_multi_opt once its children have stepped: stream order

@@ -202,6 +202,10 @@ hipError_t outlet_prepass(const lumagpu_grid *g);
 // edge/corner and pressure sites: phase 0 the neighbours' sweep-1 rho, u (erho), phase 1 the
 // sites' step; defined in api.hip
 hipError_t launch_edges(const lumagpu_grid *g, const StepArgs &a, int phase);
+// omega of the coming step under a Reynolds ramp, and its derivative with respect to the
+// omega after the ramp; defined in api.hip
+void update_reynolds(lumagpu_grid *g);
+double reynolds_omega_weight(const lumagpu_grid *g);
 // k_site_macro at a list of cells (every cell when cells is null); defined in ibm.hip
 hipError_t launch_sites(const lumagpu_grid *g, const StepArgs &a, bool odd, int n, const int *cells, int ib_force);
 

@@ -135,7 +135,8 @@ class GpuGrid:
                 positions=True, initial=False):
         """Advance nsteps with checkpoints and, with backward, run the discrete adjoint of them.
 
-        Returns a dict: J (time-averaged dir . IB body force), dJ_domega, dJ_dinlet (derivative
+        Returns a dict: J (time-averaged dir . IB body force), dJ_domega (under a Reynolds ramp:
+        with respect to the omega after the ramp), dJ_dinlet (derivative
         with respect to a scale of the inlet profile), dJ_dpos (n_markers x 3, all bodies in
         marker order) and lambda0 (dJ/df of the initial populations, LUMA layout
         (N, M, K, Q)) when requested."""

@@ -72,7 +72,7 @@ def inflow_case():
     return c, pos
 
 
-def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None, ramp=None):
+def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None, ramp=None, re_ramp=None):
     c = case
     if omega is not None:
         import dataclasses
@@ -86,6 +86,8 @@ def make_grid(case, pos, prec="fp64", omega=None, inlet_scale=1.0, f0=None, ramp
         g.upload_f(f0)
     if ramp is not None:
         g.set_velocity_ramp(ramp)
+    if re_ramp is not None:
+        g.set_reynolds_ramp(*re_ramp)
     g.ibm_add_body(pos)
     g.ibm_initialise()
     return g
@@ -180,6 +182,43 @@ def test_adjoint_with_velocity_ramp(regularised):
     pred = float((adj["lambda0"] * dirn).sum())
     print("dJ/df0 . d adjoint", pred, "fd", fd, flush=True)
     assert close(pred, fd)
+
+
+def test_adjoint_with_reynolds_ramp():
+    # L_REYNOLDS_RAMP over the first 40 of the 60 steps: every reversed step relaxes with its own
+    # omega, and dJ_domega is the derivative with respect to the omega after the ramp, which
+    # L_RE sets (omega = 1 / (3 nu + 1/2), nu = dt / (dh^2 L_RE))
+    case, pos = channel_case()
+    dt, dh = case.info["dt"], case.info["dh"]
+
+    def re_of(omega):
+        return dt / (dh * dh) / ((1.0 / omega - 0.5) / 3.0)
+
+    ramp = 40 * dt
+    T, K = 60, 16
+    print("[adjoint] Reynolds ramp", flush=True)
+    with make_grid(case, pos, re_ramp=(re_of(case.omega), ramp)) as g:
+        adj = g.adj_run(T, K)
+        om_adj = g.omega
+    with make_grid(case, pos, re_ramp=(re_of(case.omega), ramp)) as g:
+        g.adj_run(T, T, backward=False)
+        om_fwd = g.omega
+    # the grid is left at the end of the run with that step's omega
+    assert om_adj == om_fwd
+    h = 1e-5 * case.omega
+    fd = (J_of(case, pos, T, re_ramp=(re_of(case.omega + h), ramp)) -
+          J_of(case, pos, T, re_ramp=(re_of(case.omega - h), ramp))) / (2 * h)
+    print("dJ/domega adjoint", adj["dJ_domega"], "fd", fd, flush=True)
+    assert close(adj["dJ_domega"], fd)
+    # against the unramped run: the weights change the answer
+    with make_grid(case, pos) as g:
+        flat = g.adj_run(T, K)["dJ_domega"]
+    assert not close(adj["dJ_domega"], flat)
+    h = 1e-5
+    fd = (J_of(case, pos, T, inlet_scale=1 + h, re_ramp=(re_of(case.omega), ramp)) -
+          J_of(case, pos, T, inlet_scale=1 - h, re_ramp=(re_of(case.omega), ramp))) / (2 * h)
+    print("dJ/dinlet adjoint", adj["dJ_dinlet"], "fd", fd, flush=True)
+    assert close(adj["dJ_dinlet"], fd)
 
 
 def test_adjoint_universal_epsilon():
