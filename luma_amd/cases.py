@@ -185,6 +185,17 @@ def run_c4_forward(precision: str, steps: int, body: bool = True, warmup: int = 
             "ms_per_step": 1e3 * wall / steps, "mlups": cells * steps / wall / 1e6}
 
 
+def run_c4_adjoint_timing(precision: str, steps: int, every: int = 200):
+    """Wall clock of the C4 gradient alone: ``steps`` forward steps with checkpoints, then the
+    discrete adjoint of them (segments recomputed from their checkpoints), no differences."""
+    pos = c4_markers()
+    a = c4_eval(precision, C4_U, c4_tau(), pos, steps, every, True)
+    cells = 384 * 128 * 128
+    return {"config": "c4_adjoint_timing", "precision": precision, "steps": steps, "checkpoint_every": every,
+            "J": a["J"], "dJ_dU": a["dJ_dU"], "dJ_dtau": a["dJ_dtau"], "wall_s": a["wall_s"],
+            "ms_per_step": 1e3 * a["wall_s"] / steps, "mlups_forward_plus_adjoint": cells * steps / a["wall_s"] / 1e6}
+
+
 def run_c4(precisions=("fp64", "fp32s"), steps=4000, every=200, h=1e-4, fd_markers=(0, 605, 1210),
            shape=(384, 128, 128)):
     pos = c4_markers()
@@ -403,6 +414,9 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--forward", action="store_true",
                     help="c4: forward-step throughput with and without the sphere instead of the adjoint check")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="c4: time forward plus adjoint without the central differences")
+    ap.add_argument("--every", type=int, default=200, help="c4 --adjoint-only: checkpoint interval")
     ap.add_argument("--grad", action="store_true",
                     help="c5: gradient of the mean tip amplitude with respect to bending stiffness")
     ap.add_argument("--check-steps", type=int, default=2000,
@@ -455,6 +469,15 @@ def main(argv=None):
                 r = run_c4_forward(p, a.steps or 400, body)
                 report["runs"].append(r)
                 print(json.dumps(r), flush=True)
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump(report, fh, indent=1)
+        return 0
+    if a.config == "c4" and a.adjoint_only:
+        for p in precs:
+            r = run_c4_adjoint_timing(p, a.steps or 4000, a.every)
+            report["runs"].append(r)
+            print(json.dumps(r), flush=True)
         if a.out:
             with open(a.out, "w") as fh:
                 json.dump(report, fh, indent=1)

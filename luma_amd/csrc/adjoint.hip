@@ -104,6 +104,7 @@ int epsilon_chain(const lumagpu_grid *g, const std::vector<const Marker *> &ms, 
     const double dh = g->d.dh;
     // A_IJ = ds_J sum_k delta(x_I; s^I_k) delta(x_J; s^I_k) area_I; solve A^T mu = epsbar
     std::vector<double> A((size_t)n * n, 0.0);
+    int kl = 0, ku = 0;   // band of A^T's column-major view, which the solvers factor: A's own
     for (int I = 0; I < n; ++I)
         for (int J = 0; J < n; ++J) {
             if (far_apart(*ms[I], *ms[J], D, dh)) continue;
@@ -111,11 +112,20 @@ int epsilon_chain(const lumagpu_grid *g, const std::vector<const Marker *> &ms, 
             for (int k = 0; k < ms[I]->ns; ++k)
                 acc += ms[I]->delta[k] * delta_at(ms[J]->pos, ms[I]->sxyz[k], dh, ms[J]->dilation, D) * ms[I]->area;
             A[(size_t)I * n + J] = acc * ms[J]->ds;
+            if (A[(size_t)I * n + J] != 0.0) {
+                kl = std::max(kl, I - J);
+                ku = std::max(ku, J - I);
+            }
         }
     std::vector<double> At((size_t)n * n), mu(epsbar, epsbar + n);
     for (int I = 0; I < n; ++I)
         for (int J = 0; J < n; ++J) At[(size_t)J * n + I] = A[(size_t)I * n + J];
-    if (int r = lu_solve(n, At, mu)) return r;
+    // markers ordered along a body (a sphere's lattice by height, a filament along its length)
+    // couple only near neighbours in index: the banded solver skips the exact zeros outside the
+    // band and otherwise does the dense solver's operations in its order (the same solution)
+    const int r = (kl + ku < n - 1) ? lu_solve_band(n, kl, ku, At, mu, "singular IB epsilon system")
+                                    : lu_solve(n, At, mu);
+    if (r) return r;
     for (int I = 0; I < n; ++I) {
         const Marker &mI = *ms[I];
         for (int J = 0; J < n; ++J) {
@@ -203,10 +213,10 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         if (int r = b.alloc(fbytes)) return r;
     DevBuf fin, accJ;
     if (int r = fin.alloc(fbytes)) return r;
-    // J; omega (fluid and face sites), inlet scale (sources), inlet scale (faces); omega and
-    // inlet scale (edge and corner sites)
-    if (int r = accJ.alloc(6 * sizeof(double))) return r;
-    HIP_TRY(hipMemsetAsync(accJ.p, 0, 6 * sizeof(double), g->stream));
+    // J; omega (fluid sites) and inlet scale (sources); omega and inlet scale (edge, corner and
+    // pressure sites); omega and inlet scale (regularised faces)
+    if (int r = accJ.alloc(7 * sizeof(double))) return r;
+    HIP_TRY(hipMemsetAsync(accJ.p, 0, 7 * sizeof(double), g->stream));
 
     // forward with checkpoints
     for (int t = 0; t < nsteps; ++t) {
@@ -224,7 +234,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     HIP_TRY(hipMemcpyAsync(fin.p, g->f, fbytes, hipMemcpyDeviceToDevice, g->stream));
     const int fin_par = g->parity, fin_t = g->t;
     const double fin_omega = g->d.omega;
-    double hacc[6] = {0, 0, 0, 0, 0, 0};
+    double hacc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (!backward) {
         HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
@@ -243,9 +253,8 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     if (int r = lam.alloc((size_t)g->Q * nc * sizeof(T))) return r;
     if (int r = gbar.alloc((size_t)g->Q * nc * sizeof(T))) return r;
     if (int r = fbar.alloc(3 * nc * sizeof(double))) return r;
-    // omega (collision), inlet scale (streaming from forced-equilibrium sources), inlet scale
-    // (regularised faces)
-    if (int r = part.alloc(3 * (size_t)nblk * sizeof(double))) return r;
+    // omega (collision), inlet scale (streaming from forced-equilibrium sources)
+    if (int r = part.alloc(2 * (size_t)nblk * sizeof(double))) return r;
     HIP_TRY(hipMemsetAsync(lam.p, 0, (size_t)g->Q * nc * sizeof(T), g->stream));
     HIP_TRY(hipMemsetAsync(fbar.p, 0, 3 * nc * sizeof(double), g->stream));
     AdjArgs<T> ab{static_cast<T *>(lam.p), static_cast<T *>(gbar.p), static_cast<double *>(fbar.p),
@@ -326,6 +335,23 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         HIP_TRY(hipMemcpy(enb_ent.p, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
     }
 
+    // regularised velocity faces (not edges or corners): their own kernel, over a list
+    DevBuf face_cells, fpart;
+    int nface = 0;
+    if (g->d.regularised) {
+        std::vector<int> fc;
+        for (size_t c = 0; c < g->hflags.size(); ++c) {
+            const unsigned fl = g->hflags[c];
+            if ((fl & kTypeMask) == kVelocity && ((fl >> kNormalShift) & 7) != kNormalEdge) fc.push_back((int)c);
+        }
+        nface = (int)fc.size();
+        if (nface > 0) {
+            if (int r = face_cells.alloc(fc.size() * sizeof(int))) return r;
+            if (int r = fpart.alloc(2 * fc.size() * sizeof(double))) return r;
+            HIP_TRY(hipMemcpy(face_cells.p, fc.data(), fc.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
+
     void *const f_own = g->f;
     int rc = 0;
     for (int seg = nck - 1; seg >= 0 && !rc; --seg) {
@@ -361,6 +387,24 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 // the edge sites' extrapolation densities of this step
                 hipError_t e = g->nedge ? launch_edges(g, a, 0) : hipSuccess;
                 if (e == hipSuccess) e = launch_adj_collide<P>(g, a, ab);
+                if (e == hipSuccess && nface) {
+                    const dim3 fb(64), fg((unsigned)((nface + 63) / 64));
+                    const int *fcp = static_cast<const int *>(face_cells.p);
+                    double *fp = static_cast<double *>(fpart.p);
+                    const bool odd = g->parity == 0;
+                    if (g->Q == 9) {
+                        if (odd) k_adj_faces<9, P, true><<<fg, fb, 0, g->stream>>>(a, ab, fcp, nface, fp);
+                        else k_adj_faces<9, P, false><<<fg, fb, 0, g->stream>>>(a, ab, fcp, nface, fp);
+                    } else {
+                        if (odd) k_adj_faces<19, P, true><<<fg, fb, 0, g->stream>>>(a, ab, fcp, nface, fp);
+                        else k_adj_faces<19, P, false><<<fg, fb, 0, g->stream>>>(a, ab, fcp, nface, fp);
+                    }
+                    e = hipGetLastError();
+                    if (e == hipSuccess) {
+                        k_adj_sum_partials<<<1, 1024, 0, g->stream>>>(fp, nface, 2, w_om, acc + 5);
+                        e = hipGetLastError();
+                    }
+                }
                 if (e == hipSuccess && g->nedge) {
                     const dim3 eb(64), eg((unsigned)((g->nedge + 63) / 64));
                     double *db = static_cast<double *>(dbar.p), *tb = static_cast<double *>(tbar.p);
@@ -385,7 +429,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                     }
                     if (e == hipSuccess) e = hipGetLastError();
                     if (e == hipSuccess) {
-                        k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ep, g->nedge, 2, w_om, acc + 4);
+                        k_adj_sum_partials<<<1, 1024, 0, g->stream>>>(ep, g->nedge, 2, w_om, acc + 3);
                         e = hipGetLastError();
                     }
                 }
@@ -405,7 +449,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 }
                 if (e == hipSuccess) e = launch_adj_stream<P>(g, a, ab);
                 if (e == hipSuccess) {
-                    k_adj_sum_partials<<<1, 256, 0, g->stream>>>(ab.part, nblk, 3, w_om, acc + 1);
+                    k_adj_sum_partials<<<1, 1024, 0, g->stream>>>(ab.part, nblk, 2, w_om, acc + 1);
                     e = hipGetLastError();
                 }
                 if (e != hipSuccess) rc = fail(LUMAGPU_EHIP, std::string("adjoint launch: ") + hipGetErrorString(e));
@@ -421,10 +465,20 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
     g->xr_ready = 0;
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(hacc, accJ.p, sizeof(hacc), hipMemcpyDeviceToHost, g->stream));
-    std::vector<T> hl;
     if (lambda0) {
-        hl.resize((size_t)g->Q * nc);
-        HIP_TRY(hipMemcpyAsync(hl.data(), lam.p, hl.size() * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+        // LUMA's layout on the device, then one copy
+        DevBuf lout;
+        if (int r = lout.alloc((size_t)g->Q * nc * sizeof(double))) return r;
+        const unsigned nb = (unsigned)((nc + 255) / 256);
+        if (g->Q == 9)
+            k_adj_to_luma<9, T><<<nb, 256, 0, g->stream>>>(static_cast<const T *>(lam.p), nc, g->d.N, g->d.M, g->d.K,
+                                                         static_cast<double *>(lout.p));
+        else
+            k_adj_to_luma<19, T><<<nb, 256, 0, g->stream>>>(static_cast<const T *>(lam.p), nc, g->d.N, g->d.M,
+                                                          g->d.K, static_cast<double *>(lout.p));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(lambda0, lout.p, (size_t)g->Q * nc * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
     }
     std::vector<double> hx, he, hd;
     if (ib && dpos) {
@@ -436,18 +490,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         HIP_TRY(hipMemcpyAsync(hd.data(), dsbar.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     }
     HIP_TRY(hipStreamSynchronize(g->stream));
-    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1] + hacc[4], hacc[2] + hacc[3] + hacc[5]};
-    if (lambda0) {
-        // LUMA layout f[v + Q*id], id = k + j*K + i*K*M
-        const int N = g->d.N, M = g->d.M, K = g->d.K, Q = g->Q;
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < M; ++j)
-                for (int k = 0; k < K; ++k) {
-                    const size_t id = (size_t)k + (size_t)j * K + (size_t)i * K * M;
-                    const size_t cell = (size_t)i + (size_t)N * ((size_t)j + (size_t)M * k);
-                    for (int v = 0; v < Q; ++v) lambda0[v + Q * id] = double(hl[(size_t)v * nc + cell]);
-                }
-    }
+    if (res) *res = lumagpu_adj_result{hacc[0], hacc[1] + hacc[3] + hacc[5], hacc[2] + hacc[4] + hacc[6]};
     if (ib && dpos) {
         for (size_t i = 0; i < hx.size(); ++i) dpos[i] = hx[i];
         if (int r = position_chains(g, he.data(), hd.data(), dpos)) return r;

@@ -90,36 +90,31 @@ __device__ void adj_regularised_face(const StepArgs &a, unsigned fl, int y, cons
 }
 
 // Transposed collision Jacobian at every fluid site: gbar = (d f'/d g)^T lam, fbar =
-// (d f'/d F)^T lam at forced sites, and the omega derivative sum_v lam_v d f'_v / d omega; at
-// regularised velocity face sites the same through adj_regularised_face, which also gives the
-// inlet-scale derivative (part[2 nblk + block]).
+// (d f'/d F)^T lam at forced sites, and the omega derivative sum_v lam_v d f'_v / d omega
+// (part[block]).  Regularised velocity faces run in k_adj_faces: their forward-mode passes
+// take several times this path's registers, and one kernel holding both ran at one wave per
+// SIMD with spills.
 template <int Q, class P, bool ODD>
 __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typename P::T> b) {
     using L = Lattice<Q>;
     using T = typename P::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
-    double om_site = 0.0, in_site = 0.0;
+    double om_site = 0.0;
     if (x < a.nx) {
         const size_t cell = cell_of(x, y, z, a.nx, a.ny);
         const unsigned fl = a.flags[cell];
-        if (regularised_face_site(a, fl)) {
+        if ((fl & kTypeMask) == kFluid) {
+            // the site's adjoint first: its loads need no flag, so they are in flight with the pulls
+            T lam[Q];
+#pragma unroll
+            for (int v = 0; v < Q; ++v) lam[v] = b.lam[(size_t)v * a.ncell + cell];
             Nbr nb;
             nb.init(x, y, z, a.nx, a.ny, a.nz);
             T fin[Q];
-            gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kVelocity, fin);
-            double g[Q], lam[Q], gb[Q];
-            for (int v = 0; v < Q; ++v) {
-                g[v] = double(fin[v]) + (P::kShift ? L::w(v) * a.rho0 : 0.0);
-                lam[v] = double(b.lam[(size_t)v * a.ncell + cell]);
-            }
-            adj_regularised_face<Q>(a, fl, y, g, lam, gb, om_site, in_site);
-            for (int v = 0; v < Q; ++v) b.gbar[(size_t)v * a.ncell + cell] = T(gb[v]);
-        } else if ((fl & kTypeMask) == kFluid) {
-            Nbr nb;
-            nb.init(x, y, z, a.nx, a.ny, a.nz);
-            T fin[Q];
-            gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kFluid, fin);
+            // plain sites (the step's bitmap, on large grids) pull without reading neighbour flags
+            if (a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u)) gather_plain<Q, P, ODD>(a, cell, nb, fin);
+            else gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kFluid, fin);
             T g[Q];
 #pragma unroll
             for (int v = 0; v < Q; ++v) g[v] = P::kShift ? fin[v] + T(L::w(v) * a.rho0) : fin[v];
@@ -148,13 +143,11 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typenam
             const T usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
             const T uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
             const T om = T(a.omega), guo = T(1) - T(0.5) * om;
-            T lam[Q];
             T S0 = T(0), Ua[3] = {T(0), T(0), T(0)}, Ga[3] = {T(0), T(0), T(0)}, Fb[3] = {T(0), T(0), T(0)};
             T omb = T(0);
 #pragma unroll
             for (int v = 0; v < Q; ++v) {
                 const T cx = T(L::c(v, 0)), cy = T(L::c(v, 1)), cz = T(L::c(v, 2)), w = T(L::w(v));
-                lam[v] = b.lam[(size_t)v * a.ncell + cell];
                 const T cu = cx * u[0] + cy * u[1] + cz * u[2];
                 const T cF = cx * F[0] + cy * F[1] + cz * F[2];
                 const T e = T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq;
@@ -197,9 +190,34 @@ __global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typenam
     }
     const double s = block_sum(om_site);
     if (threadIdx.x == 0) b.part[block_linear()] = s;
-    __syncthreads();
-    const double si = block_sum(in_site);
-    if (threadIdx.x == 0) b.part[2 * b.nblk + block_linear()] = si;
+}
+
+// Regularised velocity face sites, one thread per listed site (cells[i]): gbar from
+// adj_regularised_face at the site's streamed populations; the omega and inlet-scale
+// derivatives per site in fpart[i] and fpart[n + i].
+template <int Q, class P, bool ODD>
+__global__ void k_adj_faces(StepArgs a, AdjArgs<typename P::T> b, const int *cells, int n, double *fpart) {
+    using L = Lattice<Q>;
+    using T = typename P::T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t cell = (size_t)cells[i];
+    const int nx = a.nx, ny = a.ny;
+    const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / ((size_t)nx * ny));
+    Nbr nb;
+    nb.init(x, y, z, nx, ny, a.nz);
+    T fin[Q];
+    gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kVelocity, fin);
+    double g[Q], lam[Q], gb[Q];
+    for (int v = 0; v < Q; ++v) {
+        g[v] = double(fin[v]) + (P::kShift ? L::w(v) * a.rho0 : 0.0);
+        lam[v] = double(b.lam[(size_t)v * a.ncell + cell]);
+    }
+    double om = 0.0, in = 0.0;
+    adj_regularised_face<Q>(a, a.flags[cell], y, g, lam, gb, om, in);
+    for (int v = 0; v < Q; ++v) b.gbar[(size_t)v * a.ncell + cell] = T(gb[v]);
+    fpart[i] = om;
+    fpart[n + i] = in;
 }
 
 // Sites that extrapolate from their two inward neighbours (k_edge_sites), one thread per site:
@@ -323,22 +341,36 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
             const bool xr = (x + 2 < nx) && ((a.flags[cell + 2] & kTypeMask) == kExtrapRight);
             Nbr ns;
             if (xr) ns.init(x + 2, y, z, nx, ny, a.nz);
+            // a plain site (every neighbour fluid) is pulled by its neighbours alone and pulls from
+            // no velocity source: no neighbour flag to read
+            const bool pl = a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u);
+            // every load before the first store: a store to lam would otherwise order the
+            // following loads (of gbar, and of flags, which may alias anything) behind it
+            T acc[Q];
+            if (pl) {
+                static_for<Q>([&](auto wc) {
+                    constexpr int w = decltype(wc)::value;
+                    acc[w] = b.gbar[(size_t)w * a.ncell + nb.at(L::c(w, 0), L::c(w, 1), L::c(w, 2), nx, ny)];
+                });
+            } else {
 #pragma unroll
-            for (int w = 0; w < Q; ++w) {
-                const int cx = L::c(w, 0), cy = L::c(w, 1), cz = L::c(w, 2);
-                const size_t yn = nb.at(cx, cy, cz, nx, ny);
-                const unsigned fn = a.flags[yn];
-                const int tn = fn & kTypeMask;
-                T acc = T(0);
-                if (tn == kFluid || regularised_boundary_site(a, fn)) acc = b.gbar[(size_t)w * a.ncell + yn];
-                else if (tn == kSolid) acc = b.gbar[(size_t)L::opp(w) * a.ncell + cell];
-                if (xr) {
-                    const size_t yp = ns.at(cx, cy, cz, nx, ny);
-                    if ((a.flags[yp] & kTypeMask) == kFluid) acc += b.gbar[(size_t)w * a.ncell + yp];
+                for (int w = 0; w < Q; ++w) {
+                    const size_t yn = nb.at(L::c(w, 0), L::c(w, 1), L::c(w, 2), nx, ny);
+                    const unsigned fn = a.flags[yn];
+                    const int tn = fn & kTypeMask;
+                    acc[w] = T(0);
+                    if (tn == kFluid || regularised_boundary_site(a, fn)) acc[w] = b.gbar[(size_t)w * a.ncell + yn];
+                    else if (tn == kSolid) acc[w] = b.gbar[(size_t)L::opp(w) * a.ncell + cell];
                 }
-                b.lam[(size_t)w * a.ncell + cell] = acc;
             }
-            if (!a.regularised) {
+            if (xr) {
+#pragma unroll
+                for (int w = 0; w < Q; ++w) {
+                    const size_t yp = ns.at(L::c(w, 0), L::c(w, 1), L::c(w, 2), nx, ny);
+                    if ((a.flags[yp] & kTypeMask) == kFluid) acc[w] += b.gbar[(size_t)w * a.ncell + yp];
+                }
+            }
+            if (!a.regularised && !pl) {
                 for (int v = 0; v < Q; ++v) {
                     const size_t src = nb.at(-L::c(v, 0), -L::c(v, 1), -L::c(v, 2), nx, ny);
                     if ((a.flags[src] & kTypeMask) != kVelocity) continue;
@@ -354,6 +386,8 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
                     in_site += double(b.gbar[(size_t)v * a.ncell + cell]) * d;
                 }
             }
+#pragma unroll
+            for (int w = 0; w < Q; ++w) b.lam[(size_t)w * a.ncell + cell] = acc[w];
         } else {
             for (int w = 0; w < Q; ++w) b.lam[(size_t)w * a.ncell + cell] = T(0);
         }
@@ -452,15 +486,41 @@ static __global__ void k_adj_objective(IbmArgs ib, double dx, double dy, double 
     if (lane == 0) acc[0] += s * invT;
 }
 
-// acc[k] += sum of part[k*n .. k*n + n), k = 0..nk-1; one block of 256, fixed order
-static __global__ __launch_bounds__(256) void k_adj_sum_partials(const double *part, int n, int nk, double w0,
-                                                                  double *acc) {
+// The initial populations' adjoint in LUMA's layout f[v + Q*id], id = k + j*K + i*K*M, from
+// the [v][cell] array: one thread per cell, reads coalesced, each thread's Q values contiguous
+template <int Q, class T>
+__global__ void k_adj_to_luma(const T *lam, size_t nc, int N, int M, int K, double *out) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= nc) return;
+    const size_t i = cell % N, j = (cell / N) % M, k = cell / ((size_t)N * M);
+    const size_t id = k + j * K + i * (size_t)K * M;
+    for (int v = 0; v < Q; ++v) out[v + Q * id] = double(lam[(size_t)v * nc + cell]);
+}
+
+// acc[k] += sum of part[k*n .. k*n + n), k = 0..nk-1; one block of 1024, each thread with
+// four independent running sums (loads in flight), then a fixed tree: the order is fixed
+static __global__ __launch_bounds__(1024) void k_adj_sum_partials(const double *part, int n, int nk, double w0,
+                                                                   double *acc) {
+    __shared__ double sh[1024];
     for (int k = 0; k < nk; ++k) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < n; i += 256) s += part[(size_t)k * n + i];
-        s = block_sum(s);
+        const double *p = part + (size_t)k * n;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int i = threadIdx.x;
+        for (; i + 3 * 1024 < n; i += 4 * 1024) {
+            s0 += p[i];
+            s1 += p[i + 1024];
+            s2 += p[i + 2 * 1024];
+            s3 += p[i + 3 * 1024];
+        }
+        for (; i < n; i += 1024) s0 += p[i];
+        sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
+        __syncthreads();
+        for (int st = 512; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+            __syncthreads();
+        }
         // the first partial (omega) scaled by this step's d omega_t / d omega (1 without a ramp)
-        if (threadIdx.x == 0) acc[k] += k == 0 ? w0 * s : s;
+        if (threadIdx.x == 0) acc[k] += k == 0 ? w0 * sh[0] : sh[0];
         __syncthreads();
     }
 }

@@ -400,8 +400,8 @@ int lu_solve_band_t(int n, int kl, int ku, std::vector<Sc> &a, std::vector<Sc> &
     return 0;
 }
 
-int lu_solve_band(int n, int kl, int ku, std::vector<double> &a, std::vector<double> &x) {
-    return lu_solve_band_t(n, kl, ku, a, x);
+int lu_solve_band(int n, int kl, int ku, std::vector<double> &a, std::vector<double> &x, const char *singular) {
+    return lu_solve_band_t(n, kl, ku, a, x, singular);
 }
 int lu_solve_band(int n, int kl, int ku, std::vector<Dual> &a, std::vector<Dual> &x) {
     return lu_solve_band_t(n, kl, ku, a, x);

@@ -126,7 +126,8 @@ int apply_core(lumagpu_grid *g);
 int lu_solve(int n, std::vector<double> &a, std::vector<double> &x);
 int lu_solve(int n, std::vector<Dual> &a, std::vector<Dual> &x);
 // the same solve for a matrix with kl sub- and ku super-diagonals (zeros elsewhere)
-int lu_solve_band(int n, int kl, int ku, std::vector<double> &a, std::vector<double> &x);
+int lu_solve_band(int n, int kl, int ku, std::vector<double> &a, std::vector<double> &x,
+                  const char *singular = "singular beam system");
 int lu_solve_band(int n, int kl, int ku, std::vector<Dual> &a, std::vector<Dual> &x);
 
 }  // namespace lumagpu
