@@ -43,7 +43,7 @@ struct DevBuf {
 
 template <class P>
 hipError_t launch_adj_collide(const lumagpu_grid *g, const StepArgs &a, const AdjArgs<typename P::T> &b) {
-    const dim3 bl(256), gr((a.nx + 255) / 256, a.ny, a.nz);
+    const dim3 bl = cell_block(a.nx), gr((a.nx + bl.x - 1) / bl.x, a.ny, a.nz);
     const bool odd = g->parity == 0;
     if (g->Q == 9) {
         if (odd) k_adj_collide<9, P, true><<<gr, bl, 0, g->stream>>>(a, b);
@@ -57,7 +57,7 @@ hipError_t launch_adj_collide(const lumagpu_grid *g, const StepArgs &a, const Ad
 
 template <class P>
 hipError_t launch_adj_stream(const lumagpu_grid *g, const StepArgs &a, const AdjArgs<typename P::T> &b) {
-    const dim3 bl(256), gr((a.nx + 255) / 256, a.ny, a.nz);
+    const dim3 bl = cell_block(a.nx), gr((a.nx + bl.x - 1) / bl.x, a.ny, a.nz);
     if (g->Q == 9) k_adj_stream<9, P><<<gr, bl, 0, g->stream>>>(a, b);
     else k_adj_stream<19, P><<<gr, bl, 0, g->stream>>>(a, b);
     return hipGetLastError();
@@ -244,7 +244,8 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
 
     // backward buffers
     const size_t nc = g->ncell;
-    const int nblk = ((g->d.N + 255) / 256) * g->d.M * g->d.K;
+    // blocks of the collide and stream launches (cell_block rows)
+    const int nblk = (int)((g->d.N + cell_block(g->d.N).x - 1) / cell_block(g->d.N).x) * g->d.M * g->d.K;
     std::vector<DevBuf> tape(every);
     std::vector<int> tpar(every);
     for (auto &b : tape)

@@ -37,16 +37,17 @@ __device__ __forceinline__ int block_linear() {
     return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 }
 
-// deterministic block sum (256 threads)
+// deterministic block sum (blocks of 64 to 1024 threads, a multiple of 64): a fixed tree over
+// each wave's lanes, then the waves' sums in wave order; every thread gets the sum
 __device__ __forceinline__ double block_sum(double v) {
-    __shared__ double sh[256];
-    sh[threadIdx.x] = v;
+    __shared__ double sh[16];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    return sh[0];
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    __syncthreads();
+    return s;
 }
 
 // A regularised velocity face site (not an edge or corner) of a grid stepped with
