@@ -95,8 +95,11 @@ __device__ void adj_regularised_face(const StepArgs &a, unsigned fl, int y, cons
 // (part[block]).  Regularised velocity faces run in k_adj_faces: their forward-mode passes
 // take several times this path's registers, and one kernel holding both ran at one wave per
 // SIMD with spills.
+// FP64 D3Q19: at most 256 VGPRs, two waves per SIMD (it needs 286 for one: 1.83 ms against
+// 1.21 ms with 28 B of spills on 6.3 M cells)
 template <int Q, class P, bool ODD>
-__global__ __launch_bounds__(256) void k_adj_collide(StepArgs a, AdjArgs<typename P::T> b) {
+__global__ __launch_bounds__(256, (sizeof(typename P::T) == 8 && Q == 19) ? 2 : 1) void k_adj_collide(
+    StepArgs a, AdjArgs<typename P::T> b) {
     using L = Lattice<Q>;
     using T = typename P::T;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
