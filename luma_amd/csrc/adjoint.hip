@@ -336,6 +336,14 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
         HIP_TRY(hipMemcpy(enb_ent.p, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
     }
 
+    // the adjoint kernels read the plain-site bitmap without a null check: a grid whose flags
+    // the IB set-up made (no bitmap) gets one with no plain site
+    DevBuf nobits;
+    if (!g->plain) {
+        if (int r = nobits.alloc((nc + 31) / 32 * sizeof(uint32_t))) return r;
+        HIP_TRY(hipMemsetAsync(nobits.p, 0, (nc + 31) / 32 * sizeof(uint32_t), g->stream));
+    }
+
     // regularised velocity faces (not edges or corners): their own kernel, over a list
     DevBuf face_cells, fpart;
     int nface = 0;
@@ -383,7 +391,8 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 update_reynolds(g);
                 w_om = reynolds_omega_weight(g);
             }
-            const StepArgs a = make_args(g);
+            StepArgs a = make_args(g);
+            if (!a.plain) a.plain = static_cast<const uint32_t *>(nobits.p);
             if (!rc) {
                 // the edge sites' extrapolation densities of this step
                 hipError_t e = g->nedge ? launch_edges(g, a, 0) : hipSuccess;

@@ -107,8 +107,11 @@ __global__ __launch_bounds__(256, (sizeof(typename P::T) == 8 && Q == 19) ? 2 : 
     double om_site = 0.0;
     if (x < a.nx) {
         const size_t cell = cell_of(x, y, z, a.nx, a.ny);
+        // the flag byte and the plain-site bit in one round trip (plain sites are fluid sites
+        // whose pulls read no neighbour flag)
+        const bool pl = (a.plain[cell >> 5] >> (cell & 31)) & 1u;   // never null in the adjoint
         const unsigned fl = a.flags[cell];
-        if ((fl & kTypeMask) == kFluid) {
+        if (pl || (fl & kTypeMask) == kFluid) {   // a plain site is a fluid site
             // the site's adjoint first: its loads need no flag, so they are in flight with the pulls
             T lam[Q];
 #pragma unroll
@@ -116,8 +119,7 @@ __global__ __launch_bounds__(256, (sizeof(typename P::T) == 8 && Q == 19) ? 2 : 
             Nbr nb;
             nb.init(x, y, z, a.nx, a.ny, a.nz);
             T fin[Q];
-            // plain sites (the step's bitmap, on large grids) pull without reading neighbour flags
-            if (a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u)) gather_plain<Q, P, ODD>(a, cell, nb, fin);
+            if (pl) gather_plain<Q, P, ODD>(a, cell, nb, fin);
             else gather_in<Q, P, ODD>(a, x, y, z, cell, nb, kFluid, fin);
             T g[Q];
 #pragma unroll
@@ -337,17 +339,19 @@ __global__ __launch_bounds__(256) void k_adj_stream(StepArgs a, AdjArgs<typename
     if (x < a.nx) {
         const int nx = a.nx, ny = a.ny;
         const size_t cell = cell_of(x, y, z, nx, ny);
+        // the site's flag, the plain-site bit and the flag two sites downstream in one round trip.
+        // A plain site (every neighbour fluid) is pulled by its neighbours alone and pulls from no
+        // velocity source: no neighbour flag to read
+        const bool pl = (a.plain[cell >> 5] >> (cell & 31)) & 1u;   // never null in the adjoint
+        const unsigned fx = a.flags[x + 2 < nx ? cell + 2 : cell];
+        const bool xr = (x + 2 < nx) && ((fx & kTypeMask) == kExtrapRight);
         const unsigned fl = a.flags[cell];
         const int ty = fl & kTypeMask;
-        if (ty == kFluid || regularised_boundary_site(a, fl)) {
+        if (pl || ty == kFluid || regularised_boundary_site(a, fl)) {   // a plain site is a fluid site
             Nbr nb;
             nb.init(x, y, z, nx, ny, a.nz);
-            const bool xr = (x + 2 < nx) && ((a.flags[cell + 2] & kTypeMask) == kExtrapRight);
             Nbr ns;
             if (xr) ns.init(x + 2, y, z, nx, ny, a.nz);
-            // a plain site (every neighbour fluid) is pulled by its neighbours alone and pulls from
-            // no velocity source: no neighbour flag to read
-            const bool pl = a.plain && ((a.plain[cell >> 5] >> (cell & 31)) & 1u);
             // every load before the first store: a store to lam would otherwise order the
             // following loads (of gbar, and of flags, which may alias anything) behind it
             T acc[Q];
