@@ -445,7 +445,7 @@ int adj_run(lumagpu_grid *g, int nsteps, int every, const double dir[3], int bac
                 }
                 if (e == hipSuccess && ib) {
                     IbmArgs ia = ibm_args(g);
-                    k_adj_ibm_marker<<<(s->nm + 127) / 128, 128, 0, g->stream>>>(ia, aj);
+                    k_adj_ibm_marker<<<(s->nm + 3) / 4, 256, 0, g->stream>>>(ia, aj);
                     e = hipGetLastError();
                     if (e == hipSuccess) {
                         if (g->Q == 9)

@@ -417,19 +417,29 @@ struct AdjIbm {
     double invT;
 };
 
-static __global__ void k_adj_ibm_marker(IbmArgs ib, AdjIbm aj) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= ib.nm) return;
+// sum over a 64-lane wave, fixed tree, lane 0's result in every lane (so every lane goes on
+// with the same value)
+__device__ __forceinline__ double wave_sum_bcast(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return __shfl(v, 0, 64);
+}
+
+// One 64-lane wave per marker (blocks of 256: four markers): the lanes take the marker's
+// supports, and the two sums over them are wave reductions.
+static __global__ __launch_bounds__(256) void k_adj_ibm_marker(IbmArgs ib, AdjIbm aj) {
+    const int m = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (m >= ib.nm) return;   // the whole wave
     const int D = ib.D;
     const double eps = ib.eps[m], ds = ib.ds[m];
     const double vd = (D == 3) ? ds : 1.0;
     const double wgt = eps * ds * vd;
     double Fs[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < ib.ns[m]; ++k) {
+    for (int k = lane; k < ib.ns[m]; k += 64) {
         const size_t cell = (size_t)ib.scell[m * ib.S + k];
         const double dl = ib.sdelta[m * ib.S + k];
         for (int d = 0; d < D; ++d) Fs[d] += dl * aj.fbar[(size_t)d * ib.ncell + cell];
     }
+    for (int d = 0; d < 3; ++d) Fs[d] = wave_sum_bcast(Fs[d]);
     double Fb[3], mom[3], FdotFs = 0.0, Fdir = 0.0, rb = 0.0;
     for (int d = 0; d < 3; ++d) {
         const double Fm = (d < D) ? ib.force[3 * m + d] : 0.0;
@@ -439,12 +449,14 @@ static __global__ void k_adj_ibm_marker(IbmArgs ib, AdjIbm aj) {
         FdotFs += Fm * Fs[d];
         Fdir += Fm * aj.dir[d];
     }
-    for (int d = 0; d < 3; ++d) aj.mbar[4 * m + d] = mom[d];
-    aj.mbar[4 * m + 3] = rb;
-    aj.epsbar[m] += -ds * vd * FdotFs + Fdir * ds * vd * aj.invT;
-    aj.dsbar[m] += ((D == 3) ? 2.0 * eps * ds : eps) * (-FdotFs + Fdir * aj.invT);
+    if (lane == 0) {
+        for (int d = 0; d < 3; ++d) aj.mbar[4 * m + d] = mom[d];
+        aj.mbar[4 * m + 3] = rb;
+        aj.epsbar[m] += -ds * vd * FdotFs + Fdir * ds * vd * aj.invT;
+        aj.dsbar[m] += ((D == 3) ? 2.0 * eps * ds : eps) * (-FdotFs + Fdir * aj.invT);
+    }
     double xb[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < ib.ns[m]; ++k) {
+    for (int k = lane; k < ib.ns[m]; k += 64) {
         const size_t cell = (size_t)ib.scell[m * ib.S + k];
         const double r = ib.rho[cell];
         double term = rb * r;
@@ -454,7 +466,9 @@ static __global__ void k_adj_ibm_marker(IbmArgs ib, AdjIbm aj) {
         }
         for (int d = 0; d < D; ++d) xb[d] += aj.sddelta[((size_t)m * ib.S + k) * 3 + d] * term;
     }
-    for (int d = 0; d < D; ++d) aj.xbar[3 * m + d] += xb[d];
+    for (int d = 0; d < 3; ++d) xb[d] = wave_sum_bcast(xb[d]);
+    if (lane == 0)
+        for (int d = 0; d < D; ++d) aj.xbar[3 * m + d] += xb[d];
 }
 
 // IB adjoint, site side: the interpolation's adjoint spread through the per-site gather lists
