@@ -375,7 +375,8 @@ int lumagpu_checkpoint_free(lumagpu_checkpoint *ck);
  * ExtrapolateRight sites, body force, rigid IB bodies (supports on fluid sites), a velocity
  * ramp, a Reynolds ramp.  Refused (LUMAGPU_EUNSUPPORTED): refinement, z-slabs, gravity,
  * flexible bodies.  Memory: ceil(T/K) + K + 1
- * population arrays plus three adjoint arrays. */
+ * population arrays plus three adjoint arrays (and, while lambda0 is written, one array of
+ * Q FP64 values per cell for LUMA's layout). */
 typedef struct lumagpu_adj_result {
     double J;
     double dJ_domega;

@@ -12,7 +12,7 @@
 // Body_<id>_LD, ObjectManager_ops_io.cpp:98-121, averaged over the T steps).
 //
 // The adjoint runs the step backwards: the collision's transposed Jacobian per site
-// (k_adj_collide), then spread and interpolation swapped (the spread's adjoint gathers the
+// (k_adj_collide; regularised faces in k_adj_faces, edges and corners in k_adj_edges), then spread and interpolation swapped (the spread's adjoint gathers the
 // site force adjoint to the markers, k_adj_ibm_marker; the interpolation's adjoint spreads the
 // marker adjoint to the sites through the same per-site gather lists, k_adj_ibm_site), then
 // reverse streaming (k_adj_stream: every population adjoint is gathered from the site that
