@@ -184,11 +184,14 @@ def test_adjoint_with_velocity_ramp(regularised):
     assert close(pred, fd)
 
 
-def test_adjoint_with_reynolds_ramp():
+@pytest.mark.parametrize("regularised", [False, True])
+def test_adjoint_with_reynolds_ramp(regularised):
     # L_REYNOLDS_RAMP over the first 40 of the 60 steps: every reversed step relaxes with its own
     # omega, and dJ_domega is the derivative with respect to the omega after the ramp, which
-    # L_RE sets (omega = 1 / (3 nu + 1/2), nu = dt / (dh^2 L_RE))
-    case, pos = channel_case()
+    # L_RE sets (omega = 1 / (3 nu + 1/2), nu = dt / (dh^2 L_RE)); with the regularised inlet
+    # the faces' omega partials take the same per-step weights
+    case = channel_case()[0] if not regularised else channel_reg_case()[0]
+    pos = channel_case()[1]
     dt, dh = case.info["dt"], case.info["dh"]
 
     def re_of(omega):
@@ -196,7 +199,7 @@ def test_adjoint_with_reynolds_ramp():
 
     ramp = 40 * dt
     T, K = 60, 16
-    print("[adjoint] Reynolds ramp", flush=True)
+    print(f"[adjoint] Reynolds ramp regularised={regularised}", flush=True)
     with make_grid(case, pos, re_ramp=(re_of(case.omega), ramp)) as g:
         adj = g.adj_run(T, K)
         om_adj = g.omega
